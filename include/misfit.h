@@ -1,0 +1,213 @@
+/*
+ * misfit.h -- C ABI of the MI355X MISFIT engine (libmisfit_amd.so).
+ *
+ * Plain C types only: no HIP or torch types cross this boundary.  Every call
+ * returns 0 on success or a negative MISFIT_E_* code; the message of the last
+ * failure is available from misfit_last_error().  No call ever exits the
+ * process.  Host buffers are borrowed for the duration of the call only;
+ * device state belongs to the context.  A context is driven by one host
+ * thread at a time (callers synchronise externally); several contexts may be
+ * used from several threads.
+ *
+ * The reference (anonymous-artifacts/MISFIT) has no C ABI: its engine is the
+ * C++ class GPUVertexClusteringGraph inside src/misfit/misfit.cu (and the
+ * same class in src/misfit_async/misfit_async.cu).  Each entry point below
+ * names the reference interface it replaces.
+ */
+#ifndef MISFIT_AMD_H
+#define MISFIT_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MISFIT_OK 0
+#define MISFIT_E_ARG (-1)        /* invalid argument */
+#define MISFIT_E_HIP (-2)        /* HIP runtime failure */
+#define MISFIT_E_STATE (-3)      /* call out of order (no graph, no MIS, ...) */
+#define MISFIT_E_NOMEM (-4)      /* device or host allocation failed */
+#define MISFIT_E_UNSUPPORTED (-5)/* input the engine does not handle */
+#define MISFIT_E_IO (-6)         /* file input/output failure */
+#define MISFIT_E_CHECK (-7)      /* an internal consistency check failed */
+
+typedef struct misfit_ctx misfit_ctx;
+
+/* One edge update.  Vertices are 0-indexed.  is_insertion follows the
+ * reference's flag: computed once against the graph as loaded
+ * (src/misfit/misfit.cu:822, core/cli_setup.h:77) and then applied as
+ * "insert if absent" / "delete if present" (src/misfit/misfit.cu:708-714). */
+typedef struct {
+    int32_t u;
+    int32_t v;
+    uint8_t is_insertion;
+    uint8_t pad[3];
+} misfit_update;
+
+/* Per-batch statistics (counts from the device, times from HIP events). */
+typedef struct {
+    int64_t updates;          /* updates in the batch */
+    int64_t seeds;            /* distinct endpoints */
+    int64_t active_ins;       /* insertions with both ends in the MIS */
+    int64_t evictions;
+    int64_t admit_ins;        /* re-admissions on the insertion branch */
+    int64_t active_del;       /* deletions with exactly one end in the MIS */
+    int64_t admit_del;        /* admissions on the deletion branch */
+    int64_t completion_adds;  /* vertices added by the completion pass */
+    int64_t rounds;           /* speculative rounds of the exact replay */
+    int64_t evaluations;      /* edge evaluations over all rounds */
+    int64_t maybe_uncovered;  /* vertices the batch may uncover (PU) */
+    int64_t admissible;       /* ... of which have a row */
+    int64_t delta_ops;        /* effective overlay operations */
+    int64_t overlay_entries;  /* pool slots in use after the batch */
+    double device_ms;         /* batch time on the device (events) */
+} misfit_batch_stats;
+
+/* Execution modes of the repair. */
+#define MISFIT_MODE_PLAIN 0       /* per-phase kernels, host-driven rounds */
+#define MISFIT_MODE_PERSISTENT 1  /* one cooperative launch per batch */
+
+/* Replaces the GPUVertexClusteringGraph constructor's device selection and
+ * stream creation (src/misfit/misfit.cu:389-421, core/cli_setup.h:43-47). */
+int misfit_ctx_create(int device, misfit_ctx** out);
+
+/* Replaces ~GPUVertexClusteringGraph (src/misfit/misfit.cu:423-439). */
+int misfit_ctx_destroy(misfit_ctx* ctx);
+
+/* Message of the last failed call on ctx (never NULL). */
+const char* misfit_last_error(const misfit_ctx* ctx);
+
+/* Streams the base CSR into HBM once.  nindex has n+1 entries, int64 when
+ * nindex_is_64 else int32; nlist has m int32 entries.  Replaces the mmap'd
+ * FlashGraph reads of the reference (core/FlashGraph.h:21-170), which the
+ * engine no longer needs after this call.  Resets the overlay and the MIS. */
+int misfit_load_csr(misfit_ctx* ctx, int32_t n, int64_t m, const void* nindex, int nindex_is_64,
+                    const int32_t* nlist);
+
+/* Sets the MIS (one byte per vertex, nonzero = member).  Replaces
+ * GPUVertexClusteringGraph::setMISFromHost (src/misfit/misfit.cu:448-453). */
+int misfit_set_mis(misfit_ctx* ctx, const uint8_t* mis);
+
+/* Reads the MIS back.  Replaces getMISToHost (src/misfit/misfit.cu:455-462). */
+int misfit_get_mis(misfit_ctx* ctx, uint8_t* mis);
+
+/* completion_on = 1 (default): after the reference rule, add the greedy
+ * ascending-id completion that keeps the MIS maximal; 0: the literal rule
+ * alone.  The reference has only the literal rule (src/misfit/misfit.cu:
+ * 277-349). */
+int misfit_set_rule(misfit_ctx* ctx, int completion_on);
+
+/* MISFIT_MODE_PLAIN or MISFIT_MODE_PERSISTENT (results are identical). */
+int misfit_set_mode(misfit_ctx* ctx, int mode);
+
+/* Insertion flags against the current graph: ins[i] = !adjacent(u[i], v[i]).
+ * Replaces the HybridGraph::isAdjacent call of batch parsing
+ * (core/cli_setup.h:77, src/misfit/misfit.cu:822). */
+int misfit_flags(misfit_ctx* ctx, const int32_t* u, const int32_t* v, int64_t count, uint8_t* ins);
+
+/* Queues one batch: overlay update, exact repair, completion.  Returns at
+ * once in persistent mode (the host buffer is copied before the call
+ * returns).  Replaces the async engine's launchBatch
+ * (src/misfit_async/misfit_async.cu:484-675) together with the overlay
+ * update of launchBatchGPU (:718-740). */
+int misfit_submit_batch(misfit_ctx* ctx, const misfit_update* ups, int64_t count, uint64_t* ticket);
+
+/* Waits for a submitted batch and returns its statistics (st may be NULL).
+ * Replaces syncAndCollect (src/misfit_async/misfit_async.cu:677-701). */
+int misfit_wait(misfit_ctx* ctx, uint64_t ticket, misfit_batch_stats* st);
+
+/* submit + wait.  Replaces processBatchGPUVertexClustering /
+ * processBatchWithVertexClustering (src/misfit/misfit.cu:468-730). */
+int misfit_process_batch(misfit_ctx* ctx, const misfit_update* ups, int64_t count, misfit_batch_stats* st);
+
+/* Waits for all queued work.  Replaces synchronizeChannels
+ * (src/misfit/misfit.cu:464-466). */
+int misfit_sync(misfit_ctx* ctx);
+
+/* Full check over base plus overlay: conflicts = edges with both ends in the
+ * MIS, uncovered = non-members without an MIS neighbour, count_errors =
+ * vertices whose maintained MIS-neighbour count disagrees with a recount
+ * (always 0 unless the engine is broken).  No reference counterpart; the
+ * reference leaves checking to an external baseline
+ * (src/misfit/misfit.cu:890-891). */
+int misfit_check(misfit_ctx* ctx, int64_t* conflicts, int64_t* uncovered, int64_t* count_errors);
+
+/* Merges the overlay into a new base on the device (base order minus
+ * deletions, then insertions in insertion order) and clears the overlay.
+ * Replaces compact_flash_graph + remap + delta.clear
+ * (core/flash_compactor.h:20-86, src/misfit/misfit.cu:916-923); the file
+ * write is misfit_write_egr below. */
+int misfit_compact(misfit_ctx* ctx, int64_t* new_m);
+
+/* Current graph size: n, base entries, live entries (base + overlay). */
+int misfit_graph_size(misfit_ctx* ctx, int32_t* n, int64_t* base_m, int64_t* live_m);
+
+/* Copies the base CSR (call misfit_compact first to include the overlay) to
+ * host buffers of n+1 int64 and base_m int32. */
+int misfit_export_csr(misfit_ctx* ctx, int64_t* nindex, int32_t* nlist);
+
+/* Neighbours of v in the current graph, in the reference's order
+ * (core/FlashGraph.h:214-233).  Returns the count through *count; writes at
+ * most cap entries. */
+int misfit_neighbors(misfit_ctx* ctx, int32_t v, int32_t* out, int64_t cap, int64_t* count);
+
+/* Algorithmic bytes of the most recent batch on the current graph:
+ * 16*U + sum over V1 of (I + 4*deg) + |V2|, V1 = S ∪ N(S), V2 = V1 ∪ N(V1),
+ * I = index_bytes (8 for an int32 .egr, 16 for int64).  Measurement aid. */
+int misfit_batch_bytes(misfit_ctx* ctx, int index_bytes, int64_t* bytes, int64_t* v1, int64_t* v2);
+
+/* ---- host I/O (same library, no device needed) ---- */
+
+/* .egr reader (core/FlashGraph.h:108-169 format; both nindex widths).  The
+ * file is memory-mapped; the returned pointers stay valid until
+ * misfit_egr_close. */
+typedef struct {
+    int32_t n;
+    int64_t m;
+    int wide;               /* 1: int64 nindex (-1 sentinel form) */
+    const void* nindex;     /* n+1 entries, int64 if wide else int32 */
+    const int32_t* nlist;   /* m entries */
+    void* priv;
+} misfit_egr;
+
+int misfit_egr_open(const char* path, misfit_egr* out, char* err, int errlen);
+int misfit_egr_close(misfit_egr* g);
+
+/* Writes the wide form atomically: <path>.tmp.<pid>, fflush, fsync, rename,
+ * directory fsync (core/flash_compactor.h:40-104). */
+int misfit_write_egr(const char* path, int32_t n, int64_t m, const int64_t* nindex, const int32_t* nlist,
+                     char* err, int errlen);
+
+/* ---- synthetic workloads (seeded, deterministic) ---- */
+
+/* Erdős–Rényi G(n, M): M distinct undirected pairs, symmetric CSR with rows
+ * ascending.  Returns entries via *m; call with nindex = NULL to size. */
+int misfit_gen_er(int32_t n, int64_t M, uint64_t seed, int64_t* nindex, int32_t* nlist, int64_t* m);
+
+/* R-MAT (Chakrabarti, Zhan, Faloutsos 2004): 2^scale vertices,
+ * edgefactor*2^scale generated edges, quadrant probabilities a,b,c (d = 1-a-b-c),
+ * self-loops dropped, symmetrised, deduplicated, rows ascending. */
+int misfit_gen_rmat(int scale, int edgefactor, double a, double b, double c, uint64_t seed,
+                    int64_t* nindex, int32_t* nlist, int64_t* m);
+
+/* 2-D random geometric graph: n points uniform in the unit square, an edge
+ * when the distance is below r = sqrt(avg_degree / (pi n)). */
+int misfit_gen_rgg(int32_t n, double avg_degree, uint64_t seed, int64_t* nindex, int32_t* nlist, int64_t* m);
+
+/* Greedy MIS in ascending id on a CSR (tools/compute_initial_mis.py:32-43). */
+int misfit_greedy_mis(int32_t n, const int64_t* nindex, const int32_t* nlist, uint8_t* mis);
+
+/* Update streams (u, v 0-indexed), written to u/v arrays of count entries.
+ *   kind 0: uniform vertex pairs u != v
+ *   kind 1: half deletions of uniform existing edges, half insertions of
+ *           uniform non-edges, distinct across the whole stream, shuffled
+ *   kind 2: both endpoints drawn proportionally to (degree + 1)
+ *   kind 3: endpoints drawn from the lowest-degree quarter of the vertices */
+int misfit_gen_updates(int kind, int32_t n, const int64_t* nindex, const int32_t* nlist, int64_t count,
+                       uint64_t seed, int32_t* u, int32_t* v);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,375 @@
+"""misfit_amd -- MI355X-native incremental maximal-independent-set maintenance.
+
+Python view of the C ABI in include/misfit.h (libmisfit_amd.so, HIP kernels
+for gfx950).  The engine runs only on the GPU: if the library is missing or
+the device cannot be opened, every entry point raises; there is no CPU
+fallback.
+
+Reference interface mirrored (anonymous-artifacts/MISFIT):
+  Engine(device)              GPUVertexClusteringGraph ctor  src/misfit/misfit.cu:389-421
+  Engine.load_csr / load_egr  FlashGraph::open               core/FlashGraph.h:32-35
+  Engine.set_mis / get_mis    setMISFromHost / getMISToHost  src/misfit/misfit.cu:448-462
+  Engine.flags                isAdjacent flag at parse time  core/cli_setup.h:66-82
+  Engine.process_batch        processBatchGPUVertexClustering src/misfit/misfit.cu:704-730
+  Engine.submit / wait        launchBatch / syncAndCollect   src/misfit_async/misfit_async.cu:484-701
+  Engine.compact              compact_flash_graph            core/flash_compactor.h:20-105
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmisfit_amd.so")
+BIN_DIR = os.path.join(_HERE, "bin")
+
+MODE_PLAIN = 0
+MODE_PERSISTENT = 1
+
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+
+UPDATE_DTYPE = np.dtype([("u", np.int32), ("v", np.int32), ("ins", np.uint8), ("pad", np.uint8, 3)])
+
+ERRORS = {
+    -1: "invalid argument", -2: "HIP runtime failure", -3: "call out of order",
+    -4: "allocation failed", -5: "unsupported input", -6: "I/O failure", -7: "consistency check failed",
+}
+
+
+class MisfitError(RuntimeError):
+    def __init__(self, code, msg=""):
+        super().__init__("%s (%d)%s" % (ERRORS.get(code, "error"), code, (": " + msg) if msg else ""))
+        self.code = code
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int64) for k in (
+        "updates", "seeds", "active_ins", "evictions", "admit_ins", "active_del", "admit_del",
+        "completion_adds", "rounds", "evaluations", "maybe_uncovered", "admissible", "delta_ops",
+        "overlay_entries")] + [("device_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class EgrView(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32), ("m", ctypes.c_int64), ("wide", ctypes.c_int),
+                ("nindex", ctypes.c_void_p), ("nlist", ctypes.c_void_p), ("priv", ctypes.c_void_p)]
+
+
+# every symbol include/misfit.h declares
+EXPORTS = (
+    "misfit_ctx_create", "misfit_ctx_destroy", "misfit_last_error", "misfit_load_csr",
+    "misfit_set_mis", "misfit_get_mis", "misfit_set_rule", "misfit_set_mode", "misfit_flags",
+    "misfit_submit_batch", "misfit_wait", "misfit_process_batch", "misfit_sync", "misfit_check",
+    "misfit_compact", "misfit_graph_size", "misfit_export_csr", "misfit_neighbors",
+    "misfit_batch_bytes", "misfit_egr_open", "misfit_egr_close", "misfit_write_egr",
+    "misfit_gen_er", "misfit_gen_rmat", "misfit_gen_rgg", "misfit_greedy_mis", "misfit_gen_updates",
+)
+
+_lib = None
+
+
+def lib():
+    """The loaded libmisfit_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libmisfit_amd.so not built: run `make -C misfit_amd/csrc` "
+                              "(or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.misfit_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+        L.misfit_ctx_destroy.argtypes = [vp]
+        L.misfit_last_error.restype = ctypes.c_char_p
+        L.misfit_last_error.argtypes = [vp]
+        L.misfit_load_csr.argtypes = [vp, ctypes.c_int32, ctypes.c_int64, vp, ctypes.c_int, _i32p]
+        L.misfit_set_mis.argtypes = [vp, _u8p]
+        L.misfit_get_mis.argtypes = [vp, _u8p]
+        L.misfit_set_rule.argtypes = [vp, ctypes.c_int]
+        L.misfit_set_mode.argtypes = [vp, ctypes.c_int]
+        L.misfit_flags.argtypes = [vp, _i32p, _i32p, ctypes.c_int64, _u8p]
+        L.misfit_submit_batch.argtypes = [vp, vp, ctypes.c_int64, _u64p]
+        L.misfit_wait.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(BatchStats)]
+        L.misfit_process_batch.argtypes = [vp, vp, ctypes.c_int64, ctypes.POINTER(BatchStats)]
+        L.misfit_sync.argtypes = [vp]
+        L.misfit_check.argtypes = [vp, _i64p, _i64p, _i64p]
+        L.misfit_compact.argtypes = [vp, _i64p]
+        L.misfit_graph_size.argtypes = [vp, _i32p, _i64p, _i64p]
+        L.misfit_export_csr.argtypes = [vp, _i64p, _i32p]
+        L.misfit_neighbors.argtypes = [vp, ctypes.c_int32, _i32p, ctypes.c_int64, _i64p]
+        L.misfit_batch_bytes.argtypes = [vp, ctypes.c_int, _i64p, _i64p, _i64p]
+        L.misfit_egr_open.argtypes = [ctypes.c_char_p, ctypes.POINTER(EgrView), ctypes.c_char_p, ctypes.c_int]
+        L.misfit_egr_close.argtypes = [ctypes.POINTER(EgrView)]
+        L.misfit_write_egr.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int64, _i64p, _i32p,
+                                       ctypes.c_char_p, ctypes.c_int]
+        L.misfit_gen_er.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, _i64p, _i32p, _i64p]
+        L.misfit_gen_rmat.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_uint64, _i64p, _i32p, _i64p]
+        L.misfit_gen_rgg.argtypes = [ctypes.c_int32, ctypes.c_double, ctypes.c_uint64, _i64p, _i32p, _i64p]
+        L.misfit_greedy_mis.argtypes = [ctypes.c_int32, _i64p, _i32p, _u8p]
+        L.misfit_gen_updates.argtypes = [ctypes.c_int, ctypes.c_int32, _i64p, _i32p, ctypes.c_int64,
+                                         ctypes.c_uint64, _i32p, _i32p]
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _ck(rc, ctx=None):
+    if rc != 0:
+        msg = ""
+        if ctx:
+            msg = (lib().misfit_last_error(ctx) or b"").decode(errors="replace")
+        raise MisfitError(rc, msg)
+
+
+# ------------------------------------------------------------------ host I/O
+def read_egr(path):
+    """(nindex int64[n+1], nlist int32[m], wide) from a .egr file."""
+    v = EgrView()
+    err = ctypes.create_string_buffer(256)
+    rc = lib().misfit_egr_open(os.fsencode(path), ctypes.byref(v), err, 256)
+    if rc != 0:
+        raise MisfitError(rc, err.value.decode())
+    try:
+        n, m = v.n, v.m
+        if v.wide:
+            ni = np.ctypeslib.as_array(ctypes.cast(v.nindex, _i64p), (n + 1,)).astype(np.int64)
+        else:
+            ni = np.ctypeslib.as_array(ctypes.cast(v.nindex, _i32p), (n + 1,)).astype(np.int64)
+        nl = (np.ctypeslib.as_array(ctypes.cast(v.nlist, _i32p), (m,)).copy() if m
+              else np.zeros(0, np.int32))
+        return ni, nl, bool(v.wide)
+    finally:
+        lib().misfit_egr_close(ctypes.byref(v))
+
+
+def write_egr(path, nindex, nlist):
+    """Wide .egr, written to a temporary file then renamed over `path`."""
+    nindex = np.ascontiguousarray(nindex, np.int64)
+    nlist = np.ascontiguousarray(nlist, np.int32)
+    err = ctypes.create_string_buffer(256)
+    nl = nlist if len(nlist) else np.zeros(1, np.int32)
+    rc = lib().misfit_write_egr(os.fsencode(path), len(nindex) - 1, len(nlist), _p(nindex, _i64p),
+                                _p(nl, _i32p), err, 256)
+    if rc != 0:
+        raise MisfitError(rc, err.value.decode())
+
+
+def write_narrow_egr(path, nindex, nlist):
+    """int32-nindex .egr (the reference's narrow form), for workloads that fit."""
+    n = len(nindex) - 1
+    m = len(nlist)
+    assert m < 2 ** 31
+    with open(path, "wb") as f:
+        f.write(np.array([n, m], np.int32).tobytes())
+        f.write(np.asarray(nindex, np.int32).tobytes())
+        f.write(np.asarray(nlist, np.int32).tobytes())
+
+
+# ------------------------------------------------------------------ generators
+def _csr_from(fn, *args):
+    m = ctypes.c_int64()
+    _ck(fn(*args, None, None, ctypes.byref(m)))
+    return m.value
+
+
+def gen_er(n, M, seed):
+    m = _csr_from(lib().misfit_gen_er, n, M, seed)
+    ni = np.zeros(n + 1, np.int64)
+    nl = np.zeros(max(m, 1), np.int32)
+    mm = ctypes.c_int64()
+    _ck(lib().misfit_gen_er(n, M, seed, _p(ni, _i64p), _p(nl, _i32p), ctypes.byref(mm)))
+    return ni, nl[:m]
+
+
+def gen_rmat(scale, edgefactor, seed, a=0.57, b=0.19, c=0.19):
+    n = 1 << scale
+    ni = np.zeros(n + 1, np.int64)
+    # one pass sizes and fills (the generator is deterministic)
+    m = _csr_from(lib().misfit_gen_rmat, scale, edgefactor, a, b, c, seed)
+    nl = np.zeros(max(m, 1), np.int32)
+    mm = ctypes.c_int64()
+    _ck(lib().misfit_gen_rmat(scale, edgefactor, a, b, c, seed, _p(ni, _i64p), _p(nl, _i32p),
+                              ctypes.byref(mm)))
+    return ni, nl[:m]
+
+
+def gen_rgg(n, avg_degree, seed):
+    m = _csr_from(lib().misfit_gen_rgg, n, avg_degree, seed)
+    ni = np.zeros(n + 1, np.int64)
+    nl = np.zeros(max(m, 1), np.int32)
+    mm = ctypes.c_int64()
+    _ck(lib().misfit_gen_rgg(n, avg_degree, seed, _p(ni, _i64p), _p(nl, _i32p), ctypes.byref(mm)))
+    return ni, nl[:m]
+
+
+def greedy_mis(nindex, nlist):
+    nindex = np.ascontiguousarray(nindex, np.int64)
+    nlist = np.ascontiguousarray(nlist, np.int32)
+    mis = np.zeros(len(nindex) - 1, np.uint8)
+    nl = nlist if len(nlist) else np.zeros(1, np.int32)
+    _ck(lib().misfit_greedy_mis(len(nindex) - 1, _p(nindex, _i64p), _p(nl, _i32p), _p(mis, _u8p)))
+    return mis
+
+
+UNIFORM_PAIRS, MIXED_EXISTING, PREFERENTIAL, LOW_DEGREE = 0, 1, 2, 3
+
+
+def gen_updates(kind, nindex, nlist, count, seed):
+    nindex = np.ascontiguousarray(nindex, np.int64)
+    nlist = np.ascontiguousarray(nlist, np.int32)
+    u = np.zeros(max(count, 1), np.int32)
+    v = np.zeros(max(count, 1), np.int32)
+    nl = nlist if len(nlist) else np.zeros(1, np.int32)
+    _ck(lib().misfit_gen_updates(kind, len(nindex) - 1, _p(nindex, _i64p), _p(nl, _i32p), count, seed,
+                                 _p(u, _i32p), _p(v, _i32p)))
+    return u[:count], v[:count]
+
+
+def make_updates(u, v, ins):
+    a = np.zeros(len(u), UPDATE_DTYPE)
+    a["u"] = u
+    a["v"] = v
+    a["ins"] = ins
+    return a
+
+
+# ------------------------------------------------------------------ engine
+class Engine:
+    """One device-resident graph with its overlay and MIS on one GPU."""
+
+    def __init__(self, device=0):
+        self.ctx = ctypes.c_void_p()
+        _ck(lib().misfit_ctx_create(device, ctypes.byref(self.ctx)))
+        self.n = 0
+
+    def close(self):
+        if self.ctx:
+            lib().misfit_ctx_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _ck(self, rc):
+        _ck(rc, self.ctx)
+
+    def load_csr(self, nindex, nlist):
+        nlist = np.ascontiguousarray(nlist, np.int32)
+        wide = nindex.dtype == np.int64
+        nindex = np.ascontiguousarray(nindex, np.int64 if wide else np.int32)
+        n = len(nindex) - 1
+        nl = nlist if len(nlist) else np.zeros(1, np.int32)
+        self._ck(lib().misfit_load_csr(self.ctx, n, len(nlist), nindex.ctypes.data_as(ctypes.c_void_p),
+                                       1 if wide else 0, _p(nl, _i32p)))
+        self.n = n
+
+    def load_egr(self, path):
+        v = EgrView()
+        err = ctypes.create_string_buffer(256)
+        rc = lib().misfit_egr_open(os.fsencode(path), ctypes.byref(v), err, 256)
+        if rc != 0:
+            raise MisfitError(rc, err.value.decode())
+        try:
+            self._ck(lib().misfit_load_csr(self.ctx, v.n, v.m, v.nindex, v.wide,
+                                           ctypes.cast(v.nlist, _i32p)))
+            self.n = v.n
+        finally:
+            lib().misfit_egr_close(ctypes.byref(v))
+
+    def set_mis(self, mis):
+        mis = np.ascontiguousarray(mis, np.uint8)
+        assert len(mis) == self.n
+        self._ck(lib().misfit_set_mis(self.ctx, _p(mis, _u8p)))
+
+    def get_mis(self):
+        out = np.zeros(self.n, np.uint8)
+        self._ck(lib().misfit_get_mis(self.ctx, _p(out, _u8p)))
+        return out
+
+    def set_rule(self, completion=True):
+        self._ck(lib().misfit_set_rule(self.ctx, 1 if completion else 0))
+
+    def set_mode(self, mode):
+        self._ck(lib().misfit_set_mode(self.ctx, mode))
+
+    def flags(self, u, v):
+        u = np.ascontiguousarray(u, np.int32)
+        v = np.ascontiguousarray(v, np.int32)
+        ins = np.zeros(max(len(u), 1), np.uint8)
+        if len(u):
+            self._ck(lib().misfit_flags(self.ctx, _p(u, _i32p), _p(v, _i32p), len(u), _p(ins, _u8p)))
+        return ins[: len(u)]
+
+    def submit(self, updates):
+        updates = np.ascontiguousarray(updates, UPDATE_DTYPE)
+        t = ctypes.c_uint64()
+        self._ck(lib().misfit_submit_batch(self.ctx, updates.ctypes.data_as(ctypes.c_void_p), len(updates),
+                                           ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket):
+        st = BatchStats()
+        self._ck(lib().misfit_wait(self.ctx, ticket, ctypes.byref(st)))
+        return st.as_dict()
+
+    def process_batch(self, updates):
+        updates = np.ascontiguousarray(updates, UPDATE_DTYPE)
+        st = BatchStats()
+        self._ck(lib().misfit_process_batch(self.ctx, updates.ctypes.data_as(ctypes.c_void_p), len(updates),
+                                            ctypes.byref(st)))
+        return st.as_dict()
+
+    def sync(self):
+        self._ck(lib().misfit_sync(self.ctx))
+
+    def check(self):
+        c, u, e = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        self._ck(lib().misfit_check(self.ctx, ctypes.byref(c), ctypes.byref(u), ctypes.byref(e)))
+        return c.value, u.value, e.value
+
+    def compact(self):
+        m = ctypes.c_int64()
+        self._ck(lib().misfit_compact(self.ctx, ctypes.byref(m)))
+        return m.value
+
+    def graph_size(self):
+        n, bm, lm = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int64()
+        self._ck(lib().misfit_graph_size(self.ctx, ctypes.byref(n), ctypes.byref(bm), ctypes.byref(lm)))
+        return n.value, bm.value, lm.value
+
+    def export_csr(self):
+        n, bm, _ = self.graph_size()
+        ni = np.zeros(n + 1, np.int64)
+        nl = np.zeros(max(bm, 1), np.int32)
+        self._ck(lib().misfit_export_csr(self.ctx, _p(ni, _i64p), _p(nl, _i32p)))
+        return ni, nl[:bm]
+
+    def neighbors(self, v):
+        k = ctypes.c_int64()
+        self._ck(lib().misfit_neighbors(self.ctx, v, None, 0, ctypes.byref(k)))
+        out = np.zeros(max(k.value, 1), np.int32)
+        self._ck(lib().misfit_neighbors(self.ctx, v, _p(out, _i32p), k.value, ctypes.byref(k)))
+        return out[: k.value]
+
+    def batch_bytes(self, index_bytes=8):
+        b, v1, v2 = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        self._ck(lib().misfit_batch_bytes(self.ctx, index_bytes, ctypes.byref(b), ctypes.byref(v1),
+                                          ctypes.byref(v2)))
+        return b.value, v1.value, v2.value

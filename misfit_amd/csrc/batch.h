@@ -1,0 +1,95 @@
+// batch.h -- parameters of one update batch on the device, shared by the
+// batch kernels and the host code that launches them.
+#pragma once
+#include "dev.h"
+
+// Rows longer than this get a per-batch index of their admissible
+// neighbours instead of being scanned in full by the repair.
+#define MF_FA_T 256
+
+enum BatchCounter {
+    C_SEEDS = 0,     // distinct endpoints
+    C_WPOT,          // distinct max(u,v) of insertions
+    C_NE,            // new directed overlay entries
+    C_TOUCHED,       // neighbours of MIS vertices that may be evicted
+    C_PU,            // vertices that may become uncovered
+    C_ADM,           // ... of which have a row (may be admitted)
+    C_BIG,           // long rows with admissible neighbours
+    C_FA_TOP,        // admissible-index entries
+    C_ROUNDS,        // speculative rounds
+    C_ACT_INS,       // insertions with both endpoints in the MIS
+    C_EVICT,
+    C_ADMIT_INS,
+    C_ACT_DEL,       // deletions with exactly one endpoint in the MIS
+    C_ADMIT_DEL,
+    C_COMP_ADDS,
+    C_COMP_ROUNDS,
+    C_DELTA_OPS,     // effective overlay operations (delta_edge_count)
+    C_POOL_OVF,
+    C_FA_OVF,
+    C_CAND,
+    C_SPECS,         // edge evaluations (speculation work)
+    C_NACT0, C_NACT1,
+    C_CONF0, C_CONF1,
+    C_UNDEC0, C_UNDEC1,
+    C_START,
+    C_GUARD,         // a loop bound was hit (engine bug): results invalid
+    C_NCOUNTERS
+};
+
+struct BP {
+    DevGraph g;
+    const int32_t* bu;
+    const int32_t* bv;
+    const uint8_t* bins;
+    int64_t U;
+    uint32_t b;        // batch epoch, >= 1
+    int completion;
+    int64_t window;    // max edges speculated per round
+    // pair table of this batch
+    u64* bt_key;
+    int32_t* bt_head;
+    int32_t* bt_nxt;
+    int32_t* bt_slot;
+    u64 bt_mask;
+    // new overlay entries
+    int32_t* ne_x;
+    int32_t* ne_y;
+    int32_t* ne_seq;
+    int32_t* ne_nxt;
+    // vertex lists
+    int32_t* seeds;
+    int32_t* wpot;
+    int32_t* touched;
+    int64_t touched_cap;
+    int32_t* pu;
+    int64_t pu_cap;
+    int32_t* big;
+    int64_t big_cap;
+    int32_t* fa_z;
+    int32_t* fa_pos;
+    int64_t fa_cap;
+    // repair
+    int32_t* active;
+    uint8_t* kind;
+    int32_t* res_w;
+    int32_t* res_x;
+    // completion
+    int32_t* cand;
+    uint8_t* cdec;
+    u64* ctr;
+    uint32_t* round;   // global speculative-round counter (device-resident)
+};
+
+// host-side launchers (k_batch.hip)
+struct LaunchCfg {
+    int blocks;        // grid for grid-stride phases
+    hipStream_t stream;
+};
+void mf_launch_batch_prologue(const BP& P, const LaunchCfg& L);
+void mf_launch_round(const BP& P, const LaunchCfg& L, int64_t start, int64_t end, int par, uint32_t R);
+void mf_launch_completion_collect(const BP& P, const LaunchCfg& L);
+void mf_launch_completion_round(const BP& P, const LaunchCfg& L, int par);
+void mf_launch_set_round(const BP& P, const LaunchCfg& L, uint32_t R);
+hipError_t mf_launch_persistent(const BP& P, int grid, hipStream_t stream);
+int mf_persistent_grid(int device);

@@ -1,0 +1,798 @@
+// capi.hip -- the C ABI (include/misfit.h): context, device memory,
+// capacity management and the two ways of driving a batch.
+//
+//   plain       per-phase kernels; the host reads the round outcome after
+//               every speculative round (one batch at a time, the way the
+//               reference's sync engine drives its kernels).
+//   persistent  one cooperative launch per batch; rounds and completion
+//               iterations are grid barriers, so submit returns at once and
+//               batches queue back to back on the stream while the host
+//               stages the next ones (the generalisation of the
+//               reference's misfit_async pipelining).
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "graph.h"
+#include "../../include/misfit.h"
+
+#define MF_NSLOT 4
+
+struct Slot {
+    int32_t* h_u = nullptr;
+    int32_t* h_v = nullptr;
+    uint8_t* h_ins = nullptr;
+    int32_t* d_u = nullptr;
+    int32_t* d_v = nullptr;
+    uint8_t* d_ins = nullptr;
+    u64* h_ctr = nullptr;
+    hipEvent_t uploaded = nullptr, done = nullptr, t0 = nullptr, t1 = nullptr;
+    int64_t count = 0;
+    uint64_t ticket = 0;
+    bool busy = false;
+};
+
+struct misfit_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t copy_stream = nullptr;
+    std::string err;
+    GraphState S{};
+    bool has_graph = false;
+    int completion = 1;
+    int mode = MISFIT_MODE_PERSISTENT;
+    int persistent_grid = 0;
+    uint32_t epoch = 0;
+    uint32_t host_round = 0;
+    uint32_t bytes_tag = 0;
+    int64_t last_U = 0;
+    // capacities
+    int64_t U_cap = 0;
+    u64 bt_size = 0;
+    int64_t fa_cap = 0;
+    int64_t ph_cap = 0;
+    int64_t ins_bound = 0;    // inserted pairs since the overlay was last cleared (upper bound)
+    int64_t pool_bound = 0;   // pool slots possibly used (upper bound)
+    // owned device memory
+    std::vector<void*> graph_allocs;   // freed on reload
+    std::vector<void*> batch_allocs;   // freed on batch-capacity growth
+    int64_t* d_nindex = nullptr;
+    int32_t* d_nlist = nullptr;
+    BP P{};
+    u64* d_ctr = nullptr;
+    uint32_t* d_round = nullptr;
+    u64* d_acc = nullptr;
+    int* d_flags = nullptr;
+    u64* h_small = nullptr;    // pinned scratch for small readbacks
+    Slot slots[MF_NSLOT];
+    uint64_t next_ticket = 1;
+};
+
+// ---------------------------------------------------------------- errors
+static int fail(misfit_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(c, expr)                                                                         \
+    do {                                                                                        \
+        hipError_t _e = (expr);                                                                 \
+        if (_e != hipSuccess)                                                                   \
+            return fail((c), MISFIT_E_HIP, std::string(#expr ": ") + hipGetErrorString(_e));    \
+    } while (0)
+
+template <typename T>
+static hipError_t dalloc(std::vector<void*>& owner, T** p, int64_t count) {
+    size_t bytes = (size_t)std::max<int64_t>(count, 1) * sizeof(T);
+    hipError_t e = hipMalloc((void**)p, bytes);
+    if (e == hipSuccess) owner.push_back((void*)*p);
+    return e;
+}
+
+static void free_list(std::vector<void*>& v) {
+    for (void* p : v) hipFree(p);
+    v.clear();
+}
+
+static u64 pow2_at_least(u64 x) {
+    u64 p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+extern "C" const char* misfit_last_error(const misfit_ctx* ctx) {
+    if (!ctx) return "null context";
+    return ctx->err.c_str();
+}
+
+// ---------------------------------------------------------------- context
+extern "C" int misfit_ctx_create(int device, misfit_ctx** out) {
+    if (!out) return MISFIT_E_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return MISFIT_E_HIP;
+    if (device < 0 || device >= ndev) return MISFIT_E_ARG;
+    misfit_ctx* c = new misfit_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return MISFIT_E_HIP;
+    }
+    int coop = 0;
+    hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device);
+    c->persistent_grid = coop ? mf_persistent_grid(device) : 0;
+    if (c->persistent_grid <= 0) c->mode = MISFIT_MODE_PLAIN;
+    int cus = 0;
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    c->S.blocks = std::max(64, cus * 8);
+    c->S.stream = c->stream;
+    if (hipHostMalloc((void**)&c->h_small, 64 * sizeof(u64), hipHostMallocDefault) != hipSuccess) {
+        delete c;
+        return MISFIT_E_NOMEM;
+    }
+    for (auto& s : c->slots) {
+        hipEventCreateWithFlags(&s.uploaded, hipEventDisableTiming);
+        hipEventCreateWithFlags(&s.done, hipEventDisableTiming);
+        hipEventCreate(&s.t0);
+        hipEventCreate(&s.t1);
+        hipHostMalloc((void**)&s.h_ctr, (C_NCOUNTERS + 1) * sizeof(u64), hipHostMallocDefault);
+    }
+    *out = c;
+    return MISFIT_OK;
+}
+
+static void free_slots(misfit_ctx* c) {
+    for (auto& s : c->slots) {
+        if (s.h_u) hipHostFree(s.h_u);
+        if (s.h_v) hipHostFree(s.h_v);
+        if (s.h_ins) hipHostFree(s.h_ins);
+        s.h_u = nullptr; s.h_v = nullptr; s.h_ins = nullptr;
+        s.d_u = nullptr; s.d_v = nullptr; s.d_ins = nullptr;
+        s.busy = false;
+    }
+}
+
+extern "C" int misfit_ctx_destroy(misfit_ctx* c) {
+    if (!c) return MISFIT_E_ARG;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    hipStreamSynchronize(c->copy_stream);
+    free_slots(c);
+    for (auto& s : c->slots) {
+        if (s.h_ctr) hipHostFree(s.h_ctr);
+        hipEventDestroy(s.uploaded);
+        hipEventDestroy(s.done);
+        hipEventDestroy(s.t0);
+        hipEventDestroy(s.t1);
+    }
+    free_list(c->batch_allocs);
+    free_list(c->graph_allocs);
+    if (c->h_small) hipHostFree(c->h_small);
+    hipStreamDestroy(c->stream);
+    hipStreamDestroy(c->copy_stream);
+    delete c;
+    return MISFIT_OK;
+}
+
+extern "C" int misfit_set_rule(misfit_ctx* c, int completion_on) {
+    if (!c) return MISFIT_E_ARG;
+    c->completion = completion_on ? 1 : 0;
+    return MISFIT_OK;
+}
+
+extern "C" int misfit_set_mode(misfit_ctx* c, int mode) {
+    if (!c) return MISFIT_E_ARG;
+    if (mode != MISFIT_MODE_PLAIN && mode != MISFIT_MODE_PERSISTENT) return fail(c, MISFIT_E_ARG, "unknown mode");
+    if (mode == MISFIT_MODE_PERSISTENT && c->persistent_grid <= 0)
+        return fail(c, MISFIT_E_UNSUPPORTED, "device has no cooperative launch");
+    c->mode = mode;
+    return MISFIT_OK;
+}
+
+extern "C" int misfit_sync(misfit_ctx* c) {
+    if (!c) return MISFIT_E_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->copy_stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MISFIT_OK;
+}
+
+// ---------------------------------------------------------------- capacity
+// Per-batch scratch for batches of up to U updates.
+static int ensure_batch_capacity(misfit_ctx* c, int64_t U) {
+    if (U <= c->U_cap && c->P.ctr) return MISFIT_OK;
+    HIPCHK(c, hipStreamSynchronize(c->copy_stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int64_t cap = std::max<int64_t>(std::max<int64_t>(U, 1 << 14), c->U_cap * 2);
+    free_list(c->batch_allocs);
+    free_slots(c);
+    auto& A = c->batch_allocs;
+    BP& P = c->P;
+    c->bt_size = pow2_at_least((u64)(2 * cap));
+    HIPCHK(c, dalloc(A, &P.bt_key, (int64_t)c->bt_size));
+    HIPCHK(c, dalloc(A, &P.bt_head, (int64_t)c->bt_size));
+    HIPCHK(c, dalloc(A, &P.bt_nxt, cap));
+    HIPCHK(c, dalloc(A, &P.bt_slot, cap));
+    HIPCHK(c, dalloc(A, &P.ne_x, 2 * cap));
+    HIPCHK(c, dalloc(A, &P.ne_y, 2 * cap));
+    HIPCHK(c, dalloc(A, &P.ne_seq, 2 * cap));
+    HIPCHK(c, dalloc(A, &P.ne_nxt, 2 * cap));
+    HIPCHK(c, dalloc(A, &P.seeds, 2 * cap));
+    HIPCHK(c, dalloc(A, &P.wpot, cap));
+    HIPCHK(c, dalloc(A, &P.active, cap));
+    HIPCHK(c, dalloc(A, &P.kind, cap));
+    HIPCHK(c, dalloc(A, &P.res_w, cap));
+    HIPCHK(c, dalloc(A, &P.res_x, cap));
+    c->fa_cap = std::max<int64_t>(1 << 22, 16 * cap);
+    HIPCHK(c, dalloc(A, &P.fa_z, c->fa_cap));
+    HIPCHK(c, dalloc(A, &P.fa_pos, c->fa_cap));
+    P.bt_mask = c->bt_size - 1;
+    P.fa_cap = c->fa_cap;
+    mf_fill_u64(P.bt_key, MF_EMPTY_KEY, (int64_t)c->bt_size, c->stream);
+    mf_fill_i32(P.bt_head, -1, (int64_t)c->bt_size, c->stream);
+    for (auto& s : c->slots) {
+        HIPCHK(c, dalloc(A, &s.d_u, cap));
+        HIPCHK(c, dalloc(A, &s.d_v, cap));
+        HIPCHK(c, dalloc(A, &s.d_ins, cap));
+        HIPCHK(c, hipHostMalloc((void**)&s.h_u, cap * sizeof(int32_t), hipHostMallocDefault));
+        HIPCHK(c, hipHostMalloc((void**)&s.h_v, cap * sizeof(int32_t), hipHostMallocDefault));
+        HIPCHK(c, hipHostMalloc((void**)&s.h_ins, cap, hipHostMallocDefault));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->U_cap = cap;
+    return MISFIT_OK;
+}
+
+// Overlay pool and inserted-pair hash sized for `extra` more insertions.
+static int ensure_overlay_capacity(misfit_ctx* c, int64_t extra_ins) {
+    DevGraph& g = c->P.g;
+    int64_t need_ph = 2 * (c->ins_bound + extra_ins) + 1024;
+    int64_t need_pool = c->pool_bound + 8 * extra_ins + 16;
+    if (need_ph <= c->ph_cap && need_pool <= g.pool_cap) return MISFIT_OK;
+    HIPCHK(c, hipStreamSynchronize(c->copy_stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (need_pool > g.pool_cap) {
+        int64_t nc = std::max<int64_t>(need_pool * 2, g.pool_cap * 2);
+        int32_t* np = nullptr;
+        HIPCHK(c, dalloc(c->graph_allocs, &np, nc));
+        u64 top = 0;
+        HIPCHK(c, hipMemcpy(&top, g.pool_top, sizeof(u64), hipMemcpyDeviceToHost));
+        if (top) HIPCHK(c, hipMemcpy(np, g.pool, top * sizeof(int32_t), hipMemcpyDeviceToDevice));
+        auto& v = c->graph_allocs;
+        auto it = std::find(v.begin(), v.end(), (void*)g.pool);
+        if (it != v.end()) { hipFree(*it); v.erase(it); }
+        g.pool = np;
+        g.pool_cap = nc;
+    }
+    if (need_ph > c->ph_cap) {
+        // rehash every key into a table twice as large
+        int64_t nc = (int64_t)pow2_at_least((u64)std::max<int64_t>(need_ph * 2, c->ph_cap * 2));
+        std::vector<u64> keys((size_t)c->ph_cap);
+        std::vector<int32_t> vals((size_t)c->ph_cap);
+        HIPCHK(c, hipMemcpy(keys.data(), g.ph_key, c->ph_cap * sizeof(u64), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(vals.data(), g.ph_val, c->ph_cap * sizeof(int32_t), hipMemcpyDeviceToHost));
+        std::vector<u64> nk((size_t)nc, MF_EMPTY_KEY);
+        std::vector<int32_t> nv((size_t)nc, 0);
+        auto mixh = [](u64 z) {
+            z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+            z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+            return z ^ (z >> 31);
+        };
+        for (int64_t i = 0; i < c->ph_cap; i++) {
+            if (keys[i] == MF_EMPTY_KEY) continue;
+            u64 s = mixh(keys[i]) & (u64)(nc - 1);
+            while (nk[s] != MF_EMPTY_KEY) s = (s + 1) & (u64)(nc - 1);
+            nk[s] = keys[i];
+            nv[s] = vals[i];
+        }
+        auto& v = c->graph_allocs;
+        for (void* old : {(void*)g.ph_key, (void*)g.ph_val}) {
+            auto it = std::find(v.begin(), v.end(), old);
+            if (it != v.end()) { hipFree(*it); v.erase(it); }
+        }
+        HIPCHK(c, dalloc(v, &g.ph_key, nc));
+        HIPCHK(c, dalloc(v, &g.ph_val, nc));
+        HIPCHK(c, hipMemcpy(g.ph_key, nk.data(), nc * sizeof(u64), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(g.ph_val, nv.data(), nc * sizeof(int32_t), hipMemcpyHostToDevice));
+        c->ph_cap = nc;
+        g.ph_mask = (u64)(nc - 1);
+    }
+    c->S.g = g;
+    return MISFIT_OK;
+}
+
+static void reset_overlay(misfit_ctx* c) {
+    DevGraph& g = c->P.g;
+    hipStream_t st = c->stream;
+    mf_fill_i64(g.ov_start, 0, g.n, st);
+    mf_fill_i32(g.ov_len, 0, g.n, st);
+    mf_fill_i32(g.ov_cap, 0, g.n, st);
+    mf_fill_i32(g.ov_alive, 0, g.n, st);
+    mf_fill_i32(g.ov_tomb, 0, g.n, st);
+    mf_fill_u64(g.pool_top, 0ull, 1, st);
+    mf_fill_u64(g.ph_key, MF_EMPTY_KEY, c->ph_cap, st);
+    mf_fill_i32(g.ph_val, 0, c->ph_cap, st);
+    c->ins_bound = 0;
+    c->pool_bound = 0;
+}
+
+// ---------------------------------------------------------------- load
+extern "C" int misfit_load_csr(misfit_ctx* c, int32_t n, int64_t m, const void* nindex, int nindex_is_64,
+                               const int32_t* nlist) {
+    if (!c) return MISFIT_E_ARG;
+    if (n < 1 || m < 0 || !nindex || (m > 0 && !nlist)) return fail(c, MISFIT_E_ARG, "invalid graph arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->copy_stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // the batch scratch keeps per-batch pointers only; graph arrays are new
+    free_list(c->graph_allocs);
+    c->has_graph = false;
+    c->U_cap = 0;
+    free_list(c->batch_allocs);
+    free_slots(c);
+    c->P = BP{};
+    auto& A = c->graph_allocs;
+    DevGraph& g = c->P.g;
+    g.n = n;
+    g.m = m;
+    HIPCHK(c, dalloc(A, &c->d_nindex, (int64_t)n + 1));
+    HIPCHK(c, dalloc(A, &c->d_nlist, m));
+    if (nindex_is_64) {
+        HIPCHK(c, hipMemcpy(c->d_nindex, nindex, ((size_t)n + 1) * 8, hipMemcpyHostToDevice));
+    } else {
+        int32_t* tmp = nullptr;
+        HIPCHK(c, hipMalloc((void**)&tmp, ((size_t)n + 1) * 4));
+        hipError_t e = hipMemcpy(tmp, nindex, ((size_t)n + 1) * 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = mf_widen(tmp, c->d_nindex, (int64_t)n + 1, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        hipFree(tmp);
+        HIPCHK(c, e);
+    }
+    if (m) HIPCHK(c, hipMemcpy(c->d_nlist, nlist, (size_t)m * 4, hipMemcpyHostToDevice));
+    g.nindex = c->d_nindex;
+    g.nlist = c->d_nlist;
+    HIPCHK(c, dalloc(A, &g.ov_start, n));
+    HIPCHK(c, dalloc(A, &g.ov_len, n));
+    HIPCHK(c, dalloc(A, &g.ov_cap, n));
+    HIPCHK(c, dalloc(A, &g.ov_alive, n));
+    HIPCHK(c, dalloc(A, &g.ov_tomb, n));
+    g.pool_cap = std::max<int64_t>(1 << 20, m / 16);
+    HIPCHK(c, dalloc(A, &g.pool, g.pool_cap));
+    HIPCHK(c, dalloc(A, &g.pool_top, 1));
+    c->ph_cap = (int64_t)pow2_at_least((u64)std::max<int64_t>(1 << 16, m / 32));
+    HIPCHK(c, dalloc(A, &g.ph_key, c->ph_cap));
+    HIPCHK(c, dalloc(A, &g.ph_val, c->ph_cap));
+    g.ph_mask = (u64)(c->ph_cap - 1);
+    HIPCHK(c, dalloc(A, &g.mc, n));
+    HIPCHK(c, dalloc(A, &g.seedb, n));
+    HIPCHK(c, dalloc(A, &g.wpotb, n));
+    HIPCHK(c, dalloc(A, &g.kw, n));
+    HIPCHK(c, dalloc(A, &g.pub, n));
+    HIPCHK(c, dalloc(A, &g.evb, n));
+    HIPCHK(c, dalloc(A, &g.cstat, n));
+    HIPCHK(c, dalloc(A, &g.mmark, n));
+    HIPCHK(c, dalloc(A, &g.cmark, n));
+    HIPCHK(c, dalloc(A, &g.ne_head, n));
+    HIPCHK(c, dalloc(A, &g.fa_cnt, n));
+    HIPCHK(c, dalloc(A, &g.fa_start, n));
+    HIPCHK(c, dalloc(A, &g.fa_fill, n));
+    BP& P = c->P;
+    HIPCHK(c, dalloc(A, &P.touched, n));
+    HIPCHK(c, dalloc(A, &P.pu, n));
+    HIPCHK(c, dalloc(A, &P.big, n));
+    HIPCHK(c, dalloc(A, &P.cand, n));
+    HIPCHK(c, dalloc(A, &P.cdec, n));
+    P.touched_cap = n;
+    P.pu_cap = n;
+    P.big_cap = n;
+    HIPCHK(c, dalloc(A, &c->d_ctr, C_NCOUNTERS));
+    HIPCHK(c, dalloc(A, &c->d_round, 1));
+    HIPCHK(c, dalloc(A, &c->d_acc, 8));
+    HIPCHK(c, dalloc(A, &c->d_flags, 1));
+    P.ctr = c->d_ctr;
+    P.round = c->d_round;
+    GraphState& S = c->S;
+    HIPCHK(c, dalloc(A, &S.d_cnt, n));
+    HIPCHK(c, dalloc(A, &S.d_bmark, n));
+    HIPCHK(c, dalloc(A, &S.d_deg, n));
+    HIPCHK(c, dalloc(A, &S.d_tomb_list, n));
+    HIPCHK(c, dalloc(A, &S.d_scan_part, ((int64_t)n + 1) / 2048 + 2));
+    S.walk_task_cap = m / 4096 + n / 64 + 1024;
+    HIPCHK(c, dalloc(A, (char**)&S.d_walk_tasks, S.walk_task_cap * 16));
+    HIPCHK(c, dalloc(A, &S.d_walk_ntasks, 1));
+    hipStream_t st = c->stream;
+    mf_fill_u32(g.mc, 0u, n, st);
+    mf_fill_u32(g.seedb, 0u, n, st);
+    mf_fill_u32(g.wpotb, 0u, n, st);
+    mf_fill_u64(g.kw, 0ull, n, st);
+    mf_fill_u32(g.pub, 0u, n, st);
+    mf_fill_u32(g.evb, 0u, n, st);
+    mf_fill_u32(g.cstat, 0u, n, st);
+    mf_fill_u64(g.mmark, 0ull, n, st);
+    mf_fill_u64(g.cmark, 0ull, n, st);
+    mf_fill_i32(g.ne_head, -1, n, st);
+    mf_fill_u64(g.fa_cnt, 0ull, n, st);
+    mf_fill_i64(g.fa_start, -1, n, st);
+    mf_fill_i32(g.fa_fill, 0, n, st);
+    mf_fill_u32(S.d_bmark, 0u, n, st);
+    mf_fill_u32(c->d_round, 0u, 1, st);
+    mf_fill_u64(c->d_ctr, 0ull, C_NCOUNTERS, st);
+    c->epoch = 0;
+    c->host_round = 0;
+    c->bytes_tag = 0;
+    S.g = g;
+    reset_overlay(c);
+    // base validation: ranges, monotone offsets, row order
+    HIPCHK(c, mf_validate_csr(S, c->d_flags));
+    int flags = 0;
+    HIPCHK(c, hipMemcpyAsync(&flags, c->d_flags, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    if (flags & (4 | 8)) return fail(c, MISFIT_E_ARG, "CSR has out-of-range entries or non-monotone offsets");
+    g.sorted = (flags & 1) ? 0 : 1;
+    if (!g.sorted) {
+        // unsorted rows: the engine needs rows without duplicate entries
+        std::vector<int64_t> ni((size_t)n + 1);
+        HIPCHK(c, hipMemcpy(ni.data(), c->d_nindex, ni.size() * 8, hipMemcpyDeviceToHost));
+        std::vector<int32_t> row;
+        for (int32_t x = 0; x < n; x++) {
+            int64_t b = ni[x], e = ni[x + 1];
+            if (e - b < 2) continue;
+            row.assign(nlist + b, nlist + e);
+            std::sort(row.begin(), row.end());
+            if (std::adjacent_find(row.begin(), row.end()) != row.end())
+                return fail(c, MISFIT_E_UNSUPPORTED, "row " + std::to_string(x) + " lists a neighbour twice");
+        }
+    }
+    S.g = g;
+    c->has_graph = true;
+    return ensure_batch_capacity(c, 1 << 14);
+}
+
+// ---------------------------------------------------------------- MIS
+extern "C" int misfit_set_mis(misfit_ctx* c, const uint8_t* mis) {
+    if (!c || !mis) return MISFIT_E_ARG;
+    if (!c->has_graph) return fail(c, MISFIT_E_STATE, "no graph loaded");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int32_t n = c->P.g.n;
+    uint8_t* d = nullptr;
+    HIPCHK(c, hipMalloc((void**)&d, n));
+    hipError_t e = hipMemcpy(d, mis, n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = mf_set_mis(c->S, d);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(d);
+    HIPCHK(c, e);
+    return MISFIT_OK;
+}
+
+extern "C" int misfit_get_mis(misfit_ctx* c, uint8_t* mis) {
+    if (!c || !mis) return MISFIT_E_ARG;
+    if (!c->has_graph) return fail(c, MISFIT_E_STATE, "no graph loaded");
+    HIPCHK(c, hipSetDevice(c->device));
+    int32_t n = c->P.g.n;
+    uint8_t* d = nullptr;
+    HIPCHK(c, hipMalloc((void**)&d, n));
+    hipError_t e = mf_get_mis(c->S, d);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(mis, d, n, hipMemcpyDeviceToHost);
+    hipFree(d);
+    HIPCHK(c, e);
+    return MISFIT_OK;
+}
+
+extern "C" int misfit_flags(misfit_ctx* c, const int32_t* u, const int32_t* v, int64_t count, uint8_t* ins) {
+    if (!c || count < 0 || (count && (!u || !v || !ins))) return MISFIT_E_ARG;
+    if (!c->has_graph) return fail(c, MISFIT_E_STATE, "no graph loaded");
+    if (!count) return MISFIT_OK;
+    int32_t n = c->P.g.n;
+    for (int64_t i = 0; i < count; i++)
+        if (u[i] < 0 || u[i] >= n || v[i] < 0 || v[i] >= n) return fail(c, MISFIT_E_ARG, "vertex out of range");
+    HIPCHK(c, hipSetDevice(c->device));
+    int32_t *du = nullptr, *dv = nullptr;
+    uint8_t* di = nullptr;
+    hipError_t e = hipMalloc((void**)&du, count * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&dv, count * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&di, count);
+    if (e == hipSuccess) e = hipMemcpy(du, u, count * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dv, v, count * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = mf_flags(c->S, du, dv, count, di);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(ins, di, count, hipMemcpyDeviceToHost);
+    hipFree(du);
+    hipFree(dv);
+    hipFree(di);
+    HIPCHK(c, e);
+    return MISFIT_OK;
+}
+
+// ---------------------------------------------------------------- batches
+static void fill_stats(const misfit_ctx* c, const Slot& s, misfit_batch_stats* st) {
+    const u64* k = s.h_ctr;
+    memset(st, 0, sizeof(*st));
+    st->updates = s.count;
+    st->seeds = (int64_t)k[C_SEEDS];
+    st->active_ins = (int64_t)k[C_ACT_INS];
+    st->evictions = (int64_t)k[C_EVICT];
+    st->admit_ins = (int64_t)k[C_ADMIT_INS];
+    st->active_del = (int64_t)k[C_ACT_DEL];
+    st->admit_del = (int64_t)k[C_ADMIT_DEL];
+    st->completion_adds = (int64_t)k[C_COMP_ADDS];
+    st->rounds = (int64_t)k[C_ROUNDS];
+    st->evaluations = (int64_t)k[C_SPECS];
+    st->maybe_uncovered = (int64_t)k[C_PU];
+    st->admissible = (int64_t)k[C_ADM];
+    st->delta_ops = (int64_t)k[C_DELTA_OPS];
+    st->overlay_entries = (int64_t)k[C_NCOUNTERS];
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, s.t0, s.t1) == hipSuccess) st->device_ms = ms;
+    (void)c;
+}
+
+// Host-driven rounds: one round = 4 kernels, then the host reads where the
+// committed prefix ends.
+static int drive_plain(misfit_ctx* c, BP& P) {
+    LaunchCfg L{c->S.blocks, c->stream};
+    mf_launch_batch_prologue(P, L);
+    int64_t start = 0;
+    uint32_t r = 0;
+    // round ids must never repeat (stale marks would read as current), so
+    // continue from the device counter that the persistent path advances
+    HIPCHK(c, hipMemcpyAsync(c->h_small, P.round, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    uint32_t R0 = (uint32_t)(c->h_small[0] & 0xFFFFFFFFull);
+    while (start < P.U) {
+        if ((int64_t)r > P.U + 1) return fail(c, MISFIT_E_CHECK, "speculative rounds made no progress");
+        int par = (int)(r & 1u);
+        int64_t end = std::min<int64_t>(P.U, start + P.window);
+        mf_launch_round(P, L, start, end, par, R0 + r + 1);
+        HIPCHK(c, hipMemcpyAsync(c->h_small, P.ctr + C_START, sizeof(u64), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        int64_t conflict = (int64_t)std::min<u64>(c->h_small[0], (u64)INT64_MAX);
+        start = conflict < end ? conflict : end;
+        r++;
+    }
+    c->host_round = R0 + r;
+    mf_launch_set_round(P, L, c->host_round);
+    if (P.completion) {
+        mf_launch_completion_collect(P, L);
+        for (uint32_t k = 0;; k++) {
+            int par = (int)(k & 1u);
+            mf_launch_completion_round(P, L, par);
+            HIPCHK(c, hipMemcpyAsync(c->h_small, P.ctr + C_UNDEC0 + par, sizeof(u64), hipMemcpyDeviceToHost,
+                                     c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            if (c->h_small[0] == 0) break;
+            if (k > (uint32_t)P.g.n + 1) return fail(c, MISFIT_E_CHECK, "completion made no progress");
+        }
+    }
+    HIPCHK(c, hipGetLastError());
+    return MISFIT_OK;
+}
+
+extern "C" int misfit_submit_batch(misfit_ctx* c, const misfit_update* ups, int64_t count, uint64_t* ticket) {
+    if (!c || count < 0 || (count && !ups)) return MISFIT_E_ARG;
+    if (!c->has_graph) return fail(c, MISFIT_E_STATE, "no graph loaded");
+    if (count > INT32_MAX / 4) return fail(c, MISFIT_E_ARG, "batch too large");
+    int32_t n = c->P.g.n;
+    int64_t nins = 0;
+    for (int64_t i = 0; i < count; i++) {
+        if (ups[i].u < 0 || ups[i].u >= n || ups[i].v < 0 || ups[i].v >= n || ups[i].u == ups[i].v)
+            return fail(c, MISFIT_E_ARG, "update " + std::to_string(i) + ": endpoints out of range or equal");
+        nins += ups[i].is_insertion ? 1 : 0;
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = ensure_batch_capacity(c, count);
+    if (rc) return rc;
+    rc = ensure_overlay_capacity(c, nins);
+    if (rc) return rc;
+    uint64_t t = c->next_ticket++;
+    Slot& s = c->slots[t % MF_NSLOT];
+    if (s.busy) HIPCHK(c, hipEventSynchronize(s.done));  // staging still owned by an older batch
+    for (int64_t i = 0; i < count; i++) {
+        s.h_u[i] = ups[i].u;
+        s.h_v[i] = ups[i].v;
+        s.h_ins[i] = ups[i].is_insertion ? 1 : 0;
+    }
+    s.count = count;
+    s.ticket = t;
+    s.busy = true;
+    if (count) {
+        HIPCHK(c, hipMemcpyAsync(s.d_u, s.h_u, count * 4, hipMemcpyHostToDevice, c->copy_stream));
+        HIPCHK(c, hipMemcpyAsync(s.d_v, s.h_v, count * 4, hipMemcpyHostToDevice, c->copy_stream));
+        HIPCHK(c, hipMemcpyAsync(s.d_ins, s.h_ins, count, hipMemcpyHostToDevice, c->copy_stream));
+    }
+    HIPCHK(c, hipEventRecord(s.uploaded, c->copy_stream));
+    HIPCHK(c, hipStreamWaitEvent(c->stream, s.uploaded, 0));
+    c->ins_bound += nins;
+    c->pool_bound += 8 * nins;
+    c->epoch++;
+    BP P = c->P;
+    P.g = c->S.g;
+    P.bu = s.d_u;
+    P.bv = s.d_v;
+    P.bins = s.d_ins;
+    P.U = count;
+    P.b = c->epoch;
+    P.completion = c->completion;
+    P.window = std::max<int64_t>(count, 1);
+    c->last_U = count;
+    HIPCHK(c, hipEventRecord(s.t0, c->stream));
+    if (c->mode == MISFIT_MODE_PERSISTENT) {
+        HIPCHK(c, mf_launch_persistent(P, c->persistent_grid, c->stream));
+    } else {
+        rc = drive_plain(c, P);
+        if (rc) return rc;
+    }
+    HIPCHK(c, hipEventRecord(s.t1, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s.h_ctr, P.ctr, C_NCOUNTERS * sizeof(u64), hipMemcpyDeviceToHost, c->stream));
+    // the pool top rides after the counters in the readback
+    HIPCHK(c, hipMemcpyAsync(s.h_ctr + C_NCOUNTERS, P.g.pool_top, sizeof(u64), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipEventRecord(s.done, c->stream));
+    if (ticket) *ticket = t;
+    return MISFIT_OK;
+}
+
+extern "C" int misfit_wait(misfit_ctx* c, uint64_t ticket, misfit_batch_stats* st) {
+    if (!c) return MISFIT_E_ARG;
+    Slot& s = c->slots[ticket % MF_NSLOT];
+    if (!s.busy || s.ticket != ticket) return fail(c, MISFIT_E_ARG, "unknown or expired ticket");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipEventSynchronize(s.done));
+    s.busy = false;
+    if (s.h_ctr[C_POOL_OVF]) return fail(c, MISFIT_E_CHECK, "overlay pool overflow");
+    if (s.h_ctr[C_GUARD]) return fail(c, MISFIT_E_CHECK, "repair loop bound exceeded");
+    if (st) fill_stats(c, s, st);
+    return MISFIT_OK;
+}
+
+extern "C" int misfit_process_batch(misfit_ctx* c, const misfit_update* ups, int64_t count, misfit_batch_stats* st) {
+    uint64_t t = 0;
+    int rc = misfit_submit_batch(c, ups, count, &t);
+    if (rc) return rc;
+    return misfit_wait(c, t, st);
+}
+
+// ---------------------------------------------------------------- checks
+extern "C" int misfit_check(misfit_ctx* c, int64_t* conflicts, int64_t* uncovered, int64_t* count_errors) {
+    if (!c) return MISFIT_E_ARG;
+    if (!c->has_graph) return fail(c, MISFIT_E_STATE, "no graph loaded");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, mf_check(c->S, c->d_acc));
+    HIPCHK(c, hipMemcpyAsync(c->h_small, c->d_acc, 4 * sizeof(u64), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (conflicts) *conflicts = (int64_t)c->h_small[0];
+    if (uncovered) *uncovered = (int64_t)c->h_small[1];
+    if (count_errors) *count_errors = (int64_t)c->h_small[2];
+    return MISFIT_OK;
+}
+
+extern "C" int misfit_batch_bytes(misfit_ctx* c, int index_bytes, int64_t* bytes, int64_t* v1, int64_t* v2) {
+    if (!c) return MISFIT_E_ARG;
+    if (!c->has_graph) return fail(c, MISFIT_E_STATE, "no graph loaded");
+    HIPCHK(c, hipSetDevice(c->device));
+    c->bytes_tag++;
+    HIPCHK(c, mf_batch_bytes(c->S, c->P.seeds, c->P.ctr + C_SEEDS, c->bytes_tag, c->d_acc));
+    HIPCHK(c, hipMemcpyAsync(c->h_small, c->d_acc, 4 * sizeof(u64), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int64_t n1 = (int64_t)c->h_small[0], n2 = (int64_t)c->h_small[1], deg = (int64_t)c->h_small[2];
+    if (bytes) *bytes = 16 * c->last_U + (int64_t)index_bytes * n1 + 4 * deg + n2;
+    if (v1) *v1 = n1;
+    if (v2) *v2 = n2;
+    return MISFIT_OK;
+}
+
+// ---------------------------------------------------------------- compaction
+extern "C" int misfit_compact(misfit_ctx* c, int64_t* new_m) {
+    if (!c) return MISFIT_E_ARG;
+    if (!c->has_graph) return fail(c, MISFIT_E_STATE, "no graph loaded");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->copy_stream));
+    DevGraph& g = c->P.g;
+    int64_t* nidx = nullptr;
+    HIPCHK(c, hipMalloc((void**)&nidx, ((size_t)g.n + 1) * 8));
+    hipError_t e = mf_compact_plan(c->S, nidx);
+    int64_t total = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&total, nidx + g.n, 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    int32_t* nl = nullptr;
+    if (e == hipSuccess) e = hipMalloc((void**)&nl, (size_t)std::max<int64_t>(total, 1) * 4);
+    if (e == hipSuccess) e = mf_compact_copy(c->S, nidx, nl);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        hipFree(nidx);
+        if (nl) hipFree(nl);
+        HIPCHK(c, e);
+    }
+    auto& v = c->graph_allocs;
+    for (void* old : {(void*)c->d_nindex, (void*)c->d_nlist}) {
+        auto it = std::find(v.begin(), v.end(), old);
+        if (it != v.end()) { hipFree(*it); v.erase(it); }
+    }
+    v.push_back(nidx);
+    v.push_back(nl);
+    c->d_nindex = nidx;
+    c->d_nlist = nl;
+    g.nindex = nidx;
+    g.nlist = nl;
+    g.m = total;
+    c->S.g = g;
+    reset_overlay(c);
+    // rows are base order then insertion order, so they may be unsorted now
+    HIPCHK(c, mf_validate_csr(c->S, c->d_flags));
+    int flags = 0;
+    HIPCHK(c, hipMemcpyAsync(&flags, c->d_flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    g.sorted = (flags & 1) ? 0 : 1;
+    c->S.g = g;
+    if (new_m) *new_m = total;
+    return MISFIT_OK;
+}
+
+extern "C" int misfit_graph_size(misfit_ctx* c, int32_t* n, int64_t* base_m, int64_t* live_m) {
+    if (!c) return MISFIT_E_ARG;
+    if (!c->has_graph) return fail(c, MISFIT_E_STATE, "no graph loaded");
+    DevGraph& g = c->P.g;
+    if (n) *n = g.n;
+    if (base_m) *base_m = g.m;
+    if (live_m) {
+        HIPCHK(c, hipSetDevice(c->device));
+        int64_t* nidx = nullptr;
+        HIPCHK(c, hipMalloc((void**)&nidx, ((size_t)g.n + 1) * 8));
+        hipError_t e = mf_compact_plan(c->S, nidx);
+        int64_t total = 0;
+        if (e == hipSuccess) e = hipMemcpyAsync(&total, nidx + g.n, 8, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        hipFree(nidx);
+        HIPCHK(c, e);
+        *live_m = total;
+    }
+    return MISFIT_OK;
+}
+
+extern "C" int misfit_export_csr(misfit_ctx* c, int64_t* nindex, int32_t* nlist) {
+    if (!c || !nindex) return MISFIT_E_ARG;
+    if (!c->has_graph) return fail(c, MISFIT_E_STATE, "no graph loaded");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    DevGraph& g = c->P.g;
+    HIPCHK(c, hipMemcpy(nindex, g.nindex, ((size_t)g.n + 1) * 8, hipMemcpyDeviceToHost));
+    if (g.m) {
+        if (!nlist) return MISFIT_E_ARG;
+        HIPCHK(c, hipMemcpy(nlist, g.nlist, (size_t)g.m * 4, hipMemcpyDeviceToHost));
+        // deleted base entries are stored complemented; export live values
+        for (int64_t i = 0; i < g.m; i++)
+            if (nlist[i] < 0) return fail(c, MISFIT_E_STATE, "base holds deletions: compact before export");
+    }
+    return MISFIT_OK;
+}
+
+extern "C" int misfit_neighbors(misfit_ctx* c, int32_t v, int32_t* out, int64_t cap, int64_t* count) {
+    if (!c || !count) return MISFIT_E_ARG;
+    if (!c->has_graph) return fail(c, MISFIT_E_STATE, "no graph loaded");
+    DevGraph& g = c->P.g;
+    if (v < 0 || v >= g.n) return fail(c, MISFIT_E_ARG, "vertex out of range");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int64_t be[2];
+    int64_t ovs = 0;
+    int32_t ovl = 0;
+    HIPCHK(c, hipMemcpy(be, g.nindex + v, 16, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(&ovl, g.ov_len + v, 4, hipMemcpyDeviceToHost));
+    if (ovl) HIPCHK(c, hipMemcpy(&ovs, g.ov_start + v, 8, hipMemcpyDeviceToHost));
+    std::vector<int32_t> row((size_t)(be[1] - be[0]) + ovl);
+    if (be[1] > be[0]) HIPCHK(c, hipMemcpy(row.data(), g.nlist + be[0], (be[1] - be[0]) * 4, hipMemcpyDeviceToHost));
+    if (ovl) HIPCHK(c, hipMemcpy(row.data() + (be[1] - be[0]), g.pool + ovs, (size_t)ovl * 4, hipMemcpyDeviceToHost));
+    int64_t k = 0;
+    for (int32_t y : row) {
+        if (y < 0) continue;
+        if (out && k < cap) out[k] = y;
+        k++;
+    }
+    *count = k;
+    return MISFIT_OK;
+}

@@ -1,0 +1,220 @@
+// dev.h -- device-side data layout and helpers shared by every MISFIT kernel.
+//
+// HBM layout of one graph (everything stays device-resident for the run):
+//   nindex[n+1]  int64   base CSR offsets (both .egr widths are widened once
+//                         at load)
+//   nlist[m]     int32   base CSR entries.  A deleted base edge stays in
+//                         place as its bitwise complement (~v < 0): a row read
+//                         skips it with one sign test and re-inserting it
+//                         restores the original position, which is what the
+//                         reference's tombstone sets give
+//                         (core/FlashGraph.h:175,196-233).
+//   ov_*         per-vertex overlay run in a shared pool: inserted
+//                         neighbours in insertion order; a removed insertion
+//                         becomes ~v inside its run (the reference's
+//                         added_adj vectors, core/FlashGraph.h:174,260-275).
+//   ph_key/ph_val         open-addressed hash of inserted pairs
+//                         (key lo<<32|hi; value 1 present, 0 removed).
+//   mc[n]        uint32  bit 31 = MIS membership; bits 0..30 = number of MIS
+//                         neighbours (self-loops excluded).  The count turns
+//                         the reference's "any MIS member in this row" scans
+//                         into one 4-byte gather.
+//   per-batch epoch words (a word equal to the current batch epoch b means
+//   "set in this batch", so nothing is ever cleared between batches):
+//     seedb   = b   vertex is an endpoint of batch b (it has a 2-hop row)
+//     wpotb   = b   vertex is max(u,v) of an insertion of batch b (may be
+//                   evicted by the rule)
+//     kw      = b<<32 | k   k = MIS neighbours of the vertex that may be
+//                   evicted in batch b
+//     pub     = 4b+1 | 2    vertex may become uncovered during batch b
+//                   (bit 1: it also has a row, i.e. it is admissible)
+//     evb     = b   vertex was evicted in batch b
+//     cstat   = 4b+s        completion state (1 open, 2 joined, 3 covered)
+//     fa_*    per-batch index of admissible neighbours of long rows
+//   mmark/cmark  u64 round<<32 | ~edge: earliest edge of the current
+//                   speculative round that flips the vertex's membership
+//                   (mmark) or changes its MIS-neighbour count (cmark).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define MF_MISBIT 0x80000000u
+#define MF_CMASK 0x7FFFFFFFu
+#define MF_WAVE 64
+#define MF_EMPTY_KEY 0xFFFFFFFFFFFFFFFFull
+
+typedef unsigned long long u64;
+
+struct DevGraph {
+    int32_t n;
+    int64_t m;
+    const int64_t* nindex;
+    int32_t* nlist;
+    int64_t* ov_start;
+    int32_t* ov_len;
+    int32_t* ov_cap;
+    int32_t* ov_alive;
+    int32_t* ov_tomb;
+    int32_t* pool;
+    int64_t pool_cap;
+    u64* pool_top;
+    u64* ph_key;
+    int32_t* ph_val;
+    u64 ph_mask;
+    uint32_t* mc;
+    uint32_t* seedb;
+    uint32_t* wpotb;
+    u64* kw;
+    uint32_t* pub;
+    uint32_t* evb;
+    uint32_t* cstat;
+    u64* mmark;
+    u64* cmark;
+    int32_t* ne_head;
+    u64* fa_cnt;     // b<<32 | count
+    int64_t* fa_start;
+    int32_t* fa_fill;
+    int sorted;      // every base row strictly ascending
+};
+
+struct Row {
+    const int32_t* base;
+    int64_t blen;
+    const int32_t* add;
+    int64_t alen;
+    __device__ __forceinline__ int64_t len() const { return blen + alen; }
+    __device__ __forceinline__ int32_t at(int64_t j) const { return j < blen ? base[j] : add[j - blen]; }
+};
+
+__device__ __forceinline__ Row get_row(const DevGraph& g, int32_t x) {
+    Row r;
+    int64_t s = g.nindex[x];
+    r.base = g.nlist + s;
+    r.blen = g.nindex[x + 1] - s;
+    int32_t al = g.ov_len[x];
+    r.alen = al;
+    r.add = al ? g.pool + g.ov_start[x] : nullptr;
+    return r;
+}
+
+__device__ __forceinline__ int64_t row_len(const DevGraph& g, int32_t x) {
+    return (g.nindex[x + 1] - g.nindex[x]) + g.ov_len[x];
+}
+
+__device__ __forceinline__ int32_t decode(int32_t e) { return e < 0 ? ~e : e; }
+
+__device__ __forceinline__ u64 pair_key(int32_t a, int32_t b) {
+    uint32_t lo = (uint32_t)(a < b ? a : b), hi = (uint32_t)(a < b ? b : a);
+    return ((u64)lo << 32) | (u64)hi;
+}
+
+// SplitMix64 finaliser (Steele, Lea, Flood, OOPSLA 2014).
+__device__ __forceinline__ u64 mix64(u64 z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// Value of an inserted pair: 1 present, 0 removed or never inserted.
+__device__ __forceinline__ int32_t ph_lookup(const DevGraph& g, u64 key) {
+    u64 s = mix64(key) & g.ph_mask;
+    while (true) {
+        u64 k = g.ph_key[s];
+        if (k == key) return g.ph_val[s];
+        if (k == MF_EMPTY_KEY) return 0;
+        s = (s + 1) & g.ph_mask;
+    }
+}
+
+// Slot of `key`, inserting it when absent.  The host keeps the load factor
+// at or below one half, so the probe always ends.
+__device__ __forceinline__ u64 ph_slot(const DevGraph& g, u64 key) {
+    u64 s = mix64(key) & g.ph_mask;
+    while (true) {
+        u64 prev = atomicCAS(&g.ph_key[s], MF_EMPTY_KEY, key);
+        if (prev == MF_EMPTY_KEY || prev == key) return s;
+        s = (s + 1) & g.ph_mask;
+    }
+}
+
+// Position of y (live or deleted) inside base row x, or -1.
+__device__ __forceinline__ int64_t base_find(const DevGraph& g, int32_t x, int32_t y) {
+    int64_t b = g.nindex[x], e = g.nindex[x + 1];
+    if (g.sorted) {
+        int64_t lo = b, hi = e;
+        while (lo < hi) {
+            int64_t mid = (lo + hi) >> 1;
+            int32_t d = decode(g.nlist[mid]);
+            if (d < y) lo = mid + 1;
+            else hi = mid;
+        }
+        if (lo < e && decode(g.nlist[lo]) == y) return lo;
+        return -1;
+    }
+    for (int64_t p = b; p < e; p++)
+        if (decode(g.nlist[p]) == y) return p;
+    return -1;
+}
+
+// Adjacency in the current graph (base minus deletions plus insertions): the
+// device form of HybridGraph::isAdjacent (core/FlashGraph.h:291-301).
+__device__ __forceinline__ bool adjacent(const DevGraph& g, int32_t x, int32_t y) {
+    if (ph_lookup(g, pair_key(x, y)) == 1) return true;
+    int64_t p = base_find(g, x, y);
+    return p >= 0 && g.nlist[p] >= 0;
+}
+
+// Flat position of live neighbour z inside row(x) (base part first, then
+// the overlay run), or -1.
+__device__ __forceinline__ int64_t row_pos(const DevGraph& g, int32_t x, int32_t z) {
+    int64_t p = base_find(g, x, z);
+    if (p >= 0 && g.nlist[p] >= 0) return p - g.nindex[x];
+    int32_t al = g.ov_len[x];
+    if (al) {
+        const int32_t* a = g.pool + g.ov_start[x];
+        for (int32_t i = 0; i < al; i++)
+            if (a[i] == z) return (g.nindex[x + 1] - g.nindex[x]) + i;
+    }
+    return -1;
+}
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (MF_WAVE - 1); }
+
+__device__ __forceinline__ int first_lane(u64 mask) { return mask ? __ffsll((long long)mask) - 1 : -1; }
+
+// Wave-wide inclusive prefix sum.
+__device__ __forceinline__ int64_t wave_incl_scan(int64_t v) {
+    int l = lane_id();
+#pragma unroll
+    for (int d = 1; d < MF_WAVE; d <<= 1) {
+        int64_t t = __shfl_up(v, d, MF_WAVE);
+        if (l >= d) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ u64 wave_min_u64(u64 v) {
+#pragma unroll
+    for (int d = MF_WAVE / 2; d > 0; d >>= 1) {
+        u64 o = __shfl_xor(v, d, MF_WAVE);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+// Grid coordinates for grid-stride phases (one phase body serves both the
+// per-phase kernels and the persistent kernel).
+struct GC {
+    int64_t tid, nthreads, wave, nwaves;
+    int lane;
+};
+
+__device__ __forceinline__ GC grid_coords() {
+    GC c;
+    c.tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    c.nthreads = (int64_t)gridDim.x * blockDim.x;
+    c.wave = c.tid / MF_WAVE;
+    c.nwaves = c.nthreads / MF_WAVE;
+    c.lane = lane_id();
+    return c;
+}

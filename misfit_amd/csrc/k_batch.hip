@@ -1,0 +1,957 @@
+// k_batch.hip -- one update batch on the device: overlay apply, the set of
+// vertices the batch may uncover, the exact sequential repair rule evaluated
+// in speculative rounds, and the maximality completion.
+//
+// Semantics (reference = anonymous-artifacts/MISFIT):
+//   * the whole batch is applied to the overlay first
+//     (src/misfit/misfit.cu:704-724), so the repair reads the post-batch
+//     graph, which is static while membership evolves;
+//   * the rule is applied per edge in batch order
+//     (src/misfit/misfit.cu:283-347):
+//       insertion, both ends in the MIS: evict w = max(u,v), then admit the
+//         first vertex x of row(w) (BFS order: [w] ++ N1(w) ++ N2(w)) that is
+//         outside the MIS, has a row, and whose row holds no MIS member;
+//       deletion, exactly one end a in the MIS: admit the other end b when
+//         row(b) holds no MIS member other than a;
+//     where row(s) = [s] ++ N1 ++ N2 for batch endpoints and row(x) = adj(x)
+//     for other neighbours of endpoints (src/misfit/misfit.cu:158-202,
+//     550-558); nobody else has a row;
+//   * completion: over C = endpoints ∪ evicted ∪ N(evicted), in ascending
+//     id, a vertex joins when it is outside the MIS with no MIS neighbour.
+//
+// How the GPU evaluates the sequential rule exactly:
+//   1. PU ("potentially uncovered") = W ∪ {z : z ∉ MIS, c(z) = k(z)}, where W
+//      = {max(u,v) of insertions}, c(z) = MIS neighbours, k(z) = MIS
+//      neighbours in W (all at batch start).  Only evictions remove members
+//      and only members of W can be evicted, so a vertex outside PU keeps an
+//      MIS neighbour (or stays a member) for the whole batch.  Hence only PU
+//      members with a row (ADM) can ever be admitted, and a vertex outside PU
+//      always blocks a 2-hop test.  The repair reads nothing else.
+//   2. BFS discovery order of row(w) equals first-occurrence order of
+//      [w] ++ adj(w) ++ concat_{x1 in adj(w)} adj(x1), so the first admissible
+//      vertex is found by scanning that flat sequence; long adj(x1) rows are
+//      replaced by a per-batch index of their ADM members with positions.
+//   3. Edges are evaluated speculatively in rounds: every pending edge of a
+//      window is evaluated against the current state; the writes it would do
+//      are marked per vertex with the earliest writing edge; a second walk
+//      finds the first edge that read a vertex written by an earlier edge of
+//      the round.  Every edge before it saw exactly the sequential state and
+//      commits; the rest are re-evaluated next round (the first pending edge
+//      never conflicts, so every round makes progress).
+#include <hip/hip_cooperative_groups.h>
+#include "batch.h"
+
+namespace cg = cooperative_groups;
+
+// ---------------------------------------------------------------- helpers
+__device__ __forceinline__ bool in_pu(const BP& P, int32_t x) { return (P.g.pub[x] >> 2) == P.b; }
+__device__ __forceinline__ bool is_adm(const BP& P, int32_t x) { return P.g.pub[x] == 4u * P.b + 3u; }
+
+__device__ __forceinline__ void push(int32_t* list, u64* ctr, int32_t v) {
+    u64 i = atomicAdd(ctr, 1ull);
+    list[i] = v;
+}
+
+__device__ __forceinline__ bool pu_add(const BP& P, int32_t z) {
+    uint32_t want = 4u * P.b + 1u;
+    uint32_t old = atomicMax(&P.g.pub[z], want);
+    if (old < want) {
+        u64 i = atomicAdd(&P.ctr[C_PU], 1ull);
+        if (i < (u64)P.pu_cap) P.pu[i] = z;
+        return true;
+    }
+    return false;
+}
+
+// Epoch-tagged counter word b<<32 | count.  Returns true on the first touch
+// in this batch.
+__device__ __forceinline__ bool epoch_count_add(u64* w, uint32_t b) {
+    u64 hi = (u64)b << 32;
+    u64 old = *w;
+    while (true) {
+        if ((old & 0xFFFFFFFF00000000ull) == hi) {
+            atomicAdd(w, 1ull);
+            return false;
+        }
+        u64 prev = atomicCAS(w, old, hi | 1ull);
+        if (prev == old) return true;
+        old = prev;
+    }
+}
+
+__device__ __forceinline__ bool mk_before(const u64* mk, int32_t x, uint32_t R, int32_t e) {
+    u64 m = mk[x];
+    return (uint32_t)(m >> 32) == R && (int64_t)(0xFFFFFFFFu - (uint32_t)m) < (int64_t)e;
+}
+
+template <bool V>
+__device__ __forceinline__ uint32_t read_mc(const BP& P, int32_t x, uint32_t R, int32_t e, bool& conf) {
+    uint32_t v = P.g.mc[x];
+    if (V) conf |= mk_before(P.g.mmark, x, R, e) || mk_before(P.g.cmark, x, R, e);
+    return v;
+}
+
+// ---------------------------------------------------------------- prologue
+// P1: claim pairs, tag endpoints and possible evictees.
+__device__ void ph_claim(const BP& P, const GC& c) {
+    for (int64_t i = c.tid; i < P.U; i += c.nthreads) {
+        int32_t a = P.bu[i], bb = P.bv[i];
+        u64 key = pair_key(a, bb);
+        u64 s = mix64(key) & P.bt_mask;
+        while (true) {
+            u64 prev = atomicCAS(&P.bt_key[s], MF_EMPTY_KEY, key);
+            if (prev == MF_EMPTY_KEY || prev == key) break;
+            s = (s + 1) & P.bt_mask;
+        }
+        int32_t prevh = atomicExch(&P.bt_head[s], (int32_t)i);
+        P.bt_nxt[i] = prevh;
+        P.bt_slot[i] = (int32_t)s;
+        if (atomicExch(&P.g.seedb[a], P.b) != P.b) push(P.seeds, &P.ctr[C_SEEDS], a);
+        if (atomicExch(&P.g.seedb[bb], P.b) != P.b) push(P.seeds, &P.ctr[C_SEEDS], bb);
+        if (P.bins[i]) {
+            int32_t w = a > bb ? a : bb;
+            if (atomicExch(&P.g.wpotb[w], P.b) != P.b) push(P.wpot, &P.ctr[C_WPOT], w);
+        }
+    }
+}
+
+__device__ __forceinline__ void set_base_deleted(const DevGraph& g, int32_t x, int64_t p, bool del) {
+    int32_t e = g.nlist[p];
+    if (del && e >= 0) {
+        g.nlist[p] = ~e;
+        atomicAdd(&g.ov_tomb[x], 1);
+    } else if (!del && e < 0) {
+        g.nlist[p] = ~e;
+        atomicSub(&g.ov_tomb[x], 1);
+    }
+}
+
+__device__ __forceinline__ void kill_run_entry(const DevGraph& g, int32_t x, int32_t y) {
+    int32_t len = g.ov_len[x];
+    int32_t* r = g.pool + g.ov_start[x];
+    for (int32_t i = 0; i < len; i++) {
+        if (r[i] == y) {
+            r[i] = ~y;
+            atomicSub(&g.ov_alive[x], 1);
+            return;
+        }
+    }
+}
+
+// P2: one thread per distinct pair replays the pair's updates in batch order
+// with the overlay's rules (core/FlashGraph.h:196-212 via misfit.cu:708-714)
+// and applies the net effect.
+__device__ void ph_resolve(const BP& P, const GC& c) {
+    const DevGraph& g = P.g;
+    for (int64_t i = c.tid; i < P.U; i += c.nthreads) {
+        int32_t slot = P.bt_slot[i];
+        if (P.bt_head[slot] != (int32_t)i) continue;
+        int32_t a = P.bu[i], bb = P.bv[i];
+        int32_t lo = a < bb ? a : bb, hi = a < bb ? bb : a;
+        int64_t plo = base_find(g, lo, hi);
+        bool in_base = plo >= 0;
+        bool tomb0 = in_base && g.nlist[plo] < 0;
+        u64 key = pair_key(lo, hi);
+        bool alive0 = !in_base && ph_lookup(g, key) == 1;
+        bool adj0 = (in_base && !tomb0) || alive0;
+        bool tomb = tomb0, alive = alive0, adj = adj0, old_killed = false;
+        int32_t last_ins = -1;
+        int64_t ops = 0;
+        int64_t prev_idx = -1;
+        while (true) {
+            int64_t cur = INT64_MAX;
+            for (int32_t j = (int32_t)i; j != -1; j = P.bt_nxt[j])
+                if (j > prev_idx && j < cur) cur = j;
+            if (cur == INT64_MAX) break;
+            prev_idx = cur;
+            bool ins = P.bins[cur] != 0;
+            if (ins && !adj) {
+                if (in_base && tomb) tomb = false;
+                else { alive = true; last_ins = (int32_t)cur; }
+                adj = true;
+                ops++;
+            } else if (!ins && adj) {
+                if (alive) {
+                    alive = false;
+                    if (last_ins < 0) old_killed = true;
+                    last_ins = -1;
+                } else {
+                    tomb = true;
+                }
+                adj = false;
+                ops++;
+            }
+        }
+        P.bt_key[slot] = MF_EMPTY_KEY;
+        P.bt_head[slot] = -1;
+        if (ops) atomicAdd(&P.ctr[C_DELTA_OPS], (u64)ops);
+        if (in_base && tomb != tomb0) {
+            set_base_deleted(g, lo, plo, tomb);
+            int64_t phi = base_find(g, hi, lo);
+            if (phi >= 0) set_base_deleted(g, hi, phi, tomb);
+        }
+        if (old_killed) {
+            kill_run_entry(g, lo, hi);
+            kill_run_entry(g, hi, lo);
+        }
+        if (!in_base && (alive != alive0 || last_ins >= 0)) {
+            u64 s = ph_slot(g, key);
+            g.ph_val[s] = alive ? 1 : 0;
+        }
+        if (alive && last_ins >= 0) {
+            u64 k = atomicAdd(&P.ctr[C_NE], 2ull);
+            P.ne_x[k] = lo; P.ne_y[k] = hi; P.ne_seq[k] = last_ins;
+            P.ne_x[k + 1] = hi; P.ne_y[k + 1] = lo; P.ne_seq[k + 1] = last_ins;
+        }
+        if (adj != adj0) {
+            uint32_t d = adj ? 1u : 0xFFFFFFFFu;  // +1 or -1 on the count bits
+            uint32_t mlo = g.mc[lo], mhi = g.mc[hi];
+            if (mhi & MF_MISBIT) atomicAdd(&g.mc[lo], d);
+            if (mlo & MF_MISBIT) atomicAdd(&g.mc[hi], d);
+        }
+    }
+}
+
+// P3: chain the new entries of each vertex.
+__device__ void ph_group(const BP& P, const GC& c) {
+    int64_t ne = (int64_t)P.ctr[C_NE];
+    for (int64_t k = c.tid; k < ne; k += c.nthreads) {
+        int32_t prev = atomicExch(&P.g.ne_head[P.ne_x[k]], (int32_t)k);
+        P.ne_nxt[k] = prev;
+    }
+}
+
+// P4: the chain owner appends the vertex's new entries to its run in batch
+// order, growing the run by doubling inside the pool.
+__device__ void ph_append(const BP& P, const GC& c) {
+    const DevGraph& g = P.g;
+    int64_t ne = (int64_t)P.ctr[C_NE];
+    for (int64_t k = c.tid; k < ne; k += c.nthreads) {
+        int32_t x = P.ne_x[k];
+        if (g.ne_head[x] != (int32_t)k) continue;
+        int32_t cnt = 0;
+        for (int32_t j = (int32_t)k; j != -1; j = P.ne_nxt[j]) cnt++;
+        int32_t len = g.ov_len[x], cap = g.ov_cap[x];
+        if (len + cnt > cap) {
+            int32_t nc = 2;
+            while (nc < len + cnt) nc <<= 1;
+            u64 st = atomicAdd(g.pool_top, (u64)nc);
+            if ((int64_t)(st + nc) > g.pool_cap) {
+                atomicExch(&P.ctr[C_POOL_OVF], 1ull);
+                g.ne_head[x] = -1;
+                continue;
+            }
+            if (len) {
+                const int32_t* src = g.pool + g.ov_start[x];
+                for (int32_t t = 0; t < len; t++) g.pool[st + t] = src[t];
+            }
+            g.ov_start[x] = (int64_t)st;
+            g.ov_cap[x] = nc;
+        }
+        int32_t* run = g.pool + g.ov_start[x];
+        int32_t prev_seq = -1;
+        for (int32_t t = 0; t < cnt; t++) {
+            int32_t best = -1, bseq = INT32_MAX;
+            for (int32_t j = (int32_t)k; j != -1; j = P.ne_nxt[j]) {
+                int32_t s = P.ne_seq[j];
+                if (s > prev_seq && s < bseq) { bseq = s; best = j; }
+            }
+            run[len + t] = P.ne_y[best];
+            prev_seq = bseq;
+        }
+        g.ov_len[x] = len + cnt;
+        atomicAdd(&g.ov_alive[x], cnt);
+        g.ne_head[x] = -1;
+    }
+}
+
+// P5: k(z) for neighbours of MIS vertices that may be evicted; W joins PU;
+// every currently uncovered vertex joins PU.
+__device__ void ph_kcount(const BP& P, const GC& c) {
+    const DevGraph& g = P.g;
+    int64_t nw = (int64_t)P.ctr[C_WPOT];
+    for (int64_t i = c.wave; i < nw; i += c.nwaves) {
+        int32_t w = P.wpot[i];
+        if (c.lane == 0) pu_add(P, w);
+        if (!(g.mc[w] & MF_MISBIT)) continue;
+        Row r = get_row(g, w);
+        int64_t L = r.len();
+        for (int64_t j = c.lane; j < L; j += MF_WAVE) {
+            int32_t z = r.at(j);
+            if (z < 0 || z == w) continue;
+            if (epoch_count_add(&g.kw[z], P.b)) {
+                u64 t = atomicAdd(&P.ctr[C_TOUCHED], 1ull);
+                if (t < (u64)P.touched_cap) P.touched[t] = z;
+            }
+        }
+    }
+    for (int64_t x = c.tid; x < g.n; x += c.nthreads) {
+        uint32_t m = g.mc[x];
+        if (m == 0) pu_add(P, (int32_t)x);  // outside the MIS with no MIS neighbour
+    }
+}
+
+// P6: touched z joins PU when all its MIS neighbours may be evicted.
+__device__ void ph_pu_touched(const BP& P, const GC& c) {
+    int64_t nt = (int64_t)P.ctr[C_TOUCHED];
+    if (nt > P.touched_cap) nt = P.touched_cap;
+    for (int64_t i = c.tid; i < nt; i += c.nthreads) {
+        int32_t z = P.touched[i];
+        uint32_t m = P.g.mc[z];
+        uint32_t k = (uint32_t)(P.g.kw[z] & 0xFFFFFFFFull);
+        if (!(m & MF_MISBIT) && (m & MF_CMASK) == k) pu_add(P, z);
+    }
+}
+
+// P7: a PU member has a row when it is an endpoint or next to one.
+__device__ void ph_hasrow(const BP& P, const GC& c) {
+    const DevGraph& g = P.g;
+    int64_t np = (int64_t)P.ctr[C_PU];
+    if (np > P.pu_cap) np = P.pu_cap;
+    for (int64_t i = c.wave; i < np; i += c.nwaves) {
+        int32_t z = P.pu[i];
+        bool has = g.seedb[z] == P.b;
+        if (!has) {
+            Row r = get_row(g, z);
+            int64_t L = r.len();
+            for (int64_t j0 = 0; j0 < L && !has; j0 += MF_WAVE) {
+                int64_t j = j0 + c.lane;
+                bool s = false;
+                if (j < L) {
+                    int32_t y = r.at(j);
+                    s = y >= 0 && g.seedb[y] == P.b;
+                }
+                has = __ballot(s) != 0;
+            }
+        }
+        if (has && c.lane == 0) {
+            g.pub[z] = 4u * P.b + 3u;
+            atomicAdd(&P.ctr[C_ADM], 1ull);
+        }
+    }
+}
+
+// P8: count ADM members next to each long row.
+__device__ void ph_fa_count(const BP& P, const GC& c) {
+    const DevGraph& g = P.g;
+    int64_t np = (int64_t)P.ctr[C_PU];
+    if (np > P.pu_cap) np = P.pu_cap;
+    for (int64_t i = c.wave; i < np; i += c.nwaves) {
+        int32_t z = P.pu[i];
+        if (!is_adm(P, z)) continue;
+        Row r = get_row(g, z);
+        int64_t L = r.len();
+        for (int64_t j = c.lane; j < L; j += MF_WAVE) {
+            int32_t x1 = r.at(j);
+            if (x1 < 0 || x1 == z) continue;
+            if (row_len(g, x1) <= MF_FA_T) continue;
+            if (epoch_count_add(&g.fa_cnt[x1], P.b)) {
+                u64 t = atomicAdd(&P.ctr[C_BIG], 1ull);
+                if (t < (u64)P.big_cap) P.big[t] = x1;
+                else atomicExch(&P.ctr[C_FA_OVF], 1ull);
+            }
+        }
+    }
+}
+
+// P9: reserve index space per long row (no space: the row is scanned).
+__device__ void ph_fa_alloc(const BP& P, const GC& c) {
+    int64_t nb = (int64_t)P.ctr[C_BIG];
+    bool ovf = nb > P.big_cap;
+    if (ovf) nb = P.big_cap;
+    for (int64_t i = c.tid; i < nb; i += c.nthreads) {
+        int32_t x1 = P.big[i];
+        u64 cnt = P.g.fa_cnt[x1] & 0xFFFFFFFFull;
+        u64 s = atomicAdd(&P.ctr[C_FA_TOP], cnt);
+        P.g.fa_start[x1] = (ovf || (int64_t)(s + cnt) > P.fa_cap) ? -1 : (int64_t)s;
+        P.g.fa_fill[x1] = 0;
+    }
+}
+
+// P10: fill the index: (z, position of z in row(x1)).
+__device__ void ph_fa_fill(const BP& P, const GC& c) {
+    const DevGraph& g = P.g;
+    int64_t np = (int64_t)P.ctr[C_PU];
+    if (np > P.pu_cap) np = P.pu_cap;
+    for (int64_t i = c.wave; i < np; i += c.nwaves) {
+        int32_t z = P.pu[i];
+        if (!is_adm(P, z)) continue;
+        Row r = get_row(g, z);
+        int64_t L = r.len();
+        for (int64_t j = c.lane; j < L; j += MF_WAVE) {
+            int32_t x1 = r.at(j);
+            if (x1 < 0 || x1 == z) continue;
+            if (row_len(g, x1) <= MF_FA_T) continue;
+            int64_t st = g.fa_start[x1];
+            if (st < 0) continue;
+            int64_t pos = row_pos(g, x1, z);
+            if (pos < 0) continue;
+            int32_t slot = atomicAdd(&g.fa_fill[x1], 1);
+            P.fa_z[st + slot] = z;
+            P.fa_pos[st + slot] = (int32_t)pos;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- the rule
+// Is x (an ADM vertex, x != w) admissible right after w's eviction?
+// xadj: x is a neighbour of w, so w is one of the MIS neighbours it counts.
+template <bool V>
+__device__ bool cand_ok(const BP& P, int32_t x, int32_t w, bool xadj, int32_t e, uint32_t R, bool& conf) {
+    const DevGraph& g = P.g;
+    uint32_t mx = read_mc<V>(P, x, R, e, conf);
+    if (mx & MF_MISBIT) return false;
+    uint32_t cx = mx & MF_CMASK;
+    if (xadj && cx) cx -= 1;
+    if (cx) return false;
+    if (g.seedb[x] != P.b) return true;  // candidate row = adj(x)
+    // seed row {x} ∪ N1(x) ∪ N2(x) must hold no MIS member
+    Row r = get_row(g, x);
+    int64_t L = r.len();
+    for (int64_t j = 0; j < L; j++) {
+        int32_t y = r.at(j);
+        if (y < 0 || y == x) continue;
+        if (y == w) {
+            if (read_mc<V>(P, w, R, e, conf) & MF_CMASK) return false;
+            continue;
+        }
+        if (!in_pu(P, y)) return false;
+        uint32_t my = read_mc<V>(P, y, R, e, conf);
+        if (my & MF_MISBIT) return false;
+        uint32_t cy = my & MF_CMASK;
+        if (cy == 0) continue;
+        if (cy >= 2) return false;
+        if (!adjacent(g, y, w)) return false;
+    }
+    return true;
+}
+
+// Insertion with both ends in the MIS, w evicted: first admissible vertex of
+// row(w) in BFS order, or -1.  Wave-cooperative; all lanes return the same.
+template <bool V>
+__device__ int32_t eval_insertion(const BP& P, int32_t w, int32_t e, uint32_t R, bool& conf) {
+    const DevGraph& g = P.g;
+    const int lane = lane_id();
+    // position 0: w itself
+    {
+        uint32_t mw = read_mc<V>(P, w, R, e, conf);
+        if ((mw & MF_CMASK) == 0) {
+            Row r = get_row(g, w);
+            int64_t L = r.len();
+            bool blocked = false;
+            for (int64_t j0 = 0; j0 < L && !blocked; j0 += MF_WAVE) {
+                int64_t j = j0 + lane;
+                bool bl = false;
+                if (j < L) {
+                    int32_t y = r.at(j);
+                    if (y >= 0 && y != w) {
+                        if (!in_pu(P, y)) bl = true;
+                        else {
+                            uint32_t my = read_mc<V>(P, y, R, e, conf);
+                            bl = (my & MF_MISBIT) || (my & MF_CMASK) >= 2;
+                        }
+                    }
+                }
+                blocked = __ballot(bl) != 0;
+            }
+            if (!blocked) return w;
+        }
+    }
+    Row rw = get_row(g, w);
+    const int64_t L = rw.len();
+    // part 2: N1(w) in adjacency order
+    for (int64_t j0 = 0; j0 < L; j0 += MF_WAVE) {
+        int64_t j = j0 + lane;
+        int32_t x = -1;
+        bool ok = false;
+        if (j < L) {
+            x = rw.at(j);
+            if (x >= 0 && x != w && is_adm(P, x)) ok = cand_ok<V>(P, x, w, true, e, R, conf);
+        }
+        u64 bal = __ballot(ok);
+        if (bal) return __shfl(x, first_lane(bal), MF_WAVE);
+    }
+    // part 3: concat over x1 in adj(w) of adj(x1); long rows through the index
+    const bool fa_ok = P.ctr[C_FA_OVF] == 0;
+    for (int64_t j0 = 0; j0 < L; j0 += MF_WAVE) {
+        int64_t j = j0 + lane;
+        int kindl = 0;  // 0 empty, 1 row scan, 2 index
+        const int32_t* pb = nullptr;
+        const int32_t* pa = nullptr;
+        int64_t blen = 0, slen = 0;
+        if (j < L) {
+            int32_t x1 = rw.at(j);
+            if (x1 >= 0 && x1 != w) {
+                Row r1 = get_row(g, x1);
+                int64_t l1 = r1.len();
+                if (l1 > MF_FA_T && fa_ok) {
+                    if ((uint32_t)(g.fa_cnt[x1] >> 32) == P.b) {
+                        int64_t st = g.fa_start[x1];
+                        if (st >= 0) {
+                            kindl = 2;
+                            pb = P.fa_z + st;
+                            pa = P.fa_pos + st;
+                            slen = g.fa_fill[x1];
+                        } else {
+                            kindl = 1; pb = r1.base; pa = r1.add; blen = r1.blen; slen = l1;
+                        }
+                    }
+                } else {
+                    kindl = 1; pb = r1.base; pa = r1.add; blen = r1.blen; slen = l1;
+                }
+            }
+        }
+        int64_t incl = wave_incl_scan(slen);
+        int64_t excl = incl - slen;
+        int64_t total = __shfl(incl, MF_WAVE - 1, MF_WAVE);
+        u64 best = ~0ull;
+        int32_t best_x = -1;
+        for (int64_t k0 = 0; k0 < total; k0 += MF_WAVE) {
+            int64_t k = k0 + lane;
+            int64_t kk = k < total ? k : total - 1;
+            // owner lane of flat index kk: largest lane whose segment starts
+            // at or before kk (fixed 6 halvings, all lanes active)
+            int lo = 0, hi = MF_WAVE - 1;
+#pragma unroll
+            for (int step = 0; step < 6; step++) {
+                int mid = (lo + hi + 1) >> 1;
+                int64_t om = __shfl(excl, mid, MF_WAVE);
+                if (om <= kk) lo = mid; else hi = mid - 1;
+            }
+            const int owner = lo;
+            int64_t oexcl = __shfl(excl, owner, MF_WAVE);
+            int okind = __shfl(kindl, owner, MF_WAVE);
+            const int32_t* obase = (const int32_t*)(intptr_t)__shfl((long long)(intptr_t)pb, owner, MF_WAVE);
+            const int32_t* oadd = (const int32_t*)(intptr_t)__shfl((long long)(intptr_t)pa, owner, MF_WAVE);
+            int64_t oblen = __shfl(blen, owner, MF_WAVE);
+            u64 key = ~0ull;
+            int32_t x = -1;
+            if (k < total) {
+                int64_t off = k - oexcl;
+                int64_t pos;
+                if (okind == 2) {
+                    x = obase[off];
+                    pos = oadd[off];
+                } else {
+                    x = off < oblen ? obase[off] : oadd[off - oblen];
+                    pos = off;
+                }
+                if (x >= 0 && x != w && is_adm(P, x) && cand_ok<V>(P, x, w, false, e, R, conf))
+                    key = ((u64)(uint32_t)owner << 32) | (u64)(uint32_t)pos;
+            }
+            u64 mn = wave_min_u64(key);
+            if (mn < best) {
+                u64 bal = __ballot(key == mn);
+                best = mn;
+                best_x = __shfl(x, first_lane(bal), MF_WAVE);
+            }
+            if (best != ~0ull) {
+                // stop once every element of the best owner's segment is seen
+                int bo = (int)(best >> 32);
+                int64_t end_bo = __shfl(incl, bo, MF_WAVE);
+                if (k0 + MF_WAVE >= end_bo) break;
+            }
+        }
+        if (best != ~0ull) return best_x;
+    }
+    return -1;
+}
+
+// Deletion, exactly one end `other` in the MIS: may tgt join?  row(tgt) =
+// {tgt} ∪ N1 ∪ N2 must hold no MIS member other than `other`.
+template <bool V>
+__device__ bool eval_deletion(const BP& P, int32_t tgt, int32_t other, int32_t e, uint32_t R, bool& conf) {
+    const DevGraph& g = P.g;
+    const int lane = lane_id();
+    Row r = get_row(g, tgt);
+    int64_t L = r.len();
+    for (int64_t j0 = 0; j0 < L; j0 += MF_WAVE) {
+        int64_t j = j0 + lane;
+        bool bl = false;
+        if (j < L) {
+            int32_t y = r.at(j);
+            if (y >= 0 && y != tgt) {
+                uint32_t my = read_mc<V>(P, y, R, e, conf);
+                if (y == other) bl = (my & MF_CMASK) != 0;
+                else if (my & MF_MISBIT) bl = true;
+                else {
+                    uint32_t cy = my & MF_CMASK;
+                    bl = cy >= 2 || (cy == 1 && !adjacent(g, y, other));
+                }
+            }
+        }
+        if (__ballot(bl)) return false;
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------- rounds
+__device__ void ph_classify(const BP& P, const GC& c, int64_t start, int64_t end, int par) {
+    for (int64_t e = start + c.tid; e < end; e += c.nthreads) {
+        int32_t a = P.bu[e], bb = P.bv[e];
+        bool ma = (P.g.mc[a] & MF_MISBIT) != 0, mb = (P.g.mc[bb] & MF_MISBIT) != 0;
+        uint8_t k = 0;
+        if (P.bins[e]) k = (ma && mb) ? 1 : 0;
+        else k = (ma != mb) ? 2 : 0;
+        P.kind[e] = k;
+        if (k) push(P.active, &P.ctr[C_NACT0 + par], (int32_t)e);
+    }
+}
+
+__device__ __forceinline__ void mark_vertex(const BP& P, int32_t z, u64 mv, int lane) {
+    if (lane == 0) atomicMax(&P.g.mmark[z], mv);
+    Row r = get_row(P.g, z);
+    int64_t L = r.len();
+    for (int64_t j = lane; j < L; j += MF_WAVE) {
+        int32_t y = r.at(j);
+        if (y >= 0 && y != z) atomicMax(&P.g.cmark[y], mv);
+    }
+}
+
+__device__ void ph_spec(const BP& P, const GC& c, int par, uint32_t R) {
+    int64_t na = (int64_t)P.ctr[C_NACT0 + par];
+    for (int64_t i = c.wave; i < na; i += c.nwaves) {
+        int32_t e = P.active[i];
+        int32_t a = P.bu[e], bb = P.bv[e];
+        bool dummy = false;
+        u64 mv = ((u64)R << 32) | (u64)(0xFFFFFFFFu - (uint32_t)e);
+        if (c.lane == 0) atomicAdd(&P.ctr[C_SPECS], 1ull);
+        if (P.kind[e] == 1) {
+            int32_t w = a > bb ? a : bb;
+            int32_t x = eval_insertion<false>(P, w, e, R, dummy);
+            if (c.lane == 0) { P.res_w[e] = w; P.res_x[e] = x; }
+            mark_vertex(P, w, mv, c.lane);
+            if (x >= 0) mark_vertex(P, x, mv, c.lane);
+        } else {
+            bool ma = (P.g.mc[a] & MF_MISBIT) != 0;
+            int32_t tgt = ma ? bb : a, other = ma ? a : bb;
+            bool ok = eval_deletion<false>(P, tgt, other, e, R, dummy);
+            if (c.lane == 0) { P.res_w[e] = -1; P.res_x[e] = ok ? tgt : -1; }
+            if (ok) mark_vertex(P, tgt, mv, c.lane);
+        }
+    }
+}
+
+__device__ void ph_validate(const BP& P, const GC& c, int64_t start, int64_t end, int par, uint32_t R) {
+    for (int64_t e = start + c.tid; e < end; e += c.nthreads) {
+        if (P.kind[e]) continue;
+        int32_t a = P.bu[e], bb = P.bv[e];
+        if (mk_before(P.g.mmark, a, R, (int32_t)e) || mk_before(P.g.mmark, bb, R, (int32_t)e))
+            atomicMin(&P.ctr[C_CONF0 + par], (u64)e);
+    }
+    int64_t na = (int64_t)P.ctr[C_NACT0 + par];
+    for (int64_t i = c.wave; i < na; i += c.nwaves) {
+        int32_t e = P.active[i];
+        int32_t a = P.bu[e], bb = P.bv[e];
+        bool conf = mk_before(P.g.mmark, a, R, e) || mk_before(P.g.mmark, bb, R, e);
+        if (P.kind[e] == 1) {
+            int32_t w = a > bb ? a : bb;
+            (void)eval_insertion<true>(P, w, e, R, conf);
+        } else {
+            bool ma = (P.g.mc[a] & MF_MISBIT) != 0;
+            int32_t tgt = ma ? bb : a, other = ma ? a : bb;
+            (void)eval_deletion<true>(P, tgt, other, e, R, conf);
+        }
+        if (__ballot(conf) && c.lane == 0) atomicMin(&P.ctr[C_CONF0 + par], (u64)e);
+    }
+}
+
+__device__ __forceinline__ void apply_flip(const BP& P, int32_t z, bool join, int lane) {
+    if (lane == 0) {
+        if (join) atomicOr(&P.g.mc[z], MF_MISBIT);
+        else atomicAnd(&P.g.mc[z], MF_CMASK);
+    }
+    Row r = get_row(P.g, z);
+    int64_t L = r.len();
+    for (int64_t j = lane; j < L; j += MF_WAVE) {
+        int32_t y = r.at(j);
+        if (y >= 0 && y != z) {
+            if (join) atomicAdd(&P.g.mc[y], 1u);
+            else atomicSub(&P.g.mc[y], 1u);
+        }
+    }
+}
+
+__device__ void ph_commit(const BP& P, const GC& c, int par) {
+    int64_t na = (int64_t)P.ctr[C_NACT0 + par];
+    int64_t conflict = (int64_t)P.ctr[C_CONF0 + par];
+    for (int64_t i = c.wave; i < na; i += c.nwaves) {
+        int32_t e = P.active[i];
+        if (e >= conflict) continue;
+        int32_t x = P.res_x[e];
+        if (P.kind[e] == 1) {
+            int32_t w = P.res_w[e];
+            apply_flip(P, w, false, c.lane);
+            if (c.lane == 0) {
+                P.g.evb[w] = P.b;
+                atomicAdd(&P.ctr[C_ACT_INS], 1ull);
+                atomicAdd(&P.ctr[C_EVICT], 1ull);
+                if (x >= 0) atomicAdd(&P.ctr[C_ADMIT_INS], 1ull);
+            }
+        } else if (c.lane == 0) {
+            atomicAdd(&P.ctr[C_ACT_DEL], 1ull);
+            if (x >= 0) atomicAdd(&P.ctr[C_ADMIT_DEL], 1ull);
+        }
+        if (x >= 0) apply_flip(P, x, true, c.lane);
+    }
+    if (c.tid == 0) {
+        P.ctr[C_NACT0 + (par ^ 1)] = 0;
+        P.ctr[C_CONF0 + (par ^ 1)] = ~0ull;
+        P.ctr[C_ROUNDS] += 1;
+        P.ctr[C_START] = (uint64_t)(conflict);  // host reads min(conflict, end)
+    }
+}
+
+// ---------------------------------------------------------------- completion
+__device__ void ph_comp_collect(const BP& P, const GC& c) {
+    const DevGraph& g = P.g;
+    int64_t np = (int64_t)P.ctr[C_PU];
+    if (np > P.pu_cap) np = P.pu_cap;
+    for (int64_t i = c.wave; i < np; i += c.nwaves) {
+        int32_t z = P.pu[i];
+        if (g.mc[z] != 0) continue;  // member, or has an MIS neighbour
+        bool inC = g.seedb[z] == P.b || g.evb[z] == P.b;
+        if (!inC) {
+            Row r = get_row(g, z);
+            int64_t L = r.len();
+            for (int64_t j0 = 0; j0 < L && !inC; j0 += MF_WAVE) {
+                int64_t j = j0 + c.lane;
+                bool s = false;
+                if (j < L) {
+                    int32_t y = r.at(j);
+                    s = y >= 0 && g.evb[y] == P.b;
+                }
+                inC = __ballot(s) != 0;
+            }
+        }
+        if (inC && c.lane == 0) {
+            g.cstat[z] = (P.b << 2) | 1u;
+            push(P.cand, &P.ctr[C_CAND], z);
+        }
+    }
+}
+
+// A candidate joins when no open candidate neighbour has a smaller id
+// (sequential greedy in ascending id, decided in parallel rounds).
+__device__ void ph_comp_decide(const BP& P, const GC& c) {
+    const DevGraph& g = P.g;
+    int64_t nc = (int64_t)P.ctr[C_CAND];
+    const uint32_t open = (P.b << 2) | 1u;
+    for (int64_t i = c.tid; i < nc; i += c.nthreads) {
+        int32_t z = P.cand[i];
+        uint8_t d = 0;
+        if (g.cstat[z] == open) {
+            if (g.mc[z] & MF_CMASK) d = 3;
+            else {
+                d = 2;
+                Row r = get_row(g, z);
+                int64_t L = r.len();
+                for (int64_t j = 0; j < L; j++) {
+                    int32_t y = r.at(j);
+                    if (y >= 0 && y < z && g.cstat[y] == open) { d = 0; break; }
+                }
+            }
+        }
+        P.cdec[i] = d;
+    }
+}
+
+__device__ void ph_comp_apply(const BP& P, const GC& c, int par) {
+    const DevGraph& g = P.g;
+    int64_t nc = (int64_t)P.ctr[C_CAND];
+    const uint32_t open = (P.b << 2) | 1u;
+    for (int64_t i = c.wave; i < nc; i += c.nwaves) {
+        int32_t z = P.cand[i];
+        uint8_t d = P.cdec[i];
+        if (d == 2) {
+            apply_flip(P, z, true, c.lane);
+            if (c.lane == 0) {
+                g.cstat[z] = (P.b << 2) | 2u;
+                atomicAdd(&P.ctr[C_COMP_ADDS], 1ull);
+            }
+        } else if (d == 3) {
+            if (c.lane == 0) g.cstat[z] = (P.b << 2) | 3u;
+        } else if (c.lane == 0 && g.cstat[z] == open) {
+            atomicAdd(&P.ctr[C_UNDEC0 + par], 1ull);
+        }
+    }
+    if (c.tid == 0) {
+        P.ctr[C_UNDEC0 + (par ^ 1)] = 0;
+        P.ctr[C_COMP_ROUNDS] += 1;
+    }
+}
+
+// ---------------------------------------------------------------- kernels
+__device__ void ph_init(const BP& P, const GC& c) {
+    for (int64_t i = c.tid; i < C_NCOUNTERS; i += c.nthreads) P.ctr[i] = 0;
+    if (c.tid == 0) {
+        P.ctr[C_CONF0] = ~0ull;
+        P.ctr[C_CONF1] = ~0ull;
+    }
+}
+
+#define MF_KERNEL(name, body)                                                   \
+    __global__ __launch_bounds__(256) void name(BP P) {                        \
+        GC c = grid_coords();                                                   \
+        body;                                                                   \
+    }
+
+MF_KERNEL(k_init, ph_init(P, c))
+MF_KERNEL(k_claim, ph_claim(P, c))
+MF_KERNEL(k_resolve, ph_resolve(P, c))
+MF_KERNEL(k_group, ph_group(P, c))
+MF_KERNEL(k_append, ph_append(P, c))
+MF_KERNEL(k_kcount, ph_kcount(P, c))
+MF_KERNEL(k_pu_touched, ph_pu_touched(P, c))
+MF_KERNEL(k_hasrow, ph_hasrow(P, c))
+MF_KERNEL(k_fa_count, ph_fa_count(P, c))
+MF_KERNEL(k_fa_alloc, ph_fa_alloc(P, c))
+MF_KERNEL(k_fa_fill, ph_fa_fill(P, c))
+MF_KERNEL(k_comp_collect, ph_comp_collect(P, c))
+MF_KERNEL(k_comp_decide, ph_comp_decide(P, c))
+
+__global__ __launch_bounds__(256) void k_classify(BP P, int64_t start, int64_t end, int par) {
+    GC c = grid_coords();
+    ph_classify(P, c, start, end, par);
+}
+__global__ __launch_bounds__(256) void k_spec(BP P, int par, uint32_t R) {
+    GC c = grid_coords();
+    ph_spec(P, c, par, R);
+}
+__global__ __launch_bounds__(256) void k_validate(BP P, int64_t start, int64_t end, int par, uint32_t R) {
+    GC c = grid_coords();
+    ph_validate(P, c, start, end, par, R);
+}
+__global__ __launch_bounds__(256) void k_commit(BP P, int par) {
+    GC c = grid_coords();
+    ph_commit(P, c, par);
+}
+__global__ __launch_bounds__(256) void k_comp_apply(BP P, int par) {
+    GC c = grid_coords();
+    ph_comp_apply(P, c, par);
+}
+__global__ void k_set_round(BP P, uint32_t R) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *P.round = R;
+}
+
+// The whole batch in one cooperative launch: every phase boundary is a grid
+// barrier, so the rounds and completion iterations need no host round trip.
+__global__ __launch_bounds__(256) void k_persistent(BP P) {
+    cg::grid_group grid = cg::this_grid();
+    GC c = grid_coords();
+    ph_init(P, c);
+    grid.sync();
+    ph_claim(P, c);
+    grid.sync();
+    ph_resolve(P, c);
+    grid.sync();
+    ph_group(P, c);
+    grid.sync();
+    ph_append(P, c);
+    grid.sync();
+    ph_kcount(P, c);
+    grid.sync();
+    ph_pu_touched(P, c);
+    grid.sync();
+    ph_hasrow(P, c);
+    grid.sync();
+    ph_fa_count(P, c);
+    grid.sync();
+    ph_fa_alloc(P, c);
+    grid.sync();
+    ph_fa_fill(P, c);
+    grid.sync();
+    const uint32_t R0 = *P.round;
+    uint32_t r = 0;
+    int64_t start = 0;
+    while (start < P.U) {
+        // every round commits at least its first edge, so U rounds suffice
+        if ((int64_t)r > P.U + 1) {
+            if (c.tid == 0) P.ctr[C_GUARD] = 1;
+            break;
+        }
+        int par = (int)(r & 1u);
+        uint32_t R = R0 + r + 1;
+        int64_t end = start + P.window;
+        if (end > P.U) end = P.U;
+        ph_classify(P, c, start, end, par);
+        grid.sync();
+        ph_spec(P, c, par, R);
+        grid.sync();
+        ph_validate(P, c, start, end, par, R);
+        grid.sync();
+        int64_t conflict = (int64_t)P.ctr[C_CONF0 + par];
+        ph_commit(P, c, par);
+        grid.sync();
+        start = conflict < end ? conflict : end;
+        r++;
+    }
+    if (P.completion) {
+        ph_comp_collect(P, c);
+        grid.sync();
+        uint32_t k = 0;
+        while (true) {
+            int par = (int)(k & 1u);
+            ph_comp_decide(P, c);
+            grid.sync();
+            ph_comp_apply(P, c, par);
+            grid.sync();
+            k++;
+            if (P.ctr[C_UNDEC0 + par] == 0) break;
+            // the smallest open candidate decides every iteration
+            if ((u64)k > P.ctr[C_CAND] + 1) {
+                if (c.tid == 0) P.ctr[C_GUARD] = 2;
+                break;
+            }
+        }
+    }
+    if (c.tid == 0) *P.round = R0 + r;
+}
+
+// ---------------------------------------------------------------- launchers
+#define LAUNCH(k, ...) hipLaunchKernelGGL(k, dim3(L.blocks), dim3(256), 0, L.stream, __VA_ARGS__)
+
+void mf_launch_batch_prologue(const BP& P, const LaunchCfg& L) {
+    LAUNCH(k_init, P);
+    LAUNCH(k_claim, P);
+    LAUNCH(k_resolve, P);
+    LAUNCH(k_group, P);
+    LAUNCH(k_append, P);
+    LAUNCH(k_kcount, P);
+    LAUNCH(k_pu_touched, P);
+    LAUNCH(k_hasrow, P);
+    LAUNCH(k_fa_count, P);
+    LAUNCH(k_fa_alloc, P);
+    LAUNCH(k_fa_fill, P);
+}
+
+void mf_launch_round(const BP& P, const LaunchCfg& L, int64_t start, int64_t end, int par, uint32_t R) {
+    LAUNCH(k_classify, P, start, end, par);
+    LAUNCH(k_spec, P, par, R);
+    LAUNCH(k_validate, P, start, end, par, R);
+    LAUNCH(k_commit, P, par);
+}
+
+void mf_launch_completion_collect(const BP& P, const LaunchCfg& L) { LAUNCH(k_comp_collect, P); }
+
+void mf_launch_completion_round(const BP& P, const LaunchCfg& L, int par) {
+    LAUNCH(k_comp_decide, P);
+    LAUNCH(k_comp_apply, P, par);
+}
+
+void mf_launch_set_round(const BP& P, const LaunchCfg& L, uint32_t R) {
+    hipLaunchKernelGGL(k_set_round, dim3(1), dim3(1), 0, L.stream, P, R);
+}
+
+int mf_persistent_grid(int device) {
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_persistent, 256, 0) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
+    if (per_cu > 2) per_cu = 2;
+    return per_cu * cus;
+}
+
+hipError_t mf_launch_persistent(const BP& P, int grid, hipStream_t stream) {
+    void* args[] = {(void*)&P};
+    return hipLaunchCooperativeKernel((const void*)k_persistent, dim3(grid), dim3(256), args, 0, stream);
+}

@@ -1,0 +1,472 @@
+// k_graph.hip -- whole-graph kernels: load, MIS upload with the
+// MIS-neighbour counts, adjacency flags, the full independence/maximality
+// check, compaction of the overlay into a new base, and the per-batch
+// algorithmic-byte measurement.
+//
+// Whole-graph passes walk the base CSR edge-parallel: a wave owns 64
+// consecutive vertices, whose rows are contiguous in nlist, and its lanes read
+// that range 64 entries at a time (coalesced); each lane finds the row of its
+// entry by a 6-step search over the 64 row offsets held in registers.  Ranges
+// longer than MF_SPLIT are cut into tasks that other waves pick up, so hub
+// rows do not serialise a pass.  Overlay runs are walked per vertex.
+#include "graph.h"
+
+#define MF_SPLIT 4096
+
+// ---------------------------------------------------------------- basics
+__global__ void k_widen32(const int32_t* in, int64_t* out, int64_t count) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = in[i];
+}
+
+template <typename T>
+__global__ void k_fill(T* p, T v, int64_t count) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// flags[0] |= 1: some row not strictly ascending; |= 2: self-loop;
+// |= 4: entry out of range; |= 8: nindex not monotone
+__global__ void k_validate_csr(const int64_t* nindex, const int32_t* nlist, int32_t n, int64_t m, int* flags) {
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
+        int64_t b = nindex[x], e = nindex[x + 1];
+        if (b > e || b < 0 || e > m) { atomicOr(flags, 8); continue; }
+        int32_t prev = -1;
+        int f = 0;
+        for (int64_t p = b; p < e; p++) {
+            int32_t y = nlist[p];
+            if (y < 0 || y >= n) f |= 4;
+            if (y == (int32_t)x) f |= 2;
+            if (y <= prev) f |= 1;
+            prev = y;
+        }
+        if (f) atomicOr(flags, f);
+    }
+}
+
+__global__ void k_set_mis(uint32_t* mc, const uint8_t* mis, int32_t n) {
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x)
+        mc[x] = mis[x] ? MF_MISBIT : 0u;
+}
+
+__global__ void k_get_mis(const uint32_t* mc, uint8_t* mis, int32_t n) {
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x)
+        mis[x] = (mc[x] & MF_MISBIT) ? 1 : 0;
+}
+
+__global__ void k_flags(DevGraph g, const int32_t* u, const int32_t* v, int64_t count, uint8_t* ins) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        ins[i] = adjacent(g, u[i], v[i]) ? 0 : 1;
+}
+
+// ---------------------------------------------------------------- edge walk
+// Visitor interface: visit(x, y) for every live entry y of row(x), y != x.
+struct PassCountMis {      // cnt[x] += [y in MIS]
+    const uint32_t* mc;
+    uint32_t* cnt;
+    u64* acc;              // acc[0] += conflicting entries (x<y, both members)
+    __device__ void visit(int32_t x, int32_t y) const {
+        uint32_t my = mc[y];
+        if (my & MF_MISBIT) {
+            atomicAdd(&cnt[x], 1u);
+            if (acc && x < y && (mc[x] & MF_MISBIT)) atomicAdd(&acc[0], 1ull);
+        }
+    }
+};
+
+struct PassMarkV2 {        // mark[y] = t for y adjacent to a V1 vertex
+    uint32_t* mark;
+    uint32_t v1tag, v2tag;
+    __device__ void visit(int32_t x, int32_t y) const {
+        if (mark[x] != v1tag) return;
+        if (mark[y] != v1tag) atomicMax(&mark[y], v2tag);
+    }
+};
+
+struct EdgeTask {
+    int32_t v0;
+    int64_t p0;
+};
+
+template <class Vis>
+__device__ void walk_range(const DevGraph& g, int32_t v0, int32_t v1, int64_t p_begin, int64_t p_end,
+                           const Vis& vis) {
+    // offsets of the (up to) 64 rows [v0, v1) in registers, one per lane
+    int lane = lane_id();
+    int32_t myv = v0 + lane;
+    int64_t off = (myv < v1) ? g.nindex[myv] : INT64_MAX;
+    for (int64_t p0 = p_begin; p0 < p_end; p0 += MF_WAVE) {
+        int64_t p = p0 + lane;
+        int64_t pp = p < p_end ? p : p_end - 1;
+        int lo = 0, hi = MF_WAVE - 1;
+#pragma unroll
+        for (int s = 0; s < 6; s++) {
+            int mid = (lo + hi + 1) >> 1;
+            int64_t om = __shfl(off, mid, MF_WAVE);
+            if (om <= pp) lo = mid; else hi = mid - 1;
+        }
+        if (p < p_end) {
+            int32_t x = v0 + lo;
+            int32_t y = g.nlist[p];
+            if (y >= 0 && y != x) vis.visit(x, y);
+        }
+    }
+}
+
+template <class Vis>
+__global__ __launch_bounds__(256) void k_walk_base(DevGraph g, Vis vis, EdgeTask* tasks, u64* ntasks, int64_t task_cap) {
+    GC c = grid_coords();
+    int64_t groups = ((int64_t)g.n + MF_WAVE - 1) / MF_WAVE;
+    for (int64_t gi = c.wave; gi < groups; gi += c.nwaves) {
+        int32_t v0 = (int32_t)(gi * MF_WAVE);
+        int32_t v1 = v0 + MF_WAVE < g.n ? v0 + MF_WAVE : g.n;
+        int64_t pb = g.nindex[v0], pe = g.nindex[v1];
+        int64_t first_end = pe;
+        if (pe - pb > MF_SPLIT) {
+            first_end = pb + MF_SPLIT;
+            int64_t extra = (pe - first_end + MF_SPLIT - 1) / MF_SPLIT;
+            u64 t0 = 0;
+            if (c.lane == 0) t0 = atomicAdd(ntasks, (u64)extra);
+            t0 = __shfl((long long)t0, 0, MF_WAVE);
+            for (int64_t k = c.lane; k < extra; k += MF_WAVE) {
+                if ((int64_t)(t0 + k) < task_cap) {
+                    tasks[t0 + k].v0 = v0;
+                    tasks[t0 + k].p0 = first_end + k * MF_SPLIT;
+                }
+            }
+        }
+        walk_range(g, v0, v1, pb, first_end, vis);
+    }
+}
+
+template <class Vis>
+__global__ __launch_bounds__(256) void k_walk_tasks(DevGraph g, Vis vis, const EdgeTask* tasks, const u64* ntasks,
+                                                    int64_t task_cap) {
+    GC c = grid_coords();
+    int64_t nt = (int64_t)*ntasks;
+    if (nt > task_cap) nt = task_cap;
+    for (int64_t t = c.wave; t < nt; t += c.nwaves) {
+        int32_t v0 = tasks[t].v0;
+        int32_t v1 = v0 + MF_WAVE < g.n ? v0 + MF_WAVE : g.n;
+        int64_t pe = g.nindex[v1];
+        int64_t p0 = tasks[t].p0;
+        int64_t p1 = p0 + MF_SPLIT < pe ? p0 + MF_SPLIT : pe;
+        walk_range(g, v0, v1, p0, p1, vis);
+    }
+}
+
+template <class Vis>
+__global__ __launch_bounds__(256) void k_walk_runs(DevGraph g, Vis vis) {
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < g.n; x += (int64_t)gridDim.x * blockDim.x) {
+        int32_t len = g.ov_len[x];
+        if (!len) continue;
+        const int32_t* r = g.pool + g.ov_start[x];
+        for (int32_t i = 0; i < len; i++) {
+            int32_t y = r[i];
+            if (y >= 0 && y != (int32_t)x) vis.visit((int32_t)x, y);
+        }
+    }
+}
+
+template <class Vis>
+static void walk_all(GraphState& S, const Vis& vis) {
+    const int B = S.blocks;
+    k_fill<u64><<<1, 64, 0, S.stream>>>(S.d_walk_ntasks, 0ull, 1);
+    k_walk_base<Vis><<<B, 256, 0, S.stream>>>(S.g, vis, (EdgeTask*)S.d_walk_tasks, S.d_walk_ntasks, S.walk_task_cap);
+    k_walk_tasks<Vis><<<B, 256, 0, S.stream>>>(S.g, vis, (const EdgeTask*)S.d_walk_tasks, S.d_walk_ntasks, S.walk_task_cap);
+    k_walk_runs<Vis><<<B, 256, 0, S.stream>>>(S.g, vis);
+}
+
+// ---------------------------------------------------------------- MIS
+__global__ void k_finish_counts(uint32_t* mc, const uint32_t* cnt, int32_t n) {
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x)
+        mc[x] = (mc[x] & MF_MISBIT) | cnt[x];
+}
+
+hipError_t mf_set_mis(GraphState& S, const uint8_t* d_mis) {
+    const int B = S.blocks;
+    k_set_mis<<<B, 256, 0, S.stream>>>(S.g.mc, d_mis, S.g.n);
+    k_fill<uint32_t><<<B, 256, 0, S.stream>>>(S.d_cnt, 0u, S.g.n);
+    PassCountMis vis{S.g.mc, S.d_cnt, nullptr};
+    walk_all(S, vis);
+    k_finish_counts<<<B, 256, 0, S.stream>>>(S.g.mc, S.d_cnt, S.g.n);
+    return hipGetLastError();
+}
+
+hipError_t mf_get_mis(GraphState& S, uint8_t* d_mis) {
+    k_get_mis<<<S.blocks, 256, 0, S.stream>>>(S.g.mc, d_mis, S.g.n);
+    return hipGetLastError();
+}
+
+hipError_t mf_flags(GraphState& S, const int32_t* d_u, const int32_t* d_v, int64_t count, uint8_t* d_ins) {
+    k_flags<<<S.blocks, 256, 0, S.stream>>>(S.g, d_u, d_v, count, d_ins);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- check
+__global__ void k_check_finish(const uint32_t* mc, const uint32_t* cnt, int32_t n, u64* acc) {
+    u64 unc = 0, bad = 0;
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t m = mc[x];
+        uint32_t k = cnt[x];
+        if (!(m & MF_MISBIT) && k == 0) unc++;
+        if ((m & MF_CMASK) != k) bad++;
+    }
+    if (unc) atomicAdd(&acc[1], unc);
+    if (bad) atomicAdd(&acc[2], bad);
+}
+
+hipError_t mf_check(GraphState& S, u64* d_acc) {
+    const int B = S.blocks;
+    k_fill<uint32_t><<<B, 256, 0, S.stream>>>(S.d_cnt, 0u, S.g.n);
+    k_fill<u64><<<1, 64, 0, S.stream>>>(d_acc, 0ull, 4);
+    PassCountMis vis{S.g.mc, S.d_cnt, d_acc};
+    walk_all(S, vis);
+    k_check_finish<<<B, 256, 0, S.stream>>>(S.g.mc, S.d_cnt, S.g.n, d_acc);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- scan
+// Exclusive prefix sum of int64 values, three passes (block sums, scan of
+// the block sums in one block, local rescan plus offset).
+#define SCAN_T 256
+#define SCAN_PER 8
+#define SCAN_TILE (SCAN_T * SCAN_PER)
+
+__device__ int64_t block_excl_scan(int64_t v, int64_t* sh, int64_t& total) {
+    int t = threadIdx.x;
+    int lane = t & 63, w = t >> 6;
+    int64_t incl = wave_incl_scan(v);
+    if (lane == 63) sh[w] = incl;
+    __syncthreads();
+    if (t == 0) {
+        int64_t run = 0;
+        for (int i = 0; i < SCAN_T / 64; i++) {
+            int64_t s = sh[i];
+            sh[i] = run;
+            run += s;
+        }
+        sh[SCAN_T / 64] = run;
+    }
+    __syncthreads();
+    int64_t r = sh[w] + incl - v;
+    total = sh[SCAN_T / 64];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_reduce(const int64_t* in, int64_t count, int64_t* part) {
+    __shared__ int64_t sh[SCAN_T / 64 + 1];
+    int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_PER;
+    int64_t s = 0;
+    for (int i = 0; i < SCAN_PER; i++)
+        if (base + i < count) s += in[base + i];
+    int64_t tot;
+    block_excl_scan(s, sh, tot);
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_parts(int64_t* part, int64_t nparts) {
+    __shared__ int64_t sh[SCAN_T / 64 + 1];
+    int64_t carry = 0;
+    for (int64_t b0 = 0; b0 < nparts; b0 += SCAN_T) {
+        int64_t i = b0 + threadIdx.x;
+        int64_t v = i < nparts ? part[i] : 0;
+        int64_t tot;
+        int64_t ex = block_excl_scan(v, sh, tot);
+        if (i < nparts) part[i] = carry + ex;
+        carry += tot;
+    }
+}
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_final(const int64_t* in, int64_t count, const int64_t* part,
+                                                       int64_t* out, int64_t* total_out) {
+    __shared__ int64_t sh[SCAN_T / 64 + 1];
+    int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_PER;
+    int64_t vals[SCAN_PER];
+    int64_t s = 0;
+    for (int i = 0; i < SCAN_PER; i++) {
+        vals[i] = base + i < count ? in[base + i] : 0;
+        s += vals[i];
+    }
+    int64_t tot;
+    int64_t ex = block_excl_scan(s, sh, tot) + part[blockIdx.x];
+    for (int i = 0; i < SCAN_PER; i++) {
+        if (base + i < count) out[base + i] = ex;
+        ex += vals[i];
+    }
+    if (base < count && base + SCAN_PER >= count && total_out) *total_out = ex;
+}
+
+// out[i] = sum in[0..i); out[count] = total.  part needs ceil(count/2048).
+void mf_scan_excl(const int64_t* in, int64_t count, int64_t* out, int64_t* part, hipStream_t st) {
+    int64_t nb = (count + SCAN_TILE - 1) / SCAN_TILE;
+    if (nb == 0) {
+        k_fill<int64_t><<<1, 64, 0, st>>>(out, 0, 1);
+        return;
+    }
+    k_scan_reduce<<<(unsigned)nb, SCAN_T, 0, st>>>(in, count, part);
+    k_scan_parts<<<1, SCAN_T, 0, st>>>(part, nb);
+    k_scan_final<<<(unsigned)nb, SCAN_T, 0, st>>>(in, count, part, out, out + count);
+}
+
+// ---------------------------------------------------------------- compaction
+__global__ void k_newdeg(DevGraph g, int64_t* deg, int32_t* tomb_list, u64* ntomb) {
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < g.n; x += (int64_t)gridDim.x * blockDim.x) {
+        int64_t blen = g.nindex[x + 1] - g.nindex[x];
+        int32_t t = g.ov_tomb[x];
+        deg[x] = blen - t + g.ov_alive[x];
+        if (t) {
+            u64 i = atomicAdd(ntomb, 1ull);
+            tomb_list[i] = (int32_t)x;
+        }
+    }
+}
+
+// rows without deletions: straight copy, edge-parallel over 64-vertex groups
+__global__ __launch_bounds__(256) void k_copy_clean(DevGraph g, const int64_t* nidx, int32_t* out) {
+    GC c = grid_coords();
+    int64_t groups = ((int64_t)g.n + MF_WAVE - 1) / MF_WAVE;
+    for (int64_t gi = c.wave; gi < groups; gi += c.nwaves) {
+        int32_t v0 = (int32_t)(gi * MF_WAVE);
+        int32_t v1 = v0 + MF_WAVE < g.n ? v0 + MF_WAVE : g.n;
+        int32_t myv = v0 + c.lane;
+        int64_t off = myv < v1 ? g.nindex[myv] : INT64_MAX;
+        int64_t dst = myv < v1 ? nidx[myv] : 0;
+        bool clean = myv < v1 ? g.ov_tomb[myv] == 0 : true;
+        int64_t pb = g.nindex[v0], pe = g.nindex[v1];
+        for (int64_t p0 = pb; p0 < pe; p0 += MF_WAVE) {
+            int64_t p = p0 + c.lane;
+            int64_t pp = p < pe ? p : pe - 1;
+            int lo = 0, hi = MF_WAVE - 1;
+#pragma unroll
+            for (int s = 0; s < 6; s++) {
+                int mid = (lo + hi + 1) >> 1;
+                int64_t om = __shfl(off, mid, MF_WAVE);
+                if (om <= pp) lo = mid; else hi = mid - 1;
+            }
+            int64_t ooff = __shfl(off, lo, MF_WAVE);
+            int64_t odst = __shfl(dst, lo, MF_WAVE);
+            bool oclean = __shfl((int)clean, lo, MF_WAVE) != 0;
+            if (p < pe && oclean) out[odst + (p - ooff)] = g.nlist[p];
+        }
+    }
+}
+
+// rows with deletions: order-preserving filter, one wave per row
+__global__ __launch_bounds__(256) void k_copy_tomb(DevGraph g, const int64_t* nidx, int32_t* out,
+                                                   const int32_t* tomb_list, const u64* ntomb) {
+    GC c = grid_coords();
+    int64_t nt = (int64_t)*ntomb;
+    for (int64_t i = c.wave; i < nt; i += c.nwaves) {
+        int32_t x = tomb_list[i];
+        int64_t pb = g.nindex[x], pe = g.nindex[x + 1];
+        int64_t dst = nidx[x];
+        for (int64_t p0 = pb; p0 < pe; p0 += MF_WAVE) {
+            int64_t p = p0 + c.lane;
+            int32_t y = p < pe ? g.nlist[p] : -1;
+            bool keep = y >= 0;
+            u64 bal = __ballot(keep);
+            int before = __popcll(bal & ((1ull << c.lane) - 1ull));
+            if (keep) out[dst + before] = y;
+            dst += __popcll(bal);
+        }
+    }
+}
+
+// live overlay entries follow the base part, in insertion order
+__global__ void k_copy_runs(DevGraph g, const int64_t* nidx, int32_t* out) {
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < g.n; x += (int64_t)gridDim.x * blockDim.x) {
+        int32_t len = g.ov_len[x];
+        if (!len) continue;
+        int64_t dst = nidx[x] + (g.nindex[x + 1] - g.nindex[x]) - g.ov_tomb[x];
+        const int32_t* r = g.pool + g.ov_start[x];
+        for (int32_t i = 0; i < len; i++)
+            if (r[i] >= 0) out[dst++] = r[i];
+    }
+}
+
+hipError_t mf_compact_plan(GraphState& S, int64_t* d_newidx) {
+    const int B = S.blocks;
+    k_fill<u64><<<1, 64, 0, S.stream>>>(S.d_walk_ntasks, 0ull, 1);
+    k_newdeg<<<B, 256, 0, S.stream>>>(S.g, S.d_deg, S.d_tomb_list, S.d_walk_ntasks);
+    mf_scan_excl(S.d_deg, S.g.n, d_newidx, S.d_scan_part, S.stream);
+    return hipGetLastError();
+}
+
+hipError_t mf_compact_copy(GraphState& S, const int64_t* d_newidx, int32_t* d_out) {
+    const int B = S.blocks;
+    k_copy_clean<<<B, 256, 0, S.stream>>>(S.g, d_newidx, d_out);
+    k_copy_tomb<<<B, 256, 0, S.stream>>>(S.g, d_newidx, d_out, S.d_tomb_list, S.d_walk_ntasks);
+    k_copy_runs<<<B, 256, 0, S.stream>>>(S.g, d_newidx, d_out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- bytes
+// V1 = S ∪ N(S) (S = endpoints of batch b), V2 = V1 ∪ N(V1).
+__global__ __launch_bounds__(256) void k_mark_v1(DevGraph g, const int32_t* seeds, const u64* nseeds, uint32_t* mark,
+                                                 uint32_t v1tag) {
+    GC c = grid_coords();
+    int64_t ns = (int64_t)*nseeds;
+    for (int64_t i = c.wave; i < ns; i += c.nwaves) {
+        int32_t s = seeds[i];
+        if (c.lane == 0) mark[s] = v1tag;
+        Row r = get_row(g, s);
+        int64_t L = r.len();
+        for (int64_t j = c.lane; j < L; j += MF_WAVE) {
+            int32_t y = r.at(j);
+            if (y >= 0) mark[y] = v1tag;
+        }
+    }
+}
+
+__global__ void k_count_v(DevGraph g, const uint32_t* mark, uint32_t v1tag, uint32_t v2tag, u64* acc) {
+    u64 n1 = 0, n2 = 0, deg = 0;
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < g.n; x += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t m = mark[x];
+        if (m == v1tag) {
+            n1++;
+            n2++;
+            deg += (u64)((g.nindex[x + 1] - g.nindex[x]) - g.ov_tomb[x] + g.ov_alive[x]);
+        } else if (m == v2tag) {
+            n2++;
+        }
+    }
+    if (n1) atomicAdd(&acc[0], n1);
+    if (n2) atomicAdd(&acc[1], n2);
+    if (deg) atomicAdd(&acc[2], deg);
+}
+
+hipError_t mf_batch_bytes(GraphState& S, const int32_t* d_seeds, const u64* d_nseeds, uint32_t t, u64* d_acc) {
+    const int B = S.blocks;
+    uint32_t v1tag = 2 * t + 1, v2tag = 2 * t;
+    k_fill<u64><<<1, 64, 0, S.stream>>>(d_acc, 0ull, 4);
+    k_mark_v1<<<B, 256, 0, S.stream>>>(S.g, d_seeds, d_nseeds, S.d_bmark, v1tag);
+    PassMarkV2 vis{S.d_bmark, v1tag, v2tag};
+    walk_all(S, vis);
+    k_count_v<<<B, 256, 0, S.stream>>>(S.g, S.d_bmark, v1tag, v2tag, d_acc);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- misc
+hipError_t mf_widen(const int32_t* d_in, int64_t* d_out, int64_t count, hipStream_t st) {
+    k_widen32<<<1024, 256, 0, st>>>(d_in, d_out, count);
+    return hipGetLastError();
+}
+
+hipError_t mf_validate_csr(GraphState& S, int* d_flags) {
+    k_fill<int><<<1, 64, 0, S.stream>>>(d_flags, 0, 1);
+    k_validate_csr<<<S.blocks, 256, 0, S.stream>>>(S.g.nindex, S.g.nlist, S.g.n, S.g.m, d_flags);
+    return hipGetLastError();
+}
+
+void mf_fill_u32(uint32_t* p, uint32_t v, int64_t count, hipStream_t st) {
+    k_fill<uint32_t><<<1024, 256, 0, st>>>(p, v, count);
+}
+void mf_fill_i32(int32_t* p, int32_t v, int64_t count, hipStream_t st) {
+    k_fill<int32_t><<<1024, 256, 0, st>>>(p, v, count);
+}
+void mf_fill_u64(u64* p, u64 v, int64_t count, hipStream_t st) { k_fill<u64><<<1024, 256, 0, st>>>(p, v, count); }
+void mf_fill_i64(int64_t* p, int64_t v, int64_t count, hipStream_t st) {
+    k_fill<int64_t><<<1024, 256, 0, st>>>(p, v, count);
+}

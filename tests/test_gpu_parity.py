@@ -1,0 +1,150 @@
+"""GPU engine vs the CPU oracle, bit-exact, through the C ABI.
+
+Every test replays the same batches through oracle/ (CPU restatement of the
+reference rule) and through libmisfit_amd.so on the GPU, and compares the MIS
+after every batch, the overlay's adjacency order and the compacted CSR.
+"""
+import numpy as np
+import pytest
+
+from tests import helpers as H
+from tests.conftest import SAMPLE
+
+pytestmark = pytest.mark.gpu
+
+MODES = [0, 1]  # plain, persistent
+
+
+@pytest.fixture(scope="module")
+def eng(engine_mod):
+    e = engine_mod.Engine(0)
+    yield e
+    e.close()
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("completion", [False, True])
+def test_sample_parity(oracle, engine_mod, eng, mode, completion):
+    g, mis0, batches = H.sample_inputs(oracle, SAMPLE)
+    ni, nl = g.base_csr()
+    want, wst, _ = H.oracle_replay(oracle, ni, nl, mis0, batches, completion=completion, literal=True)
+    got, gst, checks = H.engine_replay(engine_mod, eng, ni, nl, mis0, batches, completion=completion, mode=mode)
+    for b in range(len(batches)):
+        assert np.array_equal(want[b], got[b]), "batch %d differs" % b
+        assert checks[b][0] == 0 and checks[b][2] == 0
+    if completion:
+        assert all(c[1] == 0 for c in checks)
+    else:
+        # literal rule on the sample: no admissions, uncovered 4, 9, 12, 15, 19
+        assert [c[1] for c in checks] == [4, 9, 12, 15, 19]
+        assert int(got[-1].sum()) == 533
+        assert all(s["admit_del"] == 0 and s["admit_ins"] == 0 for s in gst)
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("seed", [1, 2, 3, 4])
+@pytest.mark.parametrize("general", [False, True])
+def test_random_parity(oracle, engine_mod, eng, mode, seed, general):
+    rng = np.random.default_rng(seed)
+    n = 300
+    ni, nl = H.random_graph(rng, n, 0.03)
+    mis0 = oracle.Graph.from_csr(ni, nl).greedy_mis()
+    batches = H.random_batches(rng, oracle, ni, nl, 6, 80, general=general)
+    for completion in (False, True):
+        want, _, og = H.oracle_replay(oracle, ni, nl, mis0, batches, completion=completion, literal=True)
+        got, _, checks = H.engine_replay(engine_mod, eng, ni, nl, mis0, batches, completion=completion, mode=mode)
+        for b in range(len(batches)):
+            assert np.array_equal(want[b], got[b]), "seed %d batch %d completion %s" % (seed, b, completion)
+            assert checks[b][0] == 0 and checks[b][2] == 0
+        for v in range(0, n, 7):
+            assert np.array_equal(eng.neighbors(v), og.neighbors(v)), "row %d order" % v
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("seed", [5, 6])
+def test_deletion_heavy_parity(oracle, engine_mod, eng, mode, seed):
+    rng = np.random.default_rng(seed)
+    ni, nl = H.random_graph(rng, 400, 0.05)
+    g = oracle.Graph.from_csr(ni, nl)
+    mis0 = g.greedy_mis()
+    raw = H.existing_edge_batches(rng, ni, nl, 6, 120)
+    batches = [(u, v, g.flags(u, v)) for (u, v) in raw]
+    want, _, _ = H.oracle_replay(oracle, ni, nl, mis0, batches, completion=True, literal=True)
+    got, _, checks = H.engine_replay(engine_mod, eng, ni, nl, mis0, batches, completion=True, mode=mode)
+    for b in range(len(batches)):
+        assert np.array_equal(want[b], got[b]), "batch %d" % b
+        assert checks[b] == (0, 0, 0)
+
+
+def test_flags_parity(oracle, engine_mod, eng):
+    rng = np.random.default_rng(9)
+    ni, nl = H.random_graph(rng, 500, 0.02)
+    eng.load_csr(ni, nl)
+    g = oracle.Graph.from_csr(ni, nl)
+    u = rng.integers(0, 500, 3000).astype(np.int32)
+    v = rng.integers(0, 500, 3000).astype(np.int32)
+    keep = u != v
+    u, v = u[keep], v[keep]
+    assert np.array_equal(eng.flags(u, v), g.flags(u, v))
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_compaction_parity(oracle, engine_mod, eng, mode):
+    rng = np.random.default_rng(11)
+    ni, nl = H.random_graph(rng, 350, 0.04)
+    g = oracle.Graph.from_csr(ni, nl)
+    mis0 = g.greedy_mis()
+    batches = H.random_batches(rng, oracle, ni, nl, 4, 100, general=True)
+    want, _, og = H.oracle_replay(oracle, ni, nl, mis0, batches)
+    got, _, _ = H.engine_replay(engine_mod, eng, ni, nl, mis0, batches, mode=mode)
+    assert np.array_equal(want[-1], got[-1])
+    oni, onl = og.compact()
+    m = eng.compact()
+    eni, enl = eng.export_csr()
+    assert m == len(onl)
+    assert np.array_equal(eni, oni) and np.array_equal(enl, onl)
+    # the engine keeps working on the compacted base
+    more = H.random_batches(rng, oracle, oni, onl, 2, 60, general=True)
+    mis = want[-1]
+    w2, _, _ = H.oracle_replay(oracle, oni, onl, mis, more)
+    for (u, v, ins), w in zip(more, w2):
+        eng.process_batch(engine_mod.make_updates(u, v, ins))
+        assert np.array_equal(eng.get_mis(), w)
+    assert eng.check() == (0, 0, 0)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_er_config_parity(oracle, engine_mod, eng, mode):
+    """Erdos-Renyi n=100,000, 800,000 pairs, seed 42; greedy MIS; 10 batches
+    of 2,000 uniform pairs flagged against the loaded graph."""
+    ni, nl = engine_mod.gen_er(100000, 800000, 42)
+    mis0 = engine_mod.greedy_mis(ni, nl)
+    u, v = engine_mod.gen_updates(engine_mod.UNIFORM_PAIRS, ni, nl, 20000, 42)
+    g = oracle.Graph.from_csr(ni, nl)
+    ins = g.flags(u, v)
+    batches = [(u[i:i + 2000], v[i:i + 2000], ins[i:i + 2000]) for i in range(0, 20000, 2000)]
+    want, _, _ = H.oracle_replay(oracle, ni, nl, mis0, batches, completion=True, literal=False)
+    got, gst, checks = H.engine_replay(engine_mod, eng, ni, nl, mis0, batches, completion=True, mode=mode)
+    for b in range(len(batches)):
+        assert np.array_equal(want[b], got[b]), "batch %d" % b
+        assert checks[b] == (0, 0, 0)
+
+
+def test_determinism(oracle, engine_mod, eng):
+    ni, nl = engine_mod.gen_rmat(14, 16, 7)
+    mis0 = engine_mod.greedy_mis(ni, nl)
+    u, v = engine_mod.gen_updates(engine_mod.MIXED_EXISTING, ni, nl, 8000, 7)
+    g = oracle.Graph.from_csr(ni, nl)
+    ins = g.flags(u, v)
+    batches = [(u[i:i + 2000], v[i:i + 2000], ins[i:i + 2000]) for i in range(0, 8000, 2000)]
+    runs = []
+    for mode in (1, 0, 1):
+        got, _, checks = H.engine_replay(engine_mod, eng, ni, nl, mis0, batches, mode=mode)
+        runs.append(got)
+        assert all(c == (0, 0, 0) for c in checks)
+    for r in runs[1:]:
+        for a, b in zip(runs[0], r):
+            assert np.array_equal(a, b)
+    want, _, _ = H.oracle_replay(oracle, ni, nl, mis0, batches, completion=True, literal=False)
+    for a, b in zip(want, runs[0]):
+        assert np.array_equal(a, b)
