@@ -157,6 +157,11 @@ int misfit_neighbors(misfit_ctx* ctx, int32_t v, int32_t* out, int64_t cap, int6
  * I = index_bytes (8 for an int32 .egr, 16 for int64).  Measurement aid. */
 int misfit_batch_bytes(misfit_ctx* ctx, int index_bytes, int64_t* bytes, int64_t* v1, int64_t* v2);
 
+/* Device name and memory (bytes).  Replaces the cudaGetDeviceProperties /
+ * cudaMemGetInfo calls of the reference's setup and peak-memory report
+ * (src/misfit/misfit.cu:46-67,758-761). */
+int misfit_device_info(misfit_ctx* ctx, char* name, int namelen, int64_t* total_bytes, int64_t* free_bytes);
+
 /* ---- host I/O (same library, no device needed) ---- */
 
 /* .egr reader (core/FlashGraph.h:108-169 format; both nindex widths).  The

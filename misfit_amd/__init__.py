@@ -66,7 +66,7 @@ EXPORTS = (
     "misfit_set_mis", "misfit_get_mis", "misfit_set_rule", "misfit_set_mode", "misfit_flags",
     "misfit_submit_batch", "misfit_wait", "misfit_process_batch", "misfit_sync", "misfit_check",
     "misfit_compact", "misfit_graph_size", "misfit_export_csr", "misfit_neighbors",
-    "misfit_batch_bytes", "misfit_egr_open", "misfit_egr_close", "misfit_write_egr",
+    "misfit_batch_bytes", "misfit_device_info", "misfit_egr_open", "misfit_egr_close", "misfit_write_egr",
     "misfit_gen_er", "misfit_gen_rmat", "misfit_gen_rgg", "misfit_greedy_mis", "misfit_gen_updates",
 )
 
@@ -102,6 +102,7 @@ def lib():
         L.misfit_export_csr.argtypes = [vp, _i64p, _i32p]
         L.misfit_neighbors.argtypes = [vp, ctypes.c_int32, _i32p, ctypes.c_int64, _i64p]
         L.misfit_batch_bytes.argtypes = [vp, ctypes.c_int, _i64p, _i64p, _i64p]
+        L.misfit_device_info.argtypes = [vp, ctypes.c_char_p, ctypes.c_int, _i64p, _i64p]
         L.misfit_egr_open.argtypes = [ctypes.c_char_p, ctypes.POINTER(EgrView), ctypes.c_char_p, ctypes.c_int]
         L.misfit_egr_close.argtypes = [ctypes.POINTER(EgrView)]
         L.misfit_write_egr.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int64, _i64p, _i32p,
@@ -367,6 +368,12 @@ class Engine:
         out = np.zeros(max(k.value, 1), np.int32)
         self._ck(lib().misfit_neighbors(self.ctx, v, _p(out, _i32p), k.value, ctypes.byref(k)))
         return out[: k.value]
+
+    def device_info(self):
+        name = ctypes.create_string_buffer(256)
+        tot, fr = ctypes.c_int64(), ctypes.c_int64()
+        self._ck(lib().misfit_device_info(self.ctx, name, 256, ctypes.byref(tot), ctypes.byref(fr)))
+        return name.value.decode(), tot.value, fr.value
 
     def batch_bytes(self, index_bytes=8):
         b, v1, v2 = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()

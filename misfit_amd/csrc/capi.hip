@@ -796,3 +796,18 @@ extern "C" int misfit_neighbors(misfit_ctx* c, int32_t v, int32_t* out, int64_t 
     *count = k;
     return MISFIT_OK;
 }
+
+extern "C" int misfit_device_info(misfit_ctx* c, char* name, int namelen, int64_t* total_bytes, int64_t* free_bytes) {
+    if (!c) return MISFIT_E_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (name && namelen > 0) {
+        hipDeviceProp_t prop;
+        HIPCHK(c, hipGetDeviceProperties(&prop, c->device));
+        snprintf(name, (size_t)namelen, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    size_t fr = 0, tot = 0;
+    HIPCHK(c, hipMemGetInfo(&fr, &tot));
+    if (total_bytes) *total_bytes = (int64_t)tot;
+    if (free_bytes) *free_bytes = (int64_t)fr;
+    return MISFIT_OK;
+}
