@@ -672,9 +672,16 @@ __device__ __forceinline__ void apply_flip(const BP& P, int32_t z, bool join, in
     }
 }
 
+// First conflicting edge of the round (INT64_MAX when none): the counter
+// holds ~0 until some edge conflicts.
+__device__ __forceinline__ int64_t round_conflict(const BP& P, int par) {
+    u64 cf = P.ctr[C_CONF0 + par];
+    return cf > (u64)INT64_MAX ? INT64_MAX : (int64_t)cf;
+}
+
 __device__ void ph_commit(const BP& P, const GC& c, int par) {
     int64_t na = (int64_t)P.ctr[C_NACT0 + par];
-    int64_t conflict = (int64_t)P.ctr[C_CONF0 + par];
+    const int64_t conflict = round_conflict(P, par);
     for (int64_t i = c.wave; i < na; i += c.nwaves) {
         int32_t e = P.active[i];
         if (e >= conflict) continue;
@@ -880,7 +887,7 @@ __global__ __launch_bounds__(256) void k_persistent(BP P) {
         grid.sync();
         ph_validate(P, c, start, end, par, R);
         grid.sync();
-        int64_t conflict = (int64_t)P.ctr[C_CONF0 + par];
+        const int64_t conflict = round_conflict(P, par);
         ph_commit(P, c, par);
         grid.sync();
         start = conflict < end ? conflict : end;

@@ -1,0 +1,149 @@
+"""CPU-side checks of libmisfit_amd.so: it loads, exports every entry point
+include/misfit.h declares, and its host parts (.egr I/O, generators, greedy
+MIS, update streams, the drop-in binaries' argument handling) behave as
+specified.  No call here touches a GPU."""
+import os
+import re
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from tests import helpers as H
+from tests.conftest import ROOT, SAMPLE
+
+
+def _declared():
+    hdr = open(os.path.join(ROOT, "include", "misfit.h")).read()
+    return sorted(set(re.findall(r"\b(misfit_[a-z0-9_]+)\s*\(", hdr)))
+
+
+def test_library_exports_every_declared_symbol(engine_mod):
+    L = engine_mod.lib()
+    names = _declared()
+    assert len(names) >= 25
+    for name in names:
+        assert hasattr(L, name), name
+    assert sorted(engine_mod.EXPORTS) == names
+    out = subprocess.run(["nm", "-D", "--defined-only", engine_mod.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    defined = set(re.findall(r" T (misfit_[a-z0-9_]+)", out))
+    assert set(names) <= defined
+
+
+def test_library_has_gfx950_code(engine_mod):
+    blob = open(engine_mod.LIB_PATH, "rb").read()
+    assert b"gfx950" in blob
+
+
+def test_egr_read_both_forms(engine_mod, oracle, tmp_path):
+    ni, nl, wide = engine_mod.read_egr(os.path.join(SAMPLE, "sample.egr"))
+    g = oracle.Graph.load_egr(os.path.join(SAMPLE, "sample.egr"))
+    oni, onl = g.base_csr()
+    assert wide and np.array_equal(ni, oni) and np.array_equal(nl, onl)
+    p = str(tmp_path / "narrow.egr")
+    engine_mod.write_narrow_egr(p, ni, nl)
+    raw = open(p, "rb").read()
+    assert struct.unpack("<ii", raw[:8]) == (2000, 16000)
+    ni2, nl2, wide2 = engine_mod.read_egr(p)
+    assert not wide2 and np.array_equal(ni2, ni) and np.array_equal(nl2, nl)
+    g2 = oracle.Graph.load_egr(p)
+    assert not g2.wide and np.array_equal(g2.base_csr()[1], nl)
+
+
+def test_egr_writer_layout_and_atomic_replace(engine_mod, tmp_path):
+    rng = np.random.default_rng(1)
+    ni, nl = H.random_graph(rng, 50, 0.1)
+    p = tmp_path / "g.egr"
+    p.write_bytes(b"old contents")
+    engine_mod.write_egr(str(p), ni, nl)
+    raw = p.read_bytes()
+    n, sent, m = struct.unpack("<iiq", raw[:16])
+    assert (n, sent, m) == (50, -1, len(nl))
+    assert raw[16:16 + 51 * 8] == ni.astype(np.int64).tobytes()
+    assert raw[16 + 51 * 8:] == nl.astype(np.int32).tobytes()
+    assert [f for f in os.listdir(tmp_path) if ".tmp." in f] == []
+
+
+@pytest.mark.parametrize("content,err", [
+    (struct.pack("<ii", 0, 5), "invalid header"),
+    (struct.pack("<ii", 3, -2), "invalid header"),
+    (struct.pack("<ii", 3, 4) + b"\0" * 8, "truncated"),
+    (b"\1\0", "header"),
+])
+def test_egr_reader_rejects_bad_files(engine_mod, tmp_path, content, err):
+    p = tmp_path / "bad.egr"
+    p.write_bytes(content)
+    with pytest.raises(engine_mod.MisfitError) as ei:
+        engine_mod.read_egr(str(p))
+    assert err in str(ei.value)
+
+
+def _csr_ok(ni, nl):
+    n = len(ni) - 1
+    src = np.repeat(np.arange(n), np.diff(ni))
+    assert not np.any(src == nl), "self-loop"
+    for x in range(0, n, max(1, n // 500)):
+        r = nl[ni[x]:ni[x + 1]]
+        assert np.all(np.diff(r) > 0), "row not strictly ascending"
+    key = src.astype(np.int64) * n + nl
+    rkey = nl.astype(np.int64) * n + src
+    assert np.array_equal(np.sort(key), np.sort(rkey)), "not symmetric"
+
+
+def test_generators_valid_and_deterministic(engine_mod):
+    M = engine_mod
+    ni, nl = M.gen_er(5000, 40000, 42)
+    _csr_ok(ni, nl)
+    assert len(nl) == 80000
+    a, b = M.gen_er(5000, 40000, 42)
+    assert np.array_equal(a, ni) and np.array_equal(b, nl)
+    ri, rl = M.gen_rmat(12, 16, 7)
+    _csr_ok(ri, rl)
+    assert len(ri) == 4097
+    deg = np.diff(ri)
+    assert deg[:16].mean() > 10 * deg.mean()  # skew towards low ids (a = 0.57)
+    gi, gl = M.gen_rgg(20000, 32.0, 11)
+    _csr_ok(gi, gl)
+    assert 28 < len(gl) / 20000 < 34  # boundary effects trim the mean a little
+    gi2, gl2 = M.gen_rgg(20000, 32.0, 11)
+    assert np.array_equal(gl, gl2)
+
+
+def test_greedy_mis_matches_oracle(engine_mod, oracle):
+    rng = np.random.default_rng(2)
+    for _ in range(3):
+        ni, nl = H.random_graph(rng, 300, 0.03)
+        assert np.array_equal(engine_mod.greedy_mis(ni, nl), oracle.Graph.from_csr(ni, nl).greedy_mis())
+    ni, nl, _ = engine_mod.read_egr(os.path.join(SAMPLE, "sample.egr"))
+    mis, _ = oracle.read_mis(os.path.join(SAMPLE, "sample_mis.txt"), 2000)
+    assert np.array_equal(engine_mod.greedy_mis(ni, nl), mis)
+
+
+def test_update_streams(engine_mod, oracle):
+    M = engine_mod
+    ni, nl = M.gen_rmat(12, 16, 7)
+    g = oracle.Graph.from_csr(ni, nl)
+    u, v = M.gen_updates(M.MIXED_EXISTING, ni, nl, 4000, 7)
+    ins = g.flags(u, v)
+    assert ins.sum() == 2000  # half insertions of non-edges, half deletions
+    pairs = {(min(a, b), max(a, b)) for a, b in zip(u.tolist(), v.tolist())}
+    assert len(pairs) == 4000  # distinct across the stream
+    u0, v0 = M.gen_updates(M.UNIFORM_PAIRS, ni, nl, 5000, 1)
+    assert np.all(u0 != v0) and u0.min() >= 0 and u0.max() < 4096
+    u2, v2 = M.gen_updates(M.PREFERENTIAL, ni, nl, 5000, 1)
+    deg = np.diff(ni)
+    assert deg[u2].mean() > 2 * deg.mean()
+    u3, v3 = M.gen_updates(M.LOW_DEGREE, ni, nl, 5000, 1)
+    q = np.sort(deg)[len(deg) // 4 - 1]
+    assert deg[u3].max() <= q and deg[v3].max() <= q
+
+
+def test_binaries_usage_without_gpu(engine_mod):
+    for b in ("misfit", "misfit_async"):
+        exe = os.path.join(engine_mod.BIN_DIR, b)
+        assert os.path.exists(exe), exe
+        r = subprocess.run([exe], capture_output=True, text=True)
+        assert r.returncode == 1
+        assert "Usage:" in r.stderr and "<num_batches>" in r.stderr
