@@ -64,9 +64,9 @@ typedef struct {
     double device_ms;         /* batch time on the device (events) */
 } misfit_batch_stats;
 
-/* Execution modes of the repair. */
-#define MISFIT_MODE_PLAIN 0       /* per-phase kernels, host-driven rounds */
-#define MISFIT_MODE_PERSISTENT 1  /* one cooperative launch per batch */
+/* Execution modes of the repair (results are identical). */
+#define MISFIT_MODE_PLAIN 0   /* one batch at a time: the host drives every round */
+#define MISFIT_MODE_QUEUED 1  /* whole batches enqueued, device-driven rounds (default) */
 
 /* Replaces the GPUVertexClusteringGraph constructor's device selection and
  * stream creation (src/misfit/misfit.cu:389-421, core/cli_setup.h:43-47). */
@@ -98,7 +98,7 @@ int misfit_get_mis(misfit_ctx* ctx, uint8_t* mis);
  * 277-349). */
 int misfit_set_rule(misfit_ctx* ctx, int completion_on);
 
-/* MISFIT_MODE_PLAIN or MISFIT_MODE_PERSISTENT (results are identical). */
+/* MISFIT_MODE_PLAIN or MISFIT_MODE_QUEUED. */
 int misfit_set_mode(misfit_ctx* ctx, int mode);
 
 /* Insertion flags against the current graph: ins[i] = !adjacent(u[i], v[i]).
@@ -107,8 +107,7 @@ int misfit_set_mode(misfit_ctx* ctx, int mode);
 int misfit_flags(misfit_ctx* ctx, const int32_t* u, const int32_t* v, int64_t count, uint8_t* ins);
 
 /* Queues one batch: overlay update, exact repair, completion.  Returns at
- * once in persistent mode (the host buffer is copied before the call
- * returns).  Replaces the async engine's launchBatch
+ * once in queued mode (the host buffer is copied before the call returns).  Replaces the async engine's launchBatch
  * (src/misfit_async/misfit_async.cu:484-675) together with the overlay
  * update of launchBatchGPU (:718-740). */
 int misfit_submit_batch(misfit_ctx* ctx, const misfit_update* ups, int64_t count, uint64_t* ticket);

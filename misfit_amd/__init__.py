@@ -24,7 +24,7 @@ LIB_PATH = os.path.join(_HERE, "libmisfit_amd.so")
 BIN_DIR = os.path.join(_HERE, "bin")
 
 MODE_PLAIN = 0
-MODE_PERSISTENT = 1
+MODE_QUEUED = 1
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _i64p = ctypes.POINTER(ctypes.c_int64)

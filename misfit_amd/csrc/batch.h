@@ -4,8 +4,9 @@
 #include "dev.h"
 
 // Rows longer than this get a per-batch index of their admissible
-// neighbours instead of being scanned in full by the repair.
-#define MF_FA_T 256
+// neighbours instead of being scanned in full by the repair.  Admissible
+// vertices are few, so every row uses the index.
+#define MF_FA_T 0
 
 enum BatchCounter {
     C_SEEDS = 0,     // distinct endpoints
@@ -87,9 +88,8 @@ struct LaunchCfg {
     hipStream_t stream;
 };
 void mf_launch_batch_prologue(const BP& P, const LaunchCfg& L);
-void mf_launch_round(const BP& P, const LaunchCfg& L, int64_t start, int64_t end, int par, uint32_t R);
+void mf_launch_round(const BP& P, const LaunchCfg& L, int par);
+void mf_launch_finish(const BP& P, const LaunchCfg& L, int par);
 void mf_launch_completion_collect(const BP& P, const LaunchCfg& L);
+void mf_launch_stats(const BP& P, const LaunchCfg& L);
 void mf_launch_completion_round(const BP& P, const LaunchCfg& L, int par);
-void mf_launch_set_round(const BP& P, const LaunchCfg& L, uint32_t R);
-hipError_t mf_launch_persistent(const BP& P, int grid, hipStream_t stream);
-int mf_persistent_grid(int device);

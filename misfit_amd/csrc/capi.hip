@@ -1,14 +1,18 @@
 // capi.hip -- the C ABI (include/misfit.h): context, device memory,
 // capacity management and the two ways of driving a batch.
 //
-//   plain       per-phase kernels; the host reads the round outcome after
-//               every speculative round (one batch at a time, the way the
-//               reference's sync engine drives its kernels).
-//   persistent  one cooperative launch per batch; rounds and completion
-//               iterations are grid barriers, so submit returns at once and
-//               batches queue back to back on the stream while the host
-//               stages the next ones (the generalisation of the
-//               reference's misfit_async pipelining).
+//   plain   per-phase kernels; the host reads the round outcome after every
+//           speculative round and every completion iteration (one batch at
+//           a time, the way the reference's sync engine drives its kernels).
+//   queued  the round state lives on the device, so a batch is enqueued in
+//           full without host round trips: the prologue, a few grid-wide
+//           rounds (no-ops once the batch is done) and one single-workgroup
+//           kernel that finishes the remaining rounds and the completion.
+//           submit returns at once and batches queue back to back while the
+//           host stages the next ones (the reference's misfit_async
+//           pipelining, generalised).  Kernel boundaries are the grid-wide
+//           barrier: on a multi-die MI355X a stream launch costs ~2.4 us,
+//           an in-kernel grid barrier 14-80 us.
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -43,10 +47,13 @@ struct misfit_ctx {
     GraphState S{};
     bool has_graph = false;
     int completion = 1;
-    int mode = MISFIT_MODE_PERSISTENT;
-    int persistent_grid = 0;
+    int mode = MISFIT_MODE_QUEUED;
+    int grid_rounds = 12;      // grid-wide rounds queued before the finisher
+    bool grid_rounds_fixed = false;
+    int rounds_hist[16] = {0};  // rounds of recent batches
+    int rounds_pos = 0;
+    bool debug_rounds = false;
     uint32_t epoch = 0;
-    uint32_t host_round = 0;
     uint32_t bytes_tag = 0;
     int64_t last_U = 0;
     // capacities
@@ -122,10 +129,13 @@ extern "C" int misfit_ctx_create(int device, misfit_ctx** out) {
         delete c;
         return MISFIT_E_HIP;
     }
-    int coop = 0;
-    hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device);
-    c->persistent_grid = coop ? mf_persistent_grid(device) : 0;
-    if (c->persistent_grid <= 0) c->mode = MISFIT_MODE_PLAIN;
+    const char* dbg = getenv("MISFIT_DEBUG_ROUNDS");
+    c->debug_rounds = dbg && dbg[0] == '1';
+    const char* gr = getenv("MISFIT_GRID_ROUNDS");
+    if (gr) {
+        c->grid_rounds = std::max(0, atoi(gr));
+        c->grid_rounds_fixed = true;
+    }
     int cus = 0;
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
     c->S.blocks = std::max(64, cus * 8);
@@ -186,9 +196,7 @@ extern "C" int misfit_set_rule(misfit_ctx* c, int completion_on) {
 
 extern "C" int misfit_set_mode(misfit_ctx* c, int mode) {
     if (!c) return MISFIT_E_ARG;
-    if (mode != MISFIT_MODE_PLAIN && mode != MISFIT_MODE_PERSISTENT) return fail(c, MISFIT_E_ARG, "unknown mode");
-    if (mode == MISFIT_MODE_PERSISTENT && c->persistent_grid <= 0)
-        return fail(c, MISFIT_E_UNSUPPORTED, "device has no cooperative launch");
+    if (mode != MISFIT_MODE_PLAIN && mode != MISFIT_MODE_QUEUED) return fail(c, MISFIT_E_ARG, "unknown mode");
     c->mode = mode;
     return MISFIT_OK;
 }
@@ -373,9 +381,11 @@ extern "C" int misfit_load_csr(misfit_ctx* c, int32_t n, int64_t m, const void* 
     HIPCHK(c, dalloc(A, &g.kw, n));
     HIPCHK(c, dalloc(A, &g.pub, n));
     HIPCHK(c, dalloc(A, &g.evb, n));
+    HIPCHK(c, dalloc(A, &g.pfb, n));
     HIPCHK(c, dalloc(A, &g.cstat, n));
     HIPCHK(c, dalloc(A, &g.mmark, n));
     HIPCHK(c, dalloc(A, &g.cmark, n));
+    HIPCHK(c, dalloc(A, &g.cdec, n));
     HIPCHK(c, dalloc(A, &g.ne_head, n));
     HIPCHK(c, dalloc(A, &g.fa_cnt, n));
     HIPCHK(c, dalloc(A, &g.fa_start, n));
@@ -411,9 +421,11 @@ extern "C" int misfit_load_csr(misfit_ctx* c, int32_t n, int64_t m, const void* 
     mf_fill_u64(g.kw, 0ull, n, st);
     mf_fill_u32(g.pub, 0u, n, st);
     mf_fill_u32(g.evb, 0u, n, st);
+    mf_fill_u32(g.pfb, 0u, n, st);
     mf_fill_u32(g.cstat, 0u, n, st);
     mf_fill_u64(g.mmark, 0ull, n, st);
     mf_fill_u64(g.cmark, 0ull, n, st);
+    mf_fill_u64(g.cdec, 0ull, n, st);
     mf_fill_i32(g.ne_head, -1, n, st);
     mf_fill_u64(g.fa_cnt, 0ull, n, st);
     mf_fill_i64(g.fa_start, -1, n, st);
@@ -422,16 +434,20 @@ extern "C" int misfit_load_csr(misfit_ctx* c, int32_t n, int64_t m, const void* 
     mf_fill_u32(c->d_round, 0u, 1, st);
     mf_fill_u64(c->d_ctr, 0ull, C_NCOUNTERS, st);
     c->epoch = 0;
-    c->host_round = 0;
     c->bytes_tag = 0;
     S.g = g;
     reset_overlay(c);
-    // base validation: ranges, monotone offsets, row order
+    // base validation: monotone offsets first (entry checks index through
+    // them), then entry ranges and row order
     HIPCHK(c, mf_validate_csr(S, c->d_flags));
     int flags = 0;
     HIPCHK(c, hipMemcpyAsync(&flags, c->d_flags, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHK(c, hipStreamSynchronize(st));
-    if (flags & (4 | 8)) return fail(c, MISFIT_E_ARG, "CSR has out-of-range entries or non-monotone offsets");
+    if (flags & 8) return fail(c, MISFIT_E_ARG, "CSR offsets are not monotone or do not span the entries");
+    HIPCHK(c, mf_validate_entries(S, c->d_flags));
+    HIPCHK(c, hipMemcpyAsync(&flags, c->d_flags, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    if (flags & 4) return fail(c, MISFIT_E_ARG, "CSR has out-of-range entries");
     g.sorted = (flags & 1) ? 0 : 1;
     if (!g.sorted) {
         // unsorted rows: the engine needs rows without duplicate entries
@@ -534,30 +550,27 @@ static void fill_stats(const misfit_ctx* c, const Slot& s, misfit_batch_stats* s
 }
 
 // Host-driven rounds: one round = 4 kernels, then the host reads where the
-// committed prefix ends.
+// committed prefix ends; the completion iterations likewise.
 static int drive_plain(misfit_ctx* c, BP& P) {
     LaunchCfg L{c->S.blocks, c->stream};
     mf_launch_batch_prologue(P, L);
-    int64_t start = 0;
-    uint32_t r = 0;
-    // round ids must never repeat (stale marks would read as current), so
-    // continue from the device counter that the persistent path advances
-    HIPCHK(c, hipMemcpyAsync(c->h_small, P.round, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    uint32_t R0 = (uint32_t)(c->h_small[0] & 0xFFFFFFFFull);
-    while (start < P.U) {
+    for (uint32_t r = 0;; r++) {
         if ((int64_t)r > P.U + 1) return fail(c, MISFIT_E_CHECK, "speculative rounds made no progress");
         int par = (int)(r & 1u);
-        int64_t end = std::min<int64_t>(P.U, start + P.window);
-        mf_launch_round(P, L, start, end, par, R0 + r + 1);
-        HIPCHK(c, hipMemcpyAsync(c->h_small, P.ctr + C_START, sizeof(u64), hipMemcpyDeviceToHost, c->stream));
+        if (P.U == 0) break;
+        mf_launch_round(P, L, par);
+        HIPCHK(c, hipMemcpyAsync(c->h_small, P.ctr, C_NCOUNTERS * sizeof(u64), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
-        int64_t conflict = (int64_t)std::min<u64>(c->h_small[0], (u64)INT64_MAX);
-        start = conflict < end ? conflict : end;
-        r++;
+        int64_t start = (int64_t)c->h_small[C_START];
+        if (c->debug_rounds) {
+            u64 cf = c->h_small[C_CONF0 + par];
+            fprintf(stderr, "[rounds] batch %u round %u: active %llu, conflict %lld, next start %lld / %lld\n",
+                    P.b, r, (unsigned long long)c->h_small[C_NACT0 + par],
+                    cf > (u64)INT64_MAX ? -1LL : (long long)cf, (long long)start, (long long)P.U);
+        }
+        if (start >= P.U) break;
     }
-    c->host_round = R0 + r;
-    mf_launch_set_round(P, L, c->host_round);
+    mf_launch_stats(P, L);
     if (P.completion) {
         mf_launch_completion_collect(P, L);
         for (uint32_t k = 0;; k++) {
@@ -570,6 +583,17 @@ static int drive_plain(misfit_ctx* c, BP& P) {
             if (k > (uint32_t)P.g.n + 1) return fail(c, MISFIT_E_CHECK, "completion made no progress");
         }
     }
+    HIPCHK(c, hipGetLastError());
+    return MISFIT_OK;
+}
+
+// Whole batch enqueued: prologue, grid-wide rounds, then the finisher.
+static int drive_queued(misfit_ctx* c, BP& P) {
+    LaunchCfg L{c->S.blocks, c->stream};
+    mf_launch_batch_prologue(P, L);
+    int k = c->grid_rounds;
+    for (int r = 0; r < k; r++) mf_launch_round(P, L, r & 1);
+    mf_launch_finish(P, L, k & 1);
     HIPCHK(c, hipGetLastError());
     return MISFIT_OK;
 }
@@ -622,12 +646,8 @@ extern "C" int misfit_submit_batch(misfit_ctx* c, const misfit_update* ups, int6
     P.window = std::max<int64_t>(count, 1);
     c->last_U = count;
     HIPCHK(c, hipEventRecord(s.t0, c->stream));
-    if (c->mode == MISFIT_MODE_PERSISTENT) {
-        HIPCHK(c, mf_launch_persistent(P, c->persistent_grid, c->stream));
-    } else {
-        rc = drive_plain(c, P);
-        if (rc) return rc;
-    }
+    rc = c->mode == MISFIT_MODE_QUEUED ? drive_queued(c, P) : drive_plain(c, P);
+    if (rc) return rc;
     HIPCHK(c, hipEventRecord(s.t1, c->stream));
     HIPCHK(c, hipMemcpyAsync(s.h_ctr, P.ctr, C_NCOUNTERS * sizeof(u64), hipMemcpyDeviceToHost, c->stream));
     // the pool top rides after the counters in the readback
@@ -644,6 +664,15 @@ extern "C" int misfit_wait(misfit_ctx* c, uint64_t ticket, misfit_batch_stats* s
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipEventSynchronize(s.done));
     s.busy = false;
+    if (!c->grid_rounds_fixed && s.count) {
+        // queue enough grid-wide rounds for the recent batches plus a margin;
+        // rounds past the end cost one empty launch each, and anything left
+        // over is finished by the single-workgroup kernel
+        c->rounds_hist[c->rounds_pos++ & 15] = (int)s.h_ctr[C_ROUNDS];
+        int mx = 0;
+        for (int h : c->rounds_hist) mx = std::max(mx, h);
+        c->grid_rounds = std::min(64, mx + 2);
+    }
     if (s.h_ctr[C_POOL_OVF]) return fail(c, MISFIT_E_CHECK, "overlay pool overflow");
     if (s.h_ctr[C_GUARD]) return fail(c, MISFIT_E_CHECK, "repair loop bound exceeded");
     if (st) fill_stats(c, s, st);
@@ -724,6 +753,7 @@ extern "C" int misfit_compact(misfit_ctx* c, int64_t* new_m) {
     reset_overlay(c);
     // rows are base order then insertion order, so they may be unsorted now
     HIPCHK(c, mf_validate_csr(c->S, c->d_flags));
+    HIPCHK(c, mf_validate_entries(c->S, c->d_flags));
     int flags = 0;
     HIPCHK(c, hipMemcpyAsync(&flags, c->d_flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));

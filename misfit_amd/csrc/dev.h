@@ -29,11 +29,17 @@
 //     pub     = 4b+1 | 2    vertex may become uncovered during batch b
 //                   (bit 1: it also has a row, i.e. it is admissible)
 //     evb     = b   vertex was evicted in batch b
+//     pfb     = b   vertex is max(u,v) of an insertion of batch b whose other
+//                   end may be evicted too (or whose edge is gone after the
+//                   batch): only then can it be uncovered after an eviction
 //     cstat   = 4b+s        completion state (1 open, 2 joined, 3 covered)
 //     fa_*    per-batch index of admissible neighbours of long rows
 //   mmark/cmark  u64 round<<32 | ~edge: earliest edge of the current
 //                   speculative round that flips the vertex's membership
 //                   (mmark) or changes its MIS-neighbour count (cmark).
+//   cdec         u64 round<<32 | number of evictions next to the vertex
+//                   speculated in the current round (an upper bound on how
+//                   far earlier edges can lower its count).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -67,9 +73,11 @@ struct DevGraph {
     u64* kw;
     uint32_t* pub;
     uint32_t* evb;
+    uint32_t* pfb;   // = b: an insertion of batch b may let this vertex become uncovered after its eviction
     uint32_t* cstat;
     u64* mmark;
     u64* cmark;
+    u64* cdec;       // round<<32 | evictions next to the vertex speculated this round
     int32_t* ne_head;
     u64* fa_cnt;     // b<<32 | count
     int64_t* fa_start;
@@ -157,9 +165,15 @@ __device__ __forceinline__ int64_t base_find(const DevGraph& g, int32_t x, int32
 }
 
 // Adjacency in the current graph (base minus deletions plus insertions): the
-// device form of HybridGraph::isAdjacent (core/FlashGraph.h:291-301).
+// device form of HybridGraph::isAdjacent (core/FlashGraph.h:291-301).  The
+// base part searches the shorter of the two rows (rows are symmetric).
 __device__ __forceinline__ bool adjacent(const DevGraph& g, int32_t x, int32_t y) {
     if (ph_lookup(g, pair_key(x, y)) == 1) return true;
+    if (g.sorted && (g.nindex[y + 1] - g.nindex[y]) < (g.nindex[x + 1] - g.nindex[x])) {
+        int32_t t = x;
+        x = y;
+        y = t;
+    }
     int64_t p = base_find(g, x, y);
     return p >= 0 && g.nlist[p] >= 0;
 }
@@ -182,6 +196,19 @@ __device__ __forceinline__ int lane_id() { return threadIdx.x & (MF_WAVE - 1); }
 
 __device__ __forceinline__ int first_lane(u64 mask) { return mask ? __ffsll((long long)mask) - 1 : -1; }
 
+// Appends v for every active lane with pred set, with one atomic per wave.
+// Call from wave-uniform or divergent code alike: only active lanes take part.
+__device__ __forceinline__ void wave_push(int32_t* list, u64* ctr, int32_t v, bool pred) {
+    u64 bal = __ballot(pred);
+    if (!bal) return;
+    int lane = lane_id();
+    int leader = first_lane(bal);
+    u64 base = 0;
+    if (lane == leader) base = atomicAdd(ctr, (u64)__popcll(bal));
+    base = (u64)__shfl((long long)base, leader, MF_WAVE);
+    if (pred) list[base + __popcll(bal & ((1ull << lane) - 1ull))] = v;
+}
+
 // Wave-wide inclusive prefix sum.
 __device__ __forceinline__ int64_t wave_incl_scan(int64_t v) {
     int l = lane_id();
@@ -203,7 +230,7 @@ __device__ __forceinline__ u64 wave_min_u64(u64 v) {
 }
 
 // Grid coordinates for grid-stride phases (one phase body serves both the
-// per-phase kernels and the persistent kernel).
+// grid-wide kernels and the single-workgroup finisher).
 struct GC {
     int64_t tid, nthreads, wave, nwaves;
     int lane;

@@ -26,6 +26,7 @@ hipError_t mf_compact_copy(GraphState& S, const int64_t* d_newidx, int32_t* d_ou
 hipError_t mf_batch_bytes(GraphState& S, const int32_t* d_seeds, const u64* d_nseeds, uint32_t t, u64* d_acc);
 hipError_t mf_widen(const int32_t* d_in, int64_t* d_out, int64_t count, hipStream_t st);
 hipError_t mf_validate_csr(GraphState& S, int* d_flags);
+hipError_t mf_validate_entries(GraphState& S, int* d_flags);
 void mf_scan_excl(const int64_t* in, int64_t count, int64_t* out, int64_t* part, hipStream_t st);
 void mf_fill_u32(uint32_t* p, uint32_t v, int64_t count, hipStream_t st);
 void mf_fill_i32(int32_t* p, int32_t v, int64_t count, hipStream_t st);

@@ -19,7 +19,7 @@
 //         (misfit.cu:909-926, core/flash_compactor.h:40-104).
 // Additions (environment, never changing the 5-argument contract):
 //   MISFIT_RULE=reference   literal reference rule without completion
-//   MISFIT_MODE=plain|persistent   how batches are driven (default persistent)
+//   MISFIT_MODE=plain|queued   how batches are driven (default queued)
 //   MISFIT_CHECK=0          skip the full check after every batch
 #pragma once
 #include <algorithm>

@@ -20,13 +20,23 @@
 //     id, a vertex joins when it is outside the MIS with no MIS neighbour.
 //
 // How the GPU evaluates the sequential rule exactly:
-//   1. PU ("potentially uncovered") = W ∪ {z : z ∉ MIS, c(z) = k(z)}, where W
-//      = {max(u,v) of insertions}, c(z) = MIS neighbours, k(z) = MIS
-//      neighbours in W (all at batch start).  Only evictions remove members
-//      and only members of W can be evicted, so a vertex outside PU keeps an
-//      MIS neighbour (or stays a member) for the whole batch.  Hence only PU
-//      members with a row (ADM) can ever be admitted, and a vertex outside PU
-//      always blocks a 2-hop test.  The repair reads nothing else.
+//   1. PU ("potentially uncovered") is a superset of the vertices that are at
+//      some moment of the batch outside the MIS with no MIS neighbour.  Only
+//      evictions remove members, and only W = {max(u,v) of insertions} can be
+//      evicted.  So, with c(z) = MIS neighbours and k(z) = MIS neighbours in W
+//      at batch start:
+//        - a non-member z with c(z) > k(z) keeps a member neighbour forever;
+//        - a member z outside W stays a member;
+//        - a member z in W becomes a non-member only when some insertion
+//          (z, o), z > o, evicts it, and o, a member at that moment and a
+//          neighbour of z after the batch, covers z until o itself is
+//          evicted, which needs o in W.
+//      PU = {z not in MIS : c(z) = k(z)}
+//         ∪ {z in W ∩ MIS : some insertion (z, o), z > o, has o in W or the
+//            pair absent after the batch}.
+//      Only PU members with a row (ADM) can ever be admitted, and a vertex
+//      outside PU always blocks a 2-hop test, so the repair reads nothing
+//      else.
 //   2. BFS discovery order of row(w) equals first-occurrence order of
 //      [w] ++ adj(w) ++ concat_{x1 in adj(w)} adj(x1), so the first admissible
 //      vertex is found by scanning that flat sequence; long adj(x1) rows are
@@ -38,10 +48,7 @@
 //      the round.  Every edge before it saw exactly the sequential state and
 //      commits; the rest are re-evaluated next round (the first pending edge
 //      never conflicts, so every round makes progress).
-#include <hip/hip_cooperative_groups.h>
 #include "batch.h"
-
-namespace cg = cooperative_groups;
 
 // ---------------------------------------------------------------- helpers
 __device__ __forceinline__ bool in_pu(const BP& P, int32_t x) { return (P.g.pub[x] >> 2) == P.b; }
@@ -84,12 +91,32 @@ __device__ __forceinline__ bool mk_before(const u64* mk, int32_t x, uint32_t R, 
     return (uint32_t)(m >> 32) == R && (int64_t)(0xFFFFFFFFu - (uint32_t)m) < (int64_t)e;
 }
 
-template <bool V>
-__device__ __forceinline__ uint32_t read_mc(const BP& P, int32_t x, uint32_t R, int32_t e, bool& conf) {
-    uint32_t v = P.g.mc[x];
-    if (V) conf |= mk_before(P.g.mmark, x, R, e) || mk_before(P.g.cmark, x, R, e);
-    return v;
+// Validation of one read of mc[x] that returned `val` by edge e.  The
+// decision taken from it holds while the membership bit is unchanged and the
+// count stays at or above T (T = 0: any change of the count matters;
+// MF_T_MIS: the count does not matter).  The read conflicts when an earlier
+// edge of the round flipped the bit, or changed the count and the evictions
+// speculated next to x this round could take it below T.
+#define MF_T_MIS 0xFFFFFFFFu
+__device__ __forceinline__ void vread(const BP& P, int32_t x, uint32_t val, uint32_t T, uint32_t R, int32_t e,
+                                      bool& conf) {
+    if (mk_before(P.g.mmark, x, R, e)) {
+        conf = true;
+        return;
+    }
+    if (T == MF_T_MIS || !mk_before(P.g.cmark, x, R, e)) return;
+    uint32_t cnt = val & MF_CMASK;
+    if (T == 0 || cnt < T) {
+        conf = true;
+        return;
+    }
+    u64 d = P.g.cdec[x];
+    uint32_t dec = ((uint32_t)(d >> 32) == R) ? (uint32_t)d : 0u;
+    if ((u64)cnt < (u64)T + dec) conf = true;
 }
+
+// Round-tagged counter word R<<32 | count (see epoch_count_add).
+__device__ __forceinline__ void round_count_add(u64* w, uint32_t R) { (void)epoch_count_add(w, R); }
 
 // ---------------------------------------------------------------- prologue
 // P1: claim pairs, tag endpoints and possible evictees.
@@ -106,12 +133,10 @@ __device__ void ph_claim(const BP& P, const GC& c) {
         int32_t prevh = atomicExch(&P.bt_head[s], (int32_t)i);
         P.bt_nxt[i] = prevh;
         P.bt_slot[i] = (int32_t)s;
-        if (atomicExch(&P.g.seedb[a], P.b) != P.b) push(P.seeds, &P.ctr[C_SEEDS], a);
-        if (atomicExch(&P.g.seedb[bb], P.b) != P.b) push(P.seeds, &P.ctr[C_SEEDS], bb);
-        if (P.bins[i]) {
-            int32_t w = a > bb ? a : bb;
-            if (atomicExch(&P.g.wpotb[w], P.b) != P.b) push(P.wpot, &P.ctr[C_WPOT], w);
-        }
+        wave_push(P.seeds, &P.ctr[C_SEEDS], a, atomicExch(&P.g.seedb[a], P.b) != P.b);
+        wave_push(P.seeds, &P.ctr[C_SEEDS], bb, atomicExch(&P.g.seedb[bb], P.b) != P.b);
+        int32_t w = a > bb ? a : bb;
+        wave_push(P.wpot, &P.ctr[C_WPOT], w, P.bins[i] && atomicExch(&P.g.wpotb[w], P.b) != P.b);
     }
 }
 
@@ -265,20 +290,34 @@ __device__ void ph_append(const BP& P, const GC& c) {
     }
 }
 
-// P5: k(z) for neighbours of MIS vertices that may be evicted; W joins PU;
-// every currently uncovered vertex joins PU.
+// P5a: flag members of W that could become uncovered after an eviction.
+__device__ void ph_partner(const BP& P, const GC& c) {
+    for (int64_t i = c.tid; i < P.U; i += c.nthreads) {
+        if (!P.bins[i]) continue;
+        int32_t a = P.bu[i], bb = P.bv[i];
+        int32_t w = a > bb ? a : bb, o = a > bb ? bb : a;
+        if (P.g.pfb[w] == P.b) continue;
+        if (P.g.wpotb[o] == P.b || !adjacent(P.g, w, o)) P.g.pfb[w] = P.b;
+    }
+}
+
+// P5: k(z) for neighbours of MIS vertices that may be evicted; flagged
+// members of W join PU; every currently uncovered vertex joins PU.
 __device__ void ph_kcount(const BP& P, const GC& c) {
     const DevGraph& g = P.g;
     int64_t nw = (int64_t)P.ctr[C_WPOT];
     for (int64_t i = c.wave; i < nw; i += c.nwaves) {
         int32_t w = P.wpot[i];
-        if (c.lane == 0) pu_add(P, w);
-        if (!(g.mc[w] & MF_MISBIT)) continue;
+        if (!(g.mc[w] & MF_MISBIT)) continue;  // non-members: c = k test below
+        if (c.lane == 0 && g.pfb[w] == P.b) pu_add(P, w);
         Row r = get_row(g, w);
         int64_t L = r.len();
         for (int64_t j = c.lane; j < L; j += MF_WAVE) {
             int32_t z = r.at(j);
             if (z < 0 || z == w) continue;
+            // k(z) <= |W|: a vertex with more MIS neighbours can never reach
+            // c(z) = k(z) (this also keeps hubs out of the counting)
+            if ((u64)(g.mc[z] & MF_CMASK) > (u64)nw) continue;
             if (epoch_count_add(&g.kw[z], P.b)) {
                 u64 t = atomicAdd(&P.ctr[C_TOUCHED], 1ull);
                 if (t < (u64)P.touched_cap) P.touched[t] = z;
@@ -399,11 +438,18 @@ __device__ void ph_fa_fill(const BP& P, const GC& c) {
 template <bool V>
 __device__ bool cand_ok(const BP& P, int32_t x, int32_t w, bool xadj, int32_t e, uint32_t R, bool& conf) {
     const DevGraph& g = P.g;
-    uint32_t mx = read_mc<V>(P, x, R, e, conf);
-    if (mx & MF_MISBIT) return false;
+    uint32_t mx = g.mc[x];
+    if (mx & MF_MISBIT) {
+        if (V) vread(P, x, mx, MF_T_MIS, R, e, conf);
+        return false;
+    }
     uint32_t cx = mx & MF_CMASK;
-    if (xadj && cx) cx -= 1;
-    if (cx) return false;
+    uint32_t corr = (xadj && cx) ? 1u : 0u;
+    if (cx - corr) {
+        if (V) vread(P, x, mx, 1u + corr, R, e, conf);
+        return false;
+    }
+    if (V) vread(P, x, mx, 0u, R, e, conf);
     if (g.seedb[x] != P.b) return true;  // candidate row = adj(x)
     // seed row {x} ∪ N1(x) ∪ N2(x) must hold no MIS member
     Row r = get_row(g, x);
@@ -411,17 +457,35 @@ __device__ bool cand_ok(const BP& P, int32_t x, int32_t w, bool xadj, int32_t e,
     for (int64_t j = 0; j < L; j++) {
         int32_t y = r.at(j);
         if (y < 0 || y == x) continue;
-        if (y == w) {
-            if (read_mc<V>(P, w, R, e, conf) & MF_CMASK) return false;
+        if (y == w) {  // w is outside the MIS now; its own count is unchanged
+            uint32_t mw = g.mc[w];
+            if (mw & MF_CMASK) {
+                if (V) vread(P, w, mw, 1u, R, e, conf);
+                return false;
+            }
+            if (V) vread(P, w, mw, 0u, R, e, conf);
             continue;
         }
         if (!in_pu(P, y)) return false;
-        uint32_t my = read_mc<V>(P, y, R, e, conf);
-        if (my & MF_MISBIT) return false;
+        uint32_t my = g.mc[y];
+        if (my & MF_MISBIT) {
+            if (V) vread(P, y, my, MF_T_MIS, R, e, conf);
+            return false;
+        }
         uint32_t cy = my & MF_CMASK;
-        if (cy == 0) continue;
-        if (cy >= 2) return false;
-        if (!adjacent(g, y, w)) return false;
+        if (cy == 0) {
+            if (V) vread(P, y, my, 0u, R, e, conf);
+            continue;
+        }
+        if (cy >= 2) {
+            if (V) vread(P, y, my, 2u, R, e, conf);
+            return false;
+        }
+        if (!adjacent(g, y, w)) {  // its one MIS neighbour is not w
+            if (V) vread(P, y, my, 1u, R, e, conf);
+            return false;
+        }
+        if (V) vread(P, y, my, 0u, R, e, conf);
     }
     return true;
 }
@@ -434,8 +498,11 @@ __device__ int32_t eval_insertion(const BP& P, int32_t w, int32_t e, uint32_t R,
     const int lane = lane_id();
     // position 0: w itself
     {
-        uint32_t mw = read_mc<V>(P, w, R, e, conf);
-        if ((mw & MF_CMASK) == 0) {
+        uint32_t mw = g.mc[w];
+        if (mw & MF_CMASK) {
+            if (V) vread(P, w, mw, 1u, R, e, conf);
+        } else {
+            if (V) vread(P, w, mw, 0u, R, e, conf);
             Row r = get_row(g, w);
             int64_t L = r.len();
             bool blocked = false;
@@ -447,8 +514,16 @@ __device__ int32_t eval_insertion(const BP& P, int32_t w, int32_t e, uint32_t R,
                     if (y >= 0 && y != w) {
                         if (!in_pu(P, y)) bl = true;
                         else {
-                            uint32_t my = read_mc<V>(P, y, R, e, conf);
-                            bl = (my & MF_MISBIT) || (my & MF_CMASK) >= 2;
+                            uint32_t my = g.mc[y];
+                            if (my & MF_MISBIT) {
+                                bl = true;
+                                if (V) vread(P, y, my, MF_T_MIS, R, e, conf);
+                            } else if ((my & MF_CMASK) >= 2) {  // w was one of them
+                                bl = true;
+                                if (V) vread(P, y, my, 2u, R, e, conf);
+                            } else if (V) {
+                                vread(P, y, my, 0u, R, e, conf);
+                            }
                         }
                     }
                 }
@@ -482,22 +557,18 @@ __device__ int32_t eval_insertion(const BP& P, int32_t w, int32_t e, uint32_t R,
         if (j < L) {
             int32_t x1 = rw.at(j);
             if (x1 >= 0 && x1 != w) {
-                Row r1 = get_row(g, x1);
-                int64_t l1 = r1.len();
-                if (l1 > MF_FA_T && fa_ok) {
-                    if ((uint32_t)(g.fa_cnt[x1] >> 32) == P.b) {
-                        int64_t st = g.fa_start[x1];
-                        if (st >= 0) {
-                            kindl = 2;
-                            pb = P.fa_z + st;
-                            pa = P.fa_pos + st;
-                            slen = g.fa_fill[x1];
-                        } else {
-                            kindl = 1; pb = r1.base; pa = r1.add; blen = r1.blen; slen = l1;
-                        }
-                    }
-                } else {
-                    kindl = 1; pb = r1.base; pa = r1.add; blen = r1.blen; slen = l1;
+                // every row with an admissible neighbour has index entries
+                // this batch; only an overflowed index falls back to the row
+                bool indexed = (uint32_t)(g.fa_cnt[x1] >> 32) == P.b;
+                int64_t st = indexed && fa_ok ? g.fa_start[x1] : -1;
+                if (st >= 0) {
+                    kindl = 2;
+                    pb = P.fa_z + st;
+                    pa = P.fa_pos + st;
+                    slen = g.fa_fill[x1];
+                } else if (indexed || !fa_ok) {
+                    Row r1 = get_row(g, x1);
+                    kindl = 1; pb = r1.base; pa = r1.add; blen = r1.blen; slen = r1.len();
                 }
             }
         }
@@ -571,13 +642,21 @@ __device__ bool eval_deletion(const BP& P, int32_t tgt, int32_t other, int32_t e
         if (j < L) {
             int32_t y = r.at(j);
             if (y >= 0 && y != tgt) {
-                uint32_t my = read_mc<V>(P, y, R, e, conf);
-                if (y == other) bl = (my & MF_CMASK) != 0;
-                else if (my & MF_MISBIT) bl = true;
-                else {
+                uint32_t my = g.mc[y];
+                uint32_t T;
+                if (y == other) {  // its MIS neighbours sit two hops from tgt
+                    bl = (my & MF_CMASK) != 0;
+                    T = bl ? 1u : 0u;
+                } else if (my & MF_MISBIT) {
+                    bl = true;
+                    T = MF_T_MIS;
+                } else {
                     uint32_t cy = my & MF_CMASK;
-                    bl = cy >= 2 || (cy == 1 && !adjacent(g, y, other));
+                    if (cy >= 2) { bl = true; T = 2u; }
+                    else if (cy == 1 && !adjacent(g, y, other)) { bl = true; T = 1u; }
+                    else T = 0u;
                 }
+                if (V) vread(P, y, my, T, R, e, conf);
             }
         }
         if (__ballot(bl)) return false;
@@ -586,25 +665,54 @@ __device__ bool eval_deletion(const BP& P, int32_t tgt, int32_t other, int32_t e
 }
 
 // ---------------------------------------------------------------- rounds
-__device__ void ph_classify(const BP& P, const GC& c, int64_t start, int64_t end, int par) {
-    for (int64_t e = start + c.tid; e < end; e += c.nthreads) {
+// The round state lives on the device: ctr[C_START] is the first edge not
+// yet committed and *P.round the id of the last round.  A round whose start
+// is at or past U does nothing, so rounds can be queued without knowing how
+// many the batch needs; only the commit phase advances the state.
+struct RoundView {
+    int64_t start, end;
+    uint32_t R;
+    bool live;
+};
+
+__device__ __forceinline__ RoundView round_view(const BP& P) {
+    RoundView v;
+    v.start = (int64_t)P.ctr[C_START];
+    v.end = v.start + P.window < P.U ? v.start + P.window : P.U;
+    v.R = *P.round + 1u;
+    v.live = v.start < P.U;
+    return v;
+}
+
+__device__ void ph_classify(const BP& P, const GC& c, const RoundView& v, int par) {
+    for (int64_t e = v.start + c.tid; e < v.end; e += c.nthreads) {
         int32_t a = P.bu[e], bb = P.bv[e];
         bool ma = (P.g.mc[a] & MF_MISBIT) != 0, mb = (P.g.mc[bb] & MF_MISBIT) != 0;
         uint8_t k = 0;
         if (P.bins[e]) k = (ma && mb) ? 1 : 0;
         else k = (ma != mb) ? 2 : 0;
         P.kind[e] = k;
-        if (k) push(P.active, &P.ctr[C_NACT0 + par], (int32_t)e);
+        wave_push(P.active, &P.ctr[C_NACT0 + par], (int32_t)e, k != 0);
     }
 }
 
-__device__ __forceinline__ void mark_vertex(const BP& P, int32_t z, u64 mv, int lane) {
+// Marks the writes of a speculated membership flip of z: z's bit and the
+// counts of its neighbours (evictions also raise the neighbours' decrement
+// bound for this round).  A neighbour whose count is at least 2 + `bound`
+// (bound >= evictions this round) is left unmarked: no read of it can
+// change its outcome this round (every count threshold is at most 2).
+__device__ __forceinline__ void mark_vertex(const BP& P, int32_t z, u64 mv, bool evict, uint32_t R, u64 bound,
+                                            int lane) {
     if (lane == 0) atomicMax(&P.g.mmark[z], mv);
     Row r = get_row(P.g, z);
     int64_t L = r.len();
     for (int64_t j = lane; j < L; j += MF_WAVE) {
         int32_t y = r.at(j);
-        if (y >= 0 && y != z) atomicMax(&P.g.cmark[y], mv);
+        if (y >= 0 && y != z) {
+            if ((u64)(P.g.mc[y] & MF_CMASK) >= 2ull + bound) continue;
+            atomicMax(&P.g.cmark[y], mv);
+            if (evict) round_count_add(&P.g.cdec[y], R);
+        }
     }
 }
 
@@ -615,25 +723,25 @@ __device__ void ph_spec(const BP& P, const GC& c, int par, uint32_t R) {
         int32_t a = P.bu[e], bb = P.bv[e];
         bool dummy = false;
         u64 mv = ((u64)R << 32) | (u64)(0xFFFFFFFFu - (uint32_t)e);
-        if (c.lane == 0) atomicAdd(&P.ctr[C_SPECS], 1ull);
         if (P.kind[e] == 1) {
             int32_t w = a > bb ? a : bb;
             int32_t x = eval_insertion<false>(P, w, e, R, dummy);
             if (c.lane == 0) { P.res_w[e] = w; P.res_x[e] = x; }
-            mark_vertex(P, w, mv, c.lane);
-            if (x >= 0) mark_vertex(P, x, mv, c.lane);
+            mark_vertex(P, w, mv, true, R, (u64)na, c.lane);
+            if (x >= 0) mark_vertex(P, x, mv, false, R, (u64)na, c.lane);
         } else {
             bool ma = (P.g.mc[a] & MF_MISBIT) != 0;
             int32_t tgt = ma ? bb : a, other = ma ? a : bb;
             bool ok = eval_deletion<false>(P, tgt, other, e, R, dummy);
             if (c.lane == 0) { P.res_w[e] = -1; P.res_x[e] = ok ? tgt : -1; }
-            if (ok) mark_vertex(P, tgt, mv, c.lane);
+            if (ok) mark_vertex(P, tgt, mv, false, R, (u64)na, c.lane);
         }
     }
 }
 
-__device__ void ph_validate(const BP& P, const GC& c, int64_t start, int64_t end, int par, uint32_t R) {
-    for (int64_t e = start + c.tid; e < end; e += c.nthreads) {
+__device__ void ph_validate(const BP& P, const GC& c, const RoundView& v, int par) {
+    const uint32_t R = v.R;
+    for (int64_t e = v.start + c.tid; e < v.end; e += c.nthreads) {
         if (P.kind[e]) continue;
         int32_t a = P.bu[e], bb = P.bv[e];
         if (mk_before(P.g.mmark, a, R, (int32_t)e) || mk_before(P.g.mmark, bb, R, (int32_t)e))
@@ -679,6 +787,9 @@ __device__ __forceinline__ int64_t round_conflict(const BP& P, int par) {
     return cf > (u64)INT64_MAX ? INT64_MAX : (int64_t)cf;
 }
 
+// Commits every speculated edge before the first conflict, then (thread 0)
+// advances the round state.  Only thread 0 reads or writes the state here, so
+// the other threads never see a half-advanced round.
 __device__ void ph_commit(const BP& P, const GC& c, int par) {
     int64_t na = (int64_t)P.ctr[C_NACT0 + par];
     const int64_t conflict = round_conflict(P, par);
@@ -689,23 +800,20 @@ __device__ void ph_commit(const BP& P, const GC& c, int par) {
         if (P.kind[e] == 1) {
             int32_t w = P.res_w[e];
             apply_flip(P, w, false, c.lane);
-            if (c.lane == 0) {
-                P.g.evb[w] = P.b;
-                atomicAdd(&P.ctr[C_ACT_INS], 1ull);
-                atomicAdd(&P.ctr[C_EVICT], 1ull);
-                if (x >= 0) atomicAdd(&P.ctr[C_ADMIT_INS], 1ull);
-            }
-        } else if (c.lane == 0) {
-            atomicAdd(&P.ctr[C_ACT_DEL], 1ull);
-            if (x >= 0) atomicAdd(&P.ctr[C_ADMIT_DEL], 1ull);
+            if (c.lane == 0) P.g.evb[w] = P.b;
         }
         if (x >= 0) apply_flip(P, x, true, c.lane);
     }
     if (c.tid == 0) {
-        P.ctr[C_NACT0 + (par ^ 1)] = 0;
-        P.ctr[C_CONF0 + (par ^ 1)] = ~0ull;
-        P.ctr[C_ROUNDS] += 1;
-        P.ctr[C_START] = (uint64_t)(conflict);  // host reads min(conflict, end)
+        RoundView v = round_view(P);
+        if (v.live) {
+            P.ctr[C_NACT0 + (par ^ 1)] = 0;
+            P.ctr[C_CONF0 + (par ^ 1)] = ~0ull;
+            P.ctr[C_ROUNDS] += 1;
+            P.ctr[C_SPECS] += (u64)na;
+            P.ctr[C_START] = (u64)(conflict < v.end ? conflict : v.end);
+            *P.round = v.R;
+        }
     }
 }
 
@@ -788,6 +896,23 @@ __device__ void ph_comp_apply(const BP& P, const GC& c, int par) {
     }
 }
 
+// ---------------------------------------------------------------- stats
+// Decisions of committed edges stay in kind/res (later rounds only classify
+// edges past the committed prefix), so the rule's counters are taken once
+// per batch instead of with per-edge atomics.
+__device__ void ph_stats(const BP& P, const GC& c) {
+    u64 ai = 0, ti = 0, ad = 0, td = 0;
+    for (int64_t e = c.tid; e < P.U; e += c.nthreads) {
+        uint8_t k = P.kind[e];
+        if (k == 1) { ai++; ti += P.res_x[e] >= 0; }
+        else if (k == 2) { ad++; td += P.res_x[e] >= 0; }
+    }
+    if (ai) { atomicAdd(&P.ctr[C_ACT_INS], ai); atomicAdd(&P.ctr[C_EVICT], ai); }
+    if (ti) atomicAdd(&P.ctr[C_ADMIT_INS], ti);
+    if (ad) atomicAdd(&P.ctr[C_ACT_DEL], ad);
+    if (td) atomicAdd(&P.ctr[C_ADMIT_DEL], td);
+}
+
 // ---------------------------------------------------------------- kernels
 __device__ void ph_init(const BP& P, const GC& c) {
     for (int64_t i = c.tid; i < C_NCOUNTERS; i += c.nthreads) P.ctr[i] = 0;
@@ -795,6 +920,16 @@ __device__ void ph_init(const BP& P, const GC& c) {
         P.ctr[C_CONF0] = ~0ull;
         P.ctr[C_CONF1] = ~0ull;
     }
+}
+
+__device__ __forceinline__ GC block_coords() {
+    GC c;
+    c.tid = threadIdx.x;
+    c.nthreads = blockDim.x;
+    c.wave = c.tid / MF_WAVE;
+    c.nwaves = c.nthreads / MF_WAVE;
+    c.lane = lane_id();
+    return c;
 }
 
 #define MF_KERNEL(name, body)                                                   \
@@ -808,6 +943,7 @@ MF_KERNEL(k_claim, ph_claim(P, c))
 MF_KERNEL(k_resolve, ph_resolve(P, c))
 MF_KERNEL(k_group, ph_group(P, c))
 MF_KERNEL(k_append, ph_append(P, c))
+MF_KERNEL(k_partner, ph_partner(P, c))
 MF_KERNEL(k_kcount, ph_kcount(P, c))
 MF_KERNEL(k_pu_touched, ph_pu_touched(P, c))
 MF_KERNEL(k_hasrow, ph_hasrow(P, c))
@@ -816,18 +952,25 @@ MF_KERNEL(k_fa_alloc, ph_fa_alloc(P, c))
 MF_KERNEL(k_fa_fill, ph_fa_fill(P, c))
 MF_KERNEL(k_comp_collect, ph_comp_collect(P, c))
 MF_KERNEL(k_comp_decide, ph_comp_decide(P, c))
+MF_KERNEL(k_stats, ph_stats(P, c))
 
-__global__ __launch_bounds__(256) void k_classify(BP P, int64_t start, int64_t end, int par) {
+// A queued round that finds the batch already done empties its active list,
+// so its commit cannot replay the last live round that used the same slot.
+__global__ __launch_bounds__(256) void k_classify(BP P, int par) {
     GC c = grid_coords();
-    ph_classify(P, c, start, end, par);
+    RoundView v = round_view(P);
+    if (v.live) ph_classify(P, c, v, par);
+    else if (c.tid == 0) P.ctr[C_NACT0 + par] = 0;
 }
-__global__ __launch_bounds__(256) void k_spec(BP P, int par, uint32_t R) {
+__global__ __launch_bounds__(256) void k_spec(BP P, int par) {
     GC c = grid_coords();
-    ph_spec(P, c, par, R);
+    RoundView v = round_view(P);
+    if (v.live) ph_spec(P, c, par, v.R);
 }
-__global__ __launch_bounds__(256) void k_validate(BP P, int64_t start, int64_t end, int par, uint32_t R) {
+__global__ __launch_bounds__(256) void k_validate(BP P, int par) {
     GC c = grid_coords();
-    ph_validate(P, c, start, end, par, R);
+    RoundView v = round_view(P);
+    if (v.live) ph_validate(P, c, v, par);
 }
 __global__ __launch_bounds__(256) void k_commit(BP P, int par) {
     GC c = grid_coords();
@@ -837,82 +980,58 @@ __global__ __launch_bounds__(256) void k_comp_apply(BP P, int par) {
     GC c = grid_coords();
     ph_comp_apply(P, c, par);
 }
-__global__ void k_set_round(BP P, uint32_t R) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *P.round = R;
+
+// Barrier between phases inside one workgroup.  Atomics are performed in L2
+// while a plain load may be served by the CU's L1, and a bare workgroup
+// barrier neither waits for outstanding vector memory operations nor drops
+// L1; the agent-scope release/acquire pair does both, so every phase sees
+// the previous phase's atomics and stores.
+__device__ __forceinline__ void phase_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
-// The whole batch in one cooperative launch: every phase boundary is a grid
-// barrier, so the rounds and completion iterations need no host round trip.
-__global__ __launch_bounds__(256) void k_persistent(BP P) {
-    cg::grid_group grid = cg::this_grid();
-    GC c = grid_coords();
-    ph_init(P, c);
-    grid.sync();
-    ph_claim(P, c);
-    grid.sync();
-    ph_resolve(P, c);
-    grid.sync();
-    ph_group(P, c);
-    grid.sync();
-    ph_append(P, c);
-    grid.sync();
-    ph_kcount(P, c);
-    grid.sync();
-    ph_pu_touched(P, c);
-    grid.sync();
-    ph_hasrow(P, c);
-    grid.sync();
-    ph_fa_count(P, c);
-    grid.sync();
-    ph_fa_alloc(P, c);
-    grid.sync();
-    ph_fa_fill(P, c);
-    grid.sync();
-    const uint32_t R0 = *P.round;
-    uint32_t r = 0;
-    int64_t start = 0;
-    while (start < P.U) {
-        // every round commits at least its first edge, so U rounds suffice
-        if ((int64_t)r > P.U + 1) {
+// The tail of a batch in one workgroup: the remaining speculative rounds and
+// the completion iterations, separated by workgroup barriers (far cheaper
+// than grid-wide synchronisation on a multi-die GPU).  Runs after the queued
+// grid-wide rounds; whatever they left is finished here.
+__global__ __launch_bounds__(1024) void k_finish(BP P, int par) {
+    GC c = block_coords();
+    uint32_t guard = 0;
+    while (true) {
+        phase_barrier();
+        RoundView v = round_view(P);
+        if (!v.live) break;
+        if ((int64_t)guard++ > P.U + 1) {  // every round commits its first edge
             if (c.tid == 0) P.ctr[C_GUARD] = 1;
-            break;
+            return;
         }
-        int par = (int)(r & 1u);
-        uint32_t R = R0 + r + 1;
-        int64_t end = start + P.window;
-        if (end > P.U) end = P.U;
-        ph_classify(P, c, start, end, par);
-        grid.sync();
-        ph_spec(P, c, par, R);
-        grid.sync();
-        ph_validate(P, c, start, end, par, R);
-        grid.sync();
-        const int64_t conflict = round_conflict(P, par);
+        ph_classify(P, c, v, par);
+        phase_barrier();
+        ph_spec(P, c, par, v.R);
+        phase_barrier();
+        ph_validate(P, c, v, par);
+        phase_barrier();
         ph_commit(P, c, par);
-        grid.sync();
-        start = conflict < end ? conflict : end;
-        r++;
+        par ^= 1;
     }
-    if (P.completion) {
-        ph_comp_collect(P, c);
-        grid.sync();
-        uint32_t k = 0;
-        while (true) {
-            int par = (int)(k & 1u);
-            ph_comp_decide(P, c);
-            grid.sync();
-            ph_comp_apply(P, c, par);
-            grid.sync();
-            k++;
-            if (P.ctr[C_UNDEC0 + par] == 0) break;
-            // the smallest open candidate decides every iteration
-            if ((u64)k > P.ctr[C_CAND] + 1) {
-                if (c.tid == 0) P.ctr[C_GUARD] = 2;
-                break;
-            }
+    ph_stats(P, c);
+    if (!P.completion) return;
+    ph_comp_collect(P, c);
+    for (uint32_t k = 0;; k++) {
+        phase_barrier();
+        int cp = (int)(k & 1u);
+        ph_comp_decide(P, c);
+        phase_barrier();
+        ph_comp_apply(P, c, cp);
+        phase_barrier();
+        if (P.ctr[C_UNDEC0 + cp] == 0) break;
+        if ((u64)k > P.ctr[C_CAND] + 1) {  // the smallest open candidate decides each time
+            if (c.tid == 0) P.ctr[C_GUARD] = 2;
+            return;
         }
     }
-    if (c.tid == 0) *P.round = R0 + r;
 }
 
 // ---------------------------------------------------------------- launchers
@@ -924,6 +1043,7 @@ void mf_launch_batch_prologue(const BP& P, const LaunchCfg& L) {
     LAUNCH(k_resolve, P);
     LAUNCH(k_group, P);
     LAUNCH(k_append, P);
+    LAUNCH(k_partner, P);
     LAUNCH(k_kcount, P);
     LAUNCH(k_pu_touched, P);
     LAUNCH(k_hasrow, P);
@@ -932,33 +1052,22 @@ void mf_launch_batch_prologue(const BP& P, const LaunchCfg& L) {
     LAUNCH(k_fa_fill, P);
 }
 
-void mf_launch_round(const BP& P, const LaunchCfg& L, int64_t start, int64_t end, int par, uint32_t R) {
-    LAUNCH(k_classify, P, start, end, par);
-    LAUNCH(k_spec, P, par, R);
-    LAUNCH(k_validate, P, start, end, par, R);
+void mf_launch_round(const BP& P, const LaunchCfg& L, int par) {
+    LAUNCH(k_classify, P, par);
+    LAUNCH(k_spec, P, par);
+    LAUNCH(k_validate, P, par);
     LAUNCH(k_commit, P, par);
+}
+
+void mf_launch_finish(const BP& P, const LaunchCfg& L, int par) {
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, L.stream, P, par);
 }
 
 void mf_launch_completion_collect(const BP& P, const LaunchCfg& L) { LAUNCH(k_comp_collect, P); }
 
+void mf_launch_stats(const BP& P, const LaunchCfg& L) { LAUNCH(k_stats, P); }
+
 void mf_launch_completion_round(const BP& P, const LaunchCfg& L, int par) {
     LAUNCH(k_comp_decide, P);
     LAUNCH(k_comp_apply, P, par);
-}
-
-void mf_launch_set_round(const BP& P, const LaunchCfg& L, uint32_t R) {
-    hipLaunchKernelGGL(k_set_round, dim3(1), dim3(1), 0, L.stream, P, R);
-}
-
-int mf_persistent_grid(int device) {
-    int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_persistent, 256, 0) != hipSuccess) return 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
-    if (per_cu > 2) per_cu = 2;
-    return per_cu * cus;
-}
-
-hipError_t mf_launch_persistent(const BP& P, int grid, hipStream_t stream) {
-    void* args[] = {(void*)&P};
-    return hipLaunchCooperativeKernel((const void*)k_persistent, dim3(grid), dim3(256), args, 0, stream);
 }

@@ -26,22 +26,31 @@ __global__ void k_fill(T* p, T v, int64_t count) {
 }
 
 // flags[0] |= 1: some row not strictly ascending; |= 2: self-loop;
-// |= 4: entry out of range; |= 8: nindex not monotone
-__global__ void k_validate_csr(const int64_t* nindex, const int32_t* nlist, int32_t n, int64_t m, int* flags) {
+// |= 4: entry out of range; |= 8: nindex not monotone.  Offsets are checked
+// per vertex, entries per entry (the row of entry p by binary search), so
+// long rows do not serialise the pass.
+__global__ void k_validate_offsets(const int64_t* nindex, int32_t n, int64_t m, int* flags) {
     for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
         int64_t b = nindex[x], e = nindex[x + 1];
-        if (b > e || b < 0 || e > m) { atomicOr(flags, 8); continue; }
-        int32_t prev = -1;
-        int f = 0;
-        for (int64_t p = b; p < e; p++) {
-            int32_t y = nlist[p];
-            if (y < 0 || y >= n) f |= 4;
-            if (y == (int32_t)x) f |= 2;
-            if (y <= prev) f |= 1;
-            prev = y;
-        }
-        if (f) atomicOr(flags, f);
+        if (b > e || b < 0 || e > m) atomicOr(flags, 8);
     }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && (nindex[0] != 0 || nindex[n] != m)) atomicOr(flags, 8);
+}
+
+__global__ void k_validate_entries(const int64_t* nindex, const int32_t* nlist, int32_t n, int64_t m, int* flags) {
+    int f = 0;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < m; p += (int64_t)gridDim.x * blockDim.x) {
+        int64_t lo = 0, hi = n - 1;  // largest x with nindex[x] <= p
+        while (lo < hi) {
+            int64_t mid = (lo + hi + 1) >> 1;
+            if (nindex[mid] <= p) lo = mid; else hi = mid - 1;
+        }
+        int32_t y = nlist[p];
+        if (y < 0 || y >= n) f |= 4;
+        if (y == (int32_t)lo) f |= 2;
+        if (p > nindex[lo] && nlist[p - 1] >= y) f |= 1;
+    }
+    if (f) atomicOr(flags, f);
 }
 
 __global__ void k_set_mis(uint32_t* mc, const uint8_t* mis, int32_t n) {
@@ -456,7 +465,13 @@ hipError_t mf_widen(const int32_t* d_in, int64_t* d_out, int64_t count, hipStrea
 
 hipError_t mf_validate_csr(GraphState& S, int* d_flags) {
     k_fill<int><<<1, 64, 0, S.stream>>>(d_flags, 0, 1);
-    k_validate_csr<<<S.blocks, 256, 0, S.stream>>>(S.g.nindex, S.g.nlist, S.g.n, S.g.m, d_flags);
+    k_validate_offsets<<<S.blocks, 256, 0, S.stream>>>(S.g.nindex, S.g.n, S.g.m, d_flags);
+    // entries are only inspected when the offsets are sane (host checks flag 8 first)
+    return hipGetLastError();
+}
+
+hipError_t mf_validate_entries(GraphState& S, int* d_flags) {
+    k_validate_entries<<<S.blocks, 256, 0, S.stream>>>(S.g.nindex, S.g.nlist, S.g.n, S.g.m, d_flags);
     return hipGetLastError();
 }
 

@@ -12,7 +12,7 @@ from tests.conftest import SAMPLE
 
 pytestmark = pytest.mark.gpu
 
-MODES = [0, 1]  # plain, persistent
+MODES = [0, 1]  # plain, queued
 
 
 @pytest.fixture(scope="module")
