@@ -13,9 +13,13 @@ the graph resident in HBM.  Two drivers run the same stream from the same
 start state:
   plain   one batch at a time, host-driven speculative rounds (the
           reference's synchronous engine pattern);
-  served  one cooperative launch per batch, batches queued back to back with
-          host staging overlapped (pipeline depth 2, the reference's async
-          pattern generalised).
+  served  the queued driver: each batch is one stream-ordered sequence
+          (prologue kernels, a fixed number of grid-wide round kernels that
+          turn into no-ops once the batch is resolved, and a one-workgroup
+          finisher that runs any remaining rounds and the completion) with
+          no host synchronisation inside it; batches are queued back to back
+          with host staging overlapped (pipeline depth 2, the reference's
+          async pattern generalised).
 `value` is the served throughput.  Both drivers must end with the same MIS,
 the full check (independence, maximality, count consistency) runs after
 every plain batch, and a bounded sample is replayed by the CPU oracle (the
@@ -35,6 +39,20 @@ sys.path.insert(0, ROOT)
 
 METRIC = "edge updates/s (MIS bit-exact) + p50/p99 batch latency, served vs plain"
 HBM_PEAK_GBS = 8000.0
+
+
+def stream_copy_gbs(mib=2048, reps=20):
+    """Copy bandwidth of the STREAM-style calibration program (read + write
+    bytes per second), run as a child process before the engine starts."""
+    import subprocess
+    exe = os.path.join(ROOT, "misfit_amd", "bin", "stream_copy")
+    if not os.path.exists(exe):
+        return None
+    r = subprocess.run([exe, str(mib), str(reps)], capture_output=True, text=True, timeout=120)
+    if r.returncode != 0:
+        print("stream_copy failed: %s" % r.stderr.strip(), file=sys.stderr, flush=True)
+        return None
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
 
 def pct(v, q):
@@ -149,11 +167,15 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--served-only", action="store_true", help="skip the plain driver (profiling runs)")
+    ap.add_argument("--no-stream-copy", action="store_true", help="skip the copy-bandwidth calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; --gpus %d is not supported" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+    copy = None if args.no_stream_copy else stream_copy_gbs()
+    if copy:
+        print("stream copy: %.1f GB/s" % copy["copy_gbs"], file=sys.stderr, flush=True)
     import misfit_amd as M
 
     W, K, B = args.warmup, args.steps, args.batch
@@ -192,7 +214,9 @@ def main():
                 "kernel": "queued batch (prologue, grid rounds, finisher) timed per batch with HIP events",
                 "algorithmic_bytes_per_batch": round(tot_bytes / K, 1),
                 "algorithmic_bytes_per_update": round(tot_bytes / (K * B), 2),
-                "kernel_ms_mean": round(float(np.mean(served["dev_ms"])), 4)}
+                "kernel_ms_mean": round(float(np.mean(served["dev_ms"])), 4),
+                "stream_copy_gbs": copy["copy_gbs"] if copy else None,
+                "frac_of_stream_copy": round(ach / copy["copy_gbs"], 5) if copy else None}
     cfg = {
         "workload": "R-MAT scale 22 (4,194,304 vertices, edgefactor 16, a,b,c,d=0.57,0.19,0.19,0.05, "
                     "symmetrised, deduplicated, seed %d), greedy ascending-id MIS, %d-update batches "

@@ -8,6 +8,10 @@
 // vertices are few, so every row uses the index.
 #define MF_FA_T 0
 
+// Rows of members that may be evicted are counted in chunks of this many
+// entries, one wave per chunk, so hub rows do not serialise on one wave.
+#define MF_KCH 512
+
 enum BatchCounter {
     C_SEEDS = 0,     // distinct endpoints
     C_WPOT,          // distinct max(u,v) of insertions
@@ -33,8 +37,9 @@ enum BatchCounter {
     C_NACT0, C_NACT1,
     C_CONF0, C_CONF1,
     C_UNDEC0, C_UNDEC1,
-    C_START,
-    C_GUARD,         // a loop bound was hit (engine bug): results invalid
+    C_START0, C_START1,  // first uncommitted edge, read by the round of that parity
+    C_KTASK,         // row chunks of members that may be evicted
+    C_GUARD,         // a loop bound was hit or a list overflowed (engine bug): results invalid
     C_NCOUNTERS
 };
 
@@ -47,6 +52,7 @@ struct BP {
     uint32_t b;        // batch epoch, >= 1
     int completion;
     int64_t window;    // max edges speculated per round
+    int64_t n_ins;     // insertion updates in the batch (bounds evictions per round)
     // pair table of this batch
     u64* bt_key;
     int32_t* bt_head;
@@ -63,6 +69,8 @@ struct BP {
     int32_t* wpot;
     int32_t* touched;
     int64_t touched_cap;
+    u64* ktask;        // w<<32 | chunk
+    int64_t ktask_cap;
     int32_t* pu;
     int64_t pu_cap;
     int32_t* big;
@@ -80,6 +88,8 @@ struct BP {
     uint8_t* cdec;
     u64* ctr;
     uint32_t* round;   // global speculative-round counter (device-resident)
+    u64* h_out;        // pinned host copy of the counters + pool top, written
+                       // by the batch's last kernel (C_NCOUNTERS + 1 words)
 };
 
 // host-side launchers (k_batch.hip)
@@ -93,3 +103,4 @@ void mf_launch_finish(const BP& P, const LaunchCfg& L, int par);
 void mf_launch_completion_collect(const BP& P, const LaunchCfg& L);
 void mf_launch_stats(const BP& P, const LaunchCfg& L);
 void mf_launch_completion_round(const BP& P, const LaunchCfg& L, int par);
+void mf_launch_publish(const BP& P, const LaunchCfg& L);

@@ -61,6 +61,8 @@ struct misfit_ctx {
     u64 bt_size = 0;
     int64_t fa_cap = 0;
     int64_t ph_cap = 0;
+    u64* d_ktask = nullptr;    // row-chunk tasks, sized for the graph plus the pool
+    int64_t ktask_cap = 0;
     int64_t ins_bound = 0;    // inserted pairs since the overlay was last cleared (upper bound)
     int64_t pool_bound = 0;   // pool slots possibly used (upper bound)
     // owned device memory
@@ -149,7 +151,8 @@ extern "C" int misfit_ctx_create(int device, misfit_ctx** out) {
         hipEventCreateWithFlags(&s.done, hipEventDisableTiming);
         hipEventCreate(&s.t0);
         hipEventCreate(&s.t1);
-        hipHostMalloc((void**)&s.h_ctr, (C_NCOUNTERS + 1) * sizeof(u64), hipHostMallocDefault);
+        // written by the batch's last kernel, read after its event
+        hipHostMalloc((void**)&s.h_ctr, (C_NCOUNTERS + 1) * sizeof(u64), hipHostMallocCoherent);
     }
     *out = c;
     return MISFIT_OK;
@@ -181,6 +184,7 @@ extern "C" int misfit_ctx_destroy(misfit_ctx* c) {
     }
     free_list(c->batch_allocs);
     free_list(c->graph_allocs);
+    if (c->d_ktask) hipFree(c->d_ktask);
     if (c->h_small) hipHostFree(c->h_small);
     hipStreamDestroy(c->stream);
     hipStreamDestroy(c->copy_stream);
@@ -561,11 +565,10 @@ static int drive_plain(misfit_ctx* c, BP& P) {
         mf_launch_round(P, L, par);
         HIPCHK(c, hipMemcpyAsync(c->h_small, P.ctr, C_NCOUNTERS * sizeof(u64), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
-        int64_t start = (int64_t)c->h_small[C_START];
+        int64_t start = (int64_t)c->h_small[C_START0 + (par ^ 1)];
         if (c->debug_rounds) {
             u64 cf = c->h_small[C_CONF0 + par];
-            fprintf(stderr, "[rounds] batch %u round %u: active %llu, conflict %lld, next start %lld / %lld\n",
-                    P.b, r, (unsigned long long)c->h_small[C_NACT0 + par],
+            fprintf(stderr, "[rounds] batch %u round %u: conflict %lld, next start %lld / %lld\n", P.b, r,
                     cf > (u64)INT64_MAX ? -1LL : (long long)cf, (long long)start, (long long)P.U);
         }
         if (start >= P.U) break;
@@ -583,6 +586,7 @@ static int drive_plain(misfit_ctx* c, BP& P) {
             if (k > (uint32_t)P.g.n + 1) return fail(c, MISFIT_E_CHECK, "completion made no progress");
         }
     }
+    mf_launch_publish(P, L);
     HIPCHK(c, hipGetLastError());
     return MISFIT_OK;
 }
@@ -595,6 +599,21 @@ static int drive_queued(misfit_ctx* c, BP& P) {
     for (int r = 0; r < k; r++) mf_launch_round(P, L, r & 1);
     mf_launch_finish(P, L, k & 1);
     HIPCHK(c, hipGetLastError());
+    return MISFIT_OK;
+}
+
+// Row-chunk task list: one task per MF_KCH entries of every row of a
+// possibly evicted member, at most U + (base entries + pool slots) / MF_KCH.
+static int ensure_ktask_capacity(misfit_ctx* c, int64_t U) {
+    int64_t need = U + (c->P.g.m + c->P.g.pool_cap) / MF_KCH + 2;
+    if (need <= c->ktask_cap) return MISFIT_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->d_ktask) hipFree(c->d_ktask);
+    c->d_ktask = nullptr;
+    c->ktask_cap = 0;
+    int64_t cap = need + need / 2;
+    HIPCHK(c, hipMalloc((void**)&c->d_ktask, cap * sizeof(u64)));
+    c->ktask_cap = cap;
     return MISFIT_OK;
 }
 
@@ -613,6 +632,8 @@ extern "C" int misfit_submit_batch(misfit_ctx* c, const misfit_update* ups, int6
     int rc = ensure_batch_capacity(c, count);
     if (rc) return rc;
     rc = ensure_overlay_capacity(c, nins);
+    if (rc) return rc;
+    rc = ensure_ktask_capacity(c, count);
     if (rc) return rc;
     uint64_t t = c->next_ticket++;
     Slot& s = c->slots[t % MF_NSLOT];
@@ -644,14 +665,17 @@ extern "C" int misfit_submit_batch(misfit_ctx* c, const misfit_update* ups, int6
     P.b = c->epoch;
     P.completion = c->completion;
     P.window = std::max<int64_t>(count, 1);
+    P.n_ins = nins;
+    P.ktask = c->d_ktask;
+    P.ktask_cap = c->ktask_cap;
+    P.h_out = s.h_ctr;
     c->last_U = count;
     HIPCHK(c, hipEventRecord(s.t0, c->stream));
     rc = c->mode == MISFIT_MODE_QUEUED ? drive_queued(c, P) : drive_plain(c, P);
     if (rc) return rc;
+    // the counters and the pool top reached s.h_ctr from the batch's last
+    // kernel (the finisher, or the publish kernel of the plain driver)
     HIPCHK(c, hipEventRecord(s.t1, c->stream));
-    HIPCHK(c, hipMemcpyAsync(s.h_ctr, P.ctr, C_NCOUNTERS * sizeof(u64), hipMemcpyDeviceToHost, c->stream));
-    // the pool top rides after the counters in the readback
-    HIPCHK(c, hipMemcpyAsync(s.h_ctr + C_NCOUNTERS, P.g.pool_top, sizeof(u64), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipEventRecord(s.done, c->stream));
     if (ticket) *ticket = t;
     return MISFIT_OK;
@@ -674,7 +698,7 @@ extern "C" int misfit_wait(misfit_ctx* c, uint64_t ticket, misfit_batch_stats* s
         c->grid_rounds = std::min(64, mx + 2);
     }
     if (s.h_ctr[C_POOL_OVF]) return fail(c, MISFIT_E_CHECK, "overlay pool overflow");
-    if (s.h_ctr[C_GUARD]) return fail(c, MISFIT_E_CHECK, "repair loop bound exceeded");
+    if (s.h_ctr[C_GUARD]) return fail(c, MISFIT_E_CHECK, "repair loop bound exceeded or task list overflow");
     if (st) fill_stats(c, s, st);
     return MISFIT_OK;
 }

@@ -164,6 +164,35 @@ __device__ __forceinline__ int64_t base_find(const DevGraph& g, int32_t x, int32
     return -1;
 }
 
+// Position of y inside base row x and of x inside base row y (or -1): the
+// two binary searches advance in lockstep so their dependent load chains
+// overlap.
+__device__ __forceinline__ void base_find_pair(const DevGraph& g, int32_t x, int32_t y, int64_t& px, int64_t& py) {
+    if (!g.sorted) {
+        px = base_find(g, x, y);
+        py = base_find(g, y, x);
+        return;
+    }
+    int64_t bx = g.nindex[x], ex = g.nindex[x + 1];
+    int64_t by = g.nindex[y], ey = g.nindex[y + 1];
+    int64_t lx = bx, hx = ex, ly = by, hy = ey;
+    while (lx < hx || ly < hy) {
+        int64_t mx = (lx + hx) >> 1, my = (ly + hy) >> 1;
+        int32_t dx = lx < hx ? decode(g.nlist[mx]) : 0;
+        int32_t dy = ly < hy ? decode(g.nlist[my]) : 0;
+        if (lx < hx) {
+            if (dx < y) lx = mx + 1;
+            else hx = mx;
+        }
+        if (ly < hy) {
+            if (dy < x) ly = my + 1;
+            else hy = my;
+        }
+    }
+    px = (lx < ex && decode(g.nlist[lx]) == y) ? lx : -1;
+    py = (ly < ey && decode(g.nlist[ly]) == x) ? ly : -1;
+}
+
 // Adjacency in the current graph (base minus deletions plus insertions): the
 // device form of HybridGraph::isAdjacent (core/FlashGraph.h:291-301).  The
 // base part searches the shorter of the two rows (rows are symmetric).

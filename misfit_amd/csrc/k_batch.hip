@@ -173,7 +173,8 @@ __device__ void ph_resolve(const BP& P, const GC& c) {
         if (P.bt_head[slot] != (int32_t)i) continue;
         int32_t a = P.bu[i], bb = P.bv[i];
         int32_t lo = a < bb ? a : bb, hi = a < bb ? bb : a;
-        int64_t plo = base_find(g, lo, hi);
+        int64_t plo, phi;
+        base_find_pair(g, lo, hi, plo, phi);
         bool in_base = plo >= 0;
         bool tomb0 = in_base && g.nlist[plo] < 0;
         u64 key = pair_key(lo, hi);
@@ -212,7 +213,6 @@ __device__ void ph_resolve(const BP& P, const GC& c) {
         if (ops) atomicAdd(&P.ctr[C_DELTA_OPS], (u64)ops);
         if (in_base && tomb != tomb0) {
             set_base_deleted(g, lo, plo, tomb);
-            int64_t phi = base_find(g, hi, lo);
             if (phi >= 0) set_base_deleted(g, hi, phi, tomb);
         }
         if (old_killed) {
@@ -290,7 +290,8 @@ __device__ void ph_append(const BP& P, const GC& c) {
     }
 }
 
-// P5a: flag members of W that could become uncovered after an eviction.
+// P5a: flag members of W that could become uncovered after an eviction, and
+// cut the rows of W's MIS members into chunks for the counting pass.
 __device__ void ph_partner(const BP& P, const GC& c) {
     for (int64_t i = c.tid; i < P.U; i += c.nthreads) {
         if (!P.bins[i]) continue;
@@ -299,20 +300,42 @@ __device__ void ph_partner(const BP& P, const GC& c) {
         if (P.g.pfb[w] == P.b) continue;
         if (P.g.wpotb[o] == P.b || !adjacent(P.g, w, o)) P.g.pfb[w] = P.b;
     }
+    int64_t nw = (int64_t)P.ctr[C_WPOT];
+    for (int64_t i = c.tid; i < nw; i += c.nthreads) {
+        int32_t w = P.wpot[i];
+        if (!(P.g.mc[w] & MF_MISBIT)) continue;
+        int64_t L = row_len(P.g, w);
+        if (!L) continue;
+        int64_t nch = (L + MF_KCH - 1) / MF_KCH;
+        u64 t = atomicAdd(&P.ctr[C_KTASK], (u64)nch);
+        if ((int64_t)(t + nch) > P.ktask_cap) {
+            atomicExch(&P.ctr[C_GUARD], 3ull);
+            continue;
+        }
+        for (int64_t k = 0; k < nch; k++) P.ktask[t + k] = ((u64)(uint32_t)w << 32) | (u64)k;
+    }
 }
 
-// P5: k(z) for neighbours of MIS vertices that may be evicted; flagged
-// members of W join PU; every currently uncovered vertex joins PU.
+// P5: k(z) for neighbours of MIS vertices that may be evicted (one wave per
+// row chunk); flagged members of W join PU; every currently uncovered vertex
+// joins PU.
 __device__ void ph_kcount(const BP& P, const GC& c) {
     const DevGraph& g = P.g;
     int64_t nw = (int64_t)P.ctr[C_WPOT];
-    for (int64_t i = c.wave; i < nw; i += c.nwaves) {
+    for (int64_t i = c.tid; i < nw; i += c.nthreads) {
         int32_t w = P.wpot[i];
-        if (!(g.mc[w] & MF_MISBIT)) continue;  // non-members: c = k test below
-        if (c.lane == 0 && g.pfb[w] == P.b) pu_add(P, w);
+        if ((g.mc[w] & MF_MISBIT) && g.pfb[w] == P.b) pu_add(P, w);
+    }
+    int64_t nt = (int64_t)P.ctr[C_KTASK];
+    if (nt > P.ktask_cap) nt = P.ktask_cap;
+    for (int64_t i = c.wave; i < nt; i += c.nwaves) {
+        u64 task = P.ktask[i];
+        int32_t w = (int32_t)(task >> 32);
+        int64_t j0 = (int64_t)(uint32_t)task * MF_KCH;
         Row r = get_row(g, w);
         int64_t L = r.len();
-        for (int64_t j = c.lane; j < L; j += MF_WAVE) {
+        int64_t j1 = j0 + MF_KCH < L ? j0 + MF_KCH : L;
+        for (int64_t j = j0 + c.lane; j < j1; j += MF_WAVE) {
             int32_t z = r.at(j);
             if (z < 0 || z == w) continue;
             // k(z) <= |W|: a vertex with more MIS neighbours can never reach
@@ -665,35 +688,32 @@ __device__ bool eval_deletion(const BP& P, int32_t tgt, int32_t other, int32_t e
 }
 
 // ---------------------------------------------------------------- rounds
-// The round state lives on the device: ctr[C_START] is the first edge not
-// yet committed and *P.round the id of the last round.  A round whose start
-// is at or past U does nothing, so rounds can be queued without knowing how
-// many the batch needs; only the commit phase advances the state.
+// The round state lives on the device: ctr[C_START0 + par] is the first edge
+// not yet committed, read by the rounds of parity par, and *P.round the id of
+// the last round.  A round whose start is at or past U does nothing, so
+// rounds can be queued without knowing how many the batch needs; only the
+// commit phase advances the state, and it writes the next round's start into
+// the other parity's word, so no wave of the committing kernel can see a
+// half-advanced round.
 struct RoundView {
     int64_t start, end;
     uint32_t R;
     bool live;
 };
 
-__device__ __forceinline__ RoundView round_view(const BP& P) {
+__device__ __forceinline__ RoundView round_view(const BP& P, int par) {
     RoundView v;
-    v.start = (int64_t)P.ctr[C_START];
+    v.start = (int64_t)P.ctr[C_START0 + par];
     v.end = v.start + P.window < P.U ? v.start + P.window : P.U;
     v.R = *P.round + 1u;
     v.live = v.start < P.U;
     return v;
 }
 
-__device__ void ph_classify(const BP& P, const GC& c, const RoundView& v, int par) {
-    for (int64_t e = v.start + c.tid; e < v.end; e += c.nthreads) {
-        int32_t a = P.bu[e], bb = P.bv[e];
-        bool ma = (P.g.mc[a] & MF_MISBIT) != 0, mb = (P.g.mc[bb] & MF_MISBIT) != 0;
-        uint8_t k = 0;
-        if (P.bins[e]) k = (ma && mb) ? 1 : 0;
-        else k = (ma != mb) ? 2 : 0;
-        P.kind[e] = k;
-        wave_push(P.active, &P.ctr[C_NACT0 + par], (int32_t)e, k != 0);
-    }
+__device__ __forceinline__ uint8_t classify_edge(const BP& P, int64_t e, int32_t a, int32_t bb) {
+    bool ma = (P.g.mc[a] & MF_MISBIT) != 0, mb = (P.g.mc[bb] & MF_MISBIT) != 0;
+    if (P.bins[e]) return (ma && mb) ? 1 : 0;
+    return (ma != mb) ? 2 : 0;
 }
 
 // Marks the writes of a speculated membership flip of z: z's bit and the
@@ -716,52 +736,40 @@ __device__ __forceinline__ void mark_vertex(const BP& P, int32_t z, u64 mv, bool
     }
 }
 
-__device__ void ph_spec(const BP& P, const GC& c, int par, uint32_t R) {
-    int64_t na = (int64_t)P.ctr[C_NACT0 + par];
-    for (int64_t i = c.wave; i < na; i += c.nwaves) {
-        int32_t e = P.active[i];
-        int32_t a = P.bu[e], bb = P.bv[e];
-        bool dummy = false;
-        u64 mv = ((u64)R << 32) | (u64)(0xFFFFFFFFu - (uint32_t)e);
-        if (P.kind[e] == 1) {
-            int32_t w = a > bb ? a : bb;
-            int32_t x = eval_insertion<false>(P, w, e, R, dummy);
-            if (c.lane == 0) { P.res_w[e] = w; P.res_x[e] = x; }
-            mark_vertex(P, w, mv, true, R, (u64)na, c.lane);
-            if (x >= 0) mark_vertex(P, x, mv, false, R, (u64)na, c.lane);
-        } else {
-            bool ma = (P.g.mc[a] & MF_MISBIT) != 0;
-            int32_t tgt = ma ? bb : a, other = ma ? a : bb;
-            bool ok = eval_deletion<false>(P, tgt, other, e, R, dummy);
-            if (c.lane == 0) { P.res_w[e] = -1; P.res_x[e] = ok ? tgt : -1; }
-            if (ok) mark_vertex(P, tgt, mv, false, R, (u64)na, c.lane);
-        }
+// Speculative evaluation of active edge e (kind k) by one wave.
+__device__ __forceinline__ void spec_edge(const BP& P, int32_t e, uint8_t k, uint32_t R, u64 bound, int lane) {
+    int32_t a = P.bu[e], bb = P.bv[e];
+    bool dummy = false;
+    u64 mv = ((u64)R << 32) | (u64)(0xFFFFFFFFu - (uint32_t)e);
+    if (k == 1) {
+        int32_t w = a > bb ? a : bb;
+        int32_t x = eval_insertion<false>(P, w, e, R, dummy);
+        if (lane == 0) { P.res_w[e] = w; P.res_x[e] = x; }
+        mark_vertex(P, w, mv, true, R, bound, lane);
+        if (x >= 0) mark_vertex(P, x, mv, false, R, bound, lane);
+    } else {
+        bool ma = (P.g.mc[a] & MF_MISBIT) != 0;
+        int32_t tgt = ma ? bb : a, other = ma ? a : bb;
+        bool ok = eval_deletion<false>(P, tgt, other, e, R, dummy);
+        if (lane == 0) { P.res_w[e] = -1; P.res_x[e] = ok ? tgt : -1; }
+        if (ok) mark_vertex(P, tgt, mv, false, R, bound, lane);
     }
 }
 
-__device__ void ph_validate(const BP& P, const GC& c, const RoundView& v, int par) {
-    const uint32_t R = v.R;
-    for (int64_t e = v.start + c.tid; e < v.end; e += c.nthreads) {
-        if (P.kind[e]) continue;
-        int32_t a = P.bu[e], bb = P.bv[e];
-        if (mk_before(P.g.mmark, a, R, (int32_t)e) || mk_before(P.g.mmark, bb, R, (int32_t)e))
-            atomicMin(&P.ctr[C_CONF0 + par], (u64)e);
+// Re-walk of active edge e: true when it read something an earlier edge of
+// the round may have changed.  Wave-uniform.
+__device__ __forceinline__ bool validate_edge(const BP& P, int32_t e, uint8_t k, uint32_t R) {
+    int32_t a = P.bu[e], bb = P.bv[e];
+    bool conf = mk_before(P.g.mmark, a, R, e) || mk_before(P.g.mmark, bb, R, e);
+    if (k == 1) {
+        int32_t w = a > bb ? a : bb;
+        (void)eval_insertion<true>(P, w, e, R, conf);
+    } else {
+        bool ma = (P.g.mc[a] & MF_MISBIT) != 0;
+        int32_t tgt = ma ? bb : a, other = ma ? a : bb;
+        (void)eval_deletion<true>(P, tgt, other, e, R, conf);
     }
-    int64_t na = (int64_t)P.ctr[C_NACT0 + par];
-    for (int64_t i = c.wave; i < na; i += c.nwaves) {
-        int32_t e = P.active[i];
-        int32_t a = P.bu[e], bb = P.bv[e];
-        bool conf = mk_before(P.g.mmark, a, R, e) || mk_before(P.g.mmark, bb, R, e);
-        if (P.kind[e] == 1) {
-            int32_t w = a > bb ? a : bb;
-            (void)eval_insertion<true>(P, w, e, R, conf);
-        } else {
-            bool ma = (P.g.mc[a] & MF_MISBIT) != 0;
-            int32_t tgt = ma ? bb : a, other = ma ? a : bb;
-            (void)eval_deletion<true>(P, tgt, other, e, R, conf);
-        }
-        if (__ballot(conf) && c.lane == 0) atomicMin(&P.ctr[C_CONF0 + par], (u64)e);
-    }
+    return __ballot(conf) != 0;
 }
 
 __device__ __forceinline__ void apply_flip(const BP& P, int32_t z, bool join, int lane) {
@@ -780,6 +788,16 @@ __device__ __forceinline__ void apply_flip(const BP& P, int32_t z, bool join, in
     }
 }
 
+__device__ __forceinline__ void commit_edge(const BP& P, int32_t e, uint8_t k, int lane) {
+    int32_t x = P.res_x[e];
+    if (k == 1) {
+        int32_t w = P.res_w[e];
+        apply_flip(P, w, false, lane);
+        if (lane == 0) P.g.evb[w] = P.b;
+    }
+    if (x >= 0) apply_flip(P, x, true, lane);
+}
+
 // First conflicting edge of the round (INT64_MAX when none): the counter
 // holds ~0 until some edge conflicts.
 __device__ __forceinline__ int64_t round_conflict(const BP& P, int par) {
@@ -787,33 +805,101 @@ __device__ __forceinline__ int64_t round_conflict(const BP& P, int par) {
     return cf > (u64)INT64_MAX ? INT64_MAX : (int64_t)cf;
 }
 
-// Commits every speculated edge before the first conflict, then (thread 0)
-// advances the round state.  Only thread 0 reads or writes the state here, so
-// the other threads never see a half-advanced round.
-__device__ void ph_commit(const BP& P, const GC& c, int par) {
+// Round bookkeeping after a commit (one thread): the next round (parity
+// par ^ 1) starts at the first conflict.  A dead round carries the start
+// over unchanged.
+__device__ __forceinline__ void advance_round(const BP& P, int par, int64_t conflict) {
+    RoundView v = round_view(P, par);
+    P.ctr[C_NACT0 + (par ^ 1)] = 0;
+    P.ctr[C_CONF0 + (par ^ 1)] = ~0ull;
+    if (v.live) {
+        P.ctr[C_ROUNDS] += 1;
+        P.ctr[C_SPECS] += (u64)(v.end - v.start);
+        P.ctr[C_START0 + (par ^ 1)] = (u64)(conflict < v.end ? conflict : v.end);
+        *P.round = v.R;
+    } else {
+        P.ctr[C_START0 + (par ^ 1)] = (u64)v.start;
+    }
+}
+
+// Grid-wide rounds: one wave per edge of the window.  Speculation classifies
+// the edge first (the kind depends only on the committed state), so a round
+// is three kernels: speculate, validate, commit.
+__device__ void ph_spec_w(const BP& P, const GC& c, const RoundView& v) {
+    for (int64_t e = v.start + c.wave; e < v.end; e += c.nwaves) {
+        int32_t a = P.bu[e], bb = P.bv[e];
+        uint8_t k = classify_edge(P, e, a, bb);
+        if (c.lane == 0) P.kind[e] = k;
+        if (k) spec_edge(P, (int32_t)e, k, v.R, (u64)P.n_ins, c.lane);
+    }
+}
+
+__device__ void ph_validate_w(const BP& P, const GC& c, const RoundView& v, int par) {
+    for (int64_t e = v.start + c.wave; e < v.end; e += c.nwaves) {
+        // an edge past a conflict already found cannot move the minimum
+        if ((u64)e > P.ctr[C_CONF0 + par]) break;
+        uint8_t k = P.kind[e];
+        bool conf;
+        if (!k) {  // its classification read both endpoints' membership
+            int32_t a = P.bu[e], bb = P.bv[e];
+            conf = mk_before(P.g.mmark, a, v.R, (int32_t)e) || mk_before(P.g.mmark, bb, v.R, (int32_t)e);
+        } else {
+            conf = validate_edge(P, (int32_t)e, k, v.R);
+        }
+        if (conf && c.lane == 0) atomicMin(&P.ctr[C_CONF0 + par], (u64)e);
+    }
+}
+
+__device__ void ph_commit_w(const BP& P, const GC& c, const RoundView& v, int par) {
+    const int64_t conflict = round_conflict(P, par);
+    const int64_t lim = conflict < v.end ? conflict : v.end;
+    for (int64_t e = v.start + c.wave; e < lim; e += c.nwaves) {
+        uint8_t k = P.kind[e];
+        if (k) commit_edge(P, (int32_t)e, k, c.lane);
+    }
+    if (c.tid == 0) advance_round(P, par, conflict);
+}
+
+// One-workgroup rounds (finisher): classification by thread into an active
+// list, then a wave per active edge, which suits the short tails it gets.
+__device__ void ph_classify_l(const BP& P, const GC& c, const RoundView& v, int par) {
+    for (int64_t e = v.start + c.tid; e < v.end; e += c.nthreads) {
+        int32_t a = P.bu[e], bb = P.bv[e];
+        uint8_t k = classify_edge(P, e, a, bb);
+        P.kind[e] = k;
+        wave_push(P.active, &P.ctr[C_NACT0 + par], (int32_t)e, k != 0);
+    }
+}
+
+__device__ void ph_spec_l(const BP& P, const GC& c, int par, uint32_t R) {
+    int64_t na = (int64_t)P.ctr[C_NACT0 + par];
+    for (int64_t i = c.wave; i < na; i += c.nwaves) {
+        int32_t e = P.active[i];
+        spec_edge(P, e, P.kind[e], R, (u64)na, c.lane);
+    }
+}
+
+__device__ void ph_validate_l(const BP& P, const GC& c, const RoundView& v, int par) {
+    const uint32_t R = v.R;
+    for (int64_t e = v.start + c.tid; e < v.end; e += c.nthreads) {
+        if (P.kind[e]) continue;
+        int32_t a = P.bu[e], bb = P.bv[e];
+        if (mk_before(P.g.mmark, a, R, (int32_t)e) || mk_before(P.g.mmark, bb, R, (int32_t)e))
+            atomicMin(&P.ctr[C_CONF0 + par], (u64)e);
+    }
+    int64_t na = (int64_t)P.ctr[C_NACT0 + par];
+    for (int64_t i = c.wave; i < na; i += c.nwaves) {
+        int32_t e = P.active[i];
+        if (validate_edge(P, e, P.kind[e], R) && c.lane == 0) atomicMin(&P.ctr[C_CONF0 + par], (u64)e);
+    }
+}
+
+__device__ void ph_commit_l(const BP& P, const GC& c, int par) {
     int64_t na = (int64_t)P.ctr[C_NACT0 + par];
     const int64_t conflict = round_conflict(P, par);
     for (int64_t i = c.wave; i < na; i += c.nwaves) {
         int32_t e = P.active[i];
-        if (e >= conflict) continue;
-        int32_t x = P.res_x[e];
-        if (P.kind[e] == 1) {
-            int32_t w = P.res_w[e];
-            apply_flip(P, w, false, c.lane);
-            if (c.lane == 0) P.g.evb[w] = P.b;
-        }
-        if (x >= 0) apply_flip(P, x, true, c.lane);
-    }
-    if (c.tid == 0) {
-        RoundView v = round_view(P);
-        if (v.live) {
-            P.ctr[C_NACT0 + (par ^ 1)] = 0;
-            P.ctr[C_CONF0 + (par ^ 1)] = ~0ull;
-            P.ctr[C_ROUNDS] += 1;
-            P.ctr[C_SPECS] += (u64)na;
-            P.ctr[C_START] = (u64)(conflict < v.end ? conflict : v.end);
-            *P.round = v.R;
-        }
+        if (e < conflict) commit_edge(P, e, P.kind[e], c.lane);
     }
 }
 
@@ -847,12 +933,13 @@ __device__ void ph_comp_collect(const BP& P, const GC& c) {
 }
 
 // A candidate joins when no open candidate neighbour has a smaller id
-// (sequential greedy in ascending id, decided in parallel rounds).
+// (sequential greedy in ascending id, decided in parallel rounds).  One wave
+// per candidate, so a long row is scanned 64 entries at a time.
 __device__ void ph_comp_decide(const BP& P, const GC& c) {
     const DevGraph& g = P.g;
     int64_t nc = (int64_t)P.ctr[C_CAND];
     const uint32_t open = (P.b << 2) | 1u;
-    for (int64_t i = c.tid; i < nc; i += c.nthreads) {
+    for (int64_t i = c.wave; i < nc; i += c.nwaves) {
         int32_t z = P.cand[i];
         uint8_t d = 0;
         if (g.cstat[z] == open) {
@@ -861,13 +948,18 @@ __device__ void ph_comp_decide(const BP& P, const GC& c) {
                 d = 2;
                 Row r = get_row(g, z);
                 int64_t L = r.len();
-                for (int64_t j = 0; j < L; j++) {
-                    int32_t y = r.at(j);
-                    if (y >= 0 && y < z && g.cstat[y] == open) { d = 0; break; }
+                for (int64_t j0 = 0; j0 < L; j0 += MF_WAVE) {
+                    int64_t j = j0 + c.lane;
+                    bool sm = false;
+                    if (j < L) {
+                        int32_t y = r.at(j);
+                        sm = y >= 0 && y < z && g.cstat[y] == open;
+                    }
+                    if (__ballot(sm)) { d = 0; break; }
                 }
             }
         }
-        P.cdec[i] = d;
+        if (c.lane == 0) P.cdec[i] = d;
     }
 }
 
@@ -954,27 +1046,21 @@ MF_KERNEL(k_comp_collect, ph_comp_collect(P, c))
 MF_KERNEL(k_comp_decide, ph_comp_decide(P, c))
 MF_KERNEL(k_stats, ph_stats(P, c))
 
-// A queued round that finds the batch already done empties its active list,
-// so its commit cannot replay the last live round that used the same slot.
-__global__ __launch_bounds__(256) void k_classify(BP P, int par) {
-    GC c = grid_coords();
-    RoundView v = round_view(P);
-    if (v.live) ph_classify(P, c, v, par);
-    else if (c.tid == 0) P.ctr[C_NACT0 + par] = 0;
-}
 __global__ __launch_bounds__(256) void k_spec(BP P, int par) {
     GC c = grid_coords();
-    RoundView v = round_view(P);
-    if (v.live) ph_spec(P, c, par, v.R);
+    RoundView v = round_view(P, par);
+    if (v.live) ph_spec_w(P, c, v);
 }
 __global__ __launch_bounds__(256) void k_validate(BP P, int par) {
     GC c = grid_coords();
-    RoundView v = round_view(P);
-    if (v.live) ph_validate(P, c, v, par);
+    RoundView v = round_view(P, par);
+    if (v.live) ph_validate_w(P, c, v, par);
 }
 __global__ __launch_bounds__(256) void k_commit(BP P, int par) {
     GC c = grid_coords();
-    ph_commit(P, c, par);
+    RoundView v = round_view(P, par);
+    if (v.live) ph_commit_w(P, c, v, par);
+    else if (c.tid == 0) advance_round(P, par, INT64_MAX);
 }
 __global__ __launch_bounds__(256) void k_comp_apply(BP P, int par) {
     GC c = grid_coords();
@@ -992,28 +1078,34 @@ __device__ __forceinline__ void phase_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
-// The tail of a batch in one workgroup: the remaining speculative rounds and
-// the completion iterations, separated by workgroup barriers (far cheaper
-// than grid-wide synchronisation on a multi-die GPU).  Runs after the queued
-// grid-wide rounds; whatever they left is finished here.
-__global__ __launch_bounds__(1024) void k_finish(BP P, int par) {
-    GC c = block_coords();
+// Copies the counters and the pool top to the pinned host block of the
+// batch (one workgroup), so the host reads the batch's results without a
+// copy on the stream.
+__device__ void publish(const BP& P, const GC& c) {
+    phase_barrier();
+    for (int64_t i = c.tid; i < C_NCOUNTERS; i += c.nthreads) P.h_out[i] = P.ctr[i];
+    if (c.tid == 0) P.h_out[C_NCOUNTERS] = *P.g.pool_top;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+}
+
+__device__ void finish_body(const BP& P, const GC& c, int par) {
     uint32_t guard = 0;
     while (true) {
         phase_barrier();
-        RoundView v = round_view(P);
+        RoundView v = round_view(P, par);
         if (!v.live) break;
         if ((int64_t)guard++ > P.U + 1) {  // every round commits its first edge
             if (c.tid == 0) P.ctr[C_GUARD] = 1;
             return;
         }
-        ph_classify(P, c, v, par);
+        ph_classify_l(P, c, v, par);
         phase_barrier();
-        ph_spec(P, c, par, v.R);
+        ph_spec_l(P, c, par, v.R);
         phase_barrier();
-        ph_validate(P, c, v, par);
+        ph_validate_l(P, c, v, par);
         phase_barrier();
-        ph_commit(P, c, par);
+        ph_commit_l(P, c, par);
+        if (c.tid == 0) advance_round(P, par, round_conflict(P, par));
         par ^= 1;
     }
     ph_stats(P, c);
@@ -1032,6 +1124,22 @@ __global__ __launch_bounds__(1024) void k_finish(BP P, int par) {
             return;
         }
     }
+}
+
+// The tail of a batch in one workgroup: the remaining speculative rounds and
+// the completion iterations, separated by workgroup barriers (far cheaper
+// than grid-wide synchronisation on a multi-die GPU).  Runs after the queued
+// grid-wide rounds; whatever they left is finished here.  Ends by
+// publishing the counters to the host.
+__global__ __launch_bounds__(1024) void k_finish(BP P, int par) {
+    GC c = block_coords();
+    finish_body(P, c, par);
+    publish(P, c);
+}
+
+__global__ __launch_bounds__(256) void k_publish(BP P) {
+    GC c = block_coords();
+    publish(P, c);
 }
 
 // ---------------------------------------------------------------- launchers
@@ -1053,7 +1161,6 @@ void mf_launch_batch_prologue(const BP& P, const LaunchCfg& L) {
 }
 
 void mf_launch_round(const BP& P, const LaunchCfg& L, int par) {
-    LAUNCH(k_classify, P, par);
     LAUNCH(k_spec, P, par);
     LAUNCH(k_validate, P, par);
     LAUNCH(k_commit, P, par);
@@ -1066,6 +1173,10 @@ void mf_launch_finish(const BP& P, const LaunchCfg& L, int par) {
 void mf_launch_completion_collect(const BP& P, const LaunchCfg& L) { LAUNCH(k_comp_collect, P); }
 
 void mf_launch_stats(const BP& P, const LaunchCfg& L) { LAUNCH(k_stats, P); }
+
+void mf_launch_publish(const BP& P, const LaunchCfg& L) {
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(256), 0, L.stream, P);
+}
 
 void mf_launch_completion_round(const BP& P, const LaunchCfg& L, int par) {
     LAUNCH(k_comp_decide, P);
