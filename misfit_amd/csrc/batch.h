@@ -10,13 +10,12 @@
 
 // Rows of members that may be evicted are counted in chunks of this many
 // entries, one wave per chunk, so hub rows do not serialise on one wave.
-#define MF_KCH 512
+#define MF_KCH 64
 
 enum BatchCounter {
     C_SEEDS = 0,     // distinct endpoints
     C_WPOT,          // distinct max(u,v) of insertions
     C_NE,            // new directed overlay entries
-    C_TOUCHED,       // neighbours of MIS vertices that may be evicted
     C_PU,            // vertices that may become uncovered
     C_ADM,           // ... of which have a row (may be admitted)
     C_BIG,           // long rows with admissible neighbours
@@ -43,6 +42,10 @@ enum BatchCounter {
     C_NCOUNTERS
 };
 
+// Each counter sits on a cache line of its own (16 x 8 bytes): waves that
+// append to one list must not stall the reads of another list's length.
+#define MF_CS 16
+
 struct BP {
     DevGraph g;
     const int32_t* bu;
@@ -67,8 +70,6 @@ struct BP {
     // vertex lists
     int32_t* seeds;
     int32_t* wpot;
-    int32_t* touched;
-    int64_t touched_cap;
     u64* ktask;        // w<<32 | chunk
     int64_t ktask_cap;
     int32_t* pu;
@@ -86,7 +87,7 @@ struct BP {
     // completion
     int32_t* cand;
     uint8_t* cdec;
-    u64* ctr;
+    u64* ctr;          // counter i at ctr[MF_CS * i]
     uint32_t* round;   // global speculative-round counter (device-resident)
     u64* h_out;        // pinned host copy of the counters + pool top, written
                        // by the batch's last kernel (C_NCOUNTERS + 1 words)
@@ -100,6 +101,8 @@ struct LaunchCfg {
 void mf_launch_batch_prologue(const BP& P, const LaunchCfg& L);
 void mf_launch_round(const BP& P, const LaunchCfg& L, int par);
 void mf_launch_finish(const BP& P, const LaunchCfg& L, int par);
+void mf_launch_tail_collect(const BP& P, const LaunchCfg& L);
+void mf_launch_finish_completion(const BP& P, const LaunchCfg& L);
 void mf_launch_completion_collect(const BP& P, const LaunchCfg& L);
 void mf_launch_stats(const BP& P, const LaunchCfg& L);
 void mf_launch_completion_round(const BP& P, const LaunchCfg& L, int par);

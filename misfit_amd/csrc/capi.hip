@@ -142,7 +142,7 @@ extern "C" int misfit_ctx_create(int device, misfit_ctx** out) {
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
     c->S.blocks = std::max(64, cus * 8);
     c->S.stream = c->stream;
-    if (hipHostMalloc((void**)&c->h_small, 64 * sizeof(u64), hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void**)&c->h_small, C_NCOUNTERS * MF_CS * sizeof(u64), hipHostMallocDefault) != hipSuccess) {
         delete c;
         return MISFIT_E_NOMEM;
     }
@@ -382,10 +382,12 @@ extern "C" int misfit_load_csr(misfit_ctx* c, int32_t n, int64_t m, const void* 
     HIPCHK(c, dalloc(A, &g.mc, n));
     HIPCHK(c, dalloc(A, &g.seedb, n));
     HIPCHK(c, dalloc(A, &g.wpotb, n));
-    HIPCHK(c, dalloc(A, &g.kw, n));
     HIPCHK(c, dalloc(A, &g.pub, n));
     HIPCHK(c, dalloc(A, &g.evb, n));
     HIPCHK(c, dalloc(A, &g.pfb, n));
+    HIPCHK(c, dalloc(A, &g.kw, n));
+    HIPCHK(c, dalloc(A, &g.tagb, n));
+    HIPCHK(c, dalloc(A, &g.nevb, n));
     HIPCHK(c, dalloc(A, &g.cstat, n));
     HIPCHK(c, dalloc(A, &g.mmark, n));
     HIPCHK(c, dalloc(A, &g.cmark, n));
@@ -395,15 +397,13 @@ extern "C" int misfit_load_csr(misfit_ctx* c, int32_t n, int64_t m, const void* 
     HIPCHK(c, dalloc(A, &g.fa_start, n));
     HIPCHK(c, dalloc(A, &g.fa_fill, n));
     BP& P = c->P;
-    HIPCHK(c, dalloc(A, &P.touched, n));
     HIPCHK(c, dalloc(A, &P.pu, n));
     HIPCHK(c, dalloc(A, &P.big, n));
     HIPCHK(c, dalloc(A, &P.cand, n));
     HIPCHK(c, dalloc(A, &P.cdec, n));
-    P.touched_cap = n;
     P.pu_cap = n;
     P.big_cap = n;
-    HIPCHK(c, dalloc(A, &c->d_ctr, C_NCOUNTERS));
+    HIPCHK(c, dalloc(A, &c->d_ctr, C_NCOUNTERS * MF_CS));
     HIPCHK(c, dalloc(A, &c->d_round, 1));
     HIPCHK(c, dalloc(A, &c->d_acc, 8));
     HIPCHK(c, dalloc(A, &c->d_flags, 1));
@@ -422,10 +422,12 @@ extern "C" int misfit_load_csr(misfit_ctx* c, int32_t n, int64_t m, const void* 
     mf_fill_u32(g.mc, 0u, n, st);
     mf_fill_u32(g.seedb, 0u, n, st);
     mf_fill_u32(g.wpotb, 0u, n, st);
-    mf_fill_u64(g.kw, 0ull, n, st);
     mf_fill_u32(g.pub, 0u, n, st);
     mf_fill_u32(g.evb, 0u, n, st);
     mf_fill_u32(g.pfb, 0u, n, st);
+    mf_fill_u64(g.kw, 0ull, n, st);
+    mf_fill_u32(g.tagb, 0u, n, st);
+    mf_fill_u32(g.nevb, 0u, n, st);
     mf_fill_u32(g.cstat, 0u, n, st);
     mf_fill_u64(g.mmark, 0ull, n, st);
     mf_fill_u64(g.cmark, 0ull, n, st);
@@ -436,7 +438,7 @@ extern "C" int misfit_load_csr(misfit_ctx* c, int32_t n, int64_t m, const void* 
     mf_fill_i32(g.fa_fill, 0, n, st);
     mf_fill_u32(S.d_bmark, 0u, n, st);
     mf_fill_u32(c->d_round, 0u, 1, st);
-    mf_fill_u64(c->d_ctr, 0ull, C_NCOUNTERS, st);
+    mf_fill_u64(c->d_ctr, 0ull, C_NCOUNTERS * MF_CS, st);
     c->epoch = 0;
     c->bytes_tag = 0;
     S.g = g;
@@ -563,11 +565,12 @@ static int drive_plain(misfit_ctx* c, BP& P) {
         int par = (int)(r & 1u);
         if (P.U == 0) break;
         mf_launch_round(P, L, par);
-        HIPCHK(c, hipMemcpyAsync(c->h_small, P.ctr, C_NCOUNTERS * sizeof(u64), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->h_small, P.ctr, C_NCOUNTERS * MF_CS * sizeof(u64), hipMemcpyDeviceToHost,
+                                 c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
-        int64_t start = (int64_t)c->h_small[C_START0 + (par ^ 1)];
+        int64_t start = (int64_t)c->h_small[MF_CS * (C_START0 + (par ^ 1))];
         if (c->debug_rounds) {
-            u64 cf = c->h_small[C_CONF0 + par];
+            u64 cf = c->h_small[MF_CS * (C_CONF0 + par)];
             fprintf(stderr, "[rounds] batch %u round %u: conflict %lld, next start %lld / %lld\n", P.b, r,
                     cf > (u64)INT64_MAX ? -1LL : (long long)cf, (long long)start, (long long)P.U);
         }
@@ -579,7 +582,7 @@ static int drive_plain(misfit_ctx* c, BP& P) {
         for (uint32_t k = 0;; k++) {
             int par = (int)(k & 1u);
             mf_launch_completion_round(P, L, par);
-            HIPCHK(c, hipMemcpyAsync(c->h_small, P.ctr + C_UNDEC0 + par, sizeof(u64), hipMemcpyDeviceToHost,
+            HIPCHK(c, hipMemcpyAsync(c->h_small, P.ctr + MF_CS * (C_UNDEC0 + par), sizeof(u64), hipMemcpyDeviceToHost,
                                      c->stream));
             HIPCHK(c, hipStreamSynchronize(c->stream));
             if (c->h_small[0] == 0) break;
@@ -591,13 +594,16 @@ static int drive_plain(misfit_ctx* c, BP& P) {
     return MISFIT_OK;
 }
 
-// Whole batch enqueued: prologue, grid-wide rounds, then the finisher.
+// Whole batch enqueued: prologue, grid-wide rounds, then the tail kernels
+// (leftover rounds, counters and completion candidates, completion).
 static int drive_queued(misfit_ctx* c, BP& P) {
     LaunchCfg L{c->S.blocks, c->stream};
     mf_launch_batch_prologue(P, L);
     int k = c->grid_rounds;
     for (int r = 0; r < k; r++) mf_launch_round(P, L, r & 1);
     mf_launch_finish(P, L, k & 1);
+    mf_launch_tail_collect(P, L);
+    mf_launch_finish_completion(P, L);
     HIPCHK(c, hipGetLastError());
     return MISFIT_OK;
 }
@@ -729,7 +735,7 @@ extern "C" int misfit_batch_bytes(misfit_ctx* c, int index_bytes, int64_t* bytes
     if (!c->has_graph) return fail(c, MISFIT_E_STATE, "no graph loaded");
     HIPCHK(c, hipSetDevice(c->device));
     c->bytes_tag++;
-    HIPCHK(c, mf_batch_bytes(c->S, c->P.seeds, c->P.ctr + C_SEEDS, c->bytes_tag, c->d_acc));
+    HIPCHK(c, mf_batch_bytes(c->S, c->P.seeds, c->P.ctr + MF_CS * C_SEEDS, c->bytes_tag, c->d_acc));
     HIPCHK(c, hipMemcpyAsync(c->h_small, c->d_acc, 4 * sizeof(u64), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     int64_t n1 = (int64_t)c->h_small[0], n2 = (int64_t)c->h_small[1], deg = (int64_t)c->h_small[2];

@@ -24,11 +24,14 @@
 //     seedb   = b   vertex is an endpoint of batch b (it has a 2-hop row)
 //     wpotb   = b   vertex is max(u,v) of an insertion of batch b (may be
 //                   evicted by the rule)
-//     kw      = b<<32 | k   k = MIS neighbours of the vertex that may be
-//                   evicted in batch b
 //     pub     = 4b+1 | 2    vertex may become uncovered during batch b
 //                   (bit 1: it also has a row, i.e. it is admissible)
+//     kw      = b<<32 | k   k = MIS neighbours of the vertex that may be
+//                   evicted in batch b (counted for non-members next to
+//                   members of W)
+//     tagb    = b   the vertex has a count in kw for batch b
 //     evb     = b   vertex was evicted in batch b
+//     nevb    = b   vertex is a neighbour of one evicted in batch b
 //     pfb     = b   vertex is max(u,v) of an insertion of batch b whose other
 //                   end may be evicted too (or whose edge is gone after the
 //                   batch): only then can it be uncovered after an eviction
@@ -70,10 +73,12 @@ struct DevGraph {
     uint32_t* mc;
     uint32_t* seedb;
     uint32_t* wpotb;
-    u64* kw;
     uint32_t* pub;
     uint32_t* evb;
+    u64* kw;
+    uint32_t* tagb;
     uint32_t* pfb;   // = b: an insertion of batch b may let this vertex become uncovered after its eviction
+    uint32_t* nevb;  // = b: neighbour of a vertex evicted in batch b
     uint32_t* cstat;
     u64* mmark;
     u64* cmark;
@@ -236,6 +241,19 @@ __device__ __forceinline__ void wave_push(int32_t* list, u64* ctr, int32_t v, bo
     if (lane == leader) base = atomicAdd(ctr, (u64)__popcll(bal));
     base = (u64)__shfl((long long)base, leader, MF_WAVE);
     if (pred) list[base + __popcll(bal & ((1ull << lane) - 1ull))] = v;
+}
+
+// Wave-wide sum; every lane of the wave must be active.
+__device__ __forceinline__ u64 wave_sum_u64(u64 v) {
+#pragma unroll
+    for (int d = MF_WAVE / 2; d > 0; d >>= 1) v += (u64)__shfl_xor((long long)v, d, MF_WAVE);
+    return v;
+}
+
+// Adds v to *ctr with one atomic per wave; every lane must be active.
+__device__ __forceinline__ void wave_add(u64* ctr, u64 v) {
+    u64 t = wave_sum_u64(v);
+    if (t && lane_id() == 0) atomicAdd(ctr, t);
 }
 
 // Wave-wide inclusive prefix sum.

@@ -54,20 +54,15 @@
 __device__ __forceinline__ bool in_pu(const BP& P, int32_t x) { return (P.g.pub[x] >> 2) == P.b; }
 __device__ __forceinline__ bool is_adm(const BP& P, int32_t x) { return P.g.pub[x] == 4u * P.b + 3u; }
 
-__device__ __forceinline__ void push(int32_t* list, u64* ctr, int32_t v) {
-    u64 i = atomicAdd(ctr, 1ull);
-    list[i] = v;
-}
-
-__device__ __forceinline__ bool pu_add(const BP& P, int32_t z) {
-    uint32_t want = 4u * P.b + 1u;
-    uint32_t old = atomicMax(&P.g.pub[z], want);
-    if (old < want) {
-        u64 i = atomicAdd(&P.ctr[C_PU], 1ull);
-        if (i < (u64)P.pu_cap) P.pu[i] = z;
-        return true;
+// z joins PU when pred holds (first time in this batch: appended to the PU
+// list).  Every list below holds distinct vertices, so n entries suffice.
+__device__ __forceinline__ void pu_add(const BP& P, int32_t z, bool pred) {
+    bool first = false;
+    if (pred) {
+        uint32_t want = 4u * P.b + 1u;
+        first = atomicMax(&P.g.pub[z], want) < want;
     }
-    return false;
+    wave_push(P.pu, &P.ctr[MF_CS * (C_PU)], z, first);
 }
 
 // Epoch-tagged counter word b<<32 | count.  Returns true on the first touch
@@ -115,8 +110,13 @@ __device__ __forceinline__ void vread(const BP& P, int32_t x, uint32_t val, uint
     if ((u64)cnt < (u64)T + dec) conf = true;
 }
 
-// Round-tagged counter word R<<32 | count (see epoch_count_add).
-__device__ __forceinline__ void round_count_add(u64* w, uint32_t R) { (void)epoch_count_add(w, R); }
+// Round-tagged counter word R<<32 | count.  Rounds only grow, so raising
+// the word to R<<32 and then adding one counts this round's additions: the
+// two no-return atomics of one lane reach the word in program order.
+__device__ __forceinline__ void round_count_add(u64* w, uint32_t R) {
+    atomicMax(w, (u64)R << 32);
+    atomicAdd(w, 1ull);
+}
 
 // ---------------------------------------------------------------- prologue
 // P1: claim pairs, tag endpoints and possible evictees.
@@ -133,10 +133,10 @@ __device__ void ph_claim(const BP& P, const GC& c) {
         int32_t prevh = atomicExch(&P.bt_head[s], (int32_t)i);
         P.bt_nxt[i] = prevh;
         P.bt_slot[i] = (int32_t)s;
-        wave_push(P.seeds, &P.ctr[C_SEEDS], a, atomicExch(&P.g.seedb[a], P.b) != P.b);
-        wave_push(P.seeds, &P.ctr[C_SEEDS], bb, atomicExch(&P.g.seedb[bb], P.b) != P.b);
+        wave_push(P.seeds, &P.ctr[MF_CS * (C_SEEDS)], a, atomicExch(&P.g.seedb[a], P.b) != P.b);
+        wave_push(P.seeds, &P.ctr[MF_CS * (C_SEEDS)], bb, atomicExch(&P.g.seedb[bb], P.b) != P.b);
         int32_t w = a > bb ? a : bb;
-        wave_push(P.wpot, &P.ctr[C_WPOT], w, P.bins[i] && atomicExch(&P.g.wpotb[w], P.b) != P.b);
+        wave_push(P.wpot, &P.ctr[MF_CS * (C_WPOT)], w, P.bins[i] && atomicExch(&P.g.wpotb[w], P.b) != P.b);
     }
 }
 
@@ -168,7 +168,11 @@ __device__ __forceinline__ void kill_run_entry(const DevGraph& g, int32_t x, int
 // and applies the net effect.
 __device__ void ph_resolve(const BP& P, const GC& c) {
     const DevGraph& g = P.g;
+    u64 my_ops = 0;
     for (int64_t i = c.tid; i < P.U; i += c.nthreads) {
+        // new overlay entries of the pair headed by update i go to 2i, 2i+1
+        P.ne_x[2 * i] = -1;
+        P.ne_x[2 * i + 1] = -1;
         int32_t slot = P.bt_slot[i];
         if (P.bt_head[slot] != (int32_t)i) continue;
         int32_t a = P.bu[i], bb = P.bv[i];
@@ -210,7 +214,7 @@ __device__ void ph_resolve(const BP& P, const GC& c) {
         }
         P.bt_key[slot] = MF_EMPTY_KEY;
         P.bt_head[slot] = -1;
-        if (ops) atomicAdd(&P.ctr[C_DELTA_OPS], (u64)ops);
+        my_ops += (u64)ops;
         if (in_base && tomb != tomb0) {
             set_base_deleted(g, lo, plo, tomb);
             if (phi >= 0) set_base_deleted(g, hi, phi, tomb);
@@ -224,7 +228,7 @@ __device__ void ph_resolve(const BP& P, const GC& c) {
             g.ph_val[s] = alive ? 1 : 0;
         }
         if (alive && last_ins >= 0) {
-            u64 k = atomicAdd(&P.ctr[C_NE], 2ull);
+            int64_t k = 2 * i;
             P.ne_x[k] = lo; P.ne_y[k] = hi; P.ne_seq[k] = last_ins;
             P.ne_x[k + 1] = hi; P.ne_y[k + 1] = lo; P.ne_seq[k + 1] = last_ins;
         }
@@ -235,13 +239,16 @@ __device__ void ph_resolve(const BP& P, const GC& c) {
             if (mlo & MF_MISBIT) atomicAdd(&g.mc[hi], d);
         }
     }
+    wave_add(&P.ctr[MF_CS * (C_DELTA_OPS)], my_ops);
 }
 
 // P3: chain the new entries of each vertex.
 __device__ void ph_group(const BP& P, const GC& c) {
-    int64_t ne = (int64_t)P.ctr[C_NE];
+    int64_t ne = 2 * P.U;
     for (int64_t k = c.tid; k < ne; k += c.nthreads) {
-        int32_t prev = atomicExch(&P.g.ne_head[P.ne_x[k]], (int32_t)k);
+        int32_t x = P.ne_x[k];
+        if (x < 0) continue;
+        int32_t prev = atomicExch(&P.g.ne_head[x], (int32_t)k);
         P.ne_nxt[k] = prev;
     }
 }
@@ -250,10 +257,10 @@ __device__ void ph_group(const BP& P, const GC& c) {
 // order, growing the run by doubling inside the pool.
 __device__ void ph_append(const BP& P, const GC& c) {
     const DevGraph& g = P.g;
-    int64_t ne = (int64_t)P.ctr[C_NE];
+    int64_t ne = 2 * P.U;
     for (int64_t k = c.tid; k < ne; k += c.nthreads) {
         int32_t x = P.ne_x[k];
-        if (g.ne_head[x] != (int32_t)k) continue;
+        if (x < 0 || g.ne_head[x] != (int32_t)k) continue;
         int32_t cnt = 0;
         for (int32_t j = (int32_t)k; j != -1; j = P.ne_nxt[j]) cnt++;
         int32_t len = g.ov_len[x], cap = g.ov_cap[x];
@@ -262,7 +269,7 @@ __device__ void ph_append(const BP& P, const GC& c) {
             while (nc < len + cnt) nc <<= 1;
             u64 st = atomicAdd(g.pool_top, (u64)nc);
             if ((int64_t)(st + nc) > g.pool_cap) {
-                atomicExch(&P.ctr[C_POOL_OVF], 1ull);
+                atomicExch(&P.ctr[MF_CS * (C_POOL_OVF)], 1ull);
                 g.ne_head[x] = -1;
                 continue;
             }
@@ -300,75 +307,94 @@ __device__ void ph_partner(const BP& P, const GC& c) {
         if (P.g.pfb[w] == P.b) continue;
         if (P.g.wpotb[o] == P.b || !adjacent(P.g, w, o)) P.g.pfb[w] = P.b;
     }
-    int64_t nw = (int64_t)P.ctr[C_WPOT];
-    for (int64_t i = c.tid; i < nw; i += c.nthreads) {
-        int32_t w = P.wpot[i];
-        if (!(P.g.mc[w] & MF_MISBIT)) continue;
-        int64_t L = row_len(P.g, w);
-        if (!L) continue;
-        int64_t nch = (L + MF_KCH - 1) / MF_KCH;
-        u64 t = atomicAdd(&P.ctr[C_KTASK], (u64)nch);
-        if ((int64_t)(t + nch) > P.ktask_cap) {
-            atomicExch(&P.ctr[C_GUARD], 3ull);
+    int64_t nw = (int64_t)P.ctr[MF_CS * (C_WPOT)];
+    // whole waves step together so the reservation below is wave-wide
+    for (int64_t i0 = c.tid - c.lane; i0 < nw; i0 += c.nthreads) {
+        int64_t i = i0 + c.lane;
+        int32_t w = i < nw ? P.wpot[i] : -1;
+        int64_t nch = 0;
+        if (w >= 0 && (P.g.mc[w] & MF_MISBIT)) nch = (row_len(P.g, w) + MF_KCH - 1) / MF_KCH;
+        int64_t incl = wave_incl_scan(nch);
+        int64_t total = __shfl(incl, MF_WAVE - 1, MF_WAVE);
+        u64 base = 0;
+        if (c.lane == 0 && total) base = atomicAdd(&P.ctr[MF_CS * (C_KTASK)], (u64)total);
+        base = (u64)__shfl((long long)base, 0, MF_WAVE);
+        u64 t = base + (u64)(incl - nch);
+        if (nch && (int64_t)(t + nch) > P.ktask_cap) {
+            atomicExch(&P.ctr[MF_CS * (C_GUARD)], 3ull);
             continue;
         }
         for (int64_t k = 0; k < nch; k++) P.ktask[t + k] = ((u64)(uint32_t)w << 32) | (u64)k;
     }
 }
 
-// P5: k(z) for neighbours of MIS vertices that may be evicted (one wave per
-// row chunk); flagged members of W join PU; every currently uncovered vertex
-// joins PU.
+// P5: k(z), the MIS neighbours of z that may be evicted, for non-members z
+// next to members of W∩MIS (one wave per 64-entry chunk of such a row).  The
+// count word is raised to the batch epoch and incremented with two no-return
+// atomics (neighbours are mostly distinct, and nothing waits on the result);
+// a plain store tags z for the scan below.  Flagged members of W, and
+// endpoints whose deleted edge took their last MIS neighbour, join PU here.
 __device__ void ph_kcount(const BP& P, const GC& c) {
     const DevGraph& g = P.g;
-    int64_t nw = (int64_t)P.ctr[C_WPOT];
+    int64_t nw = (int64_t)P.ctr[MF_CS * (C_WPOT)];
     for (int64_t i = c.tid; i < nw; i += c.nthreads) {
         int32_t w = P.wpot[i];
-        if ((g.mc[w] & MF_MISBIT) && g.pfb[w] == P.b) pu_add(P, w);
+        pu_add(P, w, (g.mc[w] & MF_MISBIT) && g.pfb[w] == P.b);
     }
-    int64_t nt = (int64_t)P.ctr[C_KTASK];
+    int64_t ns = (int64_t)P.ctr[MF_CS * (C_SEEDS)];
+    for (int64_t i = c.tid; i < ns; i += c.nthreads) {
+        int32_t s = P.seeds[i];
+        pu_add(P, s, g.mc[s] == 0);
+    }
+    int64_t nt = (int64_t)P.ctr[MF_CS * (C_KTASK)];
     if (nt > P.ktask_cap) nt = P.ktask_cap;
+    const u64 ep = (u64)P.b << 32;
     for (int64_t i = c.wave; i < nt; i += c.nwaves) {
         u64 task = P.ktask[i];
         int32_t w = (int32_t)(task >> 32);
-        int64_t j0 = (int64_t)(uint32_t)task * MF_KCH;
+        int64_t j = (int64_t)(uint32_t)task * MF_KCH + c.lane;
         Row r = get_row(g, w);
-        int64_t L = r.len();
-        int64_t j1 = j0 + MF_KCH < L ? j0 + MF_KCH : L;
-        for (int64_t j = j0 + c.lane; j < j1; j += MF_WAVE) {
-            int32_t z = r.at(j);
-            if (z < 0 || z == w) continue;
-            // k(z) <= |W|: a vertex with more MIS neighbours can never reach
-            // c(z) = k(z) (this also keeps hubs out of the counting)
-            if ((u64)(g.mc[z] & MF_CMASK) > (u64)nw) continue;
-            if (epoch_count_add(&g.kw[z], P.b)) {
-                u64 t = atomicAdd(&P.ctr[C_TOUCHED], 1ull);
-                if (t < (u64)P.touched_cap) P.touched[t] = z;
-            }
-        }
-    }
-    for (int64_t x = c.tid; x < g.n; x += c.nthreads) {
-        uint32_t m = g.mc[x];
-        if (m == 0) pu_add(P, (int32_t)x);  // outside the MIS with no MIS neighbour
+        if (j >= r.len()) continue;
+        int32_t z = r.at(j);
+        if (z < 0 || z == w) continue;
+        uint32_t mz = g.mc[z];
+        // c(z) <= |W| is necessary for all MIS neighbours to be in W
+        if ((mz & MF_MISBIT) || (u64)(mz & MF_CMASK) > (u64)nw) continue;
+        atomicMax(&g.kw[z], ep);
+        atomicAdd(&g.kw[z], 1ull);
+        g.tagb[z] = P.b;
     }
 }
 
-// P6: touched z joins PU when all its MIS neighbours may be evicted.
-__device__ void ph_pu_touched(const BP& P, const GC& c) {
-    int64_t nt = (int64_t)P.ctr[C_TOUCHED];
-    if (nt > P.touched_cap) nt = P.touched_cap;
-    for (int64_t i = c.tid; i < nt; i += c.nthreads) {
-        int32_t z = P.touched[i];
-        uint32_t m = P.g.mc[z];
-        uint32_t k = (uint32_t)(P.g.kw[z] & 0xFFFFFFFFull);
-        if (!(m & MF_MISBIT) && (m & MF_CMASK) == k) pu_add(P, z);
+// P6: PU from one pass over all vertices, four per thread: a non-member
+// with no MIS neighbour, or a tagged one all of whose MIS neighbours may be
+// evicted (c(z) = k(z)).
+__device__ __forceinline__ void pu_scan_one(const BP& P, int32_t x, uint32_t m, uint32_t t) {
+    bool p = m == 0;
+    if (!p && t == P.b && !(m & MF_MISBIT)) p = (m & MF_CMASK) == (uint32_t)(P.g.kw[x] & 0xFFFFFFFFull);
+    pu_add(P, x, p);
+}
+
+__device__ void ph_pu_scan(const BP& P, const GC& c) {
+    const DevGraph& g = P.g;
+    const int64_t n4 = (int64_t)g.n / 4;
+    const uint4* mc4 = reinterpret_cast<const uint4*>(g.mc);
+    const uint4* tg4 = reinterpret_cast<const uint4*>(g.tagb);
+    for (int64_t q = c.tid; q < n4; q += c.nthreads) {
+        uint4 m = mc4[q], t = tg4[q];
+        int32_t x = (int32_t)(4 * q);
+        pu_scan_one(P, x, m.x, t.x);
+        pu_scan_one(P, x + 1, m.y, t.y);
+        pu_scan_one(P, x + 2, m.z, t.z);
+        pu_scan_one(P, x + 3, m.w, t.w);
     }
+    for (int64_t x = 4 * n4 + c.tid; x < g.n; x += c.nthreads) pu_scan_one(P, (int32_t)x, g.mc[x], g.tagb[x]);
 }
 
 // P7: a PU member has a row when it is an endpoint or next to one.
 __device__ void ph_hasrow(const BP& P, const GC& c) {
     const DevGraph& g = P.g;
-    int64_t np = (int64_t)P.ctr[C_PU];
+    int64_t np = (int64_t)P.ctr[MF_CS * (C_PU)];
     if (np > P.pu_cap) np = P.pu_cap;
     for (int64_t i = c.wave; i < np; i += c.nwaves) {
         int32_t z = P.pu[i];
@@ -388,7 +414,7 @@ __device__ void ph_hasrow(const BP& P, const GC& c) {
         }
         if (has && c.lane == 0) {
             g.pub[z] = 4u * P.b + 3u;
-            atomicAdd(&P.ctr[C_ADM], 1ull);
+            atomicAdd(&P.ctr[MF_CS * (C_ADM)], 1ull);
         }
     }
 }
@@ -396,7 +422,7 @@ __device__ void ph_hasrow(const BP& P, const GC& c) {
 // P8: count ADM members next to each long row.
 __device__ void ph_fa_count(const BP& P, const GC& c) {
     const DevGraph& g = P.g;
-    int64_t np = (int64_t)P.ctr[C_PU];
+    int64_t np = (int64_t)P.ctr[MF_CS * (C_PU)];
     if (np > P.pu_cap) np = P.pu_cap;
     for (int64_t i = c.wave; i < np; i += c.nwaves) {
         int32_t z = P.pu[i];
@@ -407,24 +433,20 @@ __device__ void ph_fa_count(const BP& P, const GC& c) {
             int32_t x1 = r.at(j);
             if (x1 < 0 || x1 == z) continue;
             if (row_len(g, x1) <= MF_FA_T) continue;
-            if (epoch_count_add(&g.fa_cnt[x1], P.b)) {
-                u64 t = atomicAdd(&P.ctr[C_BIG], 1ull);
-                if (t < (u64)P.big_cap) P.big[t] = x1;
-                else atomicExch(&P.ctr[C_FA_OVF], 1ull);
-            }
+            wave_push(P.big, &P.ctr[MF_CS * (C_BIG)], x1, epoch_count_add(&g.fa_cnt[x1], P.b));
         }
     }
 }
 
 // P9: reserve index space per long row (no space: the row is scanned).
 __device__ void ph_fa_alloc(const BP& P, const GC& c) {
-    int64_t nb = (int64_t)P.ctr[C_BIG];
+    int64_t nb = (int64_t)P.ctr[MF_CS * (C_BIG)];
     bool ovf = nb > P.big_cap;
     if (ovf) nb = P.big_cap;
     for (int64_t i = c.tid; i < nb; i += c.nthreads) {
         int32_t x1 = P.big[i];
         u64 cnt = P.g.fa_cnt[x1] & 0xFFFFFFFFull;
-        u64 s = atomicAdd(&P.ctr[C_FA_TOP], cnt);
+        u64 s = atomicAdd(&P.ctr[MF_CS * (C_FA_TOP)], cnt);
         P.g.fa_start[x1] = (ovf || (int64_t)(s + cnt) > P.fa_cap) ? -1 : (int64_t)s;
         P.g.fa_fill[x1] = 0;
     }
@@ -433,7 +455,7 @@ __device__ void ph_fa_alloc(const BP& P, const GC& c) {
 // P10: fill the index: (z, position of z in row(x1)).
 __device__ void ph_fa_fill(const BP& P, const GC& c) {
     const DevGraph& g = P.g;
-    int64_t np = (int64_t)P.ctr[C_PU];
+    int64_t np = (int64_t)P.ctr[MF_CS * (C_PU)];
     if (np > P.pu_cap) np = P.pu_cap;
     for (int64_t i = c.wave; i < np; i += c.nwaves) {
         int32_t z = P.pu[i];
@@ -570,7 +592,7 @@ __device__ int32_t eval_insertion(const BP& P, int32_t w, int32_t e, uint32_t R,
         if (bal) return __shfl(x, first_lane(bal), MF_WAVE);
     }
     // part 3: concat over x1 in adj(w) of adj(x1); long rows through the index
-    const bool fa_ok = P.ctr[C_FA_OVF] == 0;
+    const bool fa_ok = P.ctr[MF_CS * (C_FA_OVF)] == 0;
     for (int64_t j0 = 0; j0 < L; j0 += MF_WAVE) {
         int64_t j = j0 + lane;
         int kindl = 0;  // 0 empty, 1 row scan, 2 index
@@ -703,7 +725,7 @@ struct RoundView {
 
 __device__ __forceinline__ RoundView round_view(const BP& P, int par) {
     RoundView v;
-    v.start = (int64_t)P.ctr[C_START0 + par];
+    v.start = (int64_t)P.ctr[MF_CS * (C_START0 + par)];
     v.end = v.start + P.window < P.U ? v.start + P.window : P.U;
     v.R = *P.round + 1u;
     v.live = v.start < P.U;
@@ -772,6 +794,8 @@ __device__ __forceinline__ bool validate_edge(const BP& P, int32_t e, uint8_t k,
     return __ballot(conf) != 0;
 }
 
+// Membership flip of z with its neighbours' counts; an eviction also tags
+// the neighbours (completion candidates next to an evicted vertex).
 __device__ __forceinline__ void apply_flip(const BP& P, int32_t z, bool join, int lane) {
     if (lane == 0) {
         if (join) atomicOr(&P.g.mc[z], MF_MISBIT);
@@ -783,7 +807,10 @@ __device__ __forceinline__ void apply_flip(const BP& P, int32_t z, bool join, in
         int32_t y = r.at(j);
         if (y >= 0 && y != z) {
             if (join) atomicAdd(&P.g.mc[y], 1u);
-            else atomicSub(&P.g.mc[y], 1u);
+            else {
+                atomicSub(&P.g.mc[y], 1u);
+                P.g.nevb[y] = P.b;
+            }
         }
     }
 }
@@ -801,7 +828,7 @@ __device__ __forceinline__ void commit_edge(const BP& P, int32_t e, uint8_t k, i
 // First conflicting edge of the round (INT64_MAX when none): the counter
 // holds ~0 until some edge conflicts.
 __device__ __forceinline__ int64_t round_conflict(const BP& P, int par) {
-    u64 cf = P.ctr[C_CONF0 + par];
+    u64 cf = P.ctr[MF_CS * (C_CONF0 + par)];
     return cf > (u64)INT64_MAX ? INT64_MAX : (int64_t)cf;
 }
 
@@ -810,15 +837,15 @@ __device__ __forceinline__ int64_t round_conflict(const BP& P, int par) {
 // over unchanged.
 __device__ __forceinline__ void advance_round(const BP& P, int par, int64_t conflict) {
     RoundView v = round_view(P, par);
-    P.ctr[C_NACT0 + (par ^ 1)] = 0;
-    P.ctr[C_CONF0 + (par ^ 1)] = ~0ull;
+    P.ctr[MF_CS * (C_NACT0 + (par ^ 1))] = 0;
+    P.ctr[MF_CS * (C_CONF0 + (par ^ 1))] = ~0ull;
     if (v.live) {
-        P.ctr[C_ROUNDS] += 1;
-        P.ctr[C_SPECS] += (u64)(v.end - v.start);
-        P.ctr[C_START0 + (par ^ 1)] = (u64)(conflict < v.end ? conflict : v.end);
+        P.ctr[MF_CS * (C_ROUNDS)] += 1;
+        P.ctr[MF_CS * (C_SPECS)] += (u64)(v.end - v.start);
+        P.ctr[MF_CS * (C_START0 + (par ^ 1))] = (u64)(conflict < v.end ? conflict : v.end);
         *P.round = v.R;
     } else {
-        P.ctr[C_START0 + (par ^ 1)] = (u64)v.start;
+        P.ctr[MF_CS * (C_START0 + (par ^ 1))] = (u64)v.start;
     }
 }
 
@@ -837,7 +864,7 @@ __device__ void ph_spec_w(const BP& P, const GC& c, const RoundView& v) {
 __device__ void ph_validate_w(const BP& P, const GC& c, const RoundView& v, int par) {
     for (int64_t e = v.start + c.wave; e < v.end; e += c.nwaves) {
         // an edge past a conflict already found cannot move the minimum
-        if ((u64)e > P.ctr[C_CONF0 + par]) break;
+        if ((u64)e > P.ctr[MF_CS * (C_CONF0 + par)]) break;
         uint8_t k = P.kind[e];
         bool conf;
         if (!k) {  // its classification read both endpoints' membership
@@ -846,7 +873,7 @@ __device__ void ph_validate_w(const BP& P, const GC& c, const RoundView& v, int 
         } else {
             conf = validate_edge(P, (int32_t)e, k, v.R);
         }
-        if (conf && c.lane == 0) atomicMin(&P.ctr[C_CONF0 + par], (u64)e);
+        if (conf && c.lane == 0) atomicMin(&P.ctr[MF_CS * (C_CONF0 + par)], (u64)e);
     }
 }
 
@@ -867,12 +894,12 @@ __device__ void ph_classify_l(const BP& P, const GC& c, const RoundView& v, int 
         int32_t a = P.bu[e], bb = P.bv[e];
         uint8_t k = classify_edge(P, e, a, bb);
         P.kind[e] = k;
-        wave_push(P.active, &P.ctr[C_NACT0 + par], (int32_t)e, k != 0);
+        wave_push(P.active, &P.ctr[MF_CS * (C_NACT0 + par)], (int32_t)e, k != 0);
     }
 }
 
 __device__ void ph_spec_l(const BP& P, const GC& c, int par, uint32_t R) {
-    int64_t na = (int64_t)P.ctr[C_NACT0 + par];
+    int64_t na = (int64_t)P.ctr[MF_CS * (C_NACT0 + par)];
     for (int64_t i = c.wave; i < na; i += c.nwaves) {
         int32_t e = P.active[i];
         spec_edge(P, e, P.kind[e], R, (u64)na, c.lane);
@@ -885,17 +912,17 @@ __device__ void ph_validate_l(const BP& P, const GC& c, const RoundView& v, int 
         if (P.kind[e]) continue;
         int32_t a = P.bu[e], bb = P.bv[e];
         if (mk_before(P.g.mmark, a, R, (int32_t)e) || mk_before(P.g.mmark, bb, R, (int32_t)e))
-            atomicMin(&P.ctr[C_CONF0 + par], (u64)e);
+            atomicMin(&P.ctr[MF_CS * (C_CONF0 + par)], (u64)e);
     }
-    int64_t na = (int64_t)P.ctr[C_NACT0 + par];
+    int64_t na = (int64_t)P.ctr[MF_CS * (C_NACT0 + par)];
     for (int64_t i = c.wave; i < na; i += c.nwaves) {
         int32_t e = P.active[i];
-        if (validate_edge(P, e, P.kind[e], R) && c.lane == 0) atomicMin(&P.ctr[C_CONF0 + par], (u64)e);
+        if (validate_edge(P, e, P.kind[e], R) && c.lane == 0) atomicMin(&P.ctr[MF_CS * (C_CONF0 + par)], (u64)e);
     }
 }
 
 __device__ void ph_commit_l(const BP& P, const GC& c, int par) {
-    int64_t na = (int64_t)P.ctr[C_NACT0 + par];
+    int64_t na = (int64_t)P.ctr[MF_CS * (C_NACT0 + par)];
     const int64_t conflict = round_conflict(P, par);
     for (int64_t i = c.wave; i < na; i += c.nwaves) {
         int32_t e = P.active[i];
@@ -906,29 +933,14 @@ __device__ void ph_commit_l(const BP& P, const GC& c, int par) {
 // ---------------------------------------------------------------- completion
 __device__ void ph_comp_collect(const BP& P, const GC& c) {
     const DevGraph& g = P.g;
-    int64_t np = (int64_t)P.ctr[C_PU];
+    int64_t np = (int64_t)P.ctr[MF_CS * (C_PU)];
     if (np > P.pu_cap) np = P.pu_cap;
-    for (int64_t i = c.wave; i < np; i += c.nwaves) {
+    for (int64_t i = c.tid; i < np; i += c.nthreads) {
         int32_t z = P.pu[i];
-        if (g.mc[z] != 0) continue;  // member, or has an MIS neighbour
-        bool inC = g.seedb[z] == P.b || g.evb[z] == P.b;
-        if (!inC) {
-            Row r = get_row(g, z);
-            int64_t L = r.len();
-            for (int64_t j0 = 0; j0 < L && !inC; j0 += MF_WAVE) {
-                int64_t j = j0 + c.lane;
-                bool s = false;
-                if (j < L) {
-                    int32_t y = r.at(j);
-                    s = y >= 0 && g.evb[y] == P.b;
-                }
-                inC = __ballot(s) != 0;
-            }
-        }
-        if (inC && c.lane == 0) {
-            g.cstat[z] = (P.b << 2) | 1u;
-            push(P.cand, &P.ctr[C_CAND], z);
-        }
+        // outside the MIS with no MIS neighbour, and in C
+        bool open = g.mc[z] == 0 && (g.seedb[z] == P.b || g.evb[z] == P.b || g.nevb[z] == P.b);
+        if (open) g.cstat[z] = (P.b << 2) | 1u;
+        wave_push(P.cand, &P.ctr[MF_CS * (C_CAND)], z, open);
     }
 }
 
@@ -937,7 +949,7 @@ __device__ void ph_comp_collect(const BP& P, const GC& c) {
 // per candidate, so a long row is scanned 64 entries at a time.
 __device__ void ph_comp_decide(const BP& P, const GC& c) {
     const DevGraph& g = P.g;
-    int64_t nc = (int64_t)P.ctr[C_CAND];
+    int64_t nc = (int64_t)P.ctr[MF_CS * (C_CAND)];
     const uint32_t open = (P.b << 2) | 1u;
     for (int64_t i = c.wave; i < nc; i += c.nwaves) {
         int32_t z = P.cand[i];
@@ -965,7 +977,7 @@ __device__ void ph_comp_decide(const BP& P, const GC& c) {
 
 __device__ void ph_comp_apply(const BP& P, const GC& c, int par) {
     const DevGraph& g = P.g;
-    int64_t nc = (int64_t)P.ctr[C_CAND];
+    int64_t nc = (int64_t)P.ctr[MF_CS * (C_CAND)];
     const uint32_t open = (P.b << 2) | 1u;
     for (int64_t i = c.wave; i < nc; i += c.nwaves) {
         int32_t z = P.cand[i];
@@ -974,17 +986,17 @@ __device__ void ph_comp_apply(const BP& P, const GC& c, int par) {
             apply_flip(P, z, true, c.lane);
             if (c.lane == 0) {
                 g.cstat[z] = (P.b << 2) | 2u;
-                atomicAdd(&P.ctr[C_COMP_ADDS], 1ull);
+                atomicAdd(&P.ctr[MF_CS * (C_COMP_ADDS)], 1ull);
             }
         } else if (d == 3) {
             if (c.lane == 0) g.cstat[z] = (P.b << 2) | 3u;
         } else if (c.lane == 0 && g.cstat[z] == open) {
-            atomicAdd(&P.ctr[C_UNDEC0 + par], 1ull);
+            atomicAdd(&P.ctr[MF_CS * (C_UNDEC0 + par)], 1ull);
         }
     }
     if (c.tid == 0) {
-        P.ctr[C_UNDEC0 + (par ^ 1)] = 0;
-        P.ctr[C_COMP_ROUNDS] += 1;
+        P.ctr[MF_CS * (C_UNDEC0 + (par ^ 1))] = 0;
+        P.ctr[MF_CS * (C_COMP_ROUNDS)] += 1;
     }
 }
 
@@ -999,18 +1011,19 @@ __device__ void ph_stats(const BP& P, const GC& c) {
         if (k == 1) { ai++; ti += P.res_x[e] >= 0; }
         else if (k == 2) { ad++; td += P.res_x[e] >= 0; }
     }
-    if (ai) { atomicAdd(&P.ctr[C_ACT_INS], ai); atomicAdd(&P.ctr[C_EVICT], ai); }
-    if (ti) atomicAdd(&P.ctr[C_ADMIT_INS], ti);
-    if (ad) atomicAdd(&P.ctr[C_ACT_DEL], ad);
-    if (td) atomicAdd(&P.ctr[C_ADMIT_DEL], td);
+    ai = wave_sum_u64(ai);
+    if (ai && c.lane == 0) { atomicAdd(&P.ctr[MF_CS * (C_ACT_INS)], ai); atomicAdd(&P.ctr[MF_CS * (C_EVICT)], ai); }
+    wave_add(&P.ctr[MF_CS * (C_ADMIT_INS)], ti);
+    wave_add(&P.ctr[MF_CS * (C_ACT_DEL)], ad);
+    wave_add(&P.ctr[MF_CS * (C_ADMIT_DEL)], td);
 }
 
 // ---------------------------------------------------------------- kernels
 __device__ void ph_init(const BP& P, const GC& c) {
-    for (int64_t i = c.tid; i < C_NCOUNTERS; i += c.nthreads) P.ctr[i] = 0;
+    for (int64_t i = c.tid; i < C_NCOUNTERS; i += c.nthreads) P.ctr[MF_CS * (i)] = 0;
     if (c.tid == 0) {
-        P.ctr[C_CONF0] = ~0ull;
-        P.ctr[C_CONF1] = ~0ull;
+        P.ctr[MF_CS * (C_CONF0)] = ~0ull;
+        P.ctr[MF_CS * (C_CONF1)] = ~0ull;
     }
 }
 
@@ -1037,7 +1050,7 @@ MF_KERNEL(k_group, ph_group(P, c))
 MF_KERNEL(k_append, ph_append(P, c))
 MF_KERNEL(k_partner, ph_partner(P, c))
 MF_KERNEL(k_kcount, ph_kcount(P, c))
-MF_KERNEL(k_pu_touched, ph_pu_touched(P, c))
+MF_KERNEL(k_pu_scan, ph_pu_scan(P, c))
 MF_KERNEL(k_hasrow, ph_hasrow(P, c))
 MF_KERNEL(k_fa_count, ph_fa_count(P, c))
 MF_KERNEL(k_fa_alloc, ph_fa_alloc(P, c))
@@ -1070,12 +1083,15 @@ __global__ __launch_bounds__(256) void k_comp_apply(BP P, int par) {
 // Barrier between phases inside one workgroup.  Atomics are performed in L2
 // while a plain load may be served by the CU's L1, and a bare workgroup
 // barrier neither waits for outstanding vector memory operations nor drops
-// L1; the agent-scope release/acquire pair does both, so every phase sees
-// the previous phase's atomics and stores.
+// L1.  All waves of the workgroup sit on one CU and share one L2, so waiting
+// for the wave's own memory operations before the barrier and invalidating
+// the CU's L1 after it makes every phase see the previous phase's atomics
+// and stores; the L2 itself needs no write-back (an agent-scope fence would
+// write back and invalidate the whole L2 at every phase).
 __device__ __forceinline__ void phase_barrier() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __builtin_amdgcn_s_waitcnt(0);                // own stores and atomics done
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    asm volatile("buffer_inv sc0" ::: "memory");  // drop this CU's L1 lines
 }
 
 // Copies the counters and the pool top to the pinned host block of the
@@ -1083,20 +1099,22 @@ __device__ __forceinline__ void phase_barrier() {
 // copy on the stream.
 __device__ void publish(const BP& P, const GC& c) {
     phase_barrier();
-    for (int64_t i = c.tid; i < C_NCOUNTERS; i += c.nthreads) P.h_out[i] = P.ctr[i];
+    for (int64_t i = c.tid; i < C_NCOUNTERS; i += c.nthreads) P.h_out[i] = P.ctr[MF_CS * (i)];
     if (c.tid == 0) P.h_out[C_NCOUNTERS] = *P.g.pool_top;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
 }
 
-__device__ void finish_body(const BP& P, const GC& c, int par) {
+// Remaining speculative rounds in one workgroup; returns false when a loop
+// bound was hit.
+__device__ bool finish_rounds(const BP& P, const GC& c, int par) {
     uint32_t guard = 0;
     while (true) {
         phase_barrier();
         RoundView v = round_view(P, par);
-        if (!v.live) break;
+        if (!v.live) return true;
         if ((int64_t)guard++ > P.U + 1) {  // every round commits its first edge
-            if (c.tid == 0) P.ctr[C_GUARD] = 1;
-            return;
+            if (c.tid == 0) P.ctr[MF_CS * (C_GUARD)] = 1;
+            return false;
         }
         ph_classify_l(P, c, v, par);
         phase_barrier();
@@ -1108,9 +1126,11 @@ __device__ void finish_body(const BP& P, const GC& c, int par) {
         if (c.tid == 0) advance_round(P, par, round_conflict(P, par));
         par ^= 1;
     }
-    ph_stats(P, c);
+}
+
+// Completion iterations over the collected candidates in one workgroup.
+__device__ void finish_completion(const BP& P, const GC& c) {
     if (!P.completion) return;
-    ph_comp_collect(P, c);
     for (uint32_t k = 0;; k++) {
         phase_barrier();
         int cp = (int)(k & 1u);
@@ -1118,22 +1138,36 @@ __device__ void finish_body(const BP& P, const GC& c, int par) {
         phase_barrier();
         ph_comp_apply(P, c, cp);
         phase_barrier();
-        if (P.ctr[C_UNDEC0 + cp] == 0) break;
-        if ((u64)k > P.ctr[C_CAND] + 1) {  // the smallest open candidate decides each time
-            if (c.tid == 0) P.ctr[C_GUARD] = 2;
+        if (P.ctr[MF_CS * (C_UNDEC0 + cp)] == 0) break;
+        if ((u64)k > P.ctr[MF_CS * (C_CAND)] + 1) {  // the smallest open candidate decides each time
+            if (c.tid == 0) P.ctr[MF_CS * (C_GUARD)] = 2;
             return;
         }
     }
 }
 
-// The tail of a batch in one workgroup: the remaining speculative rounds and
-// the completion iterations, separated by workgroup barriers (far cheaper
-// than grid-wide synchronisation on a multi-die GPU).  Runs after the queued
-// grid-wide rounds; whatever they left is finished here.  Ends by
-// publishing the counters to the host.
-__global__ __launch_bounds__(1024) void k_finish(BP P, int par) {
+// The tail of a batch after the queued grid-wide rounds, split in three so
+// the wide scans run on the whole GPU and only the iterations with few
+// items run in one workgroup (workgroup barriers are far cheaper than
+// grid-wide synchronisation on a multi-die GPU):
+//   k_finish_rounds      whatever rounds the grid rounds left;
+//   k_tail_collect       (grid) rule counters and completion candidates;
+//   k_finish_completion  the completion iterations, then the counters are
+//                        published to the host.
+__global__ __launch_bounds__(1024) void k_finish_rounds(BP P, int par) {
     GC c = block_coords();
-    finish_body(P, c, par);
+    (void)finish_rounds(P, c, par);
+}
+
+__global__ __launch_bounds__(256) void k_tail_collect(BP P) {
+    GC c = grid_coords();
+    ph_stats(P, c);
+    if (P.completion) ph_comp_collect(P, c);
+}
+
+__global__ __launch_bounds__(1024) void k_finish_completion(BP P) {
+    GC c = block_coords();
+    finish_completion(P, c);
     publish(P, c);
 }
 
@@ -1153,7 +1187,7 @@ void mf_launch_batch_prologue(const BP& P, const LaunchCfg& L) {
     LAUNCH(k_append, P);
     LAUNCH(k_partner, P);
     LAUNCH(k_kcount, P);
-    LAUNCH(k_pu_touched, P);
+    LAUNCH(k_pu_scan, P);
     LAUNCH(k_hasrow, P);
     LAUNCH(k_fa_count, P);
     LAUNCH(k_fa_alloc, P);
@@ -1167,7 +1201,13 @@ void mf_launch_round(const BP& P, const LaunchCfg& L, int par) {
 }
 
 void mf_launch_finish(const BP& P, const LaunchCfg& L, int par) {
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, L.stream, P, par);
+    hipLaunchKernelGGL(k_finish_rounds, dim3(1), dim3(1024), 0, L.stream, P, par);
+}
+
+void mf_launch_tail_collect(const BP& P, const LaunchCfg& L) { LAUNCH(k_tail_collect, P); }
+
+void mf_launch_finish_completion(const BP& P, const LaunchCfg& L) {
+    hipLaunchKernelGGL(k_finish_completion, dim3(1), dim3(1024), 0, L.stream, P);
 }
 
 void mf_launch_completion_collect(const BP& P, const LaunchCfg& L) { LAUNCH(k_comp_collect, P); }

@@ -148,3 +148,34 @@ def test_determinism(oracle, engine_mod, eng):
     want, _, _ = H.oracle_replay(oracle, ni, nl, mis0, batches, completion=True, literal=False)
     for a, b in zip(want, runs[0]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_nonmaximal_start_parity(oracle, engine_mod, eng, mode):
+    """Start from an independent but not maximal set (a third of the greedy
+    MIS dropped), run batches without and then with the completion: vertices
+    left uncovered before a batch must stay in the rule's reach until the
+    state is maximal again."""
+    rng = np.random.default_rng(21)
+    ni, nl = H.random_graph(rng, 400, 0.02)
+    g = oracle.Graph.from_csr(ni, nl)
+    mis0 = g.greedy_mis()
+    mis0[np.flatnonzero(mis0)[rng.random(int(mis0.sum())) < 0.33]] = 0
+    batches = H.random_batches(rng, oracle, ni, nl, 6, 60, general=True)
+    for first_without in (True, False):
+        want, got = [], []
+        og = oracle.Graph.from_csr(ni, nl)
+        m = mis0.copy()
+        eng.load_csr(ni, nl)
+        eng.set_mis(mis0)
+        eng.set_mode(mode)
+        for b, (u, v, ins) in enumerate(batches):
+            comp = not (first_without and b < 2)
+            og.process_batch(m, u, v, ins, completion=comp, literal=True)
+            eng.set_rule(comp)
+            eng.process_batch(engine_mod.make_updates(u, v, ins))
+            assert np.array_equal(eng.get_mis(), m), "batch %d (completion %s)" % (b, comp)
+            # the completion only reaches the batch's neighbourhood: vertices
+            # uncovered from the start elsewhere stay so, as in the oracle
+            conf, unc, bad = eng.check()
+            assert (conf, bad) == (0, 0) and unc == og.check(m)[1]
