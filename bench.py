@@ -55,6 +55,23 @@ def stream_copy_gbs(mib=2048, reps=20):
     return json.loads(r.stdout.strip().splitlines()[-1])
 
 
+def committed_traffic(args):
+    """HBM bytes per batch measured by rocprofv3 FETCH_SIZE + WRITE_SIZE (scaled by
+    the copy-kernel calibration) for the default workload, from the committed
+    summary; None for any other workload."""
+    if (args.scale, args.batch, args.seed) != (22, 10000, 7):
+        return None, None
+    rel = os.path.join("profiles", "queued_batch_rmat22_traffic.md")
+    try:
+        with open(os.path.join(ROOT, rel)) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_batch:"):
+                    return int(line.split(":")[1]), rel
+    except OSError:
+        pass
+    return None, None
+
+
 def pct(v, q):
     return float(np.percentile(np.asarray(v, np.float64), q)) if len(v) else 0.0
 
@@ -209,8 +226,9 @@ def main():
         tot_bytes = float(np.sum(plain["bytes"]))
         tot_s = float(np.sum(served["dev_ms"])) / 1e3
         ach = tot_bytes / tot_s / 1e9
+        traffic, traffic_src = committed_traffic(args)
         roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None,
+                "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": traffic_src,
                 "kernel": "queued batch (prologue, grid rounds, finisher) timed per batch with HIP events",
                 "algorithmic_bytes_per_batch": round(tot_bytes / K, 1),
                 "algorithmic_bytes_per_update": round(tot_bytes / (K * B), 2),

@@ -863,7 +863,10 @@ extern "C" int misfit_device_info(misfit_ctx* c, char* name, int namelen, int64_
     if (name && namelen > 0) {
         hipDeviceProp_t prop;
         HIPCHK(c, hipGetDeviceProperties(&prop, c->device));
-        snprintf(name, (size_t)namelen, "%s (%s)", prop.name, prop.gcnArchName);
+        if (prop.name[0])
+            snprintf(name, (size_t)namelen, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+        else
+            snprintf(name, (size_t)namelen, "%s, %d CUs", prop.gcnArchName, prop.multiProcessorCount);
     }
     size_t fr = 0, tot = 0;
     HIPCHK(c, hipMemGetInfo(&fr, &tot));
