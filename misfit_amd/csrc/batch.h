@@ -12,6 +12,9 @@
 // entries, one wave per chunk, so hub rows do not serialise on one wave.
 #define MF_KCH 64
 
+// Guarded reads logged per wave and speculation round (see ReadLog).
+#define MF_RLOG 128
+
 enum BatchCounter {
     C_SEEDS = 0,     // distinct endpoints
     C_WPOT,          // distinct max(u,v) of insertions
@@ -84,6 +87,9 @@ struct BP {
     uint8_t* kind;
     int32_t* res_w;
     int32_t* res_x;
+    uint4* rlog;       // guarded reads of a round, MF_RLOG per wave of the grid
+    int32_t* rlog_cnt; // records per wave
+    uint32_t* relog;   // per edge: = round when its read log overflowed
     // completion
     int32_t* cand;
     uint8_t* cdec;

@@ -239,12 +239,18 @@ static int ensure_batch_capacity(misfit_ctx* c, int64_t U) {
     HIPCHK(c, dalloc(A, &P.kind, cap));
     HIPCHK(c, dalloc(A, &P.res_w, cap));
     HIPCHK(c, dalloc(A, &P.res_x, cap));
+    HIPCHK(c, dalloc(A, &P.relog, cap));
+    // read logs: one segment per wave of the round grid (256-thread workgroups)
+    const int64_t nwaves = (int64_t)c->S.blocks * (256 / MF_WAVE);
+    HIPCHK(c, dalloc(A, &P.rlog, nwaves * MF_RLOG));
+    HIPCHK(c, dalloc(A, &P.rlog_cnt, nwaves));
     c->fa_cap = std::max<int64_t>(1 << 22, 16 * cap);
     HIPCHK(c, dalloc(A, &P.fa_z, c->fa_cap));
     HIPCHK(c, dalloc(A, &P.fa_pos, c->fa_cap));
     P.bt_mask = c->bt_size - 1;
     P.fa_cap = c->fa_cap;
     mf_fill_u64(P.bt_key, MF_EMPTY_KEY, (int64_t)c->bt_size, c->stream);
+    mf_fill_u32(P.relog, 0u, cap, c->stream);
     mf_fill_i32(P.bt_head, -1, (int64_t)c->bt_size, c->stream);
     for (auto& s : c->slots) {
         HIPCHK(c, dalloc(A, &s.d_u, cap));
@@ -594,16 +600,21 @@ static int drive_plain(misfit_ctx* c, BP& P) {
     return MISFIT_OK;
 }
 
-// Whole batch enqueued: prologue, grid-wide rounds, then the tail kernels
-// (leftover rounds, counters and completion candidates, completion).
-static int drive_queued(misfit_ctx* c, BP& P) {
-    LaunchCfg L{c->S.blocks, c->stream};
+// The queued sequence of one batch: prologue, grid-wide rounds, then the
+// tail kernels (leftover rounds, counters and completion candidates,
+// completion).
+static void launch_queued(const BP& P, const LaunchCfg& L, int k) {
     mf_launch_batch_prologue(P, L);
-    int k = c->grid_rounds;
     for (int r = 0; r < k; r++) mf_launch_round(P, L, r & 1);
     mf_launch_finish(P, L, k & 1);
     mf_launch_tail_collect(P, L);
     mf_launch_finish_completion(P, L);
+}
+
+// Whole batch enqueued with no host synchronisation inside it.
+static int drive_queued(misfit_ctx* c, BP& P) {
+    LaunchCfg L{c->S.blocks, c->stream};
+    launch_queued(P, L, c->grid_rounds);
     HIPCHK(c, hipGetLastError());
     return MISFIT_OK;
 }

@@ -110,6 +110,63 @@ __device__ __forceinline__ void vread(const BP& P, int32_t x, uint32_t val, uint
     if ((u64)cnt < (u64)T + dec) conf = true;
 }
 
+// Per-wave log of the guarded reads of a speculation round: a fixed segment
+// of MF_RLOG records (vertex, value read, threshold, edge) per wave, so
+// logging needs no global atomics; the cursor lives in LDS.  A read that does
+// not fit marks its edge for a full re-walk instead.
+struct ReadLog {
+    uint4* rec;        // this wave's segment
+    int* cur;          // this wave's cursor (LDS)
+    uint32_t* relog;   // per edge: = round when its log is incomplete
+};
+
+__device__ __forceinline__ void vlog(const ReadLog& lg, int32_t x, uint32_t val, uint32_t T, int32_t e, uint32_t R) {
+    u64 bal = __ballot(1);
+    const int lane = lane_id();
+    const int leader = first_lane(bal);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(lg.cur, __popcll(bal));
+    base = __shfl(base, leader, MF_WAVE);
+    int pos = base + __popcll(bal & ((1ull << lane) - 1ull));
+    if (pos < MF_RLOG) lg.rec[pos] = make_uint4((uint32_t)x, val, T, (uint32_t)e);
+    else lg.relog[e] = R;
+}
+
+// A guarded read: vertex, the mc word read, threshold (see vread).  x < 0:
+// none (the decision rests on per-batch static data only).
+struct Rd {
+    int32_t x;
+    uint32_t val, T;
+};
+__device__ __forceinline__ Rd rd(int32_t x, uint32_t val, uint32_t T) {
+    Rd r;
+    r.x = x;
+    r.val = val;
+    r.T = T;
+    return r;
+}
+__device__ __forceinline__ Rd rd_none() { return rd(-1, 0u, 0u); }
+
+// Where the reads an outcome rests on go: nowhere (plain speculation),
+// checked against this round's marks (validation by re-walk), or appended to
+// the wave's read log (speculation before a flat validation).
+#define REC_NONE 0
+#define REC_CHECK 1
+#define REC_LOG 2
+struct Recorder {
+    int mode;
+    ReadLog lg;
+    uint32_t R;
+    int32_t e;
+    bool conf;  // REC_CHECK: a recorded read conflicts (per lane)
+};
+
+__device__ __forceinline__ void record(const BP& P, Recorder& rc, const Rd& r) {
+    if (r.x < 0) return;
+    if (rc.mode == REC_CHECK) vread(P, r.x, r.val, r.T, rc.R, rc.e, rc.conf);
+    else if (rc.mode == REC_LOG) vlog(rc.lg, r.x, r.val, r.T, rc.e, rc.R);
+}
+
 // Round-tagged counter word R<<32 | count.  Rounds only grow, so raising
 // the word to R<<32 and then adding one counts this round's additions: the
 // two no-return atomics of one lane reach the word in program order.
@@ -478,23 +535,30 @@ __device__ void ph_fa_fill(const BP& P, const GC& c) {
 }
 
 // ---------------------------------------------------------------- the rule
-// Is x (an ADM vertex, x != w) admissible right after w's eviction?
-// xadj: x is a neighbour of w, so w is one of the MIS neighbours it counts.
-template <bool V>
-__device__ bool cand_ok(const BP& P, int32_t x, int32_t w, bool xadj, int32_t e, uint32_t R, bool& conf) {
+// Every outcome records only the reads it rests on.  A blocked outcome rests
+// on one blocker (its witness): while that read holds, the outcome holds,
+// whatever else changes.  A clear outcome (an admission) rests on every read
+// of its scan, which is then walked a second time to record them.  Blocks
+// by per-batch static facts (a vertex outside PU, no row) need no record.
+
+// Is x (an ADM vertex, x != w) admissible right after w's eviction?  xadj:
+// x is a neighbour of w (w is one of the MIS neighbours x counts).  On false,
+// `wit` is the blocking read; with ALL, every read is recorded.
+template <bool ALL>
+__device__ bool cand_ok(const BP& P, int32_t x, int32_t w, bool xadj, Rd& wit, Recorder& rc) {
     const DevGraph& g = P.g;
     uint32_t mx = g.mc[x];
     if (mx & MF_MISBIT) {
-        if (V) vread(P, x, mx, MF_T_MIS, R, e, conf);
+        wit = rd(x, mx, MF_T_MIS);
         return false;
     }
     uint32_t cx = mx & MF_CMASK;
     uint32_t corr = (xadj && cx) ? 1u : 0u;
     if (cx - corr) {
-        if (V) vread(P, x, mx, 1u + corr, R, e, conf);
+        wit = rd(x, mx, 1u + corr);
         return false;
     }
-    if (V) vread(P, x, mx, 0u, R, e, conf);
+    if (ALL) record(P, rc, rd(x, mx, 0u));
     if (g.seedb[x] != P.b) return true;  // candidate row = adj(x)
     // seed row {x} ∪ N1(x) ∪ N2(x) must hold no MIS member
     Row r = get_row(g, x);
@@ -505,55 +569,54 @@ __device__ bool cand_ok(const BP& P, int32_t x, int32_t w, bool xadj, int32_t e,
         if (y == w) {  // w is outside the MIS now; its own count is unchanged
             uint32_t mw = g.mc[w];
             if (mw & MF_CMASK) {
-                if (V) vread(P, w, mw, 1u, R, e, conf);
+                wit = rd(w, mw, 1u);
                 return false;
             }
-            if (V) vread(P, w, mw, 0u, R, e, conf);
+            if (ALL) record(P, rc, rd(w, mw, 0u));
             continue;
         }
-        if (!in_pu(P, y)) return false;
+        if (!in_pu(P, y)) {
+            wit = rd_none();
+            return false;
+        }
         uint32_t my = g.mc[y];
         if (my & MF_MISBIT) {
-            if (V) vread(P, y, my, MF_T_MIS, R, e, conf);
+            wit = rd(y, my, MF_T_MIS);
             return false;
         }
         uint32_t cy = my & MF_CMASK;
-        if (cy == 0) {
-            if (V) vread(P, y, my, 0u, R, e, conf);
-            continue;
-        }
         if (cy >= 2) {
-            if (V) vread(P, y, my, 2u, R, e, conf);
+            wit = rd(y, my, 2u);
             return false;
         }
-        if (!adjacent(g, y, w)) {  // its one MIS neighbour is not w
-            if (V) vread(P, y, my, 1u, R, e, conf);
+        if (cy == 1 && !adjacent(g, y, w)) {  // its one MIS neighbour is not w
+            wit = rd(y, my, 1u);
             return false;
         }
-        if (V) vread(P, y, my, 0u, R, e, conf);
+        if (ALL) record(P, rc, rd(y, my, 0u));
     }
     return true;
 }
 
 // Insertion with both ends in the MIS, w evicted: first admissible vertex of
 // row(w) in BFS order, or -1.  Wave-cooperative; all lanes return the same.
-template <bool V>
-__device__ int32_t eval_insertion(const BP& P, int32_t w, int32_t e, uint32_t R, bool& conf) {
+__device__ int32_t eval_insertion(const BP& P, int32_t w, Recorder& rc) {
     const DevGraph& g = P.g;
     const int lane = lane_id();
+    Recorder none{REC_NONE};
     // position 0: w itself
     {
         uint32_t mw = g.mc[w];
         if (mw & MF_CMASK) {
-            if (V) vread(P, w, mw, 1u, R, e, conf);
+            if (lane == 0) record(P, rc, rd(w, mw, 1u));
         } else {
-            if (V) vread(P, w, mw, 0u, R, e, conf);
             Row r = get_row(g, w);
             int64_t L = r.len();
             bool blocked = false;
             for (int64_t j0 = 0; j0 < L && !blocked; j0 += MF_WAVE) {
                 int64_t j = j0 + lane;
                 bool bl = false;
+                Rd wt = rd_none();
                 if (j < L) {
                     int32_t y = r.at(j);
                     if (y >= 0 && y != w) {
@@ -562,19 +625,28 @@ __device__ int32_t eval_insertion(const BP& P, int32_t w, int32_t e, uint32_t R,
                             uint32_t my = g.mc[y];
                             if (my & MF_MISBIT) {
                                 bl = true;
-                                if (V) vread(P, y, my, MF_T_MIS, R, e, conf);
+                                wt = rd(y, my, MF_T_MIS);
                             } else if ((my & MF_CMASK) >= 2) {  // w was one of them
                                 bl = true;
-                                if (V) vread(P, y, my, 2u, R, e, conf);
-                            } else if (V) {
-                                vread(P, y, my, 0u, R, e, conf);
+                                wt = rd(y, my, 2u);
                             }
                         }
                     }
                 }
-                blocked = __ballot(bl) != 0;
+                u64 bal = __ballot(bl);
+                if (bal) {
+                    blocked = true;
+                    if (lane == first_lane(bal)) record(P, rc, wt);
+                }
             }
-            if (!blocked) return w;
+            if (!blocked) {  // w itself is admissible again: record every read
+                if (lane == 0) record(P, rc, rd(w, mw, 0u));
+                for (int64_t j = lane; j < L; j += MF_WAVE) {
+                    int32_t y = r.at(j);
+                    if (y >= 0 && y != w) record(P, rc, rd(y, g.mc[y], 0u));
+                }
+                return w;
+            }
         }
     }
     Row rw = get_row(g, w);
@@ -583,13 +655,22 @@ __device__ int32_t eval_insertion(const BP& P, int32_t w, int32_t e, uint32_t R,
     for (int64_t j0 = 0; j0 < L; j0 += MF_WAVE) {
         int64_t j = j0 + lane;
         int32_t x = -1;
-        bool ok = false;
+        bool cnd = false, ok = false;
+        Rd wt = rd_none();
         if (j < L) {
             x = rw.at(j);
-            if (x >= 0 && x != w && is_adm(P, x)) ok = cand_ok<V>(P, x, w, true, e, R, conf);
+            if (x >= 0 && x != w && is_adm(P, x)) {
+                cnd = true;
+                ok = cand_ok<false>(P, x, w, true, wt, none);
+            }
         }
         u64 bal = __ballot(ok);
-        if (bal) return __shfl(x, first_lane(bal), MF_WAVE);
+        const int f = bal ? first_lane(bal) : MF_WAVE;
+        if (cnd && !ok && lane < f) record(P, rc, wt);  // earlier candidates stay blocked
+        if (bal) {
+            if (lane == f) (void)cand_ok<true>(P, x, w, true, wt, rc);
+            return __shfl(x, f, MF_WAVE);
+        }
     }
     // part 3: concat over x1 in adj(w) of adj(x1); long rows through the index
     const bool fa_ok = P.ctr[MF_CS * (C_FA_OVF)] == 0;
@@ -652,8 +733,13 @@ __device__ int32_t eval_insertion(const BP& P, int32_t w, int32_t e, uint32_t R,
                     x = off < oblen ? obase[off] : oadd[off - oblen];
                     pos = off;
                 }
-                if (x >= 0 && x != w && is_adm(P, x) && cand_ok<V>(P, x, w, false, e, R, conf))
-                    key = ((u64)(uint32_t)owner << 32) | (u64)(uint32_t)pos;
+                if (x >= 0 && x != w && is_adm(P, x)) {
+                    Rd wt = rd_none();
+                    if (cand_ok<false>(P, x, w, false, wt, none))
+                        key = ((u64)(uint32_t)owner << 32) | (u64)(uint32_t)pos;
+                    else
+                        record(P, rc, wt);  // (also for candidates after the result: harmless)
+                }
             }
             u64 mn = wave_min_u64(key);
             if (mn < best) {
@@ -668,15 +754,20 @@ __device__ int32_t eval_insertion(const BP& P, int32_t w, int32_t e, uint32_t R,
                 if (k0 + MF_WAVE >= end_bo) break;
             }
         }
-        if (best != ~0ull) return best_x;
+        if (best != ~0ull) {
+            if (lane == 0) {
+                Rd wt;
+                (void)cand_ok<true>(P, best_x, w, false, wt, rc);
+            }
+            return best_x;
+        }
     }
     return -1;
 }
 
 // Deletion, exactly one end `other` in the MIS: may tgt join?  row(tgt) =
 // {tgt} ∪ N1 ∪ N2 must hold no MIS member other than `other`.
-template <bool V>
-__device__ bool eval_deletion(const BP& P, int32_t tgt, int32_t other, int32_t e, uint32_t R, bool& conf) {
+__device__ bool eval_deletion(const BP& P, int32_t tgt, int32_t other, Recorder& rc) {
     const DevGraph& g = P.g;
     const int lane = lane_id();
     Row r = get_row(g, tgt);
@@ -684,27 +775,36 @@ __device__ bool eval_deletion(const BP& P, int32_t tgt, int32_t other, int32_t e
     for (int64_t j0 = 0; j0 < L; j0 += MF_WAVE) {
         int64_t j = j0 + lane;
         bool bl = false;
+        Rd wt = rd_none();
         if (j < L) {
             int32_t y = r.at(j);
             if (y >= 0 && y != tgt) {
                 uint32_t my = g.mc[y];
-                uint32_t T;
+                uint32_t cy = my & MF_CMASK;
                 if (y == other) {  // its MIS neighbours sit two hops from tgt
-                    bl = (my & MF_CMASK) != 0;
-                    T = bl ? 1u : 0u;
+                    if (cy) { bl = true; wt = rd(y, my, 1u); }
                 } else if (my & MF_MISBIT) {
                     bl = true;
-                    T = MF_T_MIS;
-                } else {
-                    uint32_t cy = my & MF_CMASK;
-                    if (cy >= 2) { bl = true; T = 2u; }
-                    else if (cy == 1 && !adjacent(g, y, other)) { bl = true; T = 1u; }
-                    else T = 0u;
+                    wt = rd(y, my, MF_T_MIS);
+                } else if (cy >= 2) {
+                    bl = true;
+                    wt = rd(y, my, 2u);
+                } else if (cy == 1 && !adjacent(g, y, other)) {
+                    bl = true;
+                    wt = rd(y, my, 1u);
                 }
-                if (V) vread(P, y, my, T, R, e, conf);
             }
         }
-        if (__ballot(bl)) return false;
+        u64 bal = __ballot(bl);
+        if (bal) {
+            if (lane == first_lane(bal)) record(P, rc, wt);
+            return false;
+        }
+    }
+    // tgt joins: every read of the scan decides it
+    for (int64_t j = lane; j < L; j += MF_WAVE) {
+        int32_t y = r.at(j);
+        if (y >= 0 && y != tgt) record(P, rc, rd(y, g.mc[y], 0u));
     }
     return true;
 }
@@ -758,44 +858,48 @@ __device__ __forceinline__ void mark_vertex(const BP& P, int32_t z, u64 mv, bool
     }
 }
 
-// Speculative evaluation of active edge e (kind k) by one wave.
-__device__ __forceinline__ void spec_edge(const BP& P, int32_t e, uint8_t k, uint32_t R, u64 bound, int lane) {
+// Speculative evaluation of active edge e (kind k) by one wave; the reads
+// its outcome rests on go to `rc` (nowhere, or the wave's read log).
+__device__ __forceinline__ void spec_edge(const BP& P, int32_t e, uint8_t k, uint32_t R, u64 bound, int lane,
+                                          Recorder& rc) {
     int32_t a = P.bu[e], bb = P.bv[e];
-    bool dummy = false;
+    rc.R = R;
+    rc.e = e;
     u64 mv = ((u64)R << 32) | (u64)(0xFFFFFFFFu - (uint32_t)e);
     if (k == 1) {
         int32_t w = a > bb ? a : bb;
-        int32_t x = eval_insertion<false>(P, w, e, R, dummy);
+        int32_t x = eval_insertion(P, w, rc);
         if (lane == 0) { P.res_w[e] = w; P.res_x[e] = x; }
         mark_vertex(P, w, mv, true, R, bound, lane);
         if (x >= 0) mark_vertex(P, x, mv, false, R, bound, lane);
     } else {
         bool ma = (P.g.mc[a] & MF_MISBIT) != 0;
         int32_t tgt = ma ? bb : a, other = ma ? a : bb;
-        bool ok = eval_deletion<false>(P, tgt, other, e, R, dummy);
+        bool ok = eval_deletion(P, tgt, other, rc);
         if (lane == 0) { P.res_w[e] = -1; P.res_x[e] = ok ? tgt : -1; }
         if (ok) mark_vertex(P, tgt, mv, false, R, bound, lane);
     }
 }
 
-// Re-walk of active edge e: true when it read something an earlier edge of
-// the round may have changed.  Wave-uniform.
+// Re-walk of active edge e: true when a read its outcome rests on may have
+// been changed by an earlier edge of the round.  Wave-uniform.
 __device__ __forceinline__ bool validate_edge(const BP& P, int32_t e, uint8_t k, uint32_t R) {
     int32_t a = P.bu[e], bb = P.bv[e];
-    bool conf = mk_before(P.g.mmark, a, R, e) || mk_before(P.g.mmark, bb, R, e);
+    Recorder rc{REC_CHECK};
+    rc.R = R;
+    rc.e = e;
+    rc.conf = mk_before(P.g.mmark, a, R, e) || mk_before(P.g.mmark, bb, R, e);
     if (k == 1) {
         int32_t w = a > bb ? a : bb;
-        (void)eval_insertion<true>(P, w, e, R, conf);
+        (void)eval_insertion(P, w, rc);
     } else {
         bool ma = (P.g.mc[a] & MF_MISBIT) != 0;
         int32_t tgt = ma ? bb : a, other = ma ? a : bb;
-        (void)eval_deletion<true>(P, tgt, other, e, R, conf);
+        (void)eval_deletion(P, tgt, other, rc);
     }
-    return __ballot(conf) != 0;
+    return __ballot(rc.conf) != 0;
 }
 
-// Membership flip of z with its neighbours' counts; an eviction also tags
-// the neighbours (completion candidates next to an evicted vertex).
 __device__ __forceinline__ void apply_flip(const BP& P, int32_t z, bool join, int lane) {
     if (lane == 0) {
         if (join) atomicOr(&P.g.mc[z], MF_MISBIT);
@@ -850,31 +954,57 @@ __device__ __forceinline__ void advance_round(const BP& P, int par, int64_t conf
 }
 
 // Grid-wide rounds: one wave per edge of the window.  Speculation classifies
-// the edge first (the kind depends only on the committed state), so a round
-// is three kernels: speculate, validate, commit.
-__device__ void ph_spec_w(const BP& P, const GC& c, const RoundView& v) {
+// the edge first (the kind depends only on the committed state) and logs its
+// guarded reads, so a round is three kernels: speculate, validate, commit.
+// `cur` is the LDS cursor array of the workgroup's waves.
+__device__ void ph_spec_w(const BP& P, const GC& c, const RoundView& v, int* cur) {
+    const int wiw = (int)(threadIdx.x / MF_WAVE);
+    if (c.lane == 0) cur[wiw] = 0;
+    __syncthreads();
+    Recorder rc{REC_LOG, ReadLog{P.rlog + c.wave * MF_RLOG, cur + wiw, P.relog}};
     for (int64_t e = v.start + c.wave; e < v.end; e += c.nwaves) {
         int32_t a = P.bu[e], bb = P.bv[e];
         uint8_t k = classify_edge(P, e, a, bb);
         if (c.lane == 0) P.kind[e] = k;
-        if (k) spec_edge(P, (int32_t)e, k, v.R, (u64)P.n_ins, c.lane);
+        if (k) spec_edge(P, (int32_t)e, k, v.R, (u64)P.n_ins, c.lane, rc);
+    }
+    if (c.lane == 0) {
+        int n = cur[wiw];
+        P.rlog_cnt[c.wave] = n < MF_RLOG ? n : MF_RLOG;
     }
 }
 
+// Validation: every logged read is checked on its own (one thread per log
+// slot: a few independent loads instead of the evaluation's dependent
+// chain); then every edge of the window checks its endpoints' membership
+// (its classification read it), and an edge whose log overflowed is
+// re-walked.  Same grid as the speculation kernel, so wave w's log segment
+// is rlog[w * MF_RLOG ...].
 __device__ void ph_validate_w(const BP& P, const GC& c, const RoundView& v, int par) {
+    u64 mine = ~0ull;
+    const int64_t nslots = c.nwaves * MF_RLOG;
+    for (int64_t s = c.tid; s < nslots; s += c.nthreads) {
+        if ((int32_t)(s % MF_RLOG) >= P.rlog_cnt[s / MF_RLOG]) continue;
+        uint4 r = P.rlog[s];
+        bool conf = false;
+        vread(P, (int32_t)r.x, r.y, r.z, v.R, (int32_t)r.w, conf);
+        if (conf && (u64)r.w < mine) mine = (u64)r.w;
+    }
+    mine = wave_min_u64(mine);  // wave-uniform from here on
     for (int64_t e = v.start + c.wave; e < v.end; e += c.nwaves) {
         // an edge past a conflict already found cannot move the minimum
-        if ((u64)e > P.ctr[MF_CS * (C_CONF0 + par)]) break;
+        if ((u64)e > P.ctr[MF_CS * (C_CONF0 + par)] || (u64)e >= mine) break;
         uint8_t k = P.kind[e];
         bool conf;
-        if (!k) {  // its classification read both endpoints' membership
+        if (k && P.relog[e] == v.R) {
+            conf = validate_edge(P, (int32_t)e, k, v.R);
+        } else {
             int32_t a = P.bu[e], bb = P.bv[e];
             conf = mk_before(P.g.mmark, a, v.R, (int32_t)e) || mk_before(P.g.mmark, bb, v.R, (int32_t)e);
-        } else {
-            conf = validate_edge(P, (int32_t)e, k, v.R);
         }
-        if (conf && c.lane == 0) atomicMin(&P.ctr[MF_CS * (C_CONF0 + par)], (u64)e);
+        if (conf && (u64)e < mine) mine = (u64)e;
     }
+    if (mine != ~0ull && c.lane == 0) atomicMin(&P.ctr[MF_CS * (C_CONF0 + par)], mine);
 }
 
 __device__ void ph_commit_w(const BP& P, const GC& c, const RoundView& v, int par) {
@@ -902,7 +1032,8 @@ __device__ void ph_spec_l(const BP& P, const GC& c, int par, uint32_t R) {
     int64_t na = (int64_t)P.ctr[MF_CS * (C_NACT0 + par)];
     for (int64_t i = c.wave; i < na; i += c.nwaves) {
         int32_t e = P.active[i];
-        spec_edge(P, e, P.kind[e], R, (u64)na, c.lane);
+        Recorder none{REC_NONE};
+        spec_edge(P, e, P.kind[e], R, (u64)na, c.lane, none);
     }
 }
 
@@ -1060,9 +1191,10 @@ MF_KERNEL(k_comp_decide, ph_comp_decide(P, c))
 MF_KERNEL(k_stats, ph_stats(P, c))
 
 __global__ __launch_bounds__(256) void k_spec(BP P, int par) {
+    __shared__ int cur[256 / MF_WAVE];
     GC c = grid_coords();
     RoundView v = round_view(P, par);
-    if (v.live) ph_spec_w(P, c, v);
+    if (v.live) ph_spec_w(P, c, v, cur);
 }
 __global__ __launch_bounds__(256) void k_validate(BP P, int par) {
     GC c = grid_coords();
