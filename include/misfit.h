@@ -183,6 +183,33 @@ int misfit_egr_close(misfit_egr* g);
 int misfit_write_egr(const char* path, int32_t n, int64_t m, const int64_t* nindex, const int32_t* nlist,
                      char* err, int errlen);
 
+/* .mtx -> .egr conversion.  Replaces tools/efficient_converter.py:9-113 with
+ * the same result: the first non-comment line is the dimension line (n = the
+ * larger of its first two fields); every later line "u v" (1-indexed) with
+ * u <= n appends v-1 to row u-1 in file order; lines with u > n are dropped;
+ * nothing is symmetrised, sorted or deduplicated; the wide form is written.
+ * Returns n and the number of entries written. */
+int misfit_mtx_to_egr(const char* mtx_path, const char* egr_path, int32_t* n, int64_t* m, char* err, int errlen);
+
+/* .egr validator (no reference counterpart: the reference trusts its input,
+ * core/FlashGraph.h:108-169).  Counts offsets that are not a valid prefix
+ * sum, entries outside [0, n), self-loops, repeated entries in a row, rows
+ * not strictly ascending and, on ascending in-range rows, entries (u, v)
+ * without (v, u) (-1 when that is not checked). */
+typedef struct {
+    int32_t n;
+    int64_t m;
+    int wide;
+    int64_t bad_offsets;
+    int64_t out_of_range;
+    int64_t self_loops;
+    int64_t duplicates;
+    int64_t unsorted_rows;
+    int64_t asymmetric;
+} misfit_egr_report;
+
+int misfit_egr_check(const char* path, misfit_egr_report* report, char* err, int errlen);
+
 /* ---- synthetic workloads (seeded, deterministic) ---- */
 
 /* Erdős–Rényi G(n, M): M distinct undirected pairs, symmetric CSR with rows

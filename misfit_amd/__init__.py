@@ -60,6 +60,13 @@ class EgrView(ctypes.Structure):
                 ("nindex", ctypes.c_void_p), ("nlist", ctypes.c_void_p), ("priv", ctypes.c_void_p)]
 
 
+class EgrReport(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32), ("m", ctypes.c_int64), ("wide", ctypes.c_int),
+                ("bad_offsets", ctypes.c_int64), ("out_of_range", ctypes.c_int64),
+                ("self_loops", ctypes.c_int64), ("duplicates", ctypes.c_int64),
+                ("unsorted_rows", ctypes.c_int64), ("asymmetric", ctypes.c_int64)]
+
+
 # every symbol include/misfit.h declares
 EXPORTS = (
     "misfit_ctx_create", "misfit_ctx_destroy", "misfit_last_error", "misfit_load_csr",
@@ -68,6 +75,7 @@ EXPORTS = (
     "misfit_compact", "misfit_graph_size", "misfit_export_csr", "misfit_neighbors",
     "misfit_batch_bytes", "misfit_device_info", "misfit_egr_open", "misfit_egr_close", "misfit_write_egr",
     "misfit_gen_er", "misfit_gen_rmat", "misfit_gen_rgg", "misfit_greedy_mis", "misfit_gen_updates",
+    "misfit_mtx_to_egr", "misfit_egr_check",
 )
 
 _lib = None
@@ -107,6 +115,9 @@ def lib():
         L.misfit_egr_close.argtypes = [ctypes.POINTER(EgrView)]
         L.misfit_write_egr.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int64, _i64p, _i32p,
                                        ctypes.c_char_p, ctypes.c_int]
+        L.misfit_mtx_to_egr.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _i32p, _i64p, ctypes.c_char_p,
+                                        ctypes.c_int]
+        L.misfit_egr_check.argtypes = [ctypes.c_char_p, ctypes.POINTER(EgrReport), ctypes.c_char_p, ctypes.c_int]
         L.misfit_gen_er.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, _i64p, _i32p, _i64p]
         L.misfit_gen_rmat.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_uint64, _i64p, _i32p, _i64p]
@@ -172,6 +183,29 @@ def write_narrow_egr(path, nindex, nlist):
         f.write(np.array([n, m], np.int32).tobytes())
         f.write(np.asarray(nindex, np.int32).tobytes())
         f.write(np.asarray(nlist, np.int32).tobytes())
+
+
+def mtx_to_egr(mtx_path, egr_path):
+    """Convert a text edge list to the wide .egr form (tools/efficient_converter.py
+    semantics); returns (n, m)."""
+    n = ctypes.c_int32()
+    m = ctypes.c_int64()
+    err = ctypes.create_string_buffer(512)
+    rc = lib().misfit_mtx_to_egr(os.fsencode(mtx_path), os.fsencode(egr_path), ctypes.byref(n), ctypes.byref(m),
+                                 err, 512)
+    if rc != 0:
+        raise MisfitError(rc, err.value.decode())
+    return n.value, m.value
+
+
+def egr_check(path):
+    """Structure report of a .egr file (dict)."""
+    r = EgrReport()
+    err = ctypes.create_string_buffer(512)
+    rc = lib().misfit_egr_check(os.fsencode(path), ctypes.byref(r), err, 512)
+    if rc != 0:
+        raise MisfitError(rc, err.value.decode())
+    return {k: getattr(r, k) for k, _ in EgrReport._fields_}
 
 
 # ------------------------------------------------------------------ generators

@@ -101,8 +101,9 @@ struct BP {
 
 // host-side launchers (k_batch.hip)
 struct LaunchCfg {
-    int blocks;        // grid for grid-stride phases
+    int blocks;        // grid cap for grid-stride phases (8 workgroups per CU)
     hipStream_t stream;
+    int round_per_cu = 0;  // speculation/validation workgroups per CU (0: by occupancy)
 };
 void mf_launch_batch_prologue(const BP& P, const LaunchCfg& L);
 void mf_launch_round(const BP& P, const LaunchCfg& L, int par);

@@ -50,6 +50,7 @@ struct misfit_ctx {
     int mode = MISFIT_MODE_QUEUED;
     int grid_rounds = 12;      // grid-wide rounds queued before the finisher
     bool grid_rounds_fixed = false;
+    int round_per_cu = 0;      // MISFIT_ROUND_WG_PER_CU: round grid (0: by occupancy)
     int rounds_hist[16] = {0};  // rounds of recent batches
     int rounds_pos = 0;
     bool debug_rounds = false;
@@ -133,6 +134,8 @@ extern "C" int misfit_ctx_create(int device, misfit_ctx** out) {
     }
     const char* dbg = getenv("MISFIT_DEBUG_ROUNDS");
     c->debug_rounds = dbg && dbg[0] == '1';
+    const char* rw = getenv("MISFIT_ROUND_WG_PER_CU");
+    if (rw) c->round_per_cu = std::max(0, std::min(8, atoi(rw)));
     const char* gr = getenv("MISFIT_GRID_ROUNDS");
     if (gr) {
         c->grid_rounds = std::max(0, atoi(gr));
@@ -564,7 +567,7 @@ static void fill_stats(const misfit_ctx* c, const Slot& s, misfit_batch_stats* s
 // Host-driven rounds: one round = 4 kernels, then the host reads where the
 // committed prefix ends; the completion iterations likewise.
 static int drive_plain(misfit_ctx* c, BP& P) {
-    LaunchCfg L{c->S.blocks, c->stream};
+    LaunchCfg L{c->S.blocks, c->stream, c->round_per_cu};
     mf_launch_batch_prologue(P, L);
     for (uint32_t r = 0;; r++) {
         if ((int64_t)r > P.U + 1) return fail(c, MISFIT_E_CHECK, "speculative rounds made no progress");
@@ -613,7 +616,7 @@ static void launch_queued(const BP& P, const LaunchCfg& L, int k) {
 
 // Whole batch enqueued with no host synchronisation inside it.
 static int drive_queued(misfit_ctx* c, BP& P) {
-    LaunchCfg L{c->S.blocks, c->stream};
+    LaunchCfg L{c->S.blocks, c->stream, c->round_per_cu};
     launch_queued(P, L, c->grid_rounds);
     HIPCHK(c, hipGetLastError());
     return MISFIT_OK;

@@ -48,6 +48,8 @@
 //      the round.  Every edge before it saw exactly the sequential state and
 //      commits; the rest are re-evaluated next round (the first pending edge
 //      never conflicts, so every round makes progress).
+#include <mutex>
+
 #include "batch.h"
 
 // ---------------------------------------------------------------- helpers
@@ -89,13 +91,19 @@ __device__ __forceinline__ bool mk_before(const u64* mk, int32_t x, uint32_t R, 
 // Validation of one read of mc[x] that returned `val` by edge e.  The
 // decision taken from it holds while the membership bit is unchanged and the
 // count stays at or above T (T = 0: any change of the count matters;
-// MF_T_MIS: the count does not matter).  The read conflicts when an earlier
-// edge of the round flipped the bit, or changed the count and the evictions
-// speculated next to x this round could take it below T.
+// MF_T_MIS: the count does not matter).  With MF_T_NOFLIP (on a count
+// threshold of a non-member) a flip of x does not matter either: a
+// non-member can only join the MIS, and a member blocks wherever a
+// non-member with that many MIS neighbours blocks.  The read conflicts when
+// an earlier edge of the round flipped the bit, or changed the count and the
+// evictions speculated next to x this round could take it below T.
 #define MF_T_MIS 0xFFFFFFFFu
+#define MF_T_NOFLIP 0x80000000u
 __device__ __forceinline__ void vread(const BP& P, int32_t x, uint32_t val, uint32_t T, uint32_t R, int32_t e,
                                       bool& conf) {
-    if (mk_before(P.g.mmark, x, R, e)) {
+    const bool noflip = T != MF_T_MIS && (T & MF_T_NOFLIP);
+    if (noflip) T &= ~MF_T_NOFLIP;
+    if (!noflip && mk_before(P.g.mmark, x, R, e)) {
         conf = true;
         return;
     }
@@ -539,7 +547,9 @@ __device__ void ph_fa_fill(const BP& P, const GC& c) {
 // on one blocker (its witness): while that read holds, the outcome holds,
 // whatever else changes.  A clear outcome (an admission) rests on every read
 // of its scan, which is then walked a second time to record them.  Blocks
-// by per-batch static facts (a vertex outside PU, no row) need no record.
+// by per-batch static facts (a vertex outside PU, no row, an MIS member
+// outside W, which nothing can evict) need no record, and scans prefer such a
+// blocker when they have one.
 
 // Is x (an ADM vertex, x != w) admissible right after w's eviction?  xadj:
 // x is a neighbour of w (w is one of the MIS neighbours x counts).  On false,
@@ -555,7 +565,7 @@ __device__ bool cand_ok(const BP& P, int32_t x, int32_t w, bool xadj, Rd& wit, R
     uint32_t cx = mx & MF_CMASK;
     uint32_t corr = (xadj && cx) ? 1u : 0u;
     if (cx - corr) {
-        wit = rd(x, mx, 1u + corr);
+        wit = rd(x, mx, (1u + corr) | MF_T_NOFLIP);
         return false;
     }
     if (ALL) record(P, rc, rd(x, mx, 0u));
@@ -569,7 +579,7 @@ __device__ bool cand_ok(const BP& P, int32_t x, int32_t w, bool xadj, Rd& wit, R
         if (y == w) {  // w is outside the MIS now; its own count is unchanged
             uint32_t mw = g.mc[w];
             if (mw & MF_CMASK) {
-                wit = rd(w, mw, 1u);
+                wit = rd(w, mw, 1u | MF_T_NOFLIP);
                 return false;
             }
             if (ALL) record(P, rc, rd(w, mw, 0u));
@@ -581,16 +591,16 @@ __device__ bool cand_ok(const BP& P, int32_t x, int32_t w, bool xadj, Rd& wit, R
         }
         uint32_t my = g.mc[y];
         if (my & MF_MISBIT) {
-            wit = rd(y, my, MF_T_MIS);
+            wit = g.wpotb[y] == P.b ? rd(y, my, MF_T_MIS) : rd_none();
             return false;
         }
         uint32_t cy = my & MF_CMASK;
         if (cy >= 2) {
-            wit = rd(y, my, 2u);
+            wit = rd(y, my, 2u | MF_T_NOFLIP);
             return false;
         }
         if (cy == 1 && !adjacent(g, y, w)) {  // its one MIS neighbour is not w
-            wit = rd(y, my, 1u);
+            wit = rd(y, my, 1u | MF_T_NOFLIP);
             return false;
         }
         if (ALL) record(P, rc, rd(y, my, 0u));
@@ -604,31 +614,33 @@ __device__ int32_t eval_insertion(const BP& P, int32_t w, Recorder& rc) {
     const DevGraph& g = P.g;
     const int lane = lane_id();
     Recorder none{REC_NONE};
-    // position 0: w itself
-    {
+    // position 0: w itself.  Outside PU, w keeps an MIS neighbour whenever it
+    // is outside the MIS (its insertion partner), so it is blocked for sure.
+    if (in_pu(P, w)) {
         uint32_t mw = g.mc[w];
         if (mw & MF_CMASK) {
-            if (lane == 0) record(P, rc, rd(w, mw, 1u));
+            if (lane == 0) record(P, rc, rd(w, mw, 1u | MF_T_NOFLIP));
         } else {
             Row r = get_row(g, w);
             int64_t L = r.len();
             bool blocked = false;
             for (int64_t j0 = 0; j0 < L && !blocked; j0 += MF_WAVE) {
                 int64_t j = j0 + lane;
-                bool bl = false;
+                bool bl = false, fixed = false;
                 Rd wt = rd_none();
                 if (j < L) {
                     int32_t y = r.at(j);
                     if (y >= 0 && y != w) {
-                        if (!in_pu(P, y)) bl = true;
+                        if (!in_pu(P, y)) bl = fixed = true;
                         else {
                             uint32_t my = g.mc[y];
                             if (my & MF_MISBIT) {
                                 bl = true;
+                                fixed = g.wpotb[y] != P.b;  // a member outside W stays one
                                 wt = rd(y, my, MF_T_MIS);
                             } else if ((my & MF_CMASK) >= 2) {  // w was one of them
                                 bl = true;
-                                wt = rd(y, my, 2u);
+                                wt = rd(y, my, 2u | MF_T_NOFLIP);
                             }
                         }
                     }
@@ -636,7 +648,7 @@ __device__ int32_t eval_insertion(const BP& P, int32_t w, Recorder& rc) {
                 u64 bal = __ballot(bl);
                 if (bal) {
                     blocked = true;
-                    if (lane == first_lane(bal)) record(P, rc, wt);
+                    if (!__ballot(fixed) && lane == first_lane(bal)) record(P, rc, wt);
                 }
             }
             if (!blocked) {  // w itself is admissible again: record every read
@@ -774,7 +786,7 @@ __device__ bool eval_deletion(const BP& P, int32_t tgt, int32_t other, Recorder&
     int64_t L = r.len();
     for (int64_t j0 = 0; j0 < L; j0 += MF_WAVE) {
         int64_t j = j0 + lane;
-        bool bl = false;
+        bool bl = false, fixed = false;
         Rd wt = rd_none();
         if (j < L) {
             int32_t y = r.at(j);
@@ -782,22 +794,23 @@ __device__ bool eval_deletion(const BP& P, int32_t tgt, int32_t other, Recorder&
                 uint32_t my = g.mc[y];
                 uint32_t cy = my & MF_CMASK;
                 if (y == other) {  // its MIS neighbours sit two hops from tgt
-                    if (cy) { bl = true; wt = rd(y, my, 1u); }
+                    if (cy) { bl = true; wt = rd(y, my, 1u | MF_T_NOFLIP); }
                 } else if (my & MF_MISBIT) {
                     bl = true;
+                    fixed = g.wpotb[y] != P.b;  // a member outside W stays one
                     wt = rd(y, my, MF_T_MIS);
                 } else if (cy >= 2) {
                     bl = true;
-                    wt = rd(y, my, 2u);
+                    wt = rd(y, my, 2u | MF_T_NOFLIP);
                 } else if (cy == 1 && !adjacent(g, y, other)) {
                     bl = true;
-                    wt = rd(y, my, 1u);
+                    wt = rd(y, my, 1u | MF_T_NOFLIP);
                 }
             }
         }
         u64 bal = __ballot(bl);
         if (bal) {
-            if (lane == first_lane(bal)) record(P, rc, wt);
+            if (!__ballot(fixed) && lane == first_lane(bal)) record(P, rc, wt);
             return false;
         }
     }
@@ -1309,7 +1322,56 @@ __global__ __launch_bounds__(256) void k_publish(BP P) {
 }
 
 // ---------------------------------------------------------------- launchers
-#define LAUNCH(k, ...) hipLaunchKernelGGL(k, dim3(L.blocks), dim3(256), 0, L.stream, __VA_ARGS__)
+// Grids are sized to what the GPU holds at once: a kernel's workgroups beyond
+// its occupancy wait for earlier ones to finish, and in these latency-bound
+// phases that wait is a whole evaluation chain (measured 10-17 us per round
+// at 8 workgroups per CU when registers allow 7).  The occupancy of each
+// kernel is queried once per process and device.
+namespace {
+struct OccEntry {
+    const void* fn;
+    int device;
+    int per_cu;
+};
+std::mutex g_occ_mu;
+OccEntry g_occ[64];
+int g_nocc = 0;
+
+int occ_per_cu(const void* fn, int threads) {
+    int dev = 0;
+    hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lk(g_occ_mu);
+    for (int i = 0; i < g_nocc; i++)
+        if (g_occ[i].fn == fn && g_occ[i].device == dev) return g_occ[i].per_cu;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, threads, 0) != hipSuccess || nb < 1) nb = 1;
+    if (g_nocc < 64) g_occ[g_nocc++] = OccEntry{fn, dev, nb};
+    return nb;
+}
+
+// L.blocks is the cap (8 workgroups per CU); the grid is min(cap, resident).
+int grid_for(const void* fn, const LaunchCfg& L) {
+    int per_cu = occ_per_cu(fn, 256);
+    int resident = per_cu * (L.blocks / 8);
+    return resident < L.blocks ? resident : L.blocks;
+}
+}  // namespace
+
+// Speculation and validation share one grid: wave w of the speculation
+// writes read-log segment w, which wave w's slots of the validation check.
+// L.round_per_cu > 0 sets it (workgroups per CU) instead of the occupancy.
+int mf_round_grid(const LaunchCfg& L) {
+    if (L.round_per_cu > 0) {
+        int g = L.round_per_cu * (L.blocks / 8);
+        return g < L.blocks ? g : L.blocks;
+    }
+    int a = grid_for((const void*)k_spec, L), b = grid_for((const void*)k_validate, L);
+    return a < b ? a : b;
+}
+
+#define LAUNCH(k, ...) \
+    hipLaunchKernelGGL(k, dim3(grid_for((const void*)k, L)), dim3(256), 0, L.stream, __VA_ARGS__)
+#define LAUNCH_ROUND(k, ...) hipLaunchKernelGGL(k, dim3(mf_round_grid(L)), dim3(256), 0, L.stream, __VA_ARGS__)
 
 void mf_launch_batch_prologue(const BP& P, const LaunchCfg& L) {
     LAUNCH(k_init, P);
@@ -1327,8 +1389,8 @@ void mf_launch_batch_prologue(const BP& P, const LaunchCfg& L) {
 }
 
 void mf_launch_round(const BP& P, const LaunchCfg& L, int par) {
-    LAUNCH(k_spec, P, par);
-    LAUNCH(k_validate, P, par);
+    LAUNCH_ROUND(k_spec, P, par);
+    LAUNCH_ROUND(k_validate, P, par);
     LAUNCH(k_commit, P, par);
 }
 

@@ -147,3 +147,81 @@ def test_binaries_usage_without_gpu(engine_mod):
         r = subprocess.run([exe], capture_output=True, text=True)
         assert r.returncode == 1
         assert "Usage:" in r.stderr and "<num_batches>" in r.stderr
+
+
+# ------------------------------------------------------------ .mtx -> .egr and the validator
+def test_converter_reproduces_reference_sample(engine_mod, tmp_path):
+    """sample.egr in the reference's dataset is the converter's output for
+    sample.mtx (tools/efficient_converter.py): ours must give the same bytes."""
+    out = tmp_path / "s.egr"
+    n, m = engine_mod.mtx_to_egr(os.path.join(SAMPLE, "sample.mtx"), str(out))
+    assert (n, m) == (2000, 16000)
+    assert out.read_bytes() == open(os.path.join(SAMPLE, "sample.egr"), "rb").read()
+
+
+def test_converter_semantics(engine_mod, tmp_path):
+    """Dimension line = max(rows, cols); rows in file order; u > n dropped;
+    short lines and later comments skipped; no symmetrisation or dedup."""
+    src = tmp_path / "g.mtx"
+    src.write_text("%%MatrixMarket matrix coordinate pattern general\n% comment\n"
+                   "3 4\n"            # no entry count: n = 4
+                   "2 3\n1 4\n2 1\n"
+                   "% inner comment\n"
+                   "9 1\n"            # u > n: dropped
+                   "2 3\n"            # duplicate kept
+                   "4 4\n"            # self-loop kept
+                   "7\n")             # one field: skipped
+    out = tmp_path / "g.egr"
+    n, m = engine_mod.mtx_to_egr(str(src), str(out))
+    assert (n, m) == (4, 5)
+    ni, nl, wide = engine_mod.read_egr(str(out))
+    assert wide
+    assert ni.tolist() == [0, 1, 4, 4, 5]
+    assert nl.tolist() == [3, 2, 0, 2, 3]
+    rep = engine_mod.egr_check(str(out))
+    assert rep["duplicates"] == 1 and rep["self_loops"] == 1 and rep["unsorted_rows"] == 1
+    assert rep["asymmetric"] == -1 and rep["bad_offsets"] == 0 and rep["out_of_range"] == 0
+    bad = tmp_path / "z.mtx"
+    bad.write_text("2 2\n0 1\n")
+    with pytest.raises(engine_mod.MisfitError):
+        engine_mod.mtx_to_egr(str(bad), str(tmp_path / "z.egr"))
+
+
+def test_validator_flags_asymmetry_and_range(engine_mod, tmp_path):
+    ni = np.array([0, 2, 3, 3], np.int64)
+    nl = np.array([1, 2, 0], np.int32)       # 0-1 both ways, 0-2 one way
+    p = tmp_path / "a.egr"
+    engine_mod.write_egr(str(p), ni, nl)
+    rep = engine_mod.egr_check(str(p))
+    assert rep["asymmetric"] == 1 and rep["unsorted_rows"] == 0
+    nl2 = np.array([1, 5, 0], np.int32)      # 5 is outside [0, 3)
+    engine_mod.write_egr(str(p), ni, nl2)
+    rep = engine_mod.egr_check(str(p))
+    assert rep["out_of_range"] == 1 and rep["asymmetric"] == -1
+
+
+def test_egr_tool_binary(engine_mod, tmp_path):
+    exe = os.path.join(ROOT, "misfit_amd", "bin", "misfit_egr")
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 1 and "Usage" in r.stderr
+    out = tmp_path / "s.egr"
+    r = subprocess.run([exe, "convert", os.path.join(SAMPLE, "sample.mtx"), str(out)], capture_output=True, text=True)
+    assert r.returncode == 0 and "2000 nodes, 16000 edges" in r.stdout
+    r = subprocess.run([exe, "check", str(out)], capture_output=True, text=True)
+    assert r.returncode == 0 and r.stdout.strip().endswith("OK")
+
+
+def test_run_script_arguments():
+    """misfit_amd/run.sh keeps the reference's flags (run.sh:17-127)."""
+    sh = os.path.join(ROOT, "misfit_amd", "run.sh")
+    r = subprocess.run(["bash", sh, "--help"], capture_output=True, text=True)
+    assert r.returncode == 0 and "--variant=misfit|misfit_async" in r.stderr
+    r = subprocess.run(["bash", sh], capture_output=True, text=True)
+    assert r.returncode == 1 and "--variant" in r.stderr
+    r = subprocess.run(["bash", sh, "--variant=other", "--sample"], capture_output=True, text=True)
+    assert r.returncode == 1 and "must be 'misfit' or 'misfit_async'" in r.stderr
+    r = subprocess.run(["bash", sh, "--variant=misfit", "--bogus"], capture_output=True, text=True)
+    assert r.returncode == 1 and "unrecognized argument" in r.stderr
+    r = subprocess.run(["bash", sh, "--variant=misfit", "--graph=/nonexistent.egr", "--mis=/x", "--batch=/y"],
+                       capture_output=True, text=True)
+    assert r.returncode == 1 and "graph not found" in r.stderr
