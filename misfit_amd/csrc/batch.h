@@ -108,8 +108,6 @@ struct LaunchCfg {
 void mf_launch_batch_prologue(const BP& P, const LaunchCfg& L);
 void mf_launch_round(const BP& P, const LaunchCfg& L, int par);
 void mf_launch_finish(const BP& P, const LaunchCfg& L, int par);
-void mf_launch_tail_collect(const BP& P, const LaunchCfg& L);
-void mf_launch_finish_completion(const BP& P, const LaunchCfg& L);
 void mf_launch_completion_collect(const BP& P, const LaunchCfg& L);
 void mf_launch_stats(const BP& P, const LaunchCfg& L);
 void mf_launch_completion_round(const BP& P, const LaunchCfg& L, int par);

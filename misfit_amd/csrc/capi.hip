@@ -604,14 +604,11 @@ static int drive_plain(misfit_ctx* c, BP& P) {
 }
 
 // The queued sequence of one batch: prologue, grid-wide rounds, then the
-// tail kernels (leftover rounds, counters and completion candidates,
-// completion).
+// one-workgroup tail (leftover rounds, counters, completion, publication).
 static void launch_queued(const BP& P, const LaunchCfg& L, int k) {
     mf_launch_batch_prologue(P, L);
     for (int r = 0; r < k; r++) mf_launch_round(P, L, r & 1);
     mf_launch_finish(P, L, k & 1);
-    mf_launch_tail_collect(P, L);
-    mf_launch_finish_completion(P, L);
 }
 
 // Whole batch enqueued with no host synchronisation inside it.

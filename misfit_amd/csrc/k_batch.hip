@@ -292,10 +292,12 @@ __device__ void ph_resolve(const BP& P, const GC& c) {
             u64 s = ph_slot(g, key);
             g.ph_val[s] = alive ? 1 : 0;
         }
-        if (alive && last_ins >= 0) {
+        if (alive && last_ins >= 0) {  // new entries, chained per vertex
             int64_t k = 2 * i;
             P.ne_x[k] = lo; P.ne_y[k] = hi; P.ne_seq[k] = last_ins;
             P.ne_x[k + 1] = hi; P.ne_y[k + 1] = lo; P.ne_seq[k + 1] = last_ins;
+            P.ne_nxt[k] = atomicExch(&g.ne_head[lo], (int32_t)k);
+            P.ne_nxt[k + 1] = atomicExch(&g.ne_head[hi], (int32_t)(k + 1));
         }
         if (adj != adj0) {
             uint32_t d = adj ? 1u : 0xFFFFFFFFu;  // +1 or -1 on the count bits
@@ -305,17 +307,6 @@ __device__ void ph_resolve(const BP& P, const GC& c) {
         }
     }
     wave_add(&P.ctr[MF_CS * (C_DELTA_OPS)], my_ops);
-}
-
-// P3: chain the new entries of each vertex.
-__device__ void ph_group(const BP& P, const GC& c) {
-    int64_t ne = 2 * P.U;
-    for (int64_t k = c.tid; k < ne; k += c.nthreads) {
-        int32_t x = P.ne_x[k];
-        if (x < 0) continue;
-        int32_t prev = atomicExch(&P.g.ne_head[x], (int32_t)k);
-        P.ne_nxt[k] = prev;
-    }
 }
 
 // P4: the chain owner appends the vertex's new entries to its run in batch
@@ -456,44 +447,32 @@ __device__ void ph_pu_scan(const BP& P, const GC& c) {
     for (int64_t x = 4 * n4 + c.tid; x < g.n; x += c.nthreads) pu_scan_one(P, (int32_t)x, g.mc[x], g.tagb[x]);
 }
 
-// P7: a PU member has a row when it is an endpoint or next to one.
-__device__ void ph_hasrow(const BP& P, const GC& c) {
-    const DevGraph& g = P.g;
-    int64_t np = (int64_t)P.ctr[MF_CS * (C_PU)];
-    if (np > P.pu_cap) np = P.pu_cap;
-    for (int64_t i = c.wave; i < np; i += c.nwaves) {
-        int32_t z = P.pu[i];
-        bool has = g.seedb[z] == P.b;
-        if (!has) {
-            Row r = get_row(g, z);
-            int64_t L = r.len();
-            for (int64_t j0 = 0; j0 < L && !has; j0 += MF_WAVE) {
-                int64_t j = j0 + c.lane;
-                bool s = false;
-                if (j < L) {
-                    int32_t y = r.at(j);
-                    s = y >= 0 && g.seedb[y] == P.b;
-                }
-                has = __ballot(s) != 0;
-            }
-        }
-        if (has && c.lane == 0) {
-            g.pub[z] = 4u * P.b + 3u;
-            atomicAdd(&P.ctr[MF_CS * (C_ADM)], 1ull);
-        }
-    }
-}
-
-// P8: count ADM members next to each long row.
+// P7: a PU member is admissible (ADM) when it has a row: it is an endpoint
+// or next to one.  P8: for each ADM member, count it in every adjacency row
+// it sits in (the rows the eviction scan reads through the index).
 __device__ void ph_fa_count(const BP& P, const GC& c) {
     const DevGraph& g = P.g;
     int64_t np = (int64_t)P.ctr[MF_CS * (C_PU)];
     if (np > P.pu_cap) np = P.pu_cap;
     for (int64_t i = c.wave; i < np; i += c.nwaves) {
         int32_t z = P.pu[i];
-        if (!is_adm(P, z)) continue;
         Row r = get_row(g, z);
         int64_t L = r.len();
+        bool has = g.seedb[z] == P.b;
+        for (int64_t j0 = 0; j0 < L && !has; j0 += MF_WAVE) {
+            int64_t j = j0 + c.lane;
+            bool s = false;
+            if (j < L) {
+                int32_t y = r.at(j);
+                s = y >= 0 && g.seedb[y] == P.b;
+            }
+            has = __ballot(s) != 0;
+        }
+        if (!has) continue;
+        if (c.lane == 0) {
+            g.pub[z] = 4u * P.b + 3u;
+            atomicAdd(&P.ctr[MF_CS * (C_ADM)], 1ull);
+        }
         for (int64_t j = c.lane; j < L; j += MF_WAVE) {
             int32_t x1 = r.at(j);
             if (x1 < 0 || x1 == z) continue;
@@ -1190,12 +1169,10 @@ __device__ __forceinline__ GC block_coords() {
 MF_KERNEL(k_init, ph_init(P, c))
 MF_KERNEL(k_claim, ph_claim(P, c))
 MF_KERNEL(k_resolve, ph_resolve(P, c))
-MF_KERNEL(k_group, ph_group(P, c))
 MF_KERNEL(k_append, ph_append(P, c))
 MF_KERNEL(k_partner, ph_partner(P, c))
 MF_KERNEL(k_kcount, ph_kcount(P, c))
 MF_KERNEL(k_pu_scan, ph_pu_scan(P, c))
-MF_KERNEL(k_hasrow, ph_hasrow(P, c))
 MF_KERNEL(k_fa_count, ph_fa_count(P, c))
 MF_KERNEL(k_fa_alloc, ph_fa_alloc(P, c))
 MF_KERNEL(k_fa_fill, ph_fa_fill(P, c))
@@ -1291,28 +1268,21 @@ __device__ void finish_completion(const BP& P, const GC& c) {
     }
 }
 
-// The tail of a batch after the queued grid-wide rounds, split in three so
-// the wide scans run on the whole GPU and only the iterations with few
-// items run in one workgroup (workgroup barriers are far cheaper than
-// grid-wide synchronisation on a multi-die GPU):
-//   k_finish_rounds      whatever rounds the grid rounds left;
-//   k_tail_collect       (grid) rule counters and completion candidates;
-//   k_finish_completion  the completion iterations, then the counters are
-//                        published to the host.
-__global__ __launch_bounds__(1024) void k_finish_rounds(BP P, int par) {
+// The tail of a batch in one workgroup, after the queued grid-wide rounds:
+// whatever rounds they left, the rule counters, the completion candidates
+// and iterations, then the counters are published to the host.  Its phases
+// are separated by workgroup barriers, far cheaper than grid-wide
+// synchronisation on a multi-die GPU, and each of them has few items.
+__global__ __launch_bounds__(1024) void k_finish(BP P, int par) {
     GC c = block_coords();
-    (void)finish_rounds(P, c, par);
-}
-
-__global__ __launch_bounds__(256) void k_tail_collect(BP P) {
-    GC c = grid_coords();
-    ph_stats(P, c);
-    if (P.completion) ph_comp_collect(P, c);
-}
-
-__global__ __launch_bounds__(1024) void k_finish_completion(BP P) {
-    GC c = block_coords();
-    finish_completion(P, c);
+    if (finish_rounds(P, c, par)) {
+        phase_barrier();
+        ph_stats(P, c);
+        if (P.completion) {
+            ph_comp_collect(P, c);
+            finish_completion(P, c);
+        }
+    }
     publish(P, c);
 }
 
@@ -1377,12 +1347,10 @@ void mf_launch_batch_prologue(const BP& P, const LaunchCfg& L) {
     LAUNCH(k_init, P);
     LAUNCH(k_claim, P);
     LAUNCH(k_resolve, P);
-    LAUNCH(k_group, P);
     LAUNCH(k_append, P);
     LAUNCH(k_partner, P);
     LAUNCH(k_kcount, P);
     LAUNCH(k_pu_scan, P);
-    LAUNCH(k_hasrow, P);
     LAUNCH(k_fa_count, P);
     LAUNCH(k_fa_alloc, P);
     LAUNCH(k_fa_fill, P);
@@ -1395,13 +1363,7 @@ void mf_launch_round(const BP& P, const LaunchCfg& L, int par) {
 }
 
 void mf_launch_finish(const BP& P, const LaunchCfg& L, int par) {
-    hipLaunchKernelGGL(k_finish_rounds, dim3(1), dim3(1024), 0, L.stream, P, par);
-}
-
-void mf_launch_tail_collect(const BP& P, const LaunchCfg& L) { LAUNCH(k_tail_collect, P); }
-
-void mf_launch_finish_completion(const BP& P, const LaunchCfg& L) {
-    hipLaunchKernelGGL(k_finish_completion, dim3(1), dim3(1024), 0, L.stream, P);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, L.stream, P, par);
 }
 
 void mf_launch_completion_collect(const BP& P, const LaunchCfg& L) { LAUNCH(k_comp_collect, P); }
