@@ -116,6 +116,13 @@ int misfit_submit_batch(misfit_ctx* ctx, const misfit_update* ups, int64_t count
  * Replaces syncAndCollect (src/misfit_async/misfit_async.cu:677-701). */
 int misfit_wait(misfit_ctx* ctx, uint64_t ticket, misfit_batch_stats* st);
 
+/* Non-blocking: 1 when the submitted batch has finished on the device (its
+ * results can be collected with misfit_wait without blocking), 0 while it is
+ * still queued or running.  Replaces the reference's blocking event waits in
+ * syncAndCollect (src/misfit_async/misfit_async.cu:677-701) for callers that
+ * multiplex many graphs on one thread. */
+int misfit_poll(misfit_ctx* ctx, uint64_t ticket);
+
 /* submit + wait.  Replaces processBatchGPUVertexClustering /
  * processBatchWithVertexClustering (src/misfit/misfit.cu:468-730). */
 int misfit_process_batch(misfit_ctx* ctx, const misfit_update* ups, int64_t count, misfit_batch_stats* st);
@@ -160,6 +167,50 @@ int misfit_batch_bytes(misfit_ctx* ctx, int index_bytes, int64_t* bytes, int64_t
  * cudaMemGetInfo calls of the reference's setup and peak-memory report
  * (src/misfit/misfit.cu:46-67,758-761). */
 int misfit_device_info(misfit_ctx* ctx, char* name, int namelen, int64_t* total_bytes, int64_t* free_bytes);
+
+/* ---- serving: many graphs on one GPU, batches taken as they arrive ----
+ * Generalises the async engine's pipelining (src/misfit_async/
+ * misfit_async.cu:484-701,827-876): graphs are admitted by free device
+ * memory, each graph's requests run in submission order with up to
+ * max_inflight batches queued on the device, batches of different graphs
+ * overlap on the GPU, a graph is compacted when its overlay passes
+ * compact_fraction of its base entries, and every request reports its
+ * queueing time and latency.  Thread-safe: any thread may submit or wait. */
+typedef struct misfit_server misfit_server;
+
+typedef struct {
+    int64_t hbm_reserve;      /* device bytes kept free: graphs and batches that would cut into it are refused */
+    double compact_fraction;  /* compact a graph when its overlay exceeds this share of its base (0: never) */
+    int max_inflight;         /* batches per graph queued on the device at once (1..3) */
+} misfit_server_opts;
+
+typedef struct {
+    int32_t graph;
+    int64_t updates;
+    int64_t compactions_before;  /* compactions of the graph that ran before this batch started */
+    double queue_ms;             /* submission -> queued on the device */
+    double latency_ms;           /* submission -> result */
+    double device_ms;            /* the batch on the device */
+    misfit_batch_stats batch;
+} misfit_request_stats;
+
+int misfit_server_create(int device, const misfit_server_opts* opts, misfit_server** out);
+int misfit_server_destroy(misfit_server* srv);
+const char* misfit_server_last_error(const misfit_server* srv);
+/* MISFIT_E_NOMEM (nothing allocated) when the graph does not fit. */
+int misfit_server_add_graph(misfit_server* srv, int32_t n, int64_t m, const void* nindex, int nindex_is_64,
+                            const int32_t* nlist, const uint8_t* mis, int32_t* graph);
+int misfit_server_remove_graph(misfit_server* srv, int32_t graph);
+/* Returns at once; the updates are copied. */
+int misfit_server_submit(misfit_server* srv, int32_t graph, const misfit_update* ups, int64_t count,
+                         uint64_t* request);
+/* Blocks until the request has finished; its result code (MISFIT_E_NOMEM
+ * when the batch was not admitted; the graph is then unchanged). */
+int misfit_server_wait(misfit_server* srv, uint64_t request, misfit_request_stats* st);
+/* After waiting for a graph's requests: its MIS, and the full check. */
+int misfit_server_get_mis(misfit_server* srv, int32_t graph, uint8_t* mis);
+int misfit_server_check(misfit_server* srv, int32_t graph, int64_t* conflicts, int64_t* uncovered,
+                        int64_t* count_errors);
 
 /* ---- host I/O (same library, no device needed) ---- */
 

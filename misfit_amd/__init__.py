@@ -55,6 +55,22 @@ class BatchStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("hbm_reserve", ctypes.c_int64), ("compact_fraction", ctypes.c_double),
+                ("max_inflight", ctypes.c_int)]
+
+
+class RequestStats(ctypes.Structure):
+    _fields_ = [("graph", ctypes.c_int32), ("updates", ctypes.c_int64), ("compactions_before", ctypes.c_int64),
+                ("queue_ms", ctypes.c_double), ("latency_ms", ctypes.c_double), ("device_ms", ctypes.c_double),
+                ("batch", BatchStats)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "batch"}
+        d["batch"] = self.batch.as_dict()
+        return d
+
+
 class EgrView(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int32), ("m", ctypes.c_int64), ("wide", ctypes.c_int),
                 ("nindex", ctypes.c_void_p), ("nlist", ctypes.c_void_p), ("priv", ctypes.c_void_p)]
@@ -75,7 +91,10 @@ EXPORTS = (
     "misfit_compact", "misfit_graph_size", "misfit_export_csr", "misfit_neighbors",
     "misfit_batch_bytes", "misfit_device_info", "misfit_egr_open", "misfit_egr_close", "misfit_write_egr",
     "misfit_gen_er", "misfit_gen_rmat", "misfit_gen_rgg", "misfit_greedy_mis", "misfit_gen_updates",
-    "misfit_mtx_to_egr", "misfit_egr_check",
+    "misfit_mtx_to_egr", "misfit_egr_check", "misfit_poll",
+    "misfit_server_create", "misfit_server_destroy", "misfit_server_last_error", "misfit_server_add_graph",
+    "misfit_server_remove_graph", "misfit_server_submit", "misfit_server_wait", "misfit_server_get_mis",
+    "misfit_server_check",
 )
 
 _lib = None
@@ -115,6 +134,18 @@ def lib():
         L.misfit_egr_close.argtypes = [ctypes.POINTER(EgrView)]
         L.misfit_write_egr.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int64, _i64p, _i32p,
                                        ctypes.c_char_p, ctypes.c_int]
+        L.misfit_poll.argtypes = [vp, ctypes.c_uint64]
+        L.misfit_server_create.argtypes = [ctypes.c_int, ctypes.POINTER(ServerOpts), ctypes.POINTER(vp)]
+        L.misfit_server_destroy.argtypes = [vp]
+        L.misfit_server_last_error.restype = ctypes.c_char_p
+        L.misfit_server_last_error.argtypes = [vp]
+        L.misfit_server_add_graph.argtypes = [vp, ctypes.c_int32, ctypes.c_int64, vp, ctypes.c_int, _i32p, _u8p,
+                                              _i32p]
+        L.misfit_server_remove_graph.argtypes = [vp, ctypes.c_int32]
+        L.misfit_server_submit.argtypes = [vp, ctypes.c_int32, vp, ctypes.c_int64, _u64p]
+        L.misfit_server_wait.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(RequestStats)]
+        L.misfit_server_get_mis.argtypes = [vp, ctypes.c_int32, _u8p]
+        L.misfit_server_check.argtypes = [vp, ctypes.c_int32, _i64p, _i64p, _i64p]
         L.misfit_mtx_to_egr.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _i32p, _i64p, ctypes.c_char_p,
                                         ctypes.c_int]
         L.misfit_egr_check.argtypes = [ctypes.c_char_p, ctypes.POINTER(EgrReport), ctypes.c_char_p, ctypes.c_int]
@@ -414,3 +445,63 @@ class Engine:
         self._ck(lib().misfit_batch_bytes(self.ctx, index_bytes, ctypes.byref(b), ctypes.byref(v1),
                                           ctypes.byref(v2)))
         return b.value, v1.value, v2.value
+
+
+class Server:
+    """Serving layer (misfit_server_*): many graphs on one GPU, update batches
+    submitted as they arrive, continuous batching across graphs."""
+
+    def __init__(self, device=0, hbm_reserve=0, compact_fraction=0.05, max_inflight=2):
+        self.srv = ctypes.c_void_p()
+        opts = ServerOpts(hbm_reserve, compact_fraction, max_inflight)
+        _ck(lib().misfit_server_create(device, ctypes.byref(opts), ctypes.byref(self.srv)))
+
+    def _ck(self, rc):
+        if rc != 0:
+            raise MisfitError(rc, (lib().misfit_server_last_error(self.srv) or b"").decode(errors="replace"))
+
+    def close(self):
+        if self.srv:
+            lib().misfit_server_destroy(self.srv)
+            self.srv = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def add_graph(self, nindex, nlist, mis):
+        nindex = np.ascontiguousarray(nindex, np.int64)
+        nlist = np.ascontiguousarray(nlist, np.int32)
+        mis = np.ascontiguousarray(mis, np.uint8)
+        gid = ctypes.c_int32()
+        nl = nlist if len(nlist) else np.zeros(1, np.int32)
+        self._ck(lib().misfit_server_add_graph(self.srv, len(nindex) - 1, len(nlist), nindex.ctypes.data, 1,
+                                               _p(nl, _i32p), _p(mis, _u8p), ctypes.byref(gid)))
+        return gid.value
+
+    def remove_graph(self, gid):
+        self._ck(lib().misfit_server_remove_graph(self.srv, gid))
+
+    def submit(self, gid, updates):
+        updates = np.ascontiguousarray(updates, UPDATE_DTYPE)
+        t = ctypes.c_uint64()
+        self._ck(lib().misfit_server_submit(self.srv, gid, updates.ctypes.data, len(updates), ctypes.byref(t)))
+        return t.value
+
+    def wait(self, request):
+        """Request statistics (dict); raises MisfitError on a refused or failed batch."""
+        st = RequestStats()
+        self._ck(lib().misfit_server_wait(self.srv, request, ctypes.byref(st)))
+        return st.as_dict()
+
+    def get_mis(self, gid, n):
+        out = np.zeros(n, np.uint8)
+        self._ck(lib().misfit_server_get_mis(self.srv, gid, _p(out, _u8p)))
+        return out
+
+    def check(self, gid):
+        a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        self._ck(lib().misfit_server_check(self.srv, gid, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value

@@ -698,6 +698,17 @@ extern "C" int misfit_submit_batch(misfit_ctx* c, const misfit_update* ups, int6
     return MISFIT_OK;
 }
 
+extern "C" int misfit_poll(misfit_ctx* c, uint64_t ticket) {
+    if (!c) return MISFIT_E_ARG;
+    Slot& s = c->slots[ticket % MF_NSLOT];
+    if (!s.busy || s.ticket != ticket) return fail(c, MISFIT_E_ARG, "unknown or expired ticket");
+    hipError_t e = hipEventQuery(s.done);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    HIPCHK(c, e);
+    return 0;
+}
+
 extern "C" int misfit_wait(misfit_ctx* c, uint64_t ticket, misfit_batch_stats* st) {
     if (!c) return MISFIT_E_ARG;
     Slot& s = c->slots[ticket % MF_NSLOT];

@@ -1,0 +1,104 @@
+"""The serving layer (misfit_server_*) on the GPU.
+
+Requests for several graphs are submitted interleaved and served
+concurrently; graphs are compacted when their overlay passes a bound.  Every
+graph must end with the MIS the oracle computes when it replays the same
+batches with compactions at the same points (each request reports how many
+compactions of its graph ran before it), for every in-flight depth.  Graphs
+and batches that do not fit in free device memory minus the reserve are
+refused without changing anything.
+"""
+import numpy as np
+import pytest
+
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _workload(engine_mod, oracle, kind, size, seed, nb=5, bs=3000):
+    if kind == "er":
+        ni, nl = engine_mod.gen_er(size, 8 * size, seed)
+    elif kind == "rmat":
+        ni, nl = engine_mod.gen_rmat(size, 16, seed)
+    else:
+        ni, nl = engine_mod.gen_rgg(size, 32.0, seed)
+    mis0 = engine_mod.greedy_mis(ni, nl)
+    u, v = engine_mod.gen_updates(engine_mod.MIXED_EXISTING, ni, nl, nb * bs, seed)
+    ins = oracle.Graph.from_csr(ni, nl).flags(u, v)
+    batches = [(u[i:i + bs], v[i:i + bs], ins[i:i + bs]) for i in range(0, nb * bs, bs)]
+    return ni, nl, mis0, batches
+
+
+def _oracle_with_compactions(oracle, ni, nl, mis0, batches, before):
+    """Replay with a compaction wherever `before` (compactions before each
+    batch) steps up."""
+    g = oracle.Graph.from_csr(ni, nl)
+    mis = np.array(mis0, np.uint8)
+    done = 0
+    for (u, v, ins), cb in zip(batches, before):
+        while done < cb:
+            cni, cnl = g.compact()
+            g = oracle.Graph.from_csr(cni, cnl)
+            done += 1
+        g.process_batch(mis, u, v, ins, completion=True, literal=False)
+    return mis
+
+
+@pytest.mark.parametrize("inflight", [1, 3])
+def test_served_graphs_match_oracle_with_compactions(oracle, engine_mod, inflight):
+    work = [_workload(engine_mod, oracle, k, s, seed)
+            for (k, s, seed) in [("er", 1 << 14, 31), ("rmat", 14, 32), ("rgg", 1 << 14, 33)]]
+    with engine_mod.Server(0, compact_fraction=0.02, max_inflight=inflight) as srv:
+        gids = [srv.add_graph(ni, nl, mis0) for (ni, nl, mis0, _) in work]
+        reqs = {g: [] for g in gids}
+        for b in range(5):
+            for gid, (_, _, _, batches) in zip(gids, work):
+                u, v, ins = batches[b]
+                reqs[gid].append(srv.submit(gid, engine_mod.make_updates(u, v, ins)))
+        for gid, (ni, nl, mis0, batches) in zip(gids, work):
+            stats = [srv.wait(r) for r in reqs[gid]]
+            before = [s["compactions_before"] for s in stats]
+            assert before == sorted(before)
+            for s in stats:
+                assert s["graph"] == gid and s["updates"] == 3000
+                assert 0 <= s["queue_ms"] <= s["latency_ms"] and s["device_ms"] > 0
+            want = _oracle_with_compactions(oracle, ni, nl, mis0, batches, before)
+            got = srv.get_mis(gid, len(ni) - 1)
+            assert np.array_equal(got, want), "graph %d" % gid
+            assert srv.check(gid) == (0, 0, 0)
+        # the bound is low enough that compactions did run
+        assert any(srv.wait(srv.submit(g, engine_mod.make_updates(np.zeros(0, np.int32), np.zeros(0, np.int32),
+                                                                    np.zeros(0, np.uint8))))["compactions_before"] > 0
+                   for g in gids)
+
+
+def test_admission_by_free_memory(oracle, engine_mod):
+    ni, nl, mis0, batches = _workload(engine_mod, oracle, "er", 1 << 14, 41, nb=1)
+    with engine_mod.Engine(0) as probe:
+        _, total, free = probe.device_info()
+    # a reserve larger than the device: nothing fits
+    with engine_mod.Server(0, hbm_reserve=total + (1 << 30)) as srv:
+        with pytest.raises(engine_mod.MisfitError) as ei:
+            srv.add_graph(ni, nl, mis0)
+        assert ei.value.code == -4
+    # the graph fits; a batch whose buffers would cut into the reserve is
+    # refused and leaves the graph as it was, a small one is served
+    with engine_mod.Server(0) as srv:
+        gid = srv.add_graph(ni, nl, mis0)
+        with engine_mod.Engine(0) as probe:
+            _, _, free_now = probe.device_info()
+    with engine_mod.Server(0, hbm_reserve=max(free_now - (300 << 20), 0)) as srv:
+        gid = srv.add_graph(ni, nl, mis0)
+        big = np.zeros(400_000, engine_mod.UPDATE_DTYPE)
+        big["u"] = 0
+        big["v"] = 1
+        with pytest.raises(engine_mod.MisfitError) as ei:
+            srv.wait(srv.submit(gid, big))
+        assert ei.value.code == -4
+        assert np.array_equal(srv.get_mis(gid, len(ni) - 1), mis0)
+        u, v, ins = batches[0]
+        st = srv.wait(srv.submit(gid, engine_mod.make_updates(u, v, ins)))
+        assert st["updates"] == len(u)
+        want, _, _ = H.oracle_replay(oracle, ni, nl, mis0, batches, completion=True, literal=False)
+        assert np.array_equal(srv.get_mis(gid, len(ni) - 1), want[0])
