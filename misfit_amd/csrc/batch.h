@@ -106,6 +106,7 @@ struct LaunchCfg {
     int round_per_cu = 0;  // speculation/validation workgroups per CU (0: by occupancy)
 };
 void mf_launch_batch_prologue(const BP& P, const LaunchCfg& L);
+int mf_round_grid(const LaunchCfg& L);  // workgroups of the speculation/validation kernels
 void mf_launch_round(const BP& P, const LaunchCfg& L, int par);
 void mf_launch_finish(const BP& P, const LaunchCfg& L, int par);
 void mf_launch_completion_collect(const BP& P, const LaunchCfg& L);

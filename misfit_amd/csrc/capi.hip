@@ -14,6 +14,7 @@
 //           barrier: on a multi-die MI355X a stream launch costs ~2.4 us,
 //           an in-kernel grid barrier 14-80 us.
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -48,10 +49,11 @@ struct misfit_ctx {
     bool has_graph = false;
     int completion = 1;
     int mode = MISFIT_MODE_QUEUED;
-    int grid_rounds = 12;      // grid-wide rounds queued before the finisher
+    int grid_rounds = 0;       // MISFIT_GRID_ROUNDS: fixed number of queued grid rounds
     bool grid_rounds_fixed = false;
+    double rpu_hist[16] = {0}; // (rounds - 1) per update of recent batches of >= 2048 updates
+    int rpu_pos = 0;
     int round_per_cu = 0;      // MISFIT_ROUND_WG_PER_CU: round grid (0: by occupancy)
-    int rounds_hist[16] = {0};  // rounds of recent batches
     int rounds_pos = 0;
     bool debug_rounds = false;
     uint32_t epoch = 0;
@@ -63,6 +65,8 @@ struct misfit_ctx {
     int64_t fa_cap = 0;
     int64_t ph_cap = 0;
     u64* d_ktask = nullptr;    // row-chunk tasks, sized for the graph plus the pool
+    int32_t* d_sval = nullptr; // sorted copy of the base rows when they are not ascending
+    int32_t* d_spos = nullptr;
     int64_t ktask_cap = 0;
     int64_t ins_bound = 0;    // inserted pairs since the overlay was last cleared (upper bound)
     int64_t pool_bound = 0;   // pool slots possibly used (upper bound)
@@ -93,6 +97,8 @@ static int fail(misfit_ctx* c, int code, const std::string& msg) {
         if (_e != hipSuccess)                                                                   \
             return fail((c), MISFIT_E_HIP, std::string(#expr ": ") + hipGetErrorString(_e));    \
     } while (0)
+
+static int build_row_index(misfit_ctx* c);
 
 template <typename T>
 static hipError_t dalloc(std::vector<void*>& owner, T** p, int64_t count) {
@@ -161,6 +167,9 @@ extern "C" int misfit_ctx_create(int device, misfit_ctx** out) {
     return MISFIT_OK;
 }
 
+// Staging buffers only: a slot's ticket, counters and events stay, so a
+// batch submitted before the buffers were regrown can still be waited on
+// (the regrowth synchronises the streams first).
 static void free_slots(misfit_ctx* c) {
     for (auto& s : c->slots) {
         if (s.h_u) hipHostFree(s.h_u);
@@ -168,7 +177,6 @@ static void free_slots(misfit_ctx* c) {
         if (s.h_ins) hipHostFree(s.h_ins);
         s.h_u = nullptr; s.h_v = nullptr; s.h_ins = nullptr;
         s.d_u = nullptr; s.d_v = nullptr; s.d_ins = nullptr;
-        s.busy = false;
     }
 }
 
@@ -188,6 +196,8 @@ extern "C" int misfit_ctx_destroy(misfit_ctx* c) {
     free_list(c->batch_allocs);
     free_list(c->graph_allocs);
     if (c->d_ktask) hipFree(c->d_ktask);
+    if (c->d_sval) hipFree(c->d_sval);
+    if (c->d_spos) hipFree(c->d_spos);
     if (c->h_small) hipHostFree(c->h_small);
     hipStreamDestroy(c->stream);
     hipStreamDestroy(c->copy_stream);
@@ -464,21 +474,9 @@ extern "C" int misfit_load_csr(misfit_ctx* c, int32_t n, int64_t m, const void* 
     HIPCHK(c, hipStreamSynchronize(st));
     if (flags & 4) return fail(c, MISFIT_E_ARG, "CSR has out-of-range entries");
     g.sorted = (flags & 1) ? 0 : 1;
-    if (!g.sorted) {
-        // unsorted rows: the engine needs rows without duplicate entries
-        std::vector<int64_t> ni((size_t)n + 1);
-        HIPCHK(c, hipMemcpy(ni.data(), c->d_nindex, ni.size() * 8, hipMemcpyDeviceToHost));
-        std::vector<int32_t> row;
-        for (int32_t x = 0; x < n; x++) {
-            int64_t b = ni[x], e = ni[x + 1];
-            if (e - b < 2) continue;
-            row.assign(nlist + b, nlist + e);
-            std::sort(row.begin(), row.end());
-            if (std::adjacent_find(row.begin(), row.end()) != row.end())
-                return fail(c, MISFIT_E_UNSUPPORTED, "row " + std::to_string(x) + " lists a neighbour twice");
-        }
-    }
     S.g = g;
+    int rc = build_row_index(c);  // unsorted rows: sorted copy, and no neighbour twice
+    if (rc) return rc;
     c->has_graph = true;
     return ensure_batch_capacity(c, 1 << 14);
 }
@@ -611,11 +609,52 @@ static void launch_queued(const BP& P, const LaunchCfg& L, int k) {
     mf_launch_finish(P, L, k & 1);
 }
 
+// Grid rounds to queue for a batch of `count` updates: one round plus the
+// recent maximum of further rounds per update (conflicts grow with the
+// batch; measured on batches large enough to show it), with a margin, and at
+// least one round per window.  Rounds past the end of the batch cost one
+// empty launch each; rounds left over are finished by the one-workgroup
+// tail, slowly, so the estimate errs high.
+static int queued_rounds(misfit_ctx* c, int64_t count, int64_t window) {
+    if (c->grid_rounds_fixed) return c->grid_rounds;
+    double rpu = 0.0;
+    for (double h : c->rpu_hist) rpu = std::max(rpu, h);
+    if (rpu == 0.0) rpu = 1.0 / 400.0;  // before any history
+    double k = std::ceil((double)count * rpu * 1.15) + 3.0;
+    double per_window = std::ceil((double)count / (double)std::max<int64_t>(window, 1)) + 2.0;
+    k = std::max(k, per_window);
+    return (int)std::min(k, 65536.0);
+}
+
 // Whole batch enqueued with no host synchronisation inside it.
 static int drive_queued(misfit_ctx* c, BP& P) {
     LaunchCfg L{c->S.blocks, c->stream, c->round_per_cu};
-    launch_queued(P, L, c->grid_rounds);
+    launch_queued(P, L, queued_rounds(c, P.U, P.window));
     HIPCHK(c, hipGetLastError());
+    return MISFIT_OK;
+}
+
+// Base rows that are not ascending get a sorted copy with positions, so
+// adjacency tests binary-search them (any older copy is dropped first).
+static int build_row_index(misfit_ctx* c) {
+    DevGraph& g = c->P.g;
+    if (c->d_sval) hipFree(c->d_sval);
+    if (c->d_spos) hipFree(c->d_spos);
+    c->d_sval = c->d_spos = nullptr;
+    g.sval = g.spos = nullptr;
+    c->S.g = g;
+    if (g.sorted || g.m == 0) return MISFIT_OK;
+    HIPCHK(c, hipMalloc((void**)&c->d_sval, (size_t)g.m * 4));
+    HIPCHK(c, hipMalloc((void**)&c->d_spos, (size_t)g.m * 4));
+    mf_fill_i32((int32_t*)c->d_flags, 0, 1, c->stream);
+    HIPCHK(c, mf_build_row_index(c->S, c->d_sval, c->d_spos, c->d_flags));
+    int flags = 0;
+    HIPCHK(c, hipMemcpyAsync(&flags, c->d_flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (flags & 16) return fail(c, MISFIT_E_UNSUPPORTED, "a row lists a neighbour twice");
+    g.sval = c->d_sval;
+    g.spos = c->d_spos;
+    c->S.g = g;
     return MISFIT_OK;
 }
 
@@ -681,7 +720,9 @@ extern "C" int misfit_submit_batch(misfit_ctx* c, const misfit_update* ups, int6
     P.U = count;
     P.b = c->epoch;
     P.completion = c->completion;
-    P.window = std::max<int64_t>(count, 1);
+    // one edge per wave of the round grid and round
+    LaunchCfg L{c->S.blocks, c->stream, c->round_per_cu};
+    P.window = std::max<int64_t>((int64_t)mf_round_grid(L) * (256 / MF_WAVE), 1);
     P.n_ins = nins;
     P.ktask = c->d_ktask;
     P.ktask_cap = c->ktask_cap;
@@ -716,15 +757,8 @@ extern "C" int misfit_wait(misfit_ctx* c, uint64_t ticket, misfit_batch_stats* s
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipEventSynchronize(s.done));
     s.busy = false;
-    if (!c->grid_rounds_fixed && s.count) {
-        // queue enough grid-wide rounds for the recent batches plus a margin;
-        // rounds past the end cost one empty launch each, and anything left
-        // over is finished by the single-workgroup kernel
-        c->rounds_hist[c->rounds_pos++ & 15] = (int)s.h_ctr[C_ROUNDS];
-        int mx = 0;
-        for (int h : c->rounds_hist) mx = std::max(mx, h);
-        c->grid_rounds = std::min(64, mx + 2);
-    }
+    if (s.count >= 2048 && s.h_ctr[C_ROUNDS] >= 1)
+        c->rpu_hist[c->rpu_pos++ & 15] = (double)(s.h_ctr[C_ROUNDS] - 1) / (double)s.count;
     if (s.h_ctr[C_POOL_OVF]) return fail(c, MISFIT_E_CHECK, "overlay pool overflow");
     if (s.h_ctr[C_GUARD]) return fail(c, MISFIT_E_CHECK, "repair loop bound exceeded or task list overflow");
     if (st) fill_stats(c, s, st);
@@ -811,6 +845,8 @@ extern "C" int misfit_compact(misfit_ctx* c, int64_t* new_m) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     g.sorted = (flags & 1) ? 0 : 1;
     c->S.g = g;
+    int rc = build_row_index(c);
+    if (rc) return rc;
     if (new_m) *new_m = total;
     return MISFIT_OK;
 }

@@ -3,6 +3,9 @@
 // HBM layout of one graph (everything stays device-resident for the run):
 //   nindex[n+1]  int64   base CSR offsets (both .egr widths are widened once
 //                         at load)
+//   sval/spos    int32   when base rows are not ascending (input order, or
+//                         after a compaction): every row sorted, with each
+//                         entry's position in the row, for binary searches
 //   nlist[m]     int32   base CSR entries.  A deleted base edge stays in
 //                         place as its bitwise complement (~v < 0): a row read
 //                         skips it with one sign test and re-inserting it
@@ -88,6 +91,8 @@ struct DevGraph {
     int64_t* fa_start;
     int32_t* fa_fill;
     int sorted;      // every base row strictly ascending
+    const int32_t* sval;  // rows not ascending: each row sorted (plain ids) ...
+    const int32_t* spos;  // ... with each entry's position in its row
 };
 
 struct Row {
@@ -164,6 +169,16 @@ __device__ __forceinline__ int64_t base_find(const DevGraph& g, int32_t x, int32
         if (lo < e && decode(g.nlist[lo]) == y) return lo;
         return -1;
     }
+    if (g.sval) {
+        int64_t lo = b, hi = e;
+        while (lo < hi) {
+            int64_t mid = (lo + hi) >> 1;
+            if (g.sval[mid] < y) lo = mid + 1;
+            else hi = mid;
+        }
+        if (lo < e && g.sval[lo] == y) return b + g.spos[lo];
+        return -1;
+    }
     for (int64_t p = b; p < e; p++)
         if (decode(g.nlist[p]) == y) return p;
     return -1;
@@ -203,7 +218,7 @@ __device__ __forceinline__ void base_find_pair(const DevGraph& g, int32_t x, int
 // base part searches the shorter of the two rows (rows are symmetric).
 __device__ __forceinline__ bool adjacent(const DevGraph& g, int32_t x, int32_t y) {
     if (ph_lookup(g, pair_key(x, y)) == 1) return true;
-    if (g.sorted && (g.nindex[y + 1] - g.nindex[y]) < (g.nindex[x + 1] - g.nindex[x])) {
+    if ((g.sorted || g.sval) && (g.nindex[y + 1] - g.nindex[y]) < (g.nindex[x + 1] - g.nindex[x])) {
         int32_t t = x;
         x = y;
         y = t;

@@ -27,6 +27,8 @@ hipError_t mf_batch_bytes(GraphState& S, const int32_t* d_seeds, const u64* d_ns
 hipError_t mf_widen(const int32_t* d_in, int64_t* d_out, int64_t count, hipStream_t st);
 hipError_t mf_validate_csr(GraphState& S, int* d_flags);
 hipError_t mf_validate_entries(GraphState& S, int* d_flags);
+// sorted copy of every row with positions (rows not ascending); flags |= 16 on a repeated neighbour
+hipError_t mf_build_row_index(GraphState& S, int32_t* sval, int32_t* spos, int* d_flags);
 void mf_scan_excl(const int64_t* in, int64_t count, int64_t* out, int64_t* part, hipStream_t st);
 void mf_fill_u32(uint32_t* p, uint32_t v, int64_t count, hipStream_t st);
 void mf_fill_i32(int32_t* p, int32_t v, int64_t count, hipStream_t st);

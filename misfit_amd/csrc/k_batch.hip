@@ -815,10 +815,14 @@ struct RoundView {
     bool live;
 };
 
-__device__ __forceinline__ RoundView round_view(const BP& P, int par) {
+// `win` bounds the edges a round speculates: one per wave of the round grid
+// for the grid-wide rounds (P.window), MF_FIN_WINDOW in the one-workgroup
+// tail, so a round costs about the same whatever the batch size.
+#define MF_FIN_WINDOW 1024
+__device__ __forceinline__ RoundView round_view(const BP& P, int par, int64_t win) {
     RoundView v;
     v.start = (int64_t)P.ctr[MF_CS * (C_START0 + par)];
-    v.end = v.start + P.window < P.U ? v.start + P.window : P.U;
+    v.end = v.start + win < P.U ? v.start + win : P.U;
     v.R = *P.round + 1u;
     v.live = v.start < P.U;
     return v;
@@ -931,8 +935,8 @@ __device__ __forceinline__ int64_t round_conflict(const BP& P, int par) {
 // Round bookkeeping after a commit (one thread): the next round (parity
 // par ^ 1) starts at the first conflict.  A dead round carries the start
 // over unchanged.
-__device__ __forceinline__ void advance_round(const BP& P, int par, int64_t conflict) {
-    RoundView v = round_view(P, par);
+__device__ __forceinline__ void advance_round(const BP& P, int par, int64_t conflict, int64_t win) {
+    RoundView v = round_view(P, par, win);
     P.ctr[MF_CS * (C_NACT0 + (par ^ 1))] = 0;
     P.ctr[MF_CS * (C_CONF0 + (par ^ 1))] = ~0ull;
     if (v.live) {
@@ -1006,7 +1010,7 @@ __device__ void ph_commit_w(const BP& P, const GC& c, const RoundView& v, int pa
         uint8_t k = P.kind[e];
         if (k) commit_edge(P, (int32_t)e, k, c.lane);
     }
-    if (c.tid == 0) advance_round(P, par, conflict);
+    if (c.tid == 0) advance_round(P, par, conflict, P.window);
 }
 
 // One-workgroup rounds (finisher): classification by thread into an active
@@ -1183,19 +1187,19 @@ MF_KERNEL(k_stats, ph_stats(P, c))
 __global__ __launch_bounds__(256) void k_spec(BP P, int par) {
     __shared__ int cur[256 / MF_WAVE];
     GC c = grid_coords();
-    RoundView v = round_view(P, par);
+    RoundView v = round_view(P, par, P.window);
     if (v.live) ph_spec_w(P, c, v, cur);
 }
 __global__ __launch_bounds__(256) void k_validate(BP P, int par) {
     GC c = grid_coords();
-    RoundView v = round_view(P, par);
+    RoundView v = round_view(P, par, P.window);
     if (v.live) ph_validate_w(P, c, v, par);
 }
 __global__ __launch_bounds__(256) void k_commit(BP P, int par) {
     GC c = grid_coords();
-    RoundView v = round_view(P, par);
+    RoundView v = round_view(P, par, P.window);
     if (v.live) ph_commit_w(P, c, v, par);
-    else if (c.tid == 0) advance_round(P, par, INT64_MAX);
+    else if (c.tid == 0) advance_round(P, par, INT64_MAX, P.window);
 }
 __global__ __launch_bounds__(256) void k_comp_apply(BP P, int par) {
     GC c = grid_coords();
@@ -1232,7 +1236,8 @@ __device__ bool finish_rounds(const BP& P, const GC& c, int par) {
     uint32_t guard = 0;
     while (true) {
         phase_barrier();
-        RoundView v = round_view(P, par);
+        const int64_t win = P.window < MF_FIN_WINDOW ? P.window : MF_FIN_WINDOW;
+        RoundView v = round_view(P, par, win);
         if (!v.live) return true;
         if ((int64_t)guard++ > P.U + 1) {  // every round commits its first edge
             if (c.tid == 0) P.ctr[MF_CS * (C_GUARD)] = 1;
@@ -1245,7 +1250,7 @@ __device__ bool finish_rounds(const BP& P, const GC& c, int par) {
         ph_validate_l(P, c, v, par);
         phase_barrier();
         ph_commit_l(P, c, par);
-        if (c.tid == 0) advance_round(P, par, round_conflict(P, par));
+        if (c.tid == 0) advance_round(P, par, round_conflict(P, par), win);
         par ^= 1;
     }
 }

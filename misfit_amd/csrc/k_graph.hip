@@ -9,6 +9,9 @@
 // entry by a 6-step search over the 64 row offsets held in registers.  Ranges
 // longer than MF_SPLIT are cut into tasks that other waves pick up, so hub
 // rows do not serialise a pass.  Overlay runs are walked per vertex.
+#include <algorithm>
+#include <vector>
+
 #include "graph.h"
 
 #define MF_SPLIT 4096
@@ -484,4 +487,114 @@ void mf_fill_i32(int32_t* p, int32_t v, int64_t count, hipStream_t st) {
 void mf_fill_u64(u64* p, u64 v, int64_t count, hipStream_t st) { k_fill<u64><<<1024, 256, 0, st>>>(p, v, count); }
 void mf_fill_i64(int64_t* p, int64_t v, int64_t count, hipStream_t st) {
     k_fill<int64_t><<<1024, 256, 0, st>>>(p, v, count);
+}
+
+// ---------------------------------------------------------------- row index
+// Rows that are not ascending (an input in file order, or base rows after a
+// compaction: base order, then insertions in insertion order) keep their
+// order, which the rule observes; a sorted copy of every row, with each
+// entry's position in its row, lets adjacency tests binary-search anyway.
+// Built with rocPRIM's segmented radix sort over row ranges of at most 2^30
+// entries; checked for repeated neighbours.
+#include <hipcub/device/device_segmented_sort.hpp>
+#include <rocprim/iterator/transform_iterator.hpp>
+
+namespace {
+struct RebaseOffset {
+    int64_t base;
+    __host__ __device__ int operator()(const int64_t& x) const { return (int)(x - base); }
+};
+}  // namespace
+
+__global__ void k_row_iota(const int64_t* nindex, int32_t r0, int32_t r1, int32_t* pos) {
+    const int64_t base = nindex[r0];
+    for (int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64 + r0; w < r1;
+         w += (int64_t)gridDim.x * blockDim.x / 64) {
+        int64_t b = nindex[w], e = nindex[w + 1];
+        for (int64_t p = b + (threadIdx.x & 63); p < e; p += 64) pos[p - base] = (int32_t)(p - b);
+    }
+}
+
+// flags |= 16 when a row lists a neighbour twice (adjacent equal keys).
+__global__ void k_row_dups(const int64_t* nindex, const int32_t* sval, int32_t n, int64_t m, int* flags) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x + 1; p < m;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        if (sval[p] != sval[p - 1]) continue;
+        // same row?  binary search for the row of p
+        int32_t lo = 0, hi = n;
+        while (hi - lo > 1) {
+            int32_t mid = (lo + hi) >> 1;
+            if (nindex[mid] <= p) lo = mid; else hi = mid;
+        }
+        if (nindex[lo] < p) atomicOr(flags, 16);
+    }
+}
+
+hipError_t mf_build_row_index(GraphState& S, int32_t* sval, int32_t* spos, int* d_flags) {
+    const DevGraph& g = S.g;
+    hipStream_t st = S.stream;
+    if (g.m == 0) return hipSuccess;
+    std::vector<int64_t> ni;
+    const int64_t chunk = int64_t(1) << 30;
+    std::vector<int32_t> cuts{0};
+    if (g.m > chunk) {  // row ranges of at most 2^30 entries
+        ni.resize((size_t)g.n + 1);
+        hipError_t e = hipMemcpy(ni.data(), g.nindex, ni.size() * 8, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return e;
+        int64_t start = 0;
+        for (int32_t r = 0; r < g.n; r++) {
+            if (ni[r + 1] - start > chunk && ni[r] > start) {
+                cuts.push_back(r);
+                start = ni[r];
+            }
+        }
+    }
+    cuts.push_back(g.n);
+    int64_t max_items = g.m;
+    if (!ni.empty()) {
+        max_items = 0;
+        for (size_t k = 0; k + 1 < cuts.size(); k++) max_items = std::max(max_items, ni[cuts[k + 1]] - ni[cuts[k]]);
+    }
+    if (max_items > (int64_t)INT32_MAX) return hipErrorInvalidValue;  // one row of more than 2^31 entries
+    int32_t* pos_in = nullptr;
+    hipError_t e = hipMalloc((void**)&pos_in, (size_t)max_items * 4);
+    if (e != hipSuccess) return e;
+    void* tmp = nullptr;
+    size_t tmp_bytes = 0;
+    for (size_t k = 0; k + 1 < cuts.size() && e == hipSuccess; k++) {
+        int32_t r0 = cuts[k], r1 = cuts[k + 1];
+        int64_t b, en;
+        if (ni.empty()) {
+            b = 0;
+            en = g.m;
+        } else {
+            b = ni[r0];
+            en = ni[r1];
+        }
+        int items = (int)(en - b);
+        if (items == 0) continue;
+        k_row_iota<<<S.blocks, 256, 0, st>>>(g.nindex, r0, r1, pos_in);
+        rocprim::transform_iterator<const int64_t*, RebaseOffset, int> beg(g.nindex + r0, RebaseOffset{b});
+        rocprim::transform_iterator<const int64_t*, RebaseOffset, int> end(g.nindex + r0 + 1, RebaseOffset{b});
+        size_t need = 0;
+        e = hipcub::DeviceSegmentedSort::SortPairs(nullptr, need, g.nlist + b, sval + b, pos_in, spos + b, items,
+                                                    r1 - r0, beg, end, st);
+        if (e != hipSuccess) break;
+        if (need > tmp_bytes) {
+            if (tmp) hipFree(tmp);
+            tmp = nullptr;
+            tmp_bytes = 0;
+            e = hipMalloc(&tmp, need);
+            if (e != hipSuccess) break;
+            tmp_bytes = need;
+        }
+        e = hipcub::DeviceSegmentedSort::SortPairs(tmp, tmp_bytes, g.nlist + b, sval + b, pos_in, spos + b, items,
+                                                    r1 - r0, beg, end, st);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (tmp) hipFree(tmp);
+    hipFree(pos_in);
+    if (e != hipSuccess) return e;
+    k_row_dups<<<S.blocks, 256, 0, st>>>(g.nindex, sval, g.n, g.m, d_flags);
+    return hipGetLastError();
 }

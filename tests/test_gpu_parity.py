@@ -179,3 +179,64 @@ def test_nonmaximal_start_parity(oracle, engine_mod, eng, mode):
             # uncovered from the start elsewhere stay so, as in the oracle
             conf, unc, bad = eng.check()
             assert (conf, bad) == (0, 0) and unc == og.check(m)[1]
+
+
+def test_capacity_growth_with_batches_in_flight(oracle, engine_mod, eng):
+    """A batch larger than any before regrows the staging buffers while
+    earlier batches are still queued: their tickets stay valid and every
+    result is the oracle's."""
+    rng = np.random.default_rng(17)
+    ni, nl = H.random_graph(rng, 600, 0.02)
+    g = oracle.Graph.from_csr(ni, nl)
+    mis0 = g.greedy_mis()
+    sizes = [50, 40000, 30, 70000]
+    batches = []
+    for k in sizes:
+        u = rng.integers(0, 600, k).astype(np.int32)
+        v = rng.integers(0, 600, k).astype(np.int32)
+        keep = u != v
+        u, v = u[keep], v[keep]
+        batches.append((u, v, g.flags(u, v)))
+    want, _, _ = H.oracle_replay(oracle, ni, nl, mis0, batches, completion=True, literal=True)
+    eng.load_csr(ni, nl)
+    eng.set_mis(mis0)
+    eng.set_rule(True)
+    eng.set_mode(engine_mod.MODE_QUEUED)
+    t0 = eng.submit(engine_mod.make_updates(*batches[0]))
+    t1 = eng.submit(engine_mod.make_updates(*batches[1]))   # regrows the staging buffers
+    s0 = eng.wait(t0)
+    s1 = eng.wait(t1)
+    assert s0["updates"] == len(batches[0][0]) and s1["updates"] == len(batches[1][0])
+    assert np.array_equal(eng.get_mis(), want[1])
+    t2 = eng.submit(engine_mod.make_updates(*batches[2]))
+    t3 = eng.submit(engine_mod.make_updates(*batches[3]))
+    eng.wait(t2)
+    eng.wait(t3)
+    assert np.array_equal(eng.get_mis(), want[3])
+    assert eng.check() == (0, 0, 0)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_unsorted_rows_parity(oracle, engine_mod, eng, mode):
+    """Rows in arbitrary order (a graph converted from a file in file order):
+    the order is kept (it decides which vertex an eviction admits) and
+    adjacency tests go through the sorted row copy."""
+    rng = np.random.default_rng(23)
+    ni, nl = H.random_graph(rng, 500, 0.03)
+    nl = nl.copy()
+    for x in range(500):
+        rng.shuffle(nl[ni[x]:ni[x + 1]])
+    g = oracle.Graph.from_csr(ni, nl)
+    mis0 = g.greedy_mis()
+    batches = H.random_batches(rng, oracle, ni, nl, 5, 100, general=True)
+    want, _, og = H.oracle_replay(oracle, ni, nl, mis0, batches, completion=True, literal=True)
+    got, _, checks = H.engine_replay(engine_mod, eng, ni, nl, mis0, batches, mode=mode)
+    for b in range(len(batches)):
+        assert np.array_equal(want[b], got[b]), "batch %d" % b
+        assert checks[b] == (0, 0, 0)
+    for v in range(0, 500, 11):
+        assert np.array_equal(eng.neighbors(v), og.neighbors(v))
+    dup = nl.copy()
+    dup[ni[3] + 1] = dup[ni[3]]  # a row listing a neighbour twice is refused
+    with pytest.raises(engine_mod.MisfitError):
+        eng.load_csr(ni, dup)
