@@ -83,8 +83,11 @@ def main():
     plans = [schedule(rng, r, args.seconds) for r in rates]
     total = int(sum(int(s.sum()) for _, s in plans))
     t = time.time()
+    print("generating R-MAT scale %d" % args.scale, file=sys.stderr, flush=True)
     ni, nl = M.gen_rmat(args.scale, 16, args.seed)
+    print("  %d entries (%.1fs); greedy MIS" % (len(nl), time.time() - t), file=sys.stderr, flush=True)
     mis0 = M.greedy_mis(ni, nl)
+    print("  %d updates (%.1fs)" % (total, time.time() - t), file=sys.stderr, flush=True)
     u, v = M.gen_updates(M.MIXED_EXISTING, ni, nl, total, args.seed)
     print("workload: n=%d m=%d, %d updates in %d requests, generated in %.1fs" % (
         len(ni) - 1, len(nl), total, sum(len(s) for _, s in plans), time.time() - t), file=sys.stderr, flush=True)
@@ -95,6 +98,7 @@ def main():
     ups = M.make_updates(u, v, ins)
     results = []
     with M.Server(0, compact_fraction=args.compact, max_inflight=2) as srv:
+        print("loading the graph into the server", file=sys.stderr, flush=True)
         gid = srv.add_graph(ni, nl, mis0)
         del nl
         cursor = 0
