@@ -107,7 +107,13 @@ int misfit_set_mode(misfit_ctx* ctx, int mode);
 int misfit_flags(misfit_ctx* ctx, const int32_t* u, const int32_t* v, int64_t count, uint8_t* ins);
 
 /* Queues one batch: overlay update, exact repair, completion.  Returns at
- * once in queued mode (the host buffer is copied before the call returns).  Replaces the async engine's launchBatch
+ * once in queued mode (the host buffer is copied before the call returns).
+ * The speculative rounds are queued from the recent rounds per update of
+ * batches at least a quarter as large; without such history (a graph's
+ * first batches, or a much larger batch) the batch may need rounds queued
+ * later from the host: misfit_poll / misfit_wait do that, and the next
+ * submit on the context (or any call that reads the graph) first waits for
+ * such a batch to finish.  Replaces the async engine's launchBatch
  * (src/misfit_async/misfit_async.cu:484-675) together with the overlay
  * update of launchBatchGPU (:718-740). */
 int misfit_submit_batch(misfit_ctx* ctx, const misfit_update* ups, int64_t count, uint64_t* ticket);
@@ -118,7 +124,8 @@ int misfit_wait(misfit_ctx* ctx, uint64_t ticket, misfit_batch_stats* st);
 
 /* Non-blocking: 1 when the submitted batch has finished on the device (its
  * results can be collected with misfit_wait without blocking), 0 while it is
- * still queued or running.  Replaces the reference's blocking event waits in
+ * still queued or running (a batch that needs more rounds than were queued
+ * gets them here).  Replaces the reference's blocking event waits in
  * syncAndCollect (src/misfit_async/misfit_async.cu:677-701) for callers that
  * multiplex many graphs on one thread. */
 int misfit_poll(misfit_ctx* ctx, uint64_t ticket);
@@ -173,7 +180,9 @@ int misfit_device_info(misfit_ctx* ctx, char* name, int namelen, int64_t* total_
  * misfit_async.cu:484-701,827-876): graphs are admitted by free device
  * memory, each graph's requests run in submission order with up to
  * max_inflight batches queued on the device, batches of different graphs
- * overlap on the GPU, a graph is compacted when its overlay passes
+ * overlap on the GPU (fed by a pool of dispatcher threads: a launch costs
+ * the host about 4 us, and separate threads on separate streams scale), a
+ * graph is compacted when its overlay passes
  * compact_fraction of its base entries, and every request reports its
  * queueing time and latency.  Thread-safe: any thread may submit or wait. */
 typedef struct misfit_server misfit_server;
@@ -181,7 +190,10 @@ typedef struct misfit_server misfit_server;
 typedef struct {
     int64_t hbm_reserve;      /* device bytes kept free: graphs and batches that would cut into it are refused */
     double compact_fraction;  /* compact a graph when its overlay exceeds this share of its base (0: never) */
-    int max_inflight;         /* batches per graph queued on the device at once (1..3) */
+    int max_inflight;         /* batches per graph queued on the device at once (1..3); with 1, every
+                                 batch is queued with the rounds it is expected to need and gets more
+                                 from the host if it needs them, so no rounds run empty (many graphs) */
+    int workers;              /* dispatcher threads, each owning a share of the graphs (0: 8) */
 } misfit_server_opts;
 
 typedef struct {
@@ -285,7 +297,14 @@ int misfit_greedy_mis(int32_t n, const int64_t* nindex, const int32_t* nlist, ui
  *   kind 1: half deletions of uniform existing edges, half insertions of
  *           uniform non-edges, distinct across the whole stream, shuffled
  *   kind 2: both endpoints drawn proportionally to (degree + 1)
- *   kind 3: endpoints drawn from the lowest-degree quarter of the vertices */
+ *   kind 3: endpoints drawn from the lowest-degree quarter of the vertices
+ * and the reference's batch samplers (distinct existing edges as (lo, hi),
+ * shuffled; count must not exceed the entries the graph can supply):
+ *   kind 4: a uniform CSR entry (tools/generate_test_batches.py:55-78)
+ *   kind 5: a vertex drawn proportionally to 1/degree, then a uniform
+ *           neighbour (tools/generate_low_degree_batches.py:56-100)
+ *   kind 6: a vertex drawn proportionally to its degree, then a uniform
+ *           neighbour (tools/generate_preferential_batches.py:56-98) */
 int misfit_gen_updates(int kind, int32_t n, const int64_t* nindex, const int32_t* nlist, int64_t count,
                        uint64_t seed, int32_t* u, int32_t* v);
 

@@ -57,7 +57,7 @@ class BatchStats(ctypes.Structure):
 
 class ServerOpts(ctypes.Structure):
     _fields_ = [("hbm_reserve", ctypes.c_int64), ("compact_fraction", ctypes.c_double),
-                ("max_inflight", ctypes.c_int)]
+                ("max_inflight", ctypes.c_int), ("workers", ctypes.c_int)]
 
 
 class RequestStats(ctypes.Structure):
@@ -286,6 +286,9 @@ def greedy_mis(nindex, nlist):
 
 
 UNIFORM_PAIRS, MIXED_EXISTING, PREFERENTIAL, LOW_DEGREE = 0, 1, 2, 3
+# the reference's batch samplers: distinct existing edges, uniform entry /
+# vertex by 1/degree / vertex by degree, then a uniform neighbour
+EDGE_UNIFORM, EDGE_LOW_DEGREE, EDGE_PREFERENTIAL = 4, 5, 6
 
 
 def gen_updates(kind, nindex, nlist, count, seed):
@@ -395,6 +398,13 @@ class Engine:
         self._ck(lib().misfit_wait(self.ctx, ticket, ctypes.byref(st)))
         return st.as_dict()
 
+    def poll(self, ticket):
+        """True once the batch has finished on the device (never blocks)."""
+        rc = lib().misfit_poll(self.ctx, ticket)
+        if rc < 0:
+            self._ck(rc)
+        return rc == 1
+
     def process_batch(self, updates):
         updates = np.ascontiguousarray(updates, UPDATE_DTYPE)
         st = BatchStats()
@@ -451,9 +461,9 @@ class Server:
     """Serving layer (misfit_server_*): many graphs on one GPU, update batches
     submitted as they arrive, continuous batching across graphs."""
 
-    def __init__(self, device=0, hbm_reserve=0, compact_fraction=0.05, max_inflight=2):
+    def __init__(self, device=0, hbm_reserve=0, compact_fraction=0.05, max_inflight=2, workers=0):
         self.srv = ctypes.c_void_p()
-        opts = ServerOpts(hbm_reserve, compact_fraction, max_inflight)
+        opts = ServerOpts(hbm_reserve, compact_fraction, max_inflight, workers)
         _ck(lib().misfit_server_create(device, ctypes.byref(opts), ctypes.byref(self.srv)))
 
     def _ck(self, rc):

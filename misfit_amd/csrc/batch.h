@@ -42,6 +42,7 @@ enum BatchCounter {
     C_START0, C_START1,  // first uncommitted edge, read by the round of that parity
     C_KTASK,         // row chunks of members that may be evicted
     C_GUARD,         // a loop bound was hit or a list overflowed (engine bug): results invalid
+    C_HALT,          // the tail stopped with many edges left: the host queues more rounds
     C_NCOUNTERS
 };
 
@@ -59,6 +60,7 @@ struct BP {
     int completion;
     int64_t window;    // max edges speculated per round
     int64_t n_ins;     // insertion updates in the batch (bounds evictions per round)
+    int may_halt;      // the tail may stop with edges left (nothing of this graph is queued behind)
     // pair table of this batch
     u64* bt_key;
     int32_t* bt_head;
@@ -104,6 +106,13 @@ struct LaunchCfg {
     int blocks;        // grid cap for grid-stride phases (8 workgroups per CU)
     hipStream_t stream;
     int round_per_cu = 0;  // speculation/validation workgroups per CU (0: by occupancy)
+    // Per-batch sizing (U < 0: every phase gets the whole GPU).  Phases get
+    // as many workgroups as their work needs, so a small batch leaves the
+    // GPU to other graphs' batches: a thread per update, a wave per update,
+    // a thread per four vertices; the rounds get round_grid workgroups.
+    int64_t U = -1;
+    int64_t n = 0;
+    int round_grid = 0;
 };
 void mf_launch_batch_prologue(const BP& P, const LaunchCfg& L);
 int mf_round_grid(const LaunchCfg& L);  // workgroups of the speculation/validation kernels

@@ -38,6 +38,10 @@ struct Slot {
     int64_t count = 0;
     uint64_t ticket = 0;
     bool busy = false;
+    BP P{};                // the batch's parameters and grids, for rounds queued after its tail halted
+    LaunchCfg L{0, nullptr};
+    int par_next = 0;      // parity of the batch's next round and tail
+    bool may_halt = false; // queued without a reliable round estimate; not yet settled
 };
 
 struct misfit_ctx {
@@ -51,8 +55,10 @@ struct misfit_ctx {
     int mode = MISFIT_MODE_QUEUED;
     int grid_rounds = 0;       // MISFIT_GRID_ROUNDS: fixed number of queued grid rounds
     bool grid_rounds_fixed = false;
-    double rpu_hist[16] = {0}; // (rounds - 1) per update of recent batches of >= 2048 updates
-    int rpu_pos = 0;
+    double hist_rpu[16] = {0};    // (rounds - 1) per update of recent batches of >= 512 updates
+    int64_t hist_count[16] = {0}; // ... and their sizes
+    int hist_pos = 0;
+    bool always_halt = false;     // every batch may halt (nothing is ever queued behind one)
     int round_per_cu = 0;      // MISFIT_ROUND_WG_PER_CU: round grid (0: by occupancy)
     int rounds_pos = 0;
     bool debug_rounds = false;
@@ -99,6 +105,7 @@ static int fail(misfit_ctx* c, int code, const std::string& msg) {
     } while (0)
 
 static int build_row_index(misfit_ctx* c);
+static int settle(misfit_ctx* c);
 
 template <typename T>
 static hipError_t dalloc(std::vector<void*>& owner, T** p, int64_t count) {
@@ -221,6 +228,7 @@ extern "C" int misfit_set_mode(misfit_ctx* c, int mode) {
 extern "C" int misfit_sync(misfit_ctx* c) {
     if (!c) return MISFIT_E_ARG;
     HIPCHK(c, hipSetDevice(c->device));
+    if (int rc_ = settle(c)) return rc_;
     HIPCHK(c, hipStreamSynchronize(c->copy_stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return MISFIT_OK;
@@ -357,6 +365,7 @@ extern "C" int misfit_load_csr(misfit_ctx* c, int32_t n, int64_t m, const void* 
     if (!c) return MISFIT_E_ARG;
     if (n < 1 || m < 0 || !nindex || (m > 0 && !nlist)) return fail(c, MISFIT_E_ARG, "invalid graph arguments");
     HIPCHK(c, hipSetDevice(c->device));
+    if (int rc_ = settle(c)) return rc_;
     HIPCHK(c, hipStreamSynchronize(c->copy_stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     // the batch scratch keeps per-batch pointers only; graph arrays are new
@@ -486,6 +495,7 @@ extern "C" int misfit_set_mis(misfit_ctx* c, const uint8_t* mis) {
     if (!c || !mis) return MISFIT_E_ARG;
     if (!c->has_graph) return fail(c, MISFIT_E_STATE, "no graph loaded");
     HIPCHK(c, hipSetDevice(c->device));
+    if (int rc_ = settle(c)) return rc_;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     int32_t n = c->P.g.n;
     uint8_t* d = nullptr;
@@ -502,6 +512,7 @@ extern "C" int misfit_get_mis(misfit_ctx* c, uint8_t* mis) {
     if (!c || !mis) return MISFIT_E_ARG;
     if (!c->has_graph) return fail(c, MISFIT_E_STATE, "no graph loaded");
     HIPCHK(c, hipSetDevice(c->device));
+    if (int rc_ = settle(c)) return rc_;
     int32_t n = c->P.g.n;
     uint8_t* d = nullptr;
     HIPCHK(c, hipMalloc((void**)&d, n));
@@ -521,6 +532,7 @@ extern "C" int misfit_flags(misfit_ctx* c, const int32_t* u, const int32_t* v, i
     for (int64_t i = 0; i < count; i++)
         if (u[i] < 0 || u[i] >= n || v[i] < 0 || v[i] >= n) return fail(c, MISFIT_E_ARG, "vertex out of range");
     HIPCHK(c, hipSetDevice(c->device));
+    if (int rc_ = settle(c)) return rc_;
     int32_t *du = nullptr, *dv = nullptr;
     uint8_t* di = nullptr;
     hipError_t e = hipMalloc((void**)&du, count * 4);
@@ -564,8 +576,7 @@ static void fill_stats(const misfit_ctx* c, const Slot& s, misfit_batch_stats* s
 
 // Host-driven rounds: one round = 4 kernels, then the host reads where the
 // committed prefix ends; the completion iterations likewise.
-static int drive_plain(misfit_ctx* c, BP& P) {
-    LaunchCfg L{c->S.blocks, c->stream, c->round_per_cu};
+static int drive_plain(misfit_ctx* c, BP& P, const LaunchCfg& L) {
     mf_launch_batch_prologue(P, L);
     for (uint32_t r = 0;; r++) {
         if ((int64_t)r > P.U + 1) return fail(c, MISFIT_E_CHECK, "speculative rounds made no progress");
@@ -609,30 +620,145 @@ static void launch_queued(const BP& P, const LaunchCfg& L, int k) {
     mf_launch_finish(P, L, k & 1);
 }
 
-// Grid rounds to queue for a batch of `count` updates: one round plus the
-// recent maximum of further rounds per update (conflicts grow with the
-// batch; measured on batches large enough to show it), with a margin, and at
-// least one round per window.  Rounds past the end of the batch cost one
-// empty launch each; rounds left over are finished by the one-workgroup
-// tail, slowly, so the estimate errs high.
-static int queued_rounds(misfit_ctx* c, int64_t count, int64_t window) {
-    if (c->grid_rounds_fixed) return c->grid_rounds;
+// The rounds of a batch of `count` updates.  Their number: one round plus
+// the most further rounds per update seen on recent batches at least a
+// quarter as large (conflicts per update grow with the batch, so larger
+// batches bound smaller ones), with a margin, and at least one round per
+// window; rounds past the end of the batch cost one empty launch each.
+// Their window (edges speculated per round, one per wave of the round
+// grid): the whole GPU, or 16 times the edges a round is expected to commit
+// when that is less (at least 256), so a batch whose rounds commit few
+// edges each leaves the GPU to other graphs.  *sure is false without such
+// history (a graph's first batches, or a batch far larger than any before):
+// the batch's tail may then halt and have more rounds queued from the host.
+struct RoundPlan {
+    int k;
+    int64_t window;
+    bool sure;
+};
+
+static RoundPlan plan_rounds(misfit_ctx* c, int64_t count, int64_t full_window) {
+    RoundPlan p{0, full_window, true};
     double rpu = 0.0;
-    for (double h : c->rpu_hist) rpu = std::max(rpu, h);
-    if (rpu == 0.0) rpu = 1.0 / 400.0;  // before any history
+    bool any = false;
+    for (int i = 0; i < 16; i++)
+        if (c->hist_count[i] > 0 && 4 * c->hist_count[i] >= count) {
+            rpu = std::max(rpu, c->hist_rpu[i]);
+            any = true;
+        }
+    if (c->always_halt) {
+        // rounds as expected, not as many as may be needed: a batch that
+        // needs more halts and gets them from the host, so no graph's
+        // rounds are queued to run empty
+        p.sure = count == 0;
+        double recent = -1.0;
+        for (int j = 1; j <= 16 && recent < 0; j++) {
+            int i = (c->hist_pos - j + 32) & 15;
+            if (c->hist_count[i] > 0 && 4 * c->hist_count[i] >= count) recent = c->hist_rpu[i];
+        }
+        if (recent >= 0) {
+            double per_round = (double)count / (1.0 + (double)count * recent);
+            int64_t w = (int64_t)std::ceil(16.0 * per_round);
+            p.window = std::min(std::max<int64_t>(256, (w + 31) / 32 * 32), full_window);
+        }
+        double k = std::ceil((double)count * std::max(recent, 0.0)) + 1.0;
+        k = std::max(k, std::ceil((double)count / (double)std::max<int64_t>(p.window, 1)) + 1.0);
+        p.k = (int)std::min(k, 65536.0);
+        return p;
+    }
+    if (!any) {
+        rpu = 1.0 / 400.0;
+        p.sure = count == 0;
+    } else {
+        double per_round = (double)count / (1.0 + (double)count * rpu);
+        int64_t w = (int64_t)std::ceil(16.0 * per_round);
+        w = std::max<int64_t>(256, (w + 31) / 32 * 32);
+        p.window = std::min(w, full_window);
+    }
+    if (c->grid_rounds_fixed) {
+        p.k = c->grid_rounds;
+        p.sure = true;
+        return p;
+    }
     double k = std::ceil((double)count * rpu * 1.15) + 3.0;
-    double per_window = std::ceil((double)count / (double)std::max<int64_t>(window, 1)) + 2.0;
+    double per_window = std::ceil((double)count / (double)std::max<int64_t>(p.window, 1)) + 2.0;
     k = std::max(k, per_window);
-    return (int)std::min(k, 65536.0);
+    p.k = (int)std::min(k, 65536.0);
+    return p;
 }
 
 // Whole batch enqueued with no host synchronisation inside it.
-static int drive_queued(misfit_ctx* c, BP& P) {
-    LaunchCfg L{c->S.blocks, c->stream, c->round_per_cu};
-    launch_queued(P, L, queued_rounds(c, P.U, P.window));
+static int drive_queued(misfit_ctx* c, BP& P, const LaunchCfg& L, int k) {
+    launch_queued(P, L, k);
     HIPCHK(c, hipGetLastError());
     return MISFIT_OK;
 }
+
+// The tail of slot s's batch halted with edges left (C_HALT): queue more
+// grid-wide rounds, as many as the batch's own rate of rounds per committed
+// edge predicts for the rest (with a margin unless every batch may halt),
+// but at most twice the rounds run so far (at least 8): a rate taken from a
+// few rounds is noisy, and doubling bounds the empty rounds to half.  The
+// window follows the batch's measured edges per round like a planned one
+// (rounds are independent of each other's window).  Then another tail, and
+// the batch's end events are recorded again.  Nothing else of this graph
+// was queued behind it.
+static int continue_batch(misfit_ctx* c, Slot& s) {
+    const u64* h = s.h_ctr;
+    int64_t start = (int64_t)h[C_START0 + s.par_next];
+    int64_t left = std::max<int64_t>(s.P.U - start, 0);
+    if (h[C_ROUNDS] > 0) {
+        const int64_t wpg = 256 / MF_WAVE;
+        LaunchCfg full = s.L;
+        full.round_grid = 0;
+        const int64_t full_window = std::max<int64_t>((int64_t)mf_round_grid(full) * wpg, wpg);
+        int64_t w = (int64_t)std::ceil(16.0 * (double)start / (double)h[C_ROUNDS]);
+        w = std::min(std::max<int64_t>(256, (w + 31) / 32 * 32), full_window);
+        s.L.round_grid = (int)std::max<int64_t>(w / wpg, 1);
+        s.P.window = (int64_t)s.L.round_grid * wpg;
+    }
+    double per_edge = (double)h[C_ROUNDS] / (double)std::max<int64_t>(start, 1);
+    double k = c->always_halt ? std::ceil((double)left * per_edge) + 1.0 : std::ceil((double)left * per_edge * 1.25) + 2.0;
+    k = std::min(k, std::max(8.0, 2.0 * (double)h[C_ROUNDS]));
+    k = std::max(k, std::ceil((double)left / (double)std::max<int64_t>(s.P.window, 1)));
+    int K = (int)std::min(k, 65536.0);
+    for (int r = 0; r < K; r++) mf_launch_round(s.P, s.L, (s.par_next + r) & 1);
+    s.par_next = (s.par_next + K) & 1;
+    mf_launch_finish(s.P, s.L, s.par_next);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(s.t1, c->stream));
+    HIPCHK(c, hipEventRecord(s.done, c->stream));
+    return MISFIT_OK;
+}
+
+// Blocks until every batch queued without a reliable round estimate has
+// finished (queueing further rounds where its tail halted).  Called before
+// anything that queues more work on the graph or reads its state.
+static int settle(misfit_ctx* c) {
+    for (auto& s : c->slots) {
+        if (!s.busy || !s.may_halt) continue;
+        while (true) {
+            HIPCHK(c, hipEventSynchronize(s.done));
+            if (!s.h_ctr[C_HALT]) break;
+            int rc = continue_batch(c, s);
+            if (rc) return rc;
+        }
+        s.may_halt = false;
+    }
+    return MISFIT_OK;
+}
+
+// For the serving layer: 1 while a batch of this context may still need
+// rounds queued from the host (its graph should get no further batch yet).
+int mf_ctx_settling(const misfit_ctx* c) {
+    for (const auto& s : c->slots)
+        if (s.busy && s.may_halt) return 1;
+    return 0;
+}
+
+// For the serving layer with one batch per graph on the device: every batch
+// may halt, and its rounds are queued as expected rather than as needed.
+void mf_ctx_set_halting(misfit_ctx* c, bool on) { c->always_halt = on; }
 
 // Base rows that are not ascending get a sorted copy with positions, so
 // adjacency tests binary-search them (any older copy is dropped first).
@@ -685,7 +811,9 @@ extern "C" int misfit_submit_batch(misfit_ctx* c, const misfit_update* ups, int6
         nins += ups[i].is_insertion ? 1 : 0;
     }
     HIPCHK(c, hipSetDevice(c->device));
-    int rc = ensure_batch_capacity(c, count);
+    int rc = settle(c);
+    if (rc) return rc;
+    rc = ensure_batch_capacity(c, count);
     if (rc) return rc;
     rc = ensure_overlay_capacity(c, nins);
     if (rc) return rc;
@@ -720,16 +848,29 @@ extern "C" int misfit_submit_batch(misfit_ctx* c, const misfit_update* ups, int6
     P.U = count;
     P.b = c->epoch;
     P.completion = c->completion;
-    // one edge per wave of the round grid and round
+    // grids sized to the batch; one edge per wave of the round grid and round
     LaunchCfg L{c->S.blocks, c->stream, c->round_per_cu};
-    P.window = std::max<int64_t>((int64_t)mf_round_grid(L) * (256 / MF_WAVE), 1);
+    L.U = count;
+    L.n = c->P.g.n;
+    const int64_t wpg = 256 / MF_WAVE;
+    RoundPlan plan = plan_rounds(c, count, std::max<int64_t>((int64_t)mf_round_grid(L) * wpg, wpg));
+    L.round_grid = (int)std::max<int64_t>(plan.window / wpg, 1);
+    P.window = (int64_t)L.round_grid * wpg;
     P.n_ins = nins;
     P.ktask = c->d_ktask;
     P.ktask_cap = c->ktask_cap;
     P.h_out = s.h_ctr;
+    const bool queued = c->mode == MISFIT_MODE_QUEUED;
+    const bool sure = !queued || plan.sure;
+    const int k = queued ? plan.k : 0;
+    P.may_halt = sure ? 0 : 1;
+    s.P = P;
+    s.L = L;
+    s.par_next = k & 1;
+    s.may_halt = !sure;
     c->last_U = count;
     HIPCHK(c, hipEventRecord(s.t0, c->stream));
-    rc = c->mode == MISFIT_MODE_QUEUED ? drive_queued(c, P) : drive_plain(c, P);
+    rc = queued ? drive_queued(c, P, L, k) : drive_plain(c, P, L);
     if (rc) return rc;
     // the counters and the pool top reached s.h_ctr from the batch's last
     // kernel (the finisher, or the publish kernel of the plain driver)
@@ -744,10 +885,14 @@ extern "C" int misfit_poll(misfit_ctx* c, uint64_t ticket) {
     Slot& s = c->slots[ticket % MF_NSLOT];
     if (!s.busy || s.ticket != ticket) return fail(c, MISFIT_E_ARG, "unknown or expired ticket");
     hipError_t e = hipEventQuery(s.done);
-    if (e == hipSuccess) return 1;
     if (e == hipErrorNotReady) return 0;
     HIPCHK(c, e);
-    return 0;
+    if (s.may_halt && s.h_ctr[C_HALT]) {  // halted tail: queue the rest, not done yet
+        HIPCHK(c, hipSetDevice(c->device));
+        int rc = continue_batch(c, s);
+        return rc ? rc : 0;
+    }
+    return 1;
 }
 
 extern "C" int misfit_wait(misfit_ctx* c, uint64_t ticket, misfit_batch_stats* st) {
@@ -755,10 +900,19 @@ extern "C" int misfit_wait(misfit_ctx* c, uint64_t ticket, misfit_batch_stats* s
     Slot& s = c->slots[ticket % MF_NSLOT];
     if (!s.busy || s.ticket != ticket) return fail(c, MISFIT_E_ARG, "unknown or expired ticket");
     HIPCHK(c, hipSetDevice(c->device));
-    HIPCHK(c, hipEventSynchronize(s.done));
+    while (true) {
+        HIPCHK(c, hipEventSynchronize(s.done));
+        if (!(s.may_halt && s.h_ctr[C_HALT])) break;
+        int rc = continue_batch(c, s);
+        if (rc) return rc;
+    }
     s.busy = false;
-    if (s.count >= 2048 && s.h_ctr[C_ROUNDS] >= 1)
-        c->rpu_hist[c->rpu_pos++ & 15] = (double)(s.h_ctr[C_ROUNDS] - 1) / (double)s.count;
+    s.may_halt = false;
+    if (s.count >= 512 && s.h_ctr[C_ROUNDS] >= 1) {
+        c->hist_rpu[c->hist_pos & 15] = (double)(s.h_ctr[C_ROUNDS] - 1) / (double)s.count;
+        c->hist_count[c->hist_pos & 15] = s.count;
+        c->hist_pos++;
+    }
     if (s.h_ctr[C_POOL_OVF]) return fail(c, MISFIT_E_CHECK, "overlay pool overflow");
     if (s.h_ctr[C_GUARD]) return fail(c, MISFIT_E_CHECK, "repair loop bound exceeded or task list overflow");
     if (st) fill_stats(c, s, st);
@@ -777,6 +931,7 @@ extern "C" int misfit_check(misfit_ctx* c, int64_t* conflicts, int64_t* uncovere
     if (!c) return MISFIT_E_ARG;
     if (!c->has_graph) return fail(c, MISFIT_E_STATE, "no graph loaded");
     HIPCHK(c, hipSetDevice(c->device));
+    if (int rc_ = settle(c)) return rc_;
     HIPCHK(c, mf_check(c->S, c->d_acc));
     HIPCHK(c, hipMemcpyAsync(c->h_small, c->d_acc, 4 * sizeof(u64), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -790,6 +945,7 @@ extern "C" int misfit_batch_bytes(misfit_ctx* c, int index_bytes, int64_t* bytes
     if (!c) return MISFIT_E_ARG;
     if (!c->has_graph) return fail(c, MISFIT_E_STATE, "no graph loaded");
     HIPCHK(c, hipSetDevice(c->device));
+    if (int rc_ = settle(c)) return rc_;
     c->bytes_tag++;
     HIPCHK(c, mf_batch_bytes(c->S, c->P.seeds, c->P.ctr + MF_CS * C_SEEDS, c->bytes_tag, c->d_acc));
     HIPCHK(c, hipMemcpyAsync(c->h_small, c->d_acc, 4 * sizeof(u64), hipMemcpyDeviceToHost, c->stream));
@@ -806,6 +962,7 @@ extern "C" int misfit_compact(misfit_ctx* c, int64_t* new_m) {
     if (!c) return MISFIT_E_ARG;
     if (!c->has_graph) return fail(c, MISFIT_E_STATE, "no graph loaded");
     HIPCHK(c, hipSetDevice(c->device));
+    if (int rc_ = settle(c)) return rc_;
     HIPCHK(c, hipStreamSynchronize(c->copy_stream));
     DevGraph& g = c->P.g;
     int64_t* nidx = nullptr;
@@ -859,6 +1016,7 @@ extern "C" int misfit_graph_size(misfit_ctx* c, int32_t* n, int64_t* base_m, int
     if (base_m) *base_m = g.m;
     if (live_m) {
         HIPCHK(c, hipSetDevice(c->device));
+        if (int rc_ = settle(c)) return rc_;
         int64_t* nidx = nullptr;
         HIPCHK(c, hipMalloc((void**)&nidx, ((size_t)g.n + 1) * 8));
         hipError_t e = mf_compact_plan(c->S, nidx);
@@ -876,6 +1034,7 @@ extern "C" int misfit_export_csr(misfit_ctx* c, int64_t* nindex, int32_t* nlist)
     if (!c || !nindex) return MISFIT_E_ARG;
     if (!c->has_graph) return fail(c, MISFIT_E_STATE, "no graph loaded");
     HIPCHK(c, hipSetDevice(c->device));
+    if (int rc_ = settle(c)) return rc_;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     DevGraph& g = c->P.g;
     HIPCHK(c, hipMemcpy(nindex, g.nindex, ((size_t)g.n + 1) * 8, hipMemcpyDeviceToHost));
@@ -895,6 +1054,7 @@ extern "C" int misfit_neighbors(misfit_ctx* c, int32_t v, int32_t* out, int64_t 
     DevGraph& g = c->P.g;
     if (v < 0 || v >= g.n) return fail(c, MISFIT_E_ARG, "vertex out of range");
     HIPCHK(c, hipSetDevice(c->device));
+    if (int rc_ = settle(c)) return rc_;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     int64_t be[2];
     int64_t ovs = 0;

@@ -323,6 +323,68 @@ extern "C" int misfit_gen_updates(int kind, int32_t n, const int64_t* nindex, co
         }
         return MISFIT_OK;
     }
+    if (kind >= 4 && kind <= 6) {
+        // Distinct existing undirected edges as (lo, hi), shuffled; all
+        // deletions against the loaded graph.  kind 4: a uniform CSR entry
+        // (tools/generate_test_batches.py:55-78); kind 5: a vertex drawn
+        // proportionally to 1/degree, then a uniform neighbour
+        // (tools/generate_low_degree_batches.py:56-100); kind 6: a vertex
+        // drawn proportionally to its degree, then a uniform neighbour
+        // (tools/generate_preferential_batches.py:56-98).  Fails, like those
+        // scripts, when the graph cannot supply count distinct edges.
+        if (count > m) return MISFIT_E_ARG;
+        std::vector<double> cumw;
+        if (kind == 5) {
+            cumw.assign((size_t)n + 1, 0.0);
+            for (int32_t x = 0; x < n; x++) {
+                int64_t d = nindex[x + 1] - nindex[x];
+                cumw[x + 1] = cumw[x] + (d > 0 ? 1.0 / (double)d : 0.0);
+            }
+            if (cumw[n] <= 0) return MISFIT_E_ARG;
+        }
+        if (m <= 0) return MISFIT_E_ARG;
+        std::unordered_set<uint64_t> used;
+        used.reserve((size_t)count * 2);
+        std::vector<uint64_t> out;
+        out.reserve((size_t)count);
+        const int64_t max_draws = 200 * count + (1 << 20);
+        for (int64_t t = 0; (int64_t)out.size() < count; t++) {
+            if (t >= max_draws) return MISFIT_E_ARG;
+            int32_t src;
+            int64_t p;
+            if (kind == 5) {
+                double w = r.unit() * cumw[n];
+                src = (int32_t)(std::upper_bound(cumw.begin(), cumw.end(), w) - cumw.begin() - 1);
+                if (src < 0) src = 0;
+                if (src >= n) src = n - 1;
+                int64_t d = nindex[src + 1] - nindex[src];
+                if (d <= 0) continue;
+                p = nindex[src] + (int64_t)r.below((uint64_t)d);
+            } else if (kind == 6) {
+                int64_t q = (int64_t)r.below((uint64_t)m);  // proportional to degree
+                src = (int32_t)(std::upper_bound(nindex, nindex + n + 1, q) - nindex - 1);
+                int64_t d = nindex[src + 1] - nindex[src];
+                p = nindex[src] + (int64_t)r.below((uint64_t)d);
+            } else {
+                p = (int64_t)r.below((uint64_t)m);
+                src = (int32_t)(std::upper_bound(nindex, nindex + n + 1, p) - nindex - 1);
+            }
+            int32_t dst = nlist[p];
+            if (dst == src) continue;
+            uint32_t lo = (uint32_t)std::min(src, dst), hi = (uint32_t)std::max(src, dst);
+            if (!used.insert(dpair(lo, hi)).second) continue;
+            out.push_back(dpair(lo, hi));
+        }
+        for (int64_t i = (int64_t)out.size() - 1; i > 0; i--) {
+            int64_t j = (int64_t)r.below((uint64_t)(i + 1));
+            std::swap(out[(size_t)i], out[(size_t)j]);
+        }
+        for (int64_t i = 0; i < count; i++) {
+            u[i] = (int32_t)(out[(size_t)i] >> 32);
+            v[i] = (int32_t)(out[(size_t)i] & 0xFFFFFFFFu);
+        }
+        return MISFIT_OK;
+    }
     if (kind == 3) {
         std::vector<int32_t> ids((size_t)n);
         for (int32_t x = 0; x < n; x++) ids[x] = x;

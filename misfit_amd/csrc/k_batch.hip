@@ -819,6 +819,10 @@ struct RoundView {
 // for the grid-wide rounds (P.window), MF_FIN_WINDOW in the one-workgroup
 // tail, so a round costs about the same whatever the batch size.
 #define MF_FIN_WINDOW 1024
+// A tail that may halt leaves more than this many uncommitted edges to
+// further grid-wide rounds queued by the host: one workgroup is far slower
+// per round than the whole GPU.
+#define MF_HALT_EDGES 256
 __device__ __forceinline__ RoundView round_view(const BP& P, int par, int64_t win) {
     RoundView v;
     v.start = (int64_t)P.ctr[MF_CS * (C_START0 + par)];
@@ -1277,9 +1281,23 @@ __device__ void finish_completion(const BP& P, const GC& c) {
 // whatever rounds they left, the rule counters, the completion candidates
 // and iterations, then the counters are published to the host.  Its phases
 // are separated by workgroup barriers, far cheaper than grid-wide
-// synchronisation on a multi-die GPU, and each of them has few items.
+// synchronisation on a multi-die GPU, and each of them has few items.  A
+// tail that may halt (P.may_halt: the host queued the rounds without a
+// reliable estimate and nothing of this graph behind them) stops instead
+// when more than MF_HALT_EDGES edges are left, publishing C_HALT = 1, and
+// the host queues more grid-wide rounds and another tail.
 __global__ __launch_bounds__(1024) void k_finish(BP P, int par) {
     GC c = block_coords();
+    if (P.may_halt) {
+        phase_barrier();
+        const int64_t start = (int64_t)P.ctr[MF_CS * (C_START0 + par)];
+        const bool halt = P.U - start > MF_HALT_EDGES;
+        if (c.tid == 0) P.ctr[MF_CS * (C_HALT)] = halt ? 1ull : 0ull;
+        if (halt) {
+            publish(P, c);
+            return;
+        }
+    }
     if (finish_rounds(P, c, par)) {
         phase_barrier();
         ph_stats(P, c);
@@ -1332,10 +1350,24 @@ int grid_for(const void* fn, const LaunchCfg& L) {
 }
 }  // namespace
 
+enum GridClass { G_FULL, G_UPD, G_WAVE_UPD, G_SCAN };
+
+// The grid of a phase for this batch: what its work needs (see LaunchCfg),
+// at least 8 workgroups, at most what the GPU holds at once.
+int grid_cls(const void* fn, const LaunchCfg& L, GridClass cls) {
+    int full = grid_for(fn, L);
+    if (L.U < 0 || cls == G_FULL) return full;
+    int64_t want = cls == G_UPD ? (L.U + 255) / 256 : cls == G_WAVE_UPD ? (L.U + 3) / 4 : (L.n + 1023) / 1024;
+    if (want < 8) want = 8;
+    return want < full ? (int)want : full;
+}
+
 // Speculation and validation share one grid: wave w of the speculation
 // writes read-log segment w, which wave w's slots of the validation check.
-// L.round_per_cu > 0 sets it (workgroups per CU) instead of the occupancy.
+// L.round_grid > 0 sets it for the batch; otherwise L.round_per_cu > 0 sets
+// it (workgroups per CU) instead of the occupancy.
 int mf_round_grid(const LaunchCfg& L) {
+    if (L.round_grid > 0) return L.round_grid;
     if (L.round_per_cu > 0) {
         int g = L.round_per_cu * (L.blocks / 8);
         return g < L.blocks ? g : L.blocks;
@@ -1346,25 +1378,28 @@ int mf_round_grid(const LaunchCfg& L) {
 
 #define LAUNCH(k, ...) \
     hipLaunchKernelGGL(k, dim3(grid_for((const void*)k, L)), dim3(256), 0, L.stream, __VA_ARGS__)
+#define LAUNCH_C(k, cls, ...) \
+    hipLaunchKernelGGL(k, dim3(grid_cls((const void*)k, L, cls)), dim3(256), 0, L.stream, __VA_ARGS__)
 #define LAUNCH_ROUND(k, ...) hipLaunchKernelGGL(k, dim3(mf_round_grid(L)), dim3(256), 0, L.stream, __VA_ARGS__)
 
 void mf_launch_batch_prologue(const BP& P, const LaunchCfg& L) {
-    LAUNCH(k_init, P);
-    LAUNCH(k_claim, P);
-    LAUNCH(k_resolve, P);
-    LAUNCH(k_append, P);
-    LAUNCH(k_partner, P);
-    LAUNCH(k_kcount, P);
-    LAUNCH(k_pu_scan, P);
-    LAUNCH(k_fa_count, P);
-    LAUNCH(k_fa_alloc, P);
-    LAUNCH(k_fa_fill, P);
+    LAUNCH_C(k_init, G_UPD, P);
+    LAUNCH_C(k_claim, G_UPD, P);
+    LAUNCH_C(k_resolve, G_UPD, P);
+    LAUNCH_C(k_append, G_UPD, P);
+    LAUNCH_C(k_partner, G_UPD, P);
+    LAUNCH_C(k_kcount, G_WAVE_UPD, P);
+    LAUNCH_C(k_pu_scan, G_SCAN, P);
+    LAUNCH_C(k_fa_count, G_WAVE_UPD, P);
+    LAUNCH_C(k_fa_alloc, G_UPD, P);
+    LAUNCH_C(k_fa_fill, G_WAVE_UPD, P);
 }
 
 void mf_launch_round(const BP& P, const LaunchCfg& L, int par) {
     LAUNCH_ROUND(k_spec, P, par);
     LAUNCH_ROUND(k_validate, P, par);
-    LAUNCH(k_commit, P, par);
+    if (L.round_grid > 0) LAUNCH_ROUND(k_commit, P, par);
+    else LAUNCH(k_commit, P, par);
 }
 
 void mf_launch_finish(const BP& P, const LaunchCfg& L, int par) {

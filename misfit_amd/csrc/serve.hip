@@ -7,19 +7,26 @@
 //   * graphs are admitted by free HBM (their device footprint must fit in
 //     the free memory minus a reserve), each in its own context and stream;
 //   * requests (update batches) are queued per graph in arrival order; a
-//     dispatcher thread keeps up to max_inflight batches of every graph
-//     queued on the device, so batches of different graphs overlap on the
-//     GPU and a request that arrives while others run starts as soon as its
-//     graph has a free slot; a batch whose buffers may not fit in free HBM
-//     is refused (MISFIT_E_NOMEM) instead of being started;
+//     pool of dispatcher workers, each owning a share of the graphs, keeps
+//     up to max_inflight batches of every graph queued on the device, so
+//     batches of different graphs overlap on the GPU and a request that
+//     arrives while others run starts as soon as its graph has a free slot;
+//     a batch whose buffers may not fit in free HBM is refused
+//     (MISFIT_E_NOMEM) instead of being started;
+//   * with max_inflight 1 every batch is queued with the rounds it is
+//     expected to need and gets any further rounds from its worker when its
+//     tail halts, so no graph's rounds run empty on the device;
 //   * a graph whose overlay passes compact_fraction of its base entries is
 //     compacted once its in-flight batches have drained (its later requests
 //     wait); every request reports how many compactions of its graph ran
 //     before it started, which places each compaction exactly;
 //   * every request reports its queueing time, its latency from submission
 //     to result and its device time.
-// One dispatcher thread owns every context call for batches; the public
-// calls take the server lock, so any number of client threads may submit.
+// Kernel launches cost the host about 4 us each and scale with threads on
+// separate streams, so several workers feed the GPU; each graph's context
+// is only ever called by its worker (or, while no worker is inside it, by a
+// public call holding the server lock).  The workers take the server lock
+// for queue changes only, never around a HIP call.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -34,6 +41,12 @@
 #include <vector>
 
 #include "../../include/misfit.h"
+
+// capi.hip: 1 while a batch of the context may still need rounds queued
+// from the host (queued without a reliable round estimate).
+int mf_ctx_settling(const misfit_ctx* c);
+// capi.hip: every batch of the context may halt (one batch per graph).
+void mf_ctx_set_halting(misfit_ctx* c, bool on);
 
 namespace {
 using clk = std::chrono::steady_clock;
@@ -58,11 +71,13 @@ struct Request {
 
 struct Graph {
     int32_t id = -1;
+    int worker = 0;
     misfit_ctx* ctx = nullptr;
     int32_t n = 0;
     int64_t base_m = 0;
     std::deque<Request*> pending, inflight;
     bool compact_pending = false;
+    bool in_call = false;   // a worker is calling the context outside the lock
     int64_t compactions = 0;
 };
 
@@ -75,6 +90,13 @@ int64_t graph_footprint(int32_t n, int64_t m) {
 // Device bytes one batch of `count` updates may add (batch buffers doubled
 // to the new capacity, overlay pool and inserted-pair hash growth).
 int64_t batch_footprint(int64_t count) { return 1024 * count; }
+
+int64_t device_free_bytes(int device) {
+    size_t fr = 0, tot = 0;
+    hipSetDevice(device);
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 0;
+    return (int64_t)fr;
+}
 }  // namespace
 
 struct misfit_server {
@@ -86,22 +108,23 @@ struct misfit_server {
     std::map<uint64_t, std::unique_ptr<Request>> requests;
     uint64_t next_req = 1;
     int32_t next_graph = 0;
+    int64_t loading_bytes = 0;  // footprints of graphs being loaded (admitted, not yet listed)
     bool stop = false;
-    std::thread dispatcher;
+    std::vector<std::thread> workers;
     std::string err;
 
     int fail(int rc, const std::string& m) {
         err = m;
         return rc;
     }
-    bool busy() const {
+    bool busy(int w) const {
         for (auto& kv : graphs)
-            if (!kv.second->pending.empty() || !kv.second->inflight.empty()) return true;
+            if (kv.second->worker == w && (!kv.second->pending.empty() || !kv.second->inflight.empty())) return true;
         return false;
     }
-    bool anything_inflight() const {
+    bool inflight(int w) const {
         for (auto& kv : graphs)
-            if (!kv.second->inflight.empty()) return true;
+            if (kv.second->worker == w && !kv.second->inflight.empty()) return true;
         return false;
     }
     void finish(Request* r, int rc, const std::string& msg) {
@@ -111,79 +134,104 @@ struct misfit_server {
         r->st.latency_ms = ms_between(r->t_submit, clk::now());
         cv_done.notify_all();
     }
-    int64_t free_bytes() {
-        size_t fr = 0, tot = 0;
-        hipSetDevice(device);
-        if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 0;
-        return (int64_t)fr;
+    Graph* graph(int32_t id) {
+        auto it = graphs.find(id);
+        return it == graphs.end() ? nullptr : it->second.get();
     }
-    void run();
+    bool service(Graph* g, std::unique_lock<std::mutex>& lk);
+    void run(int w);
 };
 
-// One pass over the graphs: collect finished batches, compact where due,
-// start queued requests.  Returns whether anything happened.  Holds mu.
-static bool dispatch_once(misfit_server* s) {
+// One pass over graph g by its worker: collect finished batches, compact
+// if due, start queued requests.  Called and returns with the lock held;
+// every context call runs without it.  Returns whether anything happened.
+bool misfit_server::service(Graph* g, std::unique_lock<std::mutex>& lk) {
     bool progress = false;
-    for (auto& kv : s->graphs) {
-        Graph* g = kv.second.get();
-        while (!g->inflight.empty()) {
-            Request* r = g->inflight.front();
-            int p = misfit_poll(g->ctx, r->ticket);
-            if (p == 0) break;
-            g->inflight.pop_front();
-            progress = true;
-            misfit_batch_stats bs;
-            int rc = p < 0 ? p : misfit_wait(g->ctx, r->ticket, &bs);
-            if (rc != MISFIT_OK) {
-                s->finish(r, rc, misfit_last_error(g->ctx));
-                continue;
-            }
-            r->st.batch = bs;
-            r->st.device_ms = bs.device_ms;
-            if (s->opts.compact_fraction > 0 && (double)bs.overlay_entries > s->opts.compact_fraction * (double)g->base_m)
-                g->compact_pending = true;
-            s->finish(r, MISFIT_OK, "");
+    g->in_call = true;
+    while (!g->inflight.empty()) {
+        Request* r = g->inflight.front();
+        lk.unlock();
+        int p = misfit_poll(g->ctx, r->ticket);
+        misfit_batch_stats bs{};
+        int rc = p < 0 ? p : MISFIT_OK;
+        std::string em;
+        if (p == 1) rc = misfit_wait(g->ctx, r->ticket, &bs);
+        if (rc != MISFIT_OK) em = misfit_last_error(g->ctx);
+        lk.lock();
+        if (p == 0) break;
+        g->inflight.pop_front();
+        progress = true;
+        if (rc != MISFIT_OK) {
+            finish(r, rc, em);
+            continue;
         }
-        if (g->compact_pending && g->inflight.empty()) {
-            int64_t new_m = 0;
-            int rc = misfit_compact(g->ctx, &new_m);
-            g->compact_pending = false;
-            if (rc == MISFIT_OK) {
-                g->base_m = new_m;
-                g->compactions++;
-            }
-            progress = true;
-        }
-        while (!g->compact_pending && (int)g->inflight.size() < s->opts.max_inflight && !g->pending.empty()) {
-            Request* r = g->pending.front();
-            g->pending.pop_front();
-            progress = true;
-            if (s->free_bytes() - batch_footprint((int64_t)r->ups.size()) < s->opts.hbm_reserve) {
-                s->finish(r, MISFIT_E_NOMEM, "batch not admitted: not enough free device memory");
-                continue;
-            }
-            int rc = misfit_submit_batch(g->ctx, r->ups.data(), (int64_t)r->ups.size(), &r->ticket);
-            if (rc != MISFIT_OK) {
-                s->finish(r, rc, misfit_last_error(g->ctx));
-                continue;
-            }
-            r->t_enqueue = clk::now();
-            r->st.compactions_before = g->compactions;
-            r->st.queue_ms = ms_between(r->t_submit, r->t_enqueue);
-            r->state = INFLIGHT;
-            g->inflight.push_back(r);
-        }
+        r->st.batch = bs;
+        r->st.device_ms = bs.device_ms;
+        if (opts.compact_fraction > 0 && (double)bs.overlay_entries > opts.compact_fraction * (double)g->base_m)
+            g->compact_pending = true;
+        finish(r, MISFIT_OK, "");
     }
+    if (g->compact_pending && g->inflight.empty()) {
+        lk.unlock();
+        int64_t new_m = 0;
+        int rc = misfit_compact(g->ctx, &new_m);
+        lk.lock();
+        g->compact_pending = false;
+        if (rc == MISFIT_OK) {
+            g->base_m = new_m;
+            g->compactions++;
+        }
+        progress = true;
+    }
+    // a graph whose batch is still settling its round count gets no
+    // further batch until it finishes (the poll above queues its rounds)
+    while (!g->compact_pending && (int)g->inflight.size() < opts.max_inflight && !g->pending.empty() &&
+           !mf_ctx_settling(g->ctx)) {
+        Request* r = g->pending.front();
+        g->pending.pop_front();
+        progress = true;
+        lk.unlock();
+        int rc = MISFIT_OK;
+        std::string em;
+        if (device_free_bytes(device) - batch_footprint((int64_t)r->ups.size()) < opts.hbm_reserve) {
+            rc = MISFIT_E_NOMEM;
+            em = "batch not admitted: not enough free device memory";
+        } else {
+            rc = misfit_submit_batch(g->ctx, r->ups.data(), (int64_t)r->ups.size(), &r->ticket);
+            if (rc != MISFIT_OK) em = misfit_last_error(g->ctx);
+        }
+        lk.lock();
+        if (rc != MISFIT_OK) {
+            finish(r, rc, em);
+            continue;
+        }
+        r->t_enqueue = clk::now();
+        r->st.compactions_before = g->compactions;
+        r->st.queue_ms = ms_between(r->t_submit, r->t_enqueue);
+        r->state = INFLIGHT;
+        g->inflight.push_back(r);
+    }
+    g->in_call = false;
+    cv_done.notify_all();  // public calls waiting for the context
     return progress;
 }
 
-void misfit_server::run() {
+void misfit_server::run(int w) {
     std::unique_lock<std::mutex> lk(mu);
+    std::vector<int32_t> mine;
     while (true) {
-        if (stop && !busy()) break;
-        if (dispatch_once(this)) continue;
-        if (!anything_inflight()) {
-            cv_work.wait(lk);  // until a submit or the stop request
+        if (stop && !busy(w)) break;
+        mine.clear();
+        for (auto& kv : graphs)
+            if (kv.second->worker == w) mine.push_back(kv.first);
+        bool progress = false;
+        for (int32_t id : mine) {
+            Graph* g = graph(id);  // looked up again: the lock was released in between
+            if (g) progress |= service(g, lk);
+        }
+        if (progress) continue;
+        if (!inflight(w)) {
+            cv_work.wait(lk);  // until a submit, a new graph or the stop request
         } else {
             // batches on the device: look again shortly (submits wake us too)
             cv_work.wait_for(lk, std::chrono::microseconds(20));
@@ -202,7 +250,10 @@ extern "C" int misfit_server_create(int device, const misfit_server_opts* opts, 
     s->opts.compact_fraction = opts ? opts->compact_fraction : 0.05;
     int mi = opts ? opts->max_inflight : 2;
     s->opts.max_inflight = mi < 1 ? 1 : (mi > 3 ? 3 : mi);  // a context stages at most 3 batches ahead
-    s->dispatcher = std::thread([s] { s->run(); });
+    int nw = opts ? opts->workers : 0;
+    if (nw <= 0) nw = 8;
+    s->opts.workers = nw > 64 ? 64 : nw;
+    for (int w = 0; w < s->opts.workers; w++) s->workers.emplace_back([s, w] { s->run(w); });
     *out = s;
     return MISFIT_OK;
 }
@@ -214,7 +265,7 @@ extern "C" int misfit_server_destroy(misfit_server* s) {
         s->stop = true;
     }
     s->cv_work.notify_all();
-    s->dispatcher.join();
+    for (auto& t : s->workers) t.join();
     for (auto& kv : s->graphs) misfit_ctx_destroy(kv.second->ctx);
     delete s;
     return MISFIT_OK;
@@ -225,25 +276,35 @@ extern "C" const char* misfit_server_last_error(const misfit_server* s) { return
 extern "C" int misfit_server_add_graph(misfit_server* s, int32_t n, int64_t m, const void* nindex, int nindex_is_64,
                                        const int32_t* nlist, const uint8_t* mis, int32_t* graph) {
     if (!s || !graph || !mis) return MISFIT_E_ARG;
-    std::lock_guard<std::mutex> lk(s->mu);
-    int64_t need = graph_footprint(n, m);
-    int64_t fr = s->free_bytes();
-    if (fr - need < s->opts.hbm_reserve)
-        return s->fail(MISFIT_E_NOMEM, "graph not admitted: needs " + std::to_string(need >> 20) + " MiB, " +
-                                           std::to_string(fr >> 20) + " MiB free, " +
-                                           std::to_string(s->opts.hbm_reserve >> 20) + " MiB reserved");
+    const int64_t need = graph_footprint(n, m);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        int64_t fr = device_free_bytes(s->device) - s->loading_bytes;
+        if (fr - need < s->opts.hbm_reserve)
+            return s->fail(MISFIT_E_NOMEM, "graph not admitted: needs " + std::to_string(need >> 20) + " MiB, " +
+                                               std::to_string(fr >> 20) + " MiB free, " +
+                                               std::to_string(s->opts.hbm_reserve >> 20) + " MiB reserved");
+        s->loading_bytes += need;
+    }
+    // the context is created and loaded without the lock: the workers go on
     auto g = std::make_unique<Graph>();
     int rc = misfit_ctx_create(s->device, &g->ctx);
-    if (rc) return s->fail(rc, "cannot create a context");
-    rc = misfit_load_csr(g->ctx, n, m, nindex, nindex_is_64, nlist);
-    if (rc == MISFIT_OK) rc = misfit_set_mis(g->ctx, mis);
-    if (rc == MISFIT_OK) rc = misfit_set_mode(g->ctx, MISFIT_MODE_QUEUED);
-    if (rc != MISFIT_OK) {
-        std::string m2 = misfit_last_error(g->ctx);
-        misfit_ctx_destroy(g->ctx);
-        return s->fail(rc, m2);
+    std::string em = "cannot create a context";
+    if (rc == MISFIT_OK) {
+        rc = misfit_load_csr(g->ctx, n, m, nindex, nindex_is_64, nlist);
+        if (rc == MISFIT_OK) rc = misfit_set_mis(g->ctx, mis);
+        if (rc == MISFIT_OK) rc = misfit_set_mode(g->ctx, MISFIT_MODE_QUEUED);
+        if (rc != MISFIT_OK) {
+            em = misfit_last_error(g->ctx);
+            misfit_ctx_destroy(g->ctx);
+        }
     }
+    if (rc == MISFIT_OK) mf_ctx_set_halting(g->ctx, s->opts.max_inflight == 1);
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->loading_bytes -= need;
+    if (rc != MISFIT_OK) return s->fail(rc, em);
     g->id = s->next_graph++;
+    g->worker = g->id % s->opts.workers;
     g->n = n;
     g->base_m = m;
     *graph = g->id;
@@ -251,12 +312,22 @@ extern "C" int misfit_server_add_graph(misfit_server* s, int32_t n, int64_t m, c
     return MISFIT_OK;
 }
 
+// Waits (holding the lock on return) until no worker is inside graph id's
+// context; null when the graph is unknown.
+static Graph* graph_idle(misfit_server* s, std::unique_lock<std::mutex>& lk, int32_t id) {
+    while (true) {
+        Graph* g = s->graph(id);
+        if (!g || !g->in_call) return g;
+        s->cv_done.wait(lk);
+    }
+}
+
 extern "C" int misfit_server_remove_graph(misfit_server* s, int32_t graph) {
     if (!s) return MISFIT_E_ARG;
-    std::lock_guard<std::mutex> lk(s->mu);
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (!graph_idle(s, lk, graph)) return s->fail(MISFIT_E_ARG, "unknown graph");
     auto it = s->graphs.find(graph);
-    if (it == s->graphs.end()) return s->fail(MISFIT_E_ARG, "unknown graph");
-    if (!it->second->pending.empty() || !it->second->inflight.empty())
+    if (!it->second->pending.empty() || !it->second->inflight.empty() || it->second->compact_pending)
         return s->fail(MISFIT_E_STATE, "graph has requests in progress");
     misfit_ctx_destroy(it->second->ctx);
     s->graphs.erase(it);
@@ -280,7 +351,7 @@ extern "C" int misfit_server_submit(misfit_server* s, int32_t graph, const misfi
     it->second->pending.push_back(r.get());
     s->requests[r->id] = std::move(r);
     lk.unlock();
-    s->cv_work.notify_one();
+    s->cv_work.notify_all();  // the owning worker may be any of them
     return MISFIT_OK;
 }
 
@@ -302,21 +373,21 @@ extern "C" int misfit_server_wait(misfit_server* s, uint64_t request, misfit_req
 // device (call after waiting for the graph's requests for a settled state).
 extern "C" int misfit_server_get_mis(misfit_server* s, int32_t graph, uint8_t* mis) {
     if (!s || !mis) return MISFIT_E_ARG;
-    std::lock_guard<std::mutex> lk(s->mu);
-    auto it = s->graphs.find(graph);
-    if (it == s->graphs.end()) return s->fail(MISFIT_E_ARG, "unknown graph");
-    int rc = misfit_get_mis(it->second->ctx, mis);
-    if (rc) s->err = misfit_last_error(it->second->ctx);
+    std::unique_lock<std::mutex> lk(s->mu);
+    Graph* g = graph_idle(s, lk, graph);
+    if (!g) return s->fail(MISFIT_E_ARG, "unknown graph");
+    int rc = misfit_get_mis(g->ctx, mis);
+    if (rc) s->err = misfit_last_error(g->ctx);
     return rc;
 }
 
 extern "C" int misfit_server_check(misfit_server* s, int32_t graph, int64_t* conflicts, int64_t* uncovered,
                                    int64_t* count_errors) {
     if (!s) return MISFIT_E_ARG;
-    std::lock_guard<std::mutex> lk(s->mu);
-    auto it = s->graphs.find(graph);
-    if (it == s->graphs.end()) return s->fail(MISFIT_E_ARG, "unknown graph");
-    int rc = misfit_check(it->second->ctx, conflicts, uncovered, count_errors);
-    if (rc) s->err = misfit_last_error(it->second->ctx);
+    std::unique_lock<std::mutex> lk(s->mu);
+    Graph* g = graph_idle(s, lk, graph);
+    if (!g) return s->fail(MISFIT_E_ARG, "unknown graph");
+    int rc = misfit_check(g->ctx, conflicts, uncovered, count_errors);
+    if (rc) s->err = misfit_last_error(g->ctx);
     return rc;
 }

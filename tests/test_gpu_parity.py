@@ -216,6 +216,54 @@ def test_capacity_growth_with_batches_in_flight(oracle, engine_mod, eng):
     assert eng.check() == (0, 0, 0)
 
 
+def _dense_conflict_stream(oracle, engine_mod, nb=4, bs=8000):
+    """A small R-MAT graph (8,192 vertices) with batches about as large as
+    its vertex count, endpoints in its lowest-degree quarter: far more
+    speculative rounds per update than a first estimate allows."""
+    ni, nl = engine_mod.gen_rmat(13, 16, 21)
+    mis0 = engine_mod.greedy_mis(ni, nl)
+    u, v = engine_mod.gen_updates(engine_mod.LOW_DEGREE, ni, nl, nb * bs, 21)
+    ins = oracle.Graph.from_csr(ni, nl).flags(u, v)
+    batches = [(u[i:i + bs], v[i:i + bs], ins[i:i + bs]) for i in range(0, nb * bs, bs)]
+    want, _, _ = H.oracle_replay(oracle, ni, nl, mis0, batches, completion=True, literal=False)
+    return ni, nl, mis0, batches, want
+
+
+@pytest.mark.parametrize("driver", ["wait", "poll", "submit"])
+def test_rounds_beyond_first_estimate(oracle, engine_mod, driver):
+    """Queued batches on a fresh context (no round history) that need many
+    more rounds than were queued: the tail halts and the rest of the rounds
+    are queued from the host, by wait, by a polling loop, or by the next
+    submit of the same graph; every MIS is the oracle's."""
+    ni, nl, mis0, batches, want = _dense_conflict_stream(oracle, engine_mod)
+    with engine_mod.Engine(0) as e:
+        e.load_csr(ni, nl)
+        e.set_mis(mis0)
+        e.set_mode(engine_mod.MODE_QUEUED)
+        ups = [engine_mod.make_updates(*b) for b in batches]
+        if driver == "submit":
+            t = [e.submit(ups[0]), e.submit(ups[1])]   # the second submit settles the first batch
+            st = [e.wait(t[0]), e.wait(t[1])]
+            assert np.array_equal(e.get_mis(), want[1])
+            rest = range(2, len(batches))
+        else:
+            t0 = e.submit(ups[0])
+            if driver == "poll":
+                polls = 0
+                while not e.poll(t0):
+                    polls += 1
+                assert polls >= 0
+            st = [e.wait(t0)]
+            assert np.array_equal(e.get_mis(), want[0])
+            rest = range(1, len(batches))
+        # the first batch was queued with ceil(8000 / 400) + 3 grid rounds
+        assert st[0]["rounds"] > 23
+        for b in rest:
+            e.wait(e.submit(ups[b]))
+            assert np.array_equal(e.get_mis(), want[b]), "batch %d" % b
+        assert e.check() == (0, 0, 0)
+
+
 @pytest.mark.parametrize("mode", MODES)
 def test_unsorted_rows_parity(oracle, engine_mod, eng, mode):
     """Rows in arbitrary order (a graph converted from a file in file order):

@@ -73,6 +73,32 @@ def test_served_graphs_match_oracle_with_compactions(oracle, engine_mod, infligh
                    for g in gids)
 
 
+@pytest.mark.parametrize("inflight,workers", [(1, 0), (1, 1), (2, 2), (3, 0)])
+def test_served_dense_conflict_graph(oracle, engine_mod, inflight, workers):
+    """A graph whose first batches need far more rounds than first queued
+    (8,192-vertex R-MAT, 8,000-update batches of low-degree endpoints),
+    served beside a larger graph: its batches wait while one settles its
+    round count, and both graphs end with the oracle's MIS."""
+    ni, nl = engine_mod.gen_rmat(13, 16, 21)
+    mis0 = engine_mod.greedy_mis(ni, nl)
+    u, v = engine_mod.gen_updates(engine_mod.LOW_DEGREE, ni, nl, 4 * 8000, 21)
+    ins = oracle.Graph.from_csr(ni, nl).flags(u, v)
+    dense = (ni, nl, mis0, [(u[i:i + 8000], v[i:i + 8000], ins[i:i + 8000]) for i in range(0, 32000, 8000)])
+    other = _workload(engine_mod, oracle, "er", 1 << 15, 44, nb=4)
+    with engine_mod.Server(0, compact_fraction=0.0, max_inflight=inflight, workers=workers) as srv:
+        gids = [srv.add_graph(w[0], w[1], w[2]) for w in (dense, other)]
+        reqs = []
+        for b in range(4):
+            for gid, w in zip(gids, (dense, other)):
+                reqs.append((gid, srv.submit(gid, engine_mod.make_updates(*w[3][b]))))
+        for gid, r in reqs:
+            assert srv.wait(r)["graph"] == gid
+        for gid, w in zip(gids, (dense, other)):
+            want, _, _ = H.oracle_replay(oracle, w[0], w[1], w[2], w[3], completion=True, literal=False)
+            assert np.array_equal(srv.get_mis(gid, len(w[0]) - 1), want[-1]), "graph %d" % gid
+            assert srv.check(gid) == (0, 0, 0)
+
+
 def test_admission_by_free_memory(oracle, engine_mod):
     ni, nl, mis0, batches = _workload(engine_mod, oracle, "er", 1 << 14, 41, nb=1)
     with engine_mod.Engine(0) as probe:

@@ -211,6 +211,61 @@ def test_egr_tool_binary(engine_mod, tmp_path):
     assert r.returncode == 0 and r.stdout.strip().endswith("OK")
 
 
+def test_egr_tool_mis_and_batches(engine_mod, tmp_path):
+    """misfit_egr mis gives the sample's own initial MIS file
+    (tools/compute_initial_mis.py made sample_mis.txt from sample.egr);
+    misfit_egr batches writes <k>.mtx files of distinct existing edges,
+    1-indexed with u < v, which the drop-in binaries' batch parser reads."""
+    exe = os.path.join(ROOT, "misfit_amd", "bin", "misfit_egr")
+    out = tmp_path / "mis.txt"
+    r = subprocess.run([exe, "mis", os.path.join(SAMPLE, "sample.egr"), str(out)], capture_output=True, text=True)
+    assert r.returncode == 0 and "greedy MIS: 533 members" in r.stdout
+    want = sorted(int(x) for x in open(os.path.join(SAMPLE, "sample_mis.txt")).read().split())
+    assert [int(x) for x in out.read_text().split()] == want
+    ni, nl, _ = engine_mod.read_egr(os.path.join(SAMPLE, "sample.egr"))
+    adj = {(int(a), int(b)) for a in range(len(ni) - 1) for b in nl[ni[a]:ni[a + 1]]}
+    for sampler in ("uniform", "low-degree", "preferential"):
+        d = tmp_path / sampler
+        r = subprocess.run([exe, "batches", os.path.join(SAMPLE, "sample.egr"), str(d), "100", "3", sampler, "5"],
+                           capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+        seen = set()
+        for k in (1, 2, 3):
+            lines = (d / ("%d.mtx" % k)).read_text().split("\n")[:-1]
+            assert len(lines) == 100
+            for ln in lines:
+                a, b = (int(x) for x in ln.split())
+                assert 1 <= a < b <= 2000 and (a - 1, b - 1) in adj and (a, b) not in seen
+                seen.add((a, b))
+    r = subprocess.run([exe, "batches", os.path.join(SAMPLE, "sample.egr"), str(tmp_path / "x"), "9000", "1"],
+                       capture_output=True, text=True)
+    assert r.returncode == 1 and "distinct edges" in r.stderr       # 8,000 undirected edges only
+    r = subprocess.run([exe, "batches", os.path.join(SAMPLE, "sample.egr"), str(tmp_path / "x"), "10", "1", "bogus"],
+                       capture_output=True, text=True)
+    assert r.returncode == 1 and "Usage" in r.stderr
+
+
+def test_reference_samplers_bias(engine_mod):
+    """The three existing-edge samplers: distinct (lo, hi) existing edges;
+    drawing the vertex by 1/degree lowers the mean degree of the drawn
+    vertex's side well below the entry-uniform and degree-weighted
+    samplers, which draw the same distribution (a uniform CSR entry)."""
+    ni, nl = engine_mod.gen_rmat(12, 8, 3)
+    deg = np.diff(ni)
+    adj = set(zip(np.repeat(np.arange(len(deg)), deg).tolist(), nl.tolist()))
+    means = {}
+    for kind in (engine_mod.EDGE_UNIFORM, engine_mod.EDGE_LOW_DEGREE, engine_mod.EDGE_PREFERENTIAL):
+        u, v = engine_mod.gen_updates(kind, ni, nl, 3000, 9)
+        assert np.all(u < v)
+        assert len(set(zip(u.tolist(), v.tolist()))) == 3000
+        assert all((a, b) in adj for a, b in zip(u.tolist(), v.tolist()))
+        means[kind] = float(np.minimum(deg[u], deg[v]).mean())
+    assert means[engine_mod.EDGE_LOW_DEGREE] < 0.5 * means[engine_mod.EDGE_UNIFORM]
+    assert 0.8 < means[engine_mod.EDGE_PREFERENTIAL] / means[engine_mod.EDGE_UNIFORM] < 1.25
+    with pytest.raises(engine_mod.MisfitError):
+        engine_mod.gen_updates(engine_mod.EDGE_UNIFORM, ni, nl, len(nl) + 1, 1)
+
+
 def test_run_script_arguments():
     """misfit_amd/run.sh keeps the reference's flags (run.sh:17-127)."""
     sh = os.path.join(ROOT, "misfit_amd", "run.sh")
