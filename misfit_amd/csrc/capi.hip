@@ -621,10 +621,10 @@ static void launch_queued(const BP& P, const LaunchCfg& L, int k) {
 }
 
 // The rounds of a batch of `count` updates.  Their number: one round plus
-// the most further rounds per update seen on recent batches at least a
-// quarter as large (conflicts per update grow with the batch, so larger
-// batches bound smaller ones), with a margin, and at least one round per
-// window; rounds past the end of the batch cost one empty launch each.
+// a high rate of further rounds per update seen on recent batches at least
+// a quarter as large (conflicts per update grow with the batch, so larger
+// batches bound smaller ones), plus two, and at least one round per window;
+// rounds past the end of the batch cost one empty launch each.
 // Their window (edges speculated per round, one per wave of the round
 // grid): the whole GPU, or 16 times the edges a round is expected to commit
 // when that is less (at least 256), so a batch whose rounds commit few
@@ -639,13 +639,20 @@ struct RoundPlan {
 
 static RoundPlan plan_rounds(misfit_ctx* c, int64_t count, int64_t full_window) {
     RoundPlan p{0, full_window, true};
-    double rpu = 0.0;
-    bool any = false;
+    double rpu = 0.0, rpu2 = 0.0;  // largest and second largest eligible rate
+    int eligible = 0;
     for (int i = 0; i < 16; i++)
         if (c->hist_count[i] > 0 && 4 * c->hist_count[i] >= count) {
-            rpu = std::max(rpu, c->hist_rpu[i]);
-            any = true;
+            double r = c->hist_rpu[i];
+            if (r > rpu) {
+                rpu2 = rpu;
+                rpu = r;
+            } else if (r > rpu2) {
+                rpu2 = r;
+            }
+            eligible++;
         }
+    const bool any = eligible > 0;
     if (c->always_halt) {
         // rounds as expected, not as many as may be needed: a batch that
         // needs more halts and gets them from the host, so no graph's
@@ -680,7 +687,10 @@ static RoundPlan plan_rounds(misfit_ctx* c, int64_t count, int64_t full_window) 
         p.sure = true;
         return p;
     }
-    double k = std::ceil((double)count * rpu * 1.15) + 3.0;
+    // with enough history the second largest rate plus two rounds (about
+    // the 90th percentile of 16), else the largest with a margin: a round
+    // left to the one-workgroup tail costs dozens of empty grid rounds
+    double k = eligible >= 6 ? std::ceil((double)count * rpu2) + 3.0 : std::ceil((double)count * rpu * 1.15) + 3.0;
     double per_window = std::ceil((double)count / (double)std::max<int64_t>(p.window, 1)) + 2.0;
     k = std::max(k, per_window);
     p.k = (int)std::min(k, 65536.0);
