@@ -62,6 +62,7 @@ typedef struct {
     int64_t delta_ops;        /* effective overlay operations */
     int64_t overlay_entries;  /* pool slots in use after the batch */
     double device_ms;         /* batch time on the device (events) */
+    double prologue_ms;       /* ... of which the overlay update, may-uncover set and admissible index */
 } misfit_batch_stats;
 
 /* Execution modes of the repair (results are identical). */

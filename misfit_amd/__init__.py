@@ -49,7 +49,7 @@ class BatchStats(ctypes.Structure):
     _fields_ = [(k, ctypes.c_int64) for k in (
         "updates", "seeds", "active_ins", "evictions", "admit_ins", "active_del", "admit_del",
         "completion_adds", "rounds", "evaluations", "maybe_uncovered", "admissible", "delta_ops",
-        "overlay_entries")] + [("device_ms", ctypes.c_double)]
+        "overlay_entries")] + [("device_ms", ctypes.c_double), ("prologue_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -35,6 +35,7 @@ struct Slot {
     uint8_t* d_ins = nullptr;
     u64* h_ctr = nullptr;
     hipEvent_t uploaded = nullptr, done = nullptr, t0 = nullptr, t1 = nullptr;
+    hipEvent_t tp = nullptr;  // end of the batch prologue
     int64_t count = 0;
     uint64_t ticket = 0;
     bool busy = false;
@@ -59,6 +60,7 @@ struct misfit_ctx {
     int64_t hist_count[16] = {0}; // ... and their sizes
     int hist_pos = 0;
     bool always_halt = false;     // every batch may halt (nothing is ever queued behind one)
+    double window_factor = 16.0;  // MISFIT_WINDOW_FACTOR: window = this x expected edges per round
     int round_per_cu = 0;      // MISFIT_ROUND_WG_PER_CU: round grid (0: by occupancy)
     int rounds_pos = 0;
     bool debug_rounds = false;
@@ -149,6 +151,8 @@ extern "C" int misfit_ctx_create(int device, misfit_ctx** out) {
     c->debug_rounds = dbg && dbg[0] == '1';
     const char* rw = getenv("MISFIT_ROUND_WG_PER_CU");
     if (rw) c->round_per_cu = std::max(0, std::min(8, atoi(rw)));
+    const char* wf = getenv("MISFIT_WINDOW_FACTOR");
+    if (wf && atof(wf) >= 1.0) c->window_factor = atof(wf);
     const char* gr = getenv("MISFIT_GRID_ROUNDS");
     if (gr) {
         c->grid_rounds = std::max(0, atoi(gr));
@@ -167,6 +171,7 @@ extern "C" int misfit_ctx_create(int device, misfit_ctx** out) {
         hipEventCreateWithFlags(&s.done, hipEventDisableTiming);
         hipEventCreate(&s.t0);
         hipEventCreate(&s.t1);
+        hipEventCreate(&s.tp);
         // written by the batch's last kernel, read after its event
         hipHostMalloc((void**)&s.h_ctr, (C_NCOUNTERS + 1) * sizeof(u64), hipHostMallocCoherent);
     }
@@ -199,6 +204,7 @@ extern "C" int misfit_ctx_destroy(misfit_ctx* c) {
         hipEventDestroy(s.done);
         hipEventDestroy(s.t0);
         hipEventDestroy(s.t1);
+        hipEventDestroy(s.tp);
     }
     free_list(c->batch_allocs);
     free_list(c->graph_allocs);
@@ -571,13 +577,15 @@ static void fill_stats(const misfit_ctx* c, const Slot& s, misfit_batch_stats* s
     st->overlay_entries = (int64_t)k[C_NCOUNTERS];
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, s.t0, s.t1) == hipSuccess) st->device_ms = ms;
+    if (hipEventElapsedTime(&ms, s.t0, s.tp) == hipSuccess) st->prologue_ms = ms;
     (void)c;
 }
 
 // Host-driven rounds: one round = 4 kernels, then the host reads where the
 // committed prefix ends; the completion iterations likewise.
-static int drive_plain(misfit_ctx* c, BP& P, const LaunchCfg& L) {
+static int drive_plain(misfit_ctx* c, BP& P, const LaunchCfg& L, hipEvent_t tp) {
     mf_launch_batch_prologue(P, L);
+    HIPCHK(c, hipEventRecord(tp, L.stream));
     for (uint32_t r = 0;; r++) {
         if ((int64_t)r > P.U + 1) return fail(c, MISFIT_E_CHECK, "speculative rounds made no progress");
         int par = (int)(r & 1u);
@@ -614,8 +622,9 @@ static int drive_plain(misfit_ctx* c, BP& P, const LaunchCfg& L) {
 
 // The queued sequence of one batch: prologue, grid-wide rounds, then the
 // one-workgroup tail (leftover rounds, counters, completion, publication).
-static void launch_queued(const BP& P, const LaunchCfg& L, int k) {
+static void launch_queued(const BP& P, const LaunchCfg& L, int k, hipEvent_t tp) {
     mf_launch_batch_prologue(P, L);
+    hipEventRecord(tp, L.stream);
     for (int r = 0; r < k; r++) mf_launch_round(P, L, r & 1);
     mf_launch_finish(P, L, k & 1);
 }
@@ -665,7 +674,7 @@ static RoundPlan plan_rounds(misfit_ctx* c, int64_t count, int64_t full_window) 
         }
         if (recent >= 0) {
             double per_round = (double)count / (1.0 + (double)count * recent);
-            int64_t w = (int64_t)std::ceil(16.0 * per_round);
+            int64_t w = (int64_t)std::ceil(c->window_factor * per_round);
             p.window = std::min(std::max<int64_t>(256, (w + 31) / 32 * 32), full_window);
         }
         double k = std::ceil((double)count * std::max(recent, 0.0)) + 1.0;
@@ -678,7 +687,7 @@ static RoundPlan plan_rounds(misfit_ctx* c, int64_t count, int64_t full_window) 
         p.sure = count == 0;
     } else {
         double per_round = (double)count / (1.0 + (double)count * rpu);
-        int64_t w = (int64_t)std::ceil(16.0 * per_round);
+        int64_t w = (int64_t)std::ceil(c->window_factor * per_round);
         w = std::max<int64_t>(256, (w + 31) / 32 * 32);
         p.window = std::min(w, full_window);
     }
@@ -698,8 +707,8 @@ static RoundPlan plan_rounds(misfit_ctx* c, int64_t count, int64_t full_window) 
 }
 
 // Whole batch enqueued with no host synchronisation inside it.
-static int drive_queued(misfit_ctx* c, BP& P, const LaunchCfg& L, int k) {
-    launch_queued(P, L, k);
+static int drive_queued(misfit_ctx* c, BP& P, const LaunchCfg& L, int k, hipEvent_t tp) {
+    launch_queued(P, L, k, tp);
     HIPCHK(c, hipGetLastError());
     return MISFIT_OK;
 }
@@ -722,7 +731,7 @@ static int continue_batch(misfit_ctx* c, Slot& s) {
         LaunchCfg full = s.L;
         full.round_grid = 0;
         const int64_t full_window = std::max<int64_t>((int64_t)mf_round_grid(full) * wpg, wpg);
-        int64_t w = (int64_t)std::ceil(16.0 * (double)start / (double)h[C_ROUNDS]);
+        int64_t w = (int64_t)std::ceil(c->window_factor * (double)start / (double)h[C_ROUNDS]);
         w = std::min(std::max<int64_t>(256, (w + 31) / 32 * 32), full_window);
         s.L.round_grid = (int)std::max<int64_t>(w / wpg, 1);
         s.P.window = (int64_t)s.L.round_grid * wpg;
@@ -880,7 +889,7 @@ extern "C" int misfit_submit_batch(misfit_ctx* c, const misfit_update* ups, int6
     s.may_halt = !sure;
     c->last_U = count;
     HIPCHK(c, hipEventRecord(s.t0, c->stream));
-    rc = queued ? drive_queued(c, P, L, k) : drive_plain(c, P, L);
+    rc = queued ? drive_queued(c, P, L, k, s.tp) : drive_plain(c, P, L, s.tp);
     if (rc) return rc;
     // the counters and the pool top reached s.h_ctr from the batch's last
     // kernel (the finisher, or the publish kernel of the plain driver)

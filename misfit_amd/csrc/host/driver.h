@@ -239,29 +239,55 @@ inline double percentile(std::vector<double> v, double q) {
     return v[lo] + (v[hi] - v[lo]) * (r - (double)lo);
 }
 
-inline void print_batch_detail(int i, int nb, const Batch& b, const misfit_batch_stats& st, double total_ms,
-                               bool checked, int64_t conf, int64_t unc) {
-    std::cout << "=== BATCH TIMING BREAKDOWN ===" << std::endl;
+// The reference's per-batch breakdown, line for line (misfit.cu:853-870,
+// misfit_async.cu:853-870), with this engine's figures: 1 = the host's
+// staging and launching of the batch; 3 = the prologue on the device
+// (overlay update, may-uncover set, admissible index); 4 = the exact rule's
+// speculative rounds, the tail and the completion; 7 = the rest of the
+// host's time for the batch; 8 = speculative rounds.  Stages this engine
+// does not have (the upload is overlapped on the copy stream, results come
+// back through pinned memory, the overlay lives on the device, nothing is
+// clustered) print 0.000.  This engine's own detail follows on lines of
+// its own after "Finished MIS".
+struct BatchTimes {
+    double total_ms = 0;   // from the start of its submission to the end of its collection
+    double host_ms = 0;    // the submission call (staging and launches)
+};
+
+inline void print_batch_detail(int i, const misfit_batch_stats& st, const BatchTimes& t, bool async, bool checked,
+                               int64_t conf, int64_t unc) {
+    const double z = 0.0;
+    const double gpu_rounds = std::max(0.0, st.device_ms - st.prologue_ms);
+    const double overhead = std::max(0.0, t.total_ms - t.host_ms - st.device_ms);
+    std::cout << (async ? "=== BATCH TIMING BREAKDOWN (algorithm-specific detail) ===" : "=== BATCH TIMING BREAKDOWN ===")
+              << std::endl;
     std::cout << std::fixed << std::setprecision(3);
-    std::cout << "Total Batch Time: " << total_ms << " ms\n\n";
+    std::cout << "Total Batch Time: " << t.total_ms << " ms\n\n";
     std::cout << "Component Breakdown:" << std::endl;
-    std::cout << "  1. Device repair (overlay, exact rule, completion): " << st.device_ms << " ms (GPU)" << std::endl;
-    std::cout << "  2. Speculative rounds:     " << st.rounds << " (" << st.evaluations << " edge evaluations)"
-              << std::endl;
-    std::cout << "  3. Evictions / admissions: " << st.evictions << " / " << (st.admit_ins + st.admit_del)
-              << " (+" << st.completion_adds << " by completion)" << std::endl;
-    std::cout << "  4. May-uncover set:        " << st.maybe_uncovered << " (" << st.admissible << " admissible)"
-              << std::endl;
-    std::cout << "  5. Overlay operations:     " << st.delta_ops << std::endl;
-    if (checked)
-        std::cout << "  6. Check: conflicts=" << conf << " uncovered=" << unc << std::endl;
+    std::cout << "  1. CPU Conversion:        " << t.host_ms << " ms (CPU)" << std::endl;
+    std::cout << "  2. Host->Device Transfer: " << z << " ms (GPU)" << std::endl;
+    std::cout << "  3. Cluster Processing:    " << st.prologue_ms << " ms (GPU)" << std::endl;
+    std::cout << "       - Extract vertices: " << z << " ms" << std::endl;
+    std::cout << "       - Compute neighb.:  " << z << " ms" << std::endl;
+    std::cout << "       - Vertex cluster.:  " << z << " ms" << std::endl;
+    std::cout << "       - Assign edges:     " << z << " ms" << std::endl;
+    std::cout << "       - Clear assigns.:   " << z << " ms" << std::endl;
+    std::cout << "  4. GPU Process:           " << gpu_rounds << " ms (GPU)" << std::endl;
+    std::cout << "  5. Device->Host Transfer: " << z << " ms (GPU)" << std::endl;
+    std::cout << "  6. CPU Graph Update:      " << z << " ms (CPU)" << std::endl;
+    std::cout << "  7. CPU Overhead:          " << overhead << " ms (CPU)" << std::endl;
+    std::cout << "  8. Cluster count:         " << st.rounds << std::endl;
     std::cout << "Finished MIS for batch " << i << " -> slot " << (i % 2) << std::endl;
-    (void)nb;
-    (void)b;
+    std::cout << "  Engine detail: device " << st.device_ms << " ms, rounds " << st.rounds << " ("
+              << st.evaluations << " edge evaluations), evictions " << st.evictions << ", admissions "
+              << (st.admit_ins + st.admit_del) << " (+" << st.completion_adds << " by completion), may-uncover "
+              << st.maybe_uncovered << " (" << st.admissible << " admissible), overlay operations " << st.delta_ops
+              << std::endl;
+    if (checked) std::cout << "  Check: conflicts=" << conf << " uncovered=" << unc << std::endl;
 }
 
 inline void finish(Setup& S, const Args& a, const std::vector<double>& lat_ms, double time_cum, double setup_ms,
-                   bool print_summary) {
+                   bool print_summary, double sum_prologue_ms, double sum_rounds_ms, const std::string& note) {
     misfit_sync(S.ctx);
     auto d0 = clk::now();
     std::vector<uint8_t> mis((size_t)S.n);
@@ -320,12 +346,11 @@ inline void finish(Setup& S, const Args& a, const std::vector<double>& lat_ms, d
     std::cout << "  Final D2H Transfer Time: " << d2h_ms << " milliseconds" << std::endl;
     std::cout << std::setprecision(1);
     std::cout << "  Processing rate:         " << (time_cum > 0 ? denom / (time_cum / 1000.0) : 0.0)
-              << " batches/second" << std::endl;
-    std::cout << "  Edge updates/second:     " << (time_cum > 0 ? (double)updates / (time_cum / 1000.0) : 0.0)
-              << std::endl;
-    std::cout << std::setprecision(3);
-    std::cout << "  Batch latency p50/p99:   " << percentile(lat_ms, 0.50) << " / " << percentile(lat_ms, 0.99)
-              << " ms\n" << std::endl;
+              << " batches/second\n" << std::endl;
+    std::cout << "Average Component Times:" << std::endl;
+    std::cout << std::fixed << std::setprecision(3);
+    std::cout << "  Average Cluster Processing: " << (sum_prologue_ms / denom) << " ms" << std::endl;
+    std::cout << "  Average TVB Time:           " << (sum_rounds_ms / denom) << " ms" << std::endl;
     std::cout << "\nEnd-of-cycle compaction time (NOT counted in per-batch numbers above): " << compact_ms << " ms"
               << std::endl;
     sample_memory(S);
@@ -344,6 +369,16 @@ inline void finish(Setup& S, const Args& a, const std::vector<double>& lat_ms, d
         std::cout << "Initial MIS size: " << S.initial_card << std::endl;
         std::cout << "Final MIS size:   " << card << std::endl;
     }
+    // this engine's additions, after everything the reference prints
+    std::cout << "\nEngine summary:" << std::endl;
+    std::cout << std::setprecision(1);
+    std::cout << "  Edge updates/second:     " << (time_cum > 0 ? (double)updates / (time_cum / 1000.0) : 0.0)
+              << std::endl;
+    std::cout << std::setprecision(3);
+    std::cout << "  Batch latency p50/p99:   " << percentile(lat_ms, 0.50) << " / " << percentile(lat_ms, 0.99)
+              << " ms" << std::endl;
+    std::cout << "  GPU: " << S.device_name << std::endl;
+    if (!note.empty()) std::cout << note << std::endl;
     misfit_ctx_destroy(S.ctx);
 }
 

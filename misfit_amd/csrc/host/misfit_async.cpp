@@ -3,6 +3,8 @@
 // on the device before batch i is collected, so host work on one batch
 // overlaps device work on the other.  Per-batch time = from the start of its
 // submission to the end of its collection, as in the reference (:718-747).
+// Its standard output keeps the reference's lines (:756-784,836-870,
+// 933-968, core/reporting.h); this engine's additions follow them.
 #include "driver.h"
 
 int main(int argc, char** argv) {
@@ -15,18 +17,23 @@ int main(int argc, char** argv) {
     std::cout << "=== MISFIT mis ===" << std::endl;
     std::cout << "Graph: " << a.egr << std::endl;
     std::cout << "  nodes=" << S.n << " edges=" << S.m << " avg_degree=" << std::fixed << std::setprecision(2)
-              << (S.n ? (double)S.m / S.n : 0.0) << std::endl;
+              << (S.n ? (double)S.m / S.n : 1.0) << std::endl;
     std::cout << "GPU: " << S.device_name << " | discrete (explicit-transfer path)" << std::endl;
+    // no CSR rows are staged for the device, so no staging capacity is set
+    std::cout << "MAX_CSR_ELEMENTS set to: 0" << std::endl;
     double setup_ms = ms_since(t_setup);
     int nb = std::min<int>(a.num_batches, (int)S.batches.size());
     std::vector<double> lat;
     std::vector<uint64_t> tickets((size_t)nb);
     std::vector<clk::time_point> starts((size_t)nb);
+    std::vector<double> host_ms((size_t)nb, 0.0);
+    double sum_pro = 0.0, sum_rounds = 0.0;
     auto wall0 = clk::now();
     auto submit = [&](int i) {
         starts[(size_t)i] = clk::now();
         const Batch& b = S.batches[(size_t)i];
         check(S.ctx, misfit_submit_batch(S.ctx, b.ups.data(), (int64_t)b.ups.size(), &tickets[(size_t)i]), "submit");
+        host_ms[(size_t)i] = ms_since(starts[(size_t)i]);
     };
     if (nb > 0) submit(0);
     for (int i = 0; i < nb; i++) {
@@ -36,23 +43,28 @@ int main(int argc, char** argv) {
         if (i + 1 < nb) submit(i + 1);
         misfit_batch_stats st;
         check(S.ctx, misfit_wait(S.ctx, tickets[(size_t)i], &st), "collect");
-        double ms = ms_since(starts[(size_t)i]);
-        lat.push_back(ms);
+        BatchTimes t;
+        t.total_ms = ms_since(starts[(size_t)i]);
+        t.host_ms = host_ms[(size_t)i];
+        lat.push_back(t.total_ms);
+        sum_pro += st.prologue_ms;
+        sum_rounds += std::max(0.0, st.device_ms - st.prologue_ms);
         std::cout << "Batch " << (i + 1) << "/" << nb << " | edges=" << S.batches[(size_t)i].ups.size()
-                  << " | time=" << std::fixed << std::setprecision(3) << ms << "ms | rounds=" << st.rounds
+                  << " | time=" << std::fixed << std::setprecision(3) << t.total_ms << "ms | clusters=" << st.rounds
                   << std::endl;
         std::cout << "  Current MIS cardinality: [Deferred for async execution]" << std::endl;
-        print_batch_detail(i, nb, S.batches[(size_t)i], st, ms, false, 0, 0);
+        print_batch_detail(i, st, t, true, false, 0, 0);
         sample_memory(S);
     }
     double time_cum = ms_since(wall0);
+    std::string note;
     if (S.check_each) {
         // the pipeline keeps no idle point between batches; check the end state
         int64_t conf = 0, unc = 0, bad = 0;
         check(S.ctx, misfit_check(S.ctx, &conf, &unc, &bad), "check");
-        std::cout << "\nFinal check: conflicts=" << conf << " uncovered=" << unc << std::endl;
         if (conf || bad) die("check failed");
+        note = "  Final check: conflicts=" + std::to_string(conf) + " uncovered=" + std::to_string(unc);
     }
-    finish(S, a, lat, time_cum, setup_ms, true);
+    finish(S, a, lat, time_cum, setup_ms, true, sum_pro, sum_rounds, note);
     return 0;
 }

@@ -95,3 +95,83 @@ def test_run_script_sample(tmp_path):
     ids = [int(x) for x in (tmp_path / "datasets" / "Final_MISs" / "sample_flash_final_mis.txt").read_text().split()]
     assert ids == golden["final_ids"]
     assert open(os.path.join(SAMPLE, "sample.egr"), "rb").read() == before
+
+
+# The reference's report lines (src/misfit/misfit.cu:744-794,836-871,
+# 931-960; src/misfit_async/misfit_async.cu:756-784,836-870,933-968;
+# core/reporting.h:19-47), as patterns: the figures differ, the labels and
+# their formatting do not.
+_NUM = r"-?\d+\.\d{3}"
+_SYNC_HEAD = [r"=+START=+", r"=+ MISFIT \(SYNC\) =+",
+              r"Graph average degree: [\d.e+]+, MAX_CSR_ELEMENTS set to: \d+", r"Initial MIS cardinality: \d+"]
+_ASYNC_HEAD = [r"=+START=+", r"=+ MISFIT_Async \(PIPELINED\) =+", r"=== MISFIT mis ===", r"Graph: \S+",
+               r"  nodes=\d+ edges=\d+ avg_degree=\d+\.\d{2}", r"GPU: .+ \| discrete \(explicit-transfer path\)",
+               r"MAX_CSR_ELEMENTS set to: \d+"]
+
+
+def _batch_block(async_):
+    head = [r"", r"=" * 60, r"Processing batch \d+/\d+ \| Edges: \d+"]
+    if async_:
+        head += [r"Batch \d+/\d+ \| edges=\d+ \| time=%sms \| clusters=\d+" % _NUM,
+                 r"  Current MIS cardinality: \[Deferred for async execution\]",
+                 r"=== BATCH TIMING BREAKDOWN \(algorithm-specific detail\) ==="]
+    else:
+        head += [r"  Current MIS cardinality: \[not computed per-batch\]", r"=== BATCH TIMING BREAKDOWN ==="]
+    return head + [
+        r"Total Batch Time: %s ms" % _NUM, r"", r"Component Breakdown:",
+        r"  1\. CPU Conversion:        %s ms \(CPU\)" % _NUM,
+        r"  2\. Host->Device Transfer: %s ms \(GPU\)" % _NUM,
+        r"  3\. Cluster Processing:    %s ms \(GPU\)" % _NUM,
+        r"       - Extract vertices: %s ms" % _NUM,
+        r"       - Compute neighb\.:  %s ms" % _NUM,
+        r"       - Vertex cluster\.:  %s ms" % _NUM,
+        r"       - Assign edges:     %s ms" % _NUM,
+        r"       - Clear assigns\.:   %s ms" % _NUM,
+        r"  4\. GPU Process:           %s ms \(GPU\)" % _NUM,
+        r"  5\. Device->Host Transfer: %s ms \(GPU\)" % _NUM,
+        r"  6\. CPU Graph Update:      %s ms \(CPU\)" % _NUM,
+        r"  7\. CPU Overhead:          %s ms \(CPU\)" % _NUM,
+        r"  8\. Cluster count:         \d+",
+        r"Finished MIS for batch \d+ -> slot [01]"]
+
+
+_FINAL = [r"Saved final MIS to datasets/Final_MISs/sample_flash_final_mis\.txt", r"", r"=" * 80,
+          r"=== FINAL RESULTS ===", r"=" * 80, r"", r"Graph Statistics:", r"  Initial MIS cardinality: \d+",
+          r"  Final MIS cardinality:   \d+", r"  Cardinality change:      \d+", r"  Number of batches:       \d+",
+          r"", r"Overall Timing Results:", r"  Total processing time:   %s milliseconds" % _NUM,
+          r"  Average time per batch:  %s milliseconds" % _NUM, r"  Final D2H Transfer Time: %s milliseconds" % _NUM,
+          r"  Processing rate:         \d+\.\d batches/second", r"", r"Average Component Times:",
+          r"  Average Cluster Processing: %s ms" % _NUM, r"  Average TVB Time:           %s ms" % _NUM, r"",
+          r"End-of-cycle compaction time \(NOT counted in per-batch numbers above\): %s ms" % _NUM,
+          r"Peak program GPU memory: \d+\.\d{2} MB \(\d+\.\d% of \d+\.\d{2} MB total\)", r"=" * 80]
+_SUMMARY = [r"", r"=== FINAL SUMMARY ===", r"Setup time:      %s ms" % _NUM, r"Processing time: %s ms" % _NUM,
+            r"End-to-end time: %s ms" % _NUM, r"Initial MIS size: \d+", r"Final MIS size:   \d+"]
+
+
+def _match_run(lines, i, pats):
+    import re
+    for k, p in enumerate(pats):
+        assert i + k < len(lines) and re.fullmatch(p, lines[i + k]), \
+            "line %d: %r does not match %r" % (i + k, lines[i + k] if i + k < len(lines) else None, p)
+    return i + len(pats)
+
+
+@pytest.mark.parametrize("binary", ["misfit", "misfit_async"])
+def test_report_lines_match_reference(tmp_path, binary):
+    """Scripts that read the reference's standard output read ours: the
+    header, every batch block and the final block keep the reference's lines
+    in the reference's order (this engine's own lines follow a batch's
+    "Finished MIS" line and the final block)."""
+    async_ = binary == "misfit_async"
+    lines = _run(binary, _stage(tmp_path)).split("\n")
+    i = _match_run(lines, 0, _ASYNC_HEAD if async_ else _SYNC_HEAD)
+    block = _batch_block(async_)
+    for b in range(5):
+        i = _match_run(lines, i, block)
+        assert ("Finished MIS for batch %d -> slot %d" % (b, b % 2)) == lines[i - 1]
+        while i < len(lines) and lines[i].startswith("  "):   # this engine's detail lines
+            i += 1
+    i = _match_run(lines, i, _FINAL)
+    if async_:
+        i = _match_run(lines, i, _SUMMARY)
+    assert lines[i] == "" and lines[i + 1] == "Engine summary:"
