@@ -16,6 +16,58 @@
 
 #define MF_SPLIT 4096
 
+// Entries per wave task of the entry-balanced passes (entry_walk).
+#define MF_ECHUNK 8192
+
+// Visits every entry p of rows [r0, r1) once, balanced by entries: the
+// entries are cut into tasks of MF_ECHUNK, one wave per task; the wave finds
+// the row of its first entry by binary search, then walks windows of 64
+// consecutive rows (a lane holds one row's start and rv(row), a per-row
+// value) and finds each entry's row among them by a 6-step shuffle search.
+// Hub rows are spread over many waves, short rows cost no search each.
+// visit(p, row, row start, rv(row)) runs on the lanes that hold an entry;
+// it must not use cross-lane operations.
+template <class RowVal, class Visit>
+__device__ __forceinline__ void entry_walk(const int64_t* nindex, int32_t r0, int32_t r1, RowVal rv, Visit visit) {
+    GC c = grid_coords();
+    const int64_t pb = nindex[r0], pe = nindex[r1];
+    const int64_t ntask = (pe - pb + MF_ECHUNK - 1) / MF_ECHUNK;
+    for (int64_t t = c.wave; t < ntask; t += c.nwaves) {
+        int64_t p = pb + t * MF_ECHUNK;
+        const int64_t tend = p + MF_ECHUNK < pe ? p + MF_ECHUNK : pe;
+        int64_t lo = r0, hi = r1 - 1;  // largest row x with nindex[x] <= p
+        while (lo < hi) {
+            int64_t mid = (lo + hi + 1) >> 1;
+            if (nindex[mid] <= p) lo = mid; else hi = mid - 1;
+        }
+        int64_t x = lo;
+        while (p < tend && x < r1) {  // x grows every window: the walk ends
+            const int64_t myv = x + c.lane;
+            const int64_t off = myv < r1 ? nindex[myv] : INT64_MAX;
+            const int64_t val = myv < r1 ? rv(myv) : 0;
+            const int64_t xw = x + MF_WAVE < r1 ? x + MF_WAVE : r1;
+            const int64_t wend = nindex[xw];
+            const int64_t lim = wend < tend ? wend : tend;
+            for (int64_t p0 = p; p0 < lim; p0 += MF_WAVE) {
+                const int64_t pp = p0 + c.lane;
+                const int64_t q = pp < lim ? pp : lim - 1;
+                int l = 0, h = MF_WAVE - 1;
+#pragma unroll
+                for (int s = 0; s < 6; s++) {
+                    int mid = (l + h + 1) >> 1;
+                    int64_t om = __shfl(off, mid, MF_WAVE);
+                    if (om <= q) l = mid; else h = mid - 1;
+                }
+                const int64_t ostart = __shfl(off, l, MF_WAVE);
+                const int64_t oval = __shfl(val, l, MF_WAVE);
+                if (pp < lim) visit(pp, x + l, ostart, oval);
+            }
+            if (lim > p) p = lim;
+            x = xw;
+        }
+    }
+}
+
 // ---------------------------------------------------------------- basics
 __global__ void k_widen32(const int32_t* in, int64_t* out, int64_t count) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
@@ -40,19 +92,18 @@ __global__ void k_validate_offsets(const int64_t* nindex, int32_t n, int64_t m, 
     if (blockIdx.x == 0 && threadIdx.x == 0 && (nindex[0] != 0 || nindex[n] != m)) atomicOr(flags, 8);
 }
 
-__global__ void k_validate_entries(const int64_t* nindex, const int32_t* nlist, int32_t n, int64_t m, int* flags) {
+__global__ __launch_bounds__(256) void k_validate_entries(const int64_t* nindex, const int32_t* nlist, int32_t n,
+                                                          int64_t m, int* flags) {
     int f = 0;
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < m; p += (int64_t)gridDim.x * blockDim.x) {
-        int64_t lo = 0, hi = n - 1;  // largest x with nindex[x] <= p
-        while (lo < hi) {
-            int64_t mid = (lo + hi + 1) >> 1;
-            if (nindex[mid] <= p) lo = mid; else hi = mid - 1;
-        }
-        int32_t y = nlist[p];
-        if (y < 0 || y >= n) f |= 4;
-        if (y == (int32_t)lo) f |= 2;
-        if (p > nindex[lo] && nlist[p - 1] >= y) f |= 1;
-    }
+    (void)m;
+    entry_walk(
+        nindex, 0, n, [](int64_t) { return (int64_t)0; },
+        [&](int64_t p, int64_t x, int64_t start, int64_t) {
+            int32_t y = nlist[p];
+            if (y < 0 || y >= n) f |= 4;
+            if (y == (int32_t)x) f |= 2;
+            if (p > start && nlist[p - 1] >= y) f |= 1;
+        });
     if (f) atomicOr(flags, f);
 }
 
@@ -323,46 +374,29 @@ void mf_scan_excl(const int64_t* in, int64_t count, int64_t* out, int64_t* part,
 }
 
 // ---------------------------------------------------------------- compaction
+// New row lengths, and the rows with deletions listed (one atomic per wave:
+// returning atomics on one counter serialise across the chip).
 __global__ void k_newdeg(DevGraph g, int64_t* deg, int32_t* tomb_list, u64* ntomb) {
-    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < g.n; x += (int64_t)gridDim.x * blockDim.x) {
-        int64_t blen = g.nindex[x + 1] - g.nindex[x];
-        int32_t t = g.ov_tomb[x];
-        deg[x] = blen - t + g.ov_alive[x];
-        if (t) {
-            u64 i = atomicAdd(ntomb, 1ull);
-            tomb_list[i] = (int32_t)x;
+    GC c = grid_coords();
+    for (int64_t x = c.tid; x - c.lane < g.n; x += c.nthreads) {  // whole waves stay in step
+        int32_t t = 0;
+        if (x < g.n) {
+            int64_t blen = g.nindex[x + 1] - g.nindex[x];
+            t = g.ov_tomb[x];
+            deg[x] = blen - t + g.ov_alive[x];
         }
+        wave_push(tomb_list, ntomb, (int32_t)x, x < g.n && t != 0);
     }
 }
 
-// rows without deletions: straight copy, edge-parallel over 64-vertex groups
+// rows without deletions: straight copy, balanced by entries (a row with
+// deletions gets -1 and is left to k_copy_tomb)
 __global__ __launch_bounds__(256) void k_copy_clean(DevGraph g, const int64_t* nidx, int32_t* out) {
-    GC c = grid_coords();
-    int64_t groups = ((int64_t)g.n + MF_WAVE - 1) / MF_WAVE;
-    for (int64_t gi = c.wave; gi < groups; gi += c.nwaves) {
-        int32_t v0 = (int32_t)(gi * MF_WAVE);
-        int32_t v1 = v0 + MF_WAVE < g.n ? v0 + MF_WAVE : g.n;
-        int32_t myv = v0 + c.lane;
-        int64_t off = myv < v1 ? g.nindex[myv] : INT64_MAX;
-        int64_t dst = myv < v1 ? nidx[myv] : 0;
-        bool clean = myv < v1 ? g.ov_tomb[myv] == 0 : true;
-        int64_t pb = g.nindex[v0], pe = g.nindex[v1];
-        for (int64_t p0 = pb; p0 < pe; p0 += MF_WAVE) {
-            int64_t p = p0 + c.lane;
-            int64_t pp = p < pe ? p : pe - 1;
-            int lo = 0, hi = MF_WAVE - 1;
-#pragma unroll
-            for (int s = 0; s < 6; s++) {
-                int mid = (lo + hi + 1) >> 1;
-                int64_t om = __shfl(off, mid, MF_WAVE);
-                if (om <= pp) lo = mid; else hi = mid - 1;
-            }
-            int64_t ooff = __shfl(off, lo, MF_WAVE);
-            int64_t odst = __shfl(dst, lo, MF_WAVE);
-            bool oclean = __shfl((int)clean, lo, MF_WAVE) != 0;
-            if (p < pe && oclean) out[odst + (p - ooff)] = g.nlist[p];
-        }
-    }
+    entry_walk(
+        g.nindex, 0, g.n, [&](int64_t x) { return g.ov_tomb[x] == 0 ? nidx[x] : (int64_t)-1; },
+        [&](int64_t p, int64_t, int64_t start, int64_t dst) {
+            if (dst >= 0) out[dst + (p - start)] = g.nlist[p];
+        });
 }
 
 // rows with deletions: order-preserving filter, one wave per row
@@ -506,28 +540,25 @@ struct RebaseOffset {
 };
 }  // namespace
 
-__global__ void k_row_iota(const int64_t* nindex, int32_t r0, int32_t r1, int32_t* pos) {
+// Position of every entry of rows [r0, r1) inside its row, at pos[p - first entry].
+__global__ __launch_bounds__(256) void k_row_iota(const int64_t* nindex, int32_t r0, int32_t r1, int32_t* pos) {
     const int64_t base = nindex[r0];
-    for (int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64 + r0; w < r1;
-         w += (int64_t)gridDim.x * blockDim.x / 64) {
-        int64_t b = nindex[w], e = nindex[w + 1];
-        for (int64_t p = b + (threadIdx.x & 63); p < e; p += 64) pos[p - base] = (int32_t)(p - b);
-    }
+    entry_walk(
+        nindex, r0, r1, [](int64_t) { return (int64_t)0; },
+        [&](int64_t p, int64_t, int64_t start, int64_t) { pos[p - base] = (int32_t)(p - start); });
 }
 
 // flags |= 16 when a row lists a neighbour twice (adjacent equal keys).
-__global__ void k_row_dups(const int64_t* nindex, const int32_t* sval, int32_t n, int64_t m, int* flags) {
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x + 1; p < m;
-         p += (int64_t)gridDim.x * blockDim.x) {
-        if (sval[p] != sval[p - 1]) continue;
-        // same row?  binary search for the row of p
-        int32_t lo = 0, hi = n;
-        while (hi - lo > 1) {
-            int32_t mid = (lo + hi) >> 1;
-            if (nindex[mid] <= p) lo = mid; else hi = mid;
-        }
-        if (nindex[lo] < p) atomicOr(flags, 16);
-    }
+__global__ __launch_bounds__(256) void k_row_dups(const int64_t* nindex, const int32_t* sval, int32_t n, int64_t m,
+                                                  int* flags) {
+    int f = 0;
+    (void)m;
+    entry_walk(
+        nindex, 0, n, [](int64_t) { return (int64_t)0; },
+        [&](int64_t p, int64_t, int64_t start, int64_t) {
+            if (p > start && sval[p] == sval[p - 1]) f = 16;
+        });
+    if (f) atomicOr(flags, f);
 }
 
 hipError_t mf_build_row_index(GraphState& S, int32_t* sval, int32_t* spos, int* d_flags) {
