@@ -374,19 +374,32 @@ void mf_scan_excl(const int64_t* in, int64_t count, int64_t* out, int64_t* part,
 }
 
 // ---------------------------------------------------------------- compaction
-// New row lengths, and the rows with deletions listed (one atomic per wave:
-// returning atomics on one counter serialise across the chip).
-__global__ void k_newdeg(DevGraph g, int64_t* deg, int32_t* tomb_list, u64* ntomb) {
-    GC c = grid_coords();
-    for (int64_t x = c.tid; x - c.lane < g.n; x += c.nthreads) {  // whole waves stay in step
-        int32_t t = 0;
-        if (x < g.n) {
-            int64_t blen = g.nindex[x + 1] - g.nindex[x];
-            t = g.ov_tomb[x];
-            deg[x] = blen - t + g.ov_alive[x];
-        }
-        wave_push(tomb_list, ntomb, (int32_t)x, x < g.n && t != 0);
+// New row lengths, and the rows with deletions listed.  Appends to one
+// counter serialise across the chip even one per wave, so each workgroup
+// counts its rows in LDS, reserves its share with one atomic and fills it
+// from an LDS cursor (list order is free: each row is copied on its own).
+__global__ __launch_bounds__(256) void k_newdeg(DevGraph g, int64_t* deg, int32_t* tomb_list, u64* ntomb) {
+    __shared__ unsigned int cnt, cur;
+    __shared__ u64 base;
+    if (threadIdx.x == 0) {
+        cnt = 0;
+        cur = 0;
     }
+    __syncthreads();
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    unsigned int mine = 0;
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < g.n; x += stride) {
+        int32_t t = g.ov_tomb[x];
+        deg[x] = (g.nindex[x + 1] - g.nindex[x]) - t + g.ov_alive[x];
+        mine += t ? 1u : 0u;
+    }
+    if (mine) atomicAdd(&cnt, mine);
+    __syncthreads();
+    if (threadIdx.x == 0 && cnt) base = atomicAdd(ntomb, (u64)cnt);
+    __syncthreads();
+    if (!cnt) return;
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < g.n; x += stride)
+        if (g.ov_tomb[x]) tomb_list[base + atomicAdd(&cur, 1u)] = (int32_t)x;
 }
 
 // rows without deletions: straight copy, balanced by entries (a row with
