@@ -43,6 +43,13 @@ shape, with the same checks:
       requests submitted at once; plain: the graphs one after another, one
       batch at a time.  `value` is the aggregate served throughput; per-graph
       p50/p99 latency is reported.
+  compact  overlay compaction (core/flash_compactor.h:20-105) on the R-MAT
+      scale 22 graph with an overlay of 5% of its entries (the first updates
+      of the default stream): the GPU merges it into a new base in HBM
+      (value: ms per compaction) and the reference's own compact_flash_graph,
+      built from its sources into oracle/_ref, compacts the same base and
+      overlay on the host (the CPU baseline, kind "reference"); the two
+      compacted graphs must be identical.
 """
 import argparse
 import json
@@ -353,6 +360,100 @@ def single_graph_line(M, args, name, ni, nl, mis0, u, v, W, K, B, index_bytes, c
             "roofline": roof, "cpu_baseline": cpu}
 
 
+def compact_line(M, args, copy):
+    """GPU compaction of a 5% overlay on the R-MAT scale 22 graph, timed over
+    `steps` fresh loads of the same base and overlay; the reference's
+    compactor on the same input as the CPU baseline; identical results."""
+    import shutil
+    import tempfile
+    ni, nl = M.gen_rmat(22, 16, args.seed)
+    mis0 = M.greedy_mis(ni, nl)
+    n, m = len(ni) - 1, len(nl)
+    nup = int(0.05 * m)
+    u, v = M.gen_updates(M.MIXED_EXISTING, ni, nl, nup, args.seed)
+    print("workload: n=%d m=%d, %d updates" % (n, m, nup), file=sys.stderr, flush=True)
+    ms, pool, new_m, dt = [], 0, 0, 0.0
+    tmpdir = tempfile.mkdtemp(dir="/dev/shm" if os.path.isdir("/dev/shm") else None)
+    try:
+        with M.Engine(0) as eng:
+            name = eng.device_info()[0]
+            eng.load_csr(ni, nl)
+            ins = eng.flags(u, v)
+            ups = M.make_updates(u, v, ins)
+            for step in range(args.warmup + args.steps):
+                eng.load_csr(ni, nl)
+                eng.set_mis(mis0)
+                tickets, st = [], None
+                for i in range(0, len(ups), 100_000):
+                    tickets.append(eng.submit(ups[i:i + 100_000]))
+                    if len(tickets) > 2:
+                        st = eng.wait(tickets.pop(0))
+                for t in tickets:
+                    st = eng.wait(t)
+                eng.sync()
+                pool = int(st["overlay_entries"])
+                t0 = time.perf_counter()
+                new_m = eng.compact()
+                eng.sync()
+                dt = (time.perf_counter() - t0) * 1e3
+                if step >= args.warmup:
+                    ms.append(dt)
+                print("compaction %d: %.2f ms" % (step, dt), file=sys.stderr, flush=True)
+            t0 = time.perf_counter()
+            cni, cnl = eng.export_csr()
+            M.write_egr(os.path.join(tmpdir, "gpu.egr"), cni, cnl)
+            with_write = dt + (time.perf_counter() - t0) * 1e3
+            conf, unc, bad = eng.check()
+            if (conf, unc, bad) != (0, 0, 0):
+                raise SystemExit("check failed after compaction: %s" % ((conf, unc, bad),))
+        cpu = None
+        if not args.no_cpu_baseline:
+            from oracle import pyoracle as O
+            if not O.ref_available():
+                raise SystemExit("oracle/_ref (the reference's compactor) is not built")
+            ref_path = os.path.join(tmpdir, "ref.egr")
+            M.write_egr(ref_path, ni, nl)
+            ref = O.RefGraph(ref_path)
+            # every update of this stream changes the graph (deletions of edges,
+            # insertions of non-edges, each pair once), so the reference's
+            # row-scanning adjacency tests are skipped while its overlay is built
+            ref.apply(u, v, ins, check=False)
+            t0 = time.perf_counter()
+            ref.compact_to(ref_path)
+            ref_ms = (time.perf_counter() - t0) * 1e3
+            ref.close()
+            rni, rnl, _ = M.read_egr(ref_path)
+            same = bool(np.array_equal(rni, cni) and np.array_equal(rnl, cnl))
+            if not same:
+                raise SystemExit("GPU compaction differs from the reference's")
+            cpu = {"value": round(ref_ms, 2), "unit": "ms", "cores": 1, "kind": "reference",
+                   "sample": "the reference's compact_flash_graph (core/flash_compactor.h, built from its sources "
+                             "into oracle/_ref) on the same base and overlay, writing the wide .egr to %s"
+                             % ("/dev/shm" if tmpdir.startswith("/dev/shm") else "a temporary directory"),
+                   "identical_to_gpu": same}
+            print("reference compaction: %.1f ms" % ref_ms, file=sys.stderr, flush=True)
+    finally:
+        shutil.rmtree(tmpdir, ignore_errors=True)
+    mean = float(np.mean(ms))
+    bc = 8 * (n + 1) + 4 * m + 4 * pool + 8 * (n + 1) + 4 * int(new_m)
+    ach = bc / (mean / 1e3) / 1e9
+    return {"metric": "overlay compaction time (merge into a new base, sorted row copy rebuilt)",
+            "value": round(mean, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(mean, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+            "dtype": "int32", "data": "synthetic (seeded generators in misfit_amd/csrc/gen.cpp)",
+            "config": {"workload": "R-MAT scale 22 (seed %d), greedy MIS, overlay from the first %d updates of the "
+                                   "default stream (5%% of the entries), applied in 100,000-update queued batches"
+                                   % (args.seed, nup),
+                       "name": "compact", "graph_entries": m, "overlay_slots": pool, "entries_after": int(new_m),
+                       "gpu_compact_plus_export_and_write_ms": round(with_write, 2), "device": name},
+            "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None,
+                         "kernel": "misfit_compact (all its kernels) timed by the host around the call",
+                         "algorithmic_bytes_per_compaction": int(bc),
+                         "stream_copy_gbs": copy["copy_gbs"] if copy else None},
+            "cpu_baseline": cpu}
+
+
 C5_TYPES = ("er", "rmat", "rgg")
 
 
@@ -523,7 +624,7 @@ def cpu_many(M, graphs, budget_s):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=("c2", "c1", "c3", "c5"),
+    ap.add_argument("--config", default="c2", choices=("c2", "c1", "c3", "c5", "compact"),
                     help="workload (default c2: R-MAT scale 22, the headline line)")
     ap.add_argument("--steps", type=int, default=None, help="timed batches (c5: batches per graph)")
     ap.add_argument("--warmup", type=int, default=None)
@@ -547,8 +648,8 @@ def main():
         print("stream copy: %.1f GB/s" % copy["copy_gbs"], file=sys.stderr, flush=True)
     import misfit_amd as M
 
-    steps_default = {"c2": 95, "c1": 10, "c3": 40, "c5": 4}[args.config]
-    warm_default = {"c2": 5, "c1": 2, "c3": 3, "c5": 0}[args.config]
+    steps_default = {"c2": 95, "c1": 10, "c3": 40, "c5": 4, "compact": 3}[args.config]
+    warm_default = {"c2": 5, "c1": 2, "c3": 3, "c5": 0, "compact": 1}[args.config]
     args.steps = steps_default if args.steps is None else args.steps
     args.warmup = warm_default if args.warmup is None else args.warmup
     if args.config == "c1":
@@ -569,6 +670,9 @@ def main():
         return
     if args.config == "c5":
         print(json.dumps(many_graph_line(M, args, copy)))
+        return
+    if args.config == "compact":
+        print(json.dumps(compact_line(M, args, copy)))
         return
 
     W, K, B = args.warmup, args.steps, args.batch

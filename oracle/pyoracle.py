@@ -265,6 +265,7 @@ def reflib():
         L.ref_neighbors.restype = ctypes.c_int64
         L.ref_neighbors.argtypes = [vp, ctypes.c_int32, _i32p, ctypes.c_int64]
         L.ref_compact.argtypes = [vp, ctypes.c_char_p]
+        L.ref_apply.argtypes = [vp, _i32p, _i32p, ctypes.POINTER(ctypes.c_uint8), ctypes.c_int64, ctypes.c_int]
         _ref = L
     return _ref
 
@@ -318,3 +319,13 @@ class RefGraph:
 
     def compact_to(self, path):
         reflib().ref_compact(self.h, os.fsencode(path))
+
+    def apply(self, u, v, ins, check=True):
+        """Applies a batch to the overlay the way src/misfit/misfit.cu:708-714 does
+        (check=False skips the adjacency tests: the same overlay when every
+        update changes the graph, each pair once)."""
+        u = np.ascontiguousarray(u, np.int32)
+        v = np.ascontiguousarray(v, np.int32)
+        ins = np.ascontiguousarray(ins, np.uint8)
+        reflib().ref_apply(self.h, _p(u, _i32p), _p(v, _i32p), ins.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                           len(u), 1 if check else 0)

@@ -37,6 +37,24 @@ __attribute__((visibility("default"))) int64_t ref_neighbors(void* h, int32_t v,
     if (out) std::memcpy(out, nb.data(), (size_t)(n < cap ? n : cap) * sizeof(int32_t));
     return n;
 }
+// A batch applied as the reference's synchronous engine applies it to its
+// overlay (src/misfit/misfit.cu:708-714): insert when flagged as an
+// insertion and not adjacent, delete when flagged as a deletion and adjacent.
+// check = 0 skips the adjacency test (a linear scan of the base row in the
+// reference), which gives the same overlay when every update changes the
+// graph: deletions of present edges and insertions of absent ones, each pair
+// once.
+__attribute__((visibility("default"))) void ref_apply(void* h, const int32_t* u, const int32_t* v, const uint8_t* ins,
+                                                      int64_t count, int check) {
+    HybridGraph* g = (HybridGraph*)h;
+    for (int64_t i = 0; i < count; i++) {
+        if (ins[i]) {
+            if (!check || !g->isAdjacent(u[i], v[i])) g->insertEdge(u[i], v[i]);
+        } else if (!check || g->isAdjacent(u[i], v[i])) {
+            g->removeEdge(u[i], v[i]);
+        }
+    }
+}
 // Compacts base + overlay into out_path with the reference's writer.
 __attribute__((visibility("default"))) void ref_compact(void* h, const char* out_path) {
     HybridGraph* g = (HybridGraph*)h;

@@ -165,6 +165,53 @@ def test_compaction_matches_reference_file(oracle, tmp_path):
     assert open(p_ref, "rb").read() == open(p_mine, "rb").read()
 
 
+@needs_ref
+def test_reference_batch_apply(oracle, tmp_path):
+    """ref_apply (the reference's overlay update of a batch, misfit.cu:708-714)
+    gives the same rows as applying the batch operation by operation."""
+    rng = np.random.default_rng(8)
+    ni, nl = H.random_graph(rng, 80, 0.08)
+    p1, p2 = str(tmp_path / "a.egr"), str(tmp_path / "b.egr")
+    _write_wide(p1, ni, nl)
+    _write_wide(p2, ni, nl)
+    a, b = oracle.RefGraph(p1), oracle.RefGraph(p2)
+    u = rng.integers(0, 80, 3000).astype(np.int32)
+    v = rng.integers(0, 80, 3000).astype(np.int32)
+    keep = u != v
+    u, v = u[keep], v[keep]
+    ins = (rng.random(len(u)) < 0.5).astype(np.uint8)
+    a.apply(u, v, ins)
+    # without the adjacency tests on a stream whose updates all change the graph
+    ni2, nl2 = H.random_graph(rng, 80, 0.08)
+    p3, p4 = str(tmp_path / "c.egr"), str(tmp_path / "d.egr")
+    _write_wide(p3, ni2, nl2)
+    _write_wide(p4, ni2, nl2)
+    c_, d_ = oracle.RefGraph(p3), oracle.RefGraph(p4)
+    pairs = [(x, y) for x in range(80) for y in range(x + 1, 80)]
+    pick = rng.permutation(len(pairs))[:600]
+    eu = np.array([pairs[k][0] for k in pick], np.int32)
+    ev = np.array([pairs[k][1] for k in pick], np.int32)
+    eins = np.array([0 if c_.is_adjacent(int(x), int(y)) else 1 for x, y in zip(eu, ev)], np.uint8)
+    c_.apply(eu, ev, eins, check=False)
+    d_.apply(eu, ev, eins, check=True)
+    assert c_.delta_ops == d_.delta_ops
+    for x in range(80):
+        assert np.array_equal(c_.neighbors(x), d_.neighbors(x))
+    c_.close()
+    d_.close()
+    for x, y, f in zip(u.tolist(), v.tolist(), ins.tolist()):
+        adj = b.is_adjacent(x, y)
+        if f and not adj:
+            b.insert_edge(x, y)
+        elif not f and adj:
+            b.remove_edge(x, y)
+    assert a.delta_ops == b.delta_ops
+    for x in range(80):
+        assert np.array_equal(a.neighbors(x), b.neighbors(x))
+    a.close()
+    b.close()
+
+
 def _write_wide(path, ni, nl):
     with open(path, "wb") as f:
         f.write(struct.pack("<iiq", len(ni) - 1, -1, len(nl)))
