@@ -75,6 +75,18 @@ struct misfit_ctx {
     u64* d_ktask = nullptr;    // row-chunk tasks, sized for the graph plus the pool
     int32_t* d_sval = nullptr; // sorted copy of the base rows when they are not ascending
     int32_t* d_spos = nullptr;
+    // compaction scratch, kept between compactions (grown as needed, freed
+    // with the graph): per-vertex counts, offsets and row lists, and the
+    // deletion and tail lists of the merge
+    std::vector<void*> cs_allocs;
+    int64_t cs_n = 0, cs_d = 0, cs_t = 0;
+    int64_t *cs_dcnt = nullptr, *cs_tcnt = nullptr, *cs_doff = nullptr, *cs_toff = nullptr;
+    int32_t *cs_trows = nullptr, *cs_big = nullptr;
+    u64* cs_cnt = nullptr;  // [0] rows with a tail, [1] long lists
+    int32_t *cs_dval = nullptr, *cs_dpos = nullptr, *cs_dsv = nullptr;
+    int32_t *cs_tval = nullptr, *cs_tidx = nullptr, *cs_tsv = nullptr, *cs_tsi = nullptr;
+    void* cs_tmp = nullptr;
+    size_t cs_tmp_bytes = 0;
     int64_t ktask_cap = 0;
     int64_t ins_bound = 0;    // inserted pairs since the overlay was last cleared (upper bound)
     int64_t pool_bound = 0;   // pool slots possibly used (upper bound)
@@ -107,6 +119,7 @@ static int fail(misfit_ctx* c, int code, const std::string& msg) {
     } while (0)
 
 static int build_row_index(misfit_ctx* c);
+static void free_compact_scratch(misfit_ctx* c);
 static int settle(misfit_ctx* c);
 
 template <typename T>
@@ -208,6 +221,7 @@ extern "C" int misfit_ctx_destroy(misfit_ctx* c) {
     }
     free_list(c->batch_allocs);
     free_list(c->graph_allocs);
+    free_compact_scratch(c);
     if (c->d_ktask) hipFree(c->d_ktask);
     if (c->d_sval) hipFree(c->d_sval);
     if (c->d_spos) hipFree(c->d_spos);
@@ -376,6 +390,7 @@ extern "C" int misfit_load_csr(misfit_ctx* c, int32_t n, int64_t m, const void* 
     HIPCHK(c, hipStreamSynchronize(c->stream));
     // the batch scratch keeps per-batch pointers only; graph arrays are new
     free_list(c->graph_allocs);
+    free_compact_scratch(c);
     c->has_graph = false;
     c->U_cap = 0;
     free_list(c->batch_allocs);
@@ -797,6 +812,7 @@ static int build_row_index(misfit_ctx* c) {
     HIPCHK(c, hipMemcpyAsync(&flags, c->d_flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (flags & 16) return fail(c, MISFIT_E_UNSUPPORTED, "a row lists a neighbour twice");
+    if (flags & 32) return fail(c, MISFIT_E_CHECK, "sorted row copy does not match the rows");
     g.sval = c->d_sval;
     g.spos = c->d_spos;
     c->S.g = g;
@@ -977,6 +993,83 @@ extern "C" int misfit_batch_bytes(misfit_ctx* c, int index_bytes, int64_t* bytes
 }
 
 // ---------------------------------------------------------------- compaction
+// Compaction scratch: freed with the graph (reload, destroy).
+static void free_compact_scratch(misfit_ctx* c) {
+    for (void* p : c->cs_allocs) hipFree(p);
+    c->cs_allocs.clear();
+    if (c->cs_tmp) hipFree(c->cs_tmp);
+    c->cs_tmp = nullptr;
+    c->cs_tmp_bytes = 0;
+    c->cs_n = c->cs_d = c->cs_t = 0;
+    c->cs_dcnt = c->cs_tcnt = c->cs_doff = c->cs_toff = nullptr;
+    c->cs_trows = c->cs_big = nullptr;
+    c->cs_cnt = nullptr;
+    c->cs_dval = c->cs_dpos = c->cs_dsv = nullptr;
+    c->cs_tval = c->cs_tidx = c->cs_tsv = c->cs_tsi = nullptr;
+}
+
+static hipError_t cs_alloc(misfit_ctx* c, void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 8));
+    if (e == hipSuccess) c->cs_allocs.push_back(*p);
+    return e;
+}
+
+// Frees a scratch buffer the context still owns (and nothing else).
+static void cs_release(misfit_ctx* c, void* p) {
+    if (!p) return;
+    auto it = std::find(c->cs_allocs.begin(), c->cs_allocs.end(), p);
+    if (it == c->cs_allocs.end()) return;
+    c->cs_allocs.erase(it);
+    hipFree(p);
+}
+
+// Per-vertex scratch (once per graph) and the deletion and tail lists
+// (grown by half again when short).
+static hipError_t ensure_compact_scratch(misfit_ctx* c, int64_t ndel, int64_t ntail) {
+    const int64_t n = c->P.g.n;
+    hipError_t e = hipSuccess;
+    if (c->cs_n != n) {
+        free_compact_scratch(c);
+        e = cs_alloc(c, (void**)&c->cs_dcnt, (size_t)n * 8);
+        if (e == hipSuccess) e = cs_alloc(c, (void**)&c->cs_tcnt, (size_t)n * 8);
+        if (e == hipSuccess) e = cs_alloc(c, (void**)&c->cs_doff, ((size_t)n + 1) * 8);
+        if (e == hipSuccess) e = cs_alloc(c, (void**)&c->cs_toff, ((size_t)n + 1) * 8);
+        if (e == hipSuccess) e = cs_alloc(c, (void**)&c->cs_trows, (size_t)n * 4);
+        if (e == hipSuccess) e = cs_alloc(c, (void**)&c->cs_big, (size_t)n * 4);
+        if (e == hipSuccess) e = cs_alloc(c, (void**)&c->cs_cnt, 2 * 8);
+        if (e != hipSuccess) {
+            free_compact_scratch(c);
+            return e;
+        }
+        c->cs_n = n;
+    }
+    if (ndel > c->cs_d) {
+        for (int32_t** p : {&c->cs_dval, &c->cs_dpos, &c->cs_dsv}) {
+            cs_release(c, *p);
+            *p = nullptr;
+        }
+        c->cs_d = 0;
+        const int64_t want = ndel + ndel / 2;
+        for (int32_t** p : {&c->cs_dval, &c->cs_dpos, &c->cs_dsv})
+            if (e == hipSuccess) e = cs_alloc(c, (void**)p, (size_t)want * 4);
+        if (e != hipSuccess) return e;
+        c->cs_d = want;
+    }
+    if (ntail > c->cs_t) {
+        for (int32_t** p : {&c->cs_tval, &c->cs_tidx, &c->cs_tsv, &c->cs_tsi}) {
+            cs_release(c, *p);
+            *p = nullptr;
+        }
+        c->cs_t = 0;
+        const int64_t want = ntail + ntail / 2;
+        for (int32_t** p : {&c->cs_tval, &c->cs_tidx, &c->cs_tsv, &c->cs_tsi})
+            if (e == hipSuccess) e = cs_alloc(c, (void**)p, (size_t)want * 4);
+        if (e != hipSuccess) return e;
+        c->cs_t = want;
+    }
+    return hipSuccess;
+}
+
 extern "C" int misfit_compact(misfit_ctx* c, int64_t* new_m) {
     if (!c) return MISFIT_E_ARG;
     if (!c->has_graph) return fail(c, MISFIT_E_STATE, "no graph loaded");
@@ -984,21 +1077,96 @@ extern "C" int misfit_compact(misfit_ctx* c, int64_t* new_m) {
     if (int rc_ = settle(c)) return rc_;
     HIPCHK(c, hipStreamSynchronize(c->copy_stream));
     DevGraph& g = c->P.g;
+    hipStream_t st = c->stream;
+    // every device buffer made here is freed unless it becomes the new graph;
+    // until then the old graph is untouched
+    std::vector<void*> mine;
+    auto grab = [&](void** p, size_t bytes) {
+        hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 4));
+        if (e == hipSuccess) mine.push_back(*p);
+        return e;
+    };
+    auto drop_all = [&]() {
+        for (void* p : mine) hipFree(p);
+        mine.clear();
+    };
+    auto keep = [&](void* p) { mine.erase(std::find(mine.begin(), mine.end(), p)); };
     int64_t* nidx = nullptr;
-    HIPCHK(c, hipMalloc((void**)&nidx, ((size_t)g.n + 1) * 8));
-    hipError_t e = mf_compact_plan(c->S, nidx);
-    int64_t total = 0;
-    if (e == hipSuccess) e = hipMemcpyAsync(&total, nidx + g.n, 8, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     int32_t* nl = nullptr;
-    if (e == hipSuccess) e = hipMalloc((void**)&nl, (size_t)std::max<int64_t>(total, 1) * 4);
-    if (e == hipSuccess) e = mf_compact_copy(c->S, nidx, nl);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    int64_t total = 0, ndel = 0, ntail = 0;
+    MergeLists L;
+    hipError_t e = grab((void**)&nidx, ((size_t)g.n + 1) * 8);
+    if (e == hipSuccess) e = mf_compact_plan(c->S, nidx);
+    // per-row deletion and tail lists, for merging the sorted row copy
+    if (e == hipSuccess) e = ensure_compact_scratch(c, 0, 0);
+    if (e == hipSuccess)
+        e = mf_merge_counts(c->S, c->cs_dcnt, c->cs_tcnt, c->cs_doff, c->cs_toff, c->cs_trows, c->cs_cnt);
+    if (e == hipSuccess) e = hipMemcpyAsync(&total, nidx + g.n, 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(&ndel, c->cs_doff + g.n, 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(&ntail, c->cs_toff + g.n, 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = ensure_compact_scratch(c, ndel, ntail);
+    L.dval = c->cs_dval;
+    L.dpos = c->cs_dpos;
+    L.doff = c->cs_doff;
+    L.tval = c->cs_tval;
+    L.tidx = c->cs_tidx;
+    L.toff = c->cs_toff;
+    L.trows = c->cs_trows;
+    L.ntrows = c->cs_cnt;
+    L.dsv = c->cs_dsv;
+    L.tsv = c->cs_tsv;
+    L.tsi = c->cs_tsi;
+    L.big = c->cs_big;
+    L.nbig = c->cs_cnt + 1;
+    L.tmp = &c->cs_tmp;
+    L.tmp_bytes = &c->cs_tmp_bytes;
+    if (e == hipSuccess) e = grab((void**)&nl, (size_t)total * 4);
+    if (e == hipSuccess) e = mf_compact_copy(c->S, nidx, nl, L);
+    // rows are base order then insertion order, so they may be unsorted now
+    GraphState NS = c->S;
+    NS.g.nindex = nidx;
+    NS.g.nlist = nl;
+    NS.g.m = total;
+    int flags = 0;
+    if (e == hipSuccess) e = mf_validate_csr(NS, c->d_flags);
+    if (e == hipSuccess) e = mf_validate_entries(NS, c->d_flags);
+    if (e == hipSuccess) e = hipMemcpyAsync(&flags, c->d_flags, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
-        hipFree(nidx);
-        if (nl) hipFree(nl);
+        drop_all();
         HIPCHK(c, e);
     }
+    const bool sorted = (flags & 1) == 0;
+    int32_t *nsval = nullptr, *nspos = nullptr;
+    bool merged = false;
+    if (!sorted && total > 0) {
+        e = grab((void**)&nsval, (size_t)total * 4);
+        if (e == hipSuccess) e = grab((void**)&nspos, (size_t)total * 4);
+        if (e == hipSuccess) {
+            mf_fill_i32((int32_t*)c->d_flags, 0, 1, st);
+            e = mf_merge_row_index(c->S, L, ndel, ntail, nidx, nl, nsval, nspos, c->d_flags);
+            if (e == hipSuccess) e = hipMemcpyAsync(&flags, c->d_flags, sizeof(int), hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            if (e == hipSuccess) merged = true;
+            else if (e == hipErrorInvalidValue) e = hipSuccess;  // lists too long to merge: full sort below
+        }
+        if (e != hipSuccess) {
+            drop_all();
+            HIPCHK(c, e);
+        }
+        if (merged && (flags & 16)) {
+            drop_all();
+            return fail(c, MISFIT_E_UNSUPPORTED, "a row lists a neighbour twice");
+        }
+        if (merged && (flags & 32)) {
+            drop_all();
+            return fail(c, MISFIT_E_CHECK, "merged sorted row copy does not match the compacted rows");
+        }
+    }
+    // the new graph replaces the old one
+    keep(nidx);
+    keep(nl);
     auto& v = c->graph_allocs;
     for (void* old : {(void*)c->d_nindex, (void*)c->d_nlist}) {
         auto it = std::find(v.begin(), v.end(), old);
@@ -1008,21 +1176,28 @@ extern "C" int misfit_compact(misfit_ctx* c, int64_t* new_m) {
     v.push_back(nl);
     c->d_nindex = nidx;
     c->d_nlist = nl;
+    if (c->d_sval) hipFree(c->d_sval);
+    if (c->d_spos) hipFree(c->d_spos);
+    c->d_sval = c->d_spos = nullptr;
+    if (merged) {
+        keep(nsval);
+        keep(nspos);
+        c->d_sval = nsval;
+        c->d_spos = nspos;
+    }
+    drop_all();
     g.nindex = nidx;
     g.nlist = nl;
     g.m = total;
+    g.sorted = sorted ? 1 : 0;
+    g.sval = c->d_sval;
+    g.spos = c->d_spos;
     c->S.g = g;
     reset_overlay(c);
-    // rows are base order then insertion order, so they may be unsorted now
-    HIPCHK(c, mf_validate_csr(c->S, c->d_flags));
-    HIPCHK(c, mf_validate_entries(c->S, c->d_flags));
-    int flags = 0;
-    HIPCHK(c, hipMemcpyAsync(&flags, c->d_flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    g.sorted = (flags & 1) ? 0 : 1;
-    c->S.g = g;
-    int rc = build_row_index(c);
-    if (rc) return rc;
+    if (!sorted && !merged) {
+        int rc = build_row_index(c);
+        if (rc) return rc;
+    }
     if (new_m) *new_m = total;
     return MISFIT_OK;
 }

@@ -22,13 +22,44 @@ hipError_t mf_get_mis(GraphState& S, uint8_t* d_mis);
 hipError_t mf_flags(GraphState& S, const int32_t* d_u, const int32_t* d_v, int64_t count, uint8_t* d_ins);
 hipError_t mf_check(GraphState& S, u64* d_acc);
 hipError_t mf_compact_plan(GraphState& S, int64_t* d_newidx);
-hipError_t mf_compact_copy(GraphState& S, const int64_t* d_newidx, int32_t* d_out);
+// Per-row lists a compaction hands to the merge of the sorted row copy
+// (all null: not collected).  doff/toff: n+1 offsets (exclusive scans of
+// the deleted and live-overlay counts per row).
+struct MergeLists {
+    int32_t* dval = nullptr;   // deleted entries' values, row order
+    int32_t* dpos = nullptr;   // ... their positions in the old row (ascending per row)
+    const int64_t* doff = nullptr;
+    int32_t* tval = nullptr;   // live overlay entries, insertion order
+    int32_t* tidx = nullptr;   // ... their index in that order
+    const int64_t* toff = nullptr;
+    const int32_t* trows = nullptr;  // rows with a tail (rows with deletions: the tomb list)
+    const u64* ntrows = nullptr;
+    // merge scratch: sorted deletion values, sorted tail values and their
+    // insertion indices, rows whose list is long (n), a counter, and the
+    // library sort's temporary storage (grown as needed)
+    int32_t* dsv = nullptr;
+    int32_t* tsv = nullptr;
+    int32_t* tsi = nullptr;
+    int32_t* big = nullptr;
+    u64* nbig = nullptr;
+    void** tmp = nullptr;
+    size_t* tmp_bytes = nullptr;
+};
+hipError_t mf_compact_copy(GraphState& S, const int64_t* d_newidx, int32_t* d_out, const MergeLists& L);
 hipError_t mf_batch_bytes(GraphState& S, const int32_t* d_seeds, const u64* d_nseeds, uint32_t t, u64* d_acc);
 hipError_t mf_widen(const int32_t* d_in, int64_t* d_out, int64_t count, hipStream_t st);
 hipError_t mf_validate_csr(GraphState& S, int* d_flags);
 hipError_t mf_validate_entries(GraphState& S, int* d_flags);
-// sorted copy of every row with positions (rows not ascending); flags |= 16 on a repeated neighbour
+// sorted copy of every row with positions (rows not ascending); flags |= 16 on a repeated neighbour,
+// |= 32 when the copy does not match the rows
 hipError_t mf_build_row_index(GraphState& S, int32_t* sval, int32_t* spos, int* d_flags);
+// per-row deletion and tail counts of the overlay and their offsets (n+1 each)
+hipError_t mf_merge_counts(GraphState& S, int64_t* dcnt, int64_t* tcnt, int64_t* doff, int64_t* toff, int32_t* trows,
+                           u64* ntrows);
+// sorted copy of the compacted rows nidx/nnl by merging the old graph's (S) sorted order with the
+// lists compaction collected; same flags as mf_build_row_index
+hipError_t mf_merge_row_index(GraphState& S, const MergeLists& L, int64_t ndel, int64_t ntail, const int64_t* nidx,
+                              const int32_t* nnl, int32_t* nsval, int32_t* nspos, int* d_flags);
 void mf_scan_excl(const int64_t* in, int64_t count, int64_t* out, int64_t* part, hipStream_t st);
 void mf_fill_u32(uint32_t* p, uint32_t v, int64_t count, hipStream_t st);
 void mf_fill_i32(int32_t* p, int32_t v, int64_t count, hipStream_t st);

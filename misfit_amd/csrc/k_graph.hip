@@ -412,36 +412,58 @@ __global__ __launch_bounds__(256) void k_copy_clean(DevGraph g, const int64_t* n
         });
 }
 
-// rows with deletions: order-preserving filter, one wave per row
+// rows with deletions: order-preserving filter, one wave per row; with
+// L.dval, the deleted entries (value and position in the old row, in row
+// order) also go to the row's slice of the deletion lists
 __global__ __launch_bounds__(256) void k_copy_tomb(DevGraph g, const int64_t* nidx, int32_t* out,
-                                                   const int32_t* tomb_list, const u64* ntomb) {
+                                                   const int32_t* tomb_list, const u64* ntomb, MergeLists L) {
     GC c = grid_coords();
     int64_t nt = (int64_t)*ntomb;
     for (int64_t i = c.wave; i < nt; i += c.nwaves) {
         int32_t x = tomb_list[i];
         int64_t pb = g.nindex[x], pe = g.nindex[x + 1];
         int64_t dst = nidx[x];
+        int64_t ddst = L.dval ? L.doff[x] : 0;
         for (int64_t p0 = pb; p0 < pe; p0 += MF_WAVE) {
             int64_t p = p0 + c.lane;
-            int32_t y = p < pe ? g.nlist[p] : -1;
-            bool keep = y >= 0;
+            int32_t y = p < pe ? g.nlist[p] : 0;
+            bool keep = p < pe && y >= 0;
+            bool del = p < pe && y < 0;
             u64 bal = __ballot(keep);
-            int before = __popcll(bal & ((1ull << c.lane) - 1ull));
-            if (keep) out[dst + before] = y;
+            u64 bald = __ballot(del);
+            u64 below = (1ull << c.lane) - 1ull;
+            if (keep) out[dst + __popcll(bal & below)] = y;
+            if (del && L.dval) {
+                int64_t k = ddst + __popcll(bald & below);
+                L.dval[k] = ~y;
+                L.dpos[k] = (int32_t)(p - pb);
+            }
             dst += __popcll(bal);
+            ddst += __popcll(bald);
         }
     }
 }
 
-// live overlay entries follow the base part, in insertion order
-__global__ void k_copy_runs(DevGraph g, const int64_t* nidx, int32_t* out) {
+// live overlay entries follow the base part, in insertion order; with
+// L.tval, they also go to the row's slice of the tail lists with their
+// insertion index
+__global__ void k_copy_runs(DevGraph g, const int64_t* nidx, int32_t* out, MergeLists L) {
     for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < g.n; x += (int64_t)gridDim.x * blockDim.x) {
         int32_t len = g.ov_len[x];
         if (!len) continue;
         int64_t dst = nidx[x] + (g.nindex[x + 1] - g.nindex[x]) - g.ov_tomb[x];
+        int64_t t0 = L.tval ? L.toff[x] : 0;
+        int32_t j = 0;
         const int32_t* r = g.pool + g.ov_start[x];
         for (int32_t i = 0; i < len; i++)
-            if (r[i] >= 0) out[dst++] = r[i];
+            if (r[i] >= 0) {
+                out[dst++] = r[i];
+                if (L.tval) {
+                    L.tval[t0 + j] = r[i];
+                    L.tidx[t0 + j] = j;
+                }
+                j++;
+            }
     }
 }
 
@@ -453,11 +475,11 @@ hipError_t mf_compact_plan(GraphState& S, int64_t* d_newidx) {
     return hipGetLastError();
 }
 
-hipError_t mf_compact_copy(GraphState& S, const int64_t* d_newidx, int32_t* d_out) {
+hipError_t mf_compact_copy(GraphState& S, const int64_t* d_newidx, int32_t* d_out, const MergeLists& L) {
     const int B = S.blocks;
     k_copy_clean<<<B, 256, 0, S.stream>>>(S.g, d_newidx, d_out);
-    k_copy_tomb<<<B, 256, 0, S.stream>>>(S.g, d_newidx, d_out, S.d_tomb_list, S.d_walk_ntasks);
-    k_copy_runs<<<B, 256, 0, S.stream>>>(S.g, d_newidx, d_out);
+    k_copy_tomb<<<B, 256, 0, S.stream>>>(S.g, d_newidx, d_out, S.d_tomb_list, S.d_walk_ntasks, L);
+    k_copy_runs<<<B, 256, 0, S.stream>>>(S.g, d_newidx, d_out, L);
     return hipGetLastError();
 }
 
@@ -544,6 +566,7 @@ void mf_fill_i64(int64_t* p, int64_t v, int64_t count, hipStream_t st) {
 // Built with rocPRIM's segmented radix sort over row ranges of at most 2^30
 // entries; checked for repeated neighbours.
 #include <hipcub/device/device_segmented_sort.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
 #include <rocprim/iterator/transform_iterator.hpp>
 
 namespace {
@@ -561,17 +584,253 @@ __global__ __launch_bounds__(256) void k_row_iota(const int64_t* nindex, int32_t
         [&](int64_t p, int64_t, int64_t start, int64_t) { pos[p - base] = (int32_t)(p - start); });
 }
 
-// flags |= 16 when a row lists a neighbour twice (adjacent equal keys).
-__global__ __launch_bounds__(256) void k_row_dups(const int64_t* nindex, const int32_t* sval, int32_t n, int64_t m,
-                                                  int* flags) {
+// Checks a sorted row copy against its rows, entry by entry: every row of
+// sval strictly ascending (flags |= 16 on a value repeated, i.e. a row that
+// lists a neighbour twice) and every sval[p] found at its spos[p] in the row
+// (flags |= 32 otherwise: the copy is wrong).  Strictly ascending values that
+// all occur in the row, as many as the row has, make spos a permutation.
+__global__ __launch_bounds__(256) void k_check_row_index(const int64_t* nindex, const int32_t* nlist,
+                                                         const int32_t* sval, const int32_t* spos, int32_t n,
+                                                         int* flags) {
     int f = 0;
-    (void)m;
     entry_walk(
-        nindex, 0, n, [](int64_t) { return (int64_t)0; },
-        [&](int64_t p, int64_t, int64_t start, int64_t) {
-            if (p > start && sval[p] == sval[p - 1]) f = 16;
+        nindex, 0, n, [&](int64_t x) { return nindex[x + 1]; },
+        [&](int64_t p, int64_t, int64_t start, int64_t end) {
+            int32_t v = sval[p];
+            int32_t q = spos[p];
+            if (q < 0 || q >= end - start || nlist[start + q] != v) f |= 32;
+            if (p > start) {
+                int32_t pv = sval[p - 1];
+                if (pv == v) f |= 16;
+                else if (pv > v) f |= 32;
+            }
         });
     if (f) atomicOr(flags, f);
+}
+
+// ---------------------------------------------------------------- merged row copy
+// After a compaction row x is K ++ T: the old row's kept entries in old
+// order, then its live overlay entries in insertion order.  Its sorted copy
+// merges the old row's sorted order (the row itself when rows were
+// ascending, else the old sorted copy), minus the deleted entries, with the
+// sorted tail:
+//   a kept entry of old sorted rank k, value v, old position q lands at
+//     k - #{deleted values < v} + #{tail values < v}, new position
+//     q - #{deleted positions < q};
+//   the tail entry of sorted rank j, value w, insertion index i lands at
+//     #{old values < w} - #{deleted values < w} + j, new position |K| + i.
+// Tail values never occur in the old row (re-inserting a deleted base edge
+// restores it in place), so all values of the new row are distinct ranks.
+struct OldRows {
+    const int64_t* nindex;
+    const int32_t* nlist;  // with deletions as ~v
+    const int32_t* sval;   // sorted copy, or null when rows are ascending
+    const int32_t* spos;
+};
+
+__device__ __forceinline__ int32_t old_sorted_val(const OldRows& o, int64_t p) {
+    return o.sval ? o.sval[p] : decode(o.nlist[p]);
+}
+
+// first index in [lo, hi) with a[i] >= v
+__device__ __forceinline__ int64_t lower32(const int32_t* a, int64_t lo, int64_t hi, int32_t v) {
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (a[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Entries of the old rows in balanced chunks (as entry_walk), with each
+// row's list offsets and destination fetched once per 64-row window by the
+// lane that owns the row and shuffled to its entries.
+__global__ __launch_bounds__(256) void k_merge_base(OldRows o, int32_t n, MergeLists L, const int32_t* dsv,
+                                                    const int32_t* tsv, const int64_t* nidx, int32_t* nsval,
+                                                    int32_t* nspos) {
+    GC c = grid_coords();
+    const int64_t pb = o.nindex[0], pe = o.nindex[n];
+    const int64_t ntask = (pe - pb + MF_ECHUNK - 1) / MF_ECHUNK;
+    for (int64_t t = c.wave; t < ntask; t += c.nwaves) {
+        int64_t p = pb + t * MF_ECHUNK;
+        const int64_t tend = p + MF_ECHUNK < pe ? p + MF_ECHUNK : pe;
+        int64_t lo = 0, hi = n - 1;
+        while (lo < hi) {
+            int64_t mid = (lo + hi + 1) >> 1;
+            if (o.nindex[mid] <= p) lo = mid; else hi = mid - 1;
+        }
+        int64_t x = lo;
+        while (p < tend && x < n) {
+            const int64_t myv = x + c.lane;
+            int64_t off = INT64_MAX, d0 = 0, t0 = 0, dst = 0;
+            int32_t dn = 0, tn = 0;
+            if (myv < n) {
+                off = o.nindex[myv];
+                d0 = L.doff[myv];
+                dn = (int32_t)(L.doff[myv + 1] - d0);
+                t0 = L.toff[myv];
+                tn = (int32_t)(L.toff[myv + 1] - t0);
+                dst = nidx[myv];
+            }
+            const int64_t xw = x + MF_WAVE < n ? x + MF_WAVE : n;
+            const int64_t wend = o.nindex[xw];
+            const int64_t lim = wend < tend ? wend : tend;
+            for (int64_t p0 = p; p0 < lim; p0 += MF_WAVE) {
+                const int64_t pp = p0 + c.lane;
+                const int64_t q0 = pp < lim ? pp : lim - 1;
+                int l = 0, h = MF_WAVE - 1;
+#pragma unroll
+                for (int s = 0; s < 6; s++) {
+                    int mid = (l + h + 1) >> 1;
+                    int64_t om = __shfl(off, mid, MF_WAVE);
+                    if (om <= q0) l = mid; else h = mid - 1;
+                }
+                const int64_t start = __shfl(off, l, MF_WAVE);
+                const int64_t rd0 = __shfl(d0, l, MF_WAVE), rt0 = __shfl(t0, l, MF_WAVE);
+                const int64_t rdst = __shfl(dst, l, MF_WAVE);
+                const int32_t rdn = __shfl(dn, l, MF_WAVE), rtn = __shfl(tn, l, MF_WAVE);
+                if (pp >= lim) continue;
+                const int64_t k = pp - start;
+                int64_t q;
+                int32_t v;
+                if (o.sval) {
+                    q = o.spos[pp];
+                    if (o.nlist[start + q] < 0) continue;  // deleted
+                    v = o.sval[pp];
+                } else {
+                    const int32_t e = o.nlist[pp];
+                    if (e < 0) continue;
+                    v = e;
+                    q = k;
+                }
+                int64_t at = k, np = q;
+                if (rdn) {
+                    at -= lower32(dsv, rd0, rd0 + rdn, v) - rd0;
+                    np -= lower32(L.dpos, rd0, rd0 + rdn, (int32_t)q) - rd0;
+                }
+                if (rtn) at += lower32(tsv, rt0, rt0 + rtn, v) - rt0;
+                nsval[rdst + at] = v;
+                nspos[rdst + at] = (int32_t)np;
+            }
+            if (lim > p) p = lim;
+            x = xw;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_merge_tail(OldRows o, int32_t n, MergeLists L, const int32_t* dsv,
+                                                    const int32_t* tsv, const int32_t* tsi, const int64_t* nidx,
+                                                    int32_t* nsval, int32_t* nspos) {
+    entry_walk(
+        L.toff, 0, n, [](int64_t) { return (int64_t)0; },
+        [&](int64_t p, int64_t x, int64_t start, int64_t) {
+            const int32_t w = tsv[p];
+            const int64_t b = o.nindex[x], e = o.nindex[x + 1];
+            int64_t lo = b, hi = e;  // old values below w
+            while (lo < hi) {
+                int64_t mid = (lo + hi) >> 1;
+                if (old_sorted_val(o, mid) < w) lo = mid + 1;
+                else hi = mid;
+            }
+            const int64_t d0 = L.doff[x], d1 = L.doff[x + 1];
+            const int64_t at = (lo - b) - (lower32(dsv, d0, d1, w) - d0) + (p - start);
+            const int64_t kept = (e - b) - (d1 - d0);
+            nsval[nidx[x] + at] = w;
+            nspos[nidx[x] + at] = (int32_t)(kept + tsi[p]);
+        });
+}
+
+// Per-row deletion and tail counts, and the rows with a tail listed (one
+// atomic per workgroup, as in k_newdeg).
+__global__ __launch_bounds__(256) void k_merge_counts(DevGraph g, int64_t* dcnt, int64_t* tcnt, int32_t* trows,
+                                                      u64* ntrows) {
+    __shared__ unsigned int cnt, cur;
+    __shared__ u64 base;
+    if (threadIdx.x == 0) {
+        cnt = 0;
+        cur = 0;
+    }
+    __syncthreads();
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    unsigned int mine = 0;
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < g.n; x += stride) {
+        dcnt[x] = g.ov_tomb[x];
+        int32_t a = g.ov_alive[x];
+        tcnt[x] = a;
+        mine += a ? 1u : 0u;
+    }
+    if (mine) atomicAdd(&cnt, mine);
+    __syncthreads();
+    if (threadIdx.x == 0 && cnt) base = atomicAdd(ntrows, (u64)cnt);
+    __syncthreads();
+    if (!cnt) return;
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < g.n; x += stride)
+        if (g.ov_alive[x]) trows[base + atomicAdd(&cur, 1u)] = (int32_t)x;
+}
+
+// Sorts the per-row lists of the rows listed in rows[0, *nrows) by rank
+// (values are distinct within a row: each entry goes to the number of
+// smaller ones).  A wave takes 64 listed rows at a time: a list of at most
+// MF_SORT_THREAD entries is ranked by its lane alone, one of at most
+// MF_SORT_WAVE by the whole wave, longer ones are listed in big[] for the
+// library's segmented sort.
+#define MF_SORT_THREAD 8
+#define MF_SORT_WAVE 256
+__global__ __launch_bounds__(256) void k_sort_small(const int64_t* off, const int32_t* rows, const u64* nrows,
+                                                    const int32_t* kin, const int32_t* vin, int32_t* kout,
+                                                    int32_t* vout, int32_t* big, u64* nbig) {
+    GC c = grid_coords();
+    const int64_t nr = (int64_t)*nrows;
+    for (int64_t base = c.wave * MF_WAVE; base < nr; base += c.nwaves * MF_WAVE) {
+        const int64_t i = base + c.lane;
+        int64_t b = 0, L = 0;
+        int32_t x = 0;
+        if (i < nr) {
+            x = rows[i];
+            b = off[x];
+            L = off[x + 1] - b;
+        }
+        if (L > MF_SORT_WAVE) big[atomicAdd(nbig, 1ull)] = x;
+        if (L <= MF_SORT_THREAD) {
+            for (int64_t a = 0; a < L; a++) {
+                const int32_t v = kin[b + a];
+                int64_t r = 0;
+                for (int64_t j = 0; j < L; j++) r += kin[b + j] < v ? 1 : 0;
+                kout[b + r] = v;
+                if (vout) vout[b + r] = vin[b + a];
+            }
+        }
+        u64 med = __ballot(L > MF_SORT_THREAD && L <= MF_SORT_WAVE);
+        while (med) {
+            const int src = __builtin_ctzll(med);
+            med &= med - 1;
+            const int64_t rb = __shfl(b, src, MF_WAVE);
+            const int64_t rl = __shfl(L, src, MF_WAVE);
+            for (int64_t i0 = 0; i0 < rl; i0 += MF_WAVE) {
+                const int64_t me = i0 + c.lane;
+                const int32_t v = me < rl ? kin[rb + me] : 0;
+                int64_t rank = 0;
+                for (int64_t j0 = 0; j0 < rl; j0 += MF_WAVE) {
+                    const int32_t u = j0 + c.lane < rl ? kin[rb + j0 + c.lane] : INT32_MAX;
+                    const int64_t lim = rl - j0 < MF_WAVE ? rl - j0 : MF_WAVE;
+                    for (int t = 0; t < (int)lim; t++) rank += __shfl(u, t, MF_WAVE) < v ? 1 : 0;
+                }
+                if (me < rl) {
+                    kout[rb + rank] = v;
+                    if (vout) vout[rb + rank] = vin[rb + me];
+                }
+            }
+        }
+    }
+}
+
+hipError_t mf_merge_counts(GraphState& S, int64_t* dcnt, int64_t* tcnt, int64_t* doff, int64_t* toff, int32_t* trows,
+                           u64* ntrows) {
+    k_fill<u64><<<1, 64, 0, S.stream>>>(ntrows, 0ull, 1);
+    k_merge_counts<<<S.blocks, 256, 0, S.stream>>>(S.g, dcnt, tcnt, trows, ntrows);
+    mf_scan_excl(dcnt, S.g.n, doff, S.d_scan_part, S.stream);
+    mf_scan_excl(tcnt, S.g.n, toff, S.d_scan_part, S.stream);
+    return hipGetLastError();
 }
 
 hipError_t mf_build_row_index(GraphState& S, int32_t* sval, int32_t* spos, int* d_flags) {
@@ -639,6 +898,76 @@ hipError_t mf_build_row_index(GraphState& S, int32_t* sval, int32_t* spos, int* 
     if (tmp) hipFree(tmp);
     hipFree(pos_in);
     if (e != hipSuccess) return e;
-    k_row_dups<<<S.blocks, 256, 0, st>>>(g.nindex, sval, g.n, g.m, d_flags);
+    k_check_row_index<<<S.blocks, 256, 0, st>>>(g.nindex, g.nlist, sval, spos, g.n, d_flags);
     return hipGetLastError();
+}
+
+// The sorted copy of compacted rows by merging (see k_merge_base): the
+// deletion and tail lists of each row are sorted (they are a few entries a
+// row), the old sorted order is merged with them entry by entry, and the
+// result is checked.  S describes the graph before compaction (its rows, the
+// deletions still marked, its sorted copy if it has one); nidx/nnl are the
+// compacted rows.  Lists of more than 2^31 entries are refused.
+namespace {
+// begin (end: past = 1) of the i-th listed row's slice of a list, for the library sort
+struct RowOffset {
+    const int64_t* off;
+    const int32_t* rows;
+    int past;
+    __host__ __device__ int operator()(const int& i) const { return (int)off[rows[i] + past]; }
+};
+
+// Sorts the per-row lists of the listed rows: short ones a wave each, the
+// rest with the library's segmented sort over just those rows.
+hipError_t sort_lists(GraphState& S, const int64_t* off, const int32_t* rows, const u64* nrows, int64_t total,
+                      const int32_t* kin, const int32_t* vin, int32_t* kout, int32_t* vout, const MergeLists& L) {
+    hipStream_t st = S.stream;
+    if (total == 0) return hipSuccess;
+    k_fill<u64><<<1, 64, 0, st>>>(L.nbig, 0ull, 1);
+    k_sort_small<<<S.blocks, 256, 0, st>>>(off, rows, nrows, kin, vin, kout, vout, L.big, L.nbig);
+    u64 nbig = 0;
+    hipError_t e = hipMemcpyAsync(&nbig, L.nbig, 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess || nbig == 0) return e;
+    rocprim::counting_iterator<int> idx(0);
+    rocprim::transform_iterator<rocprim::counting_iterator<int>, RowOffset, int> beg(idx, RowOffset{off, L.big, 0});
+    rocprim::transform_iterator<rocprim::counting_iterator<int>, RowOffset, int> end(idx, RowOffset{off, L.big, 1});
+    size_t need = 0;
+    if (vin) e = hipcub::DeviceSegmentedSort::SortPairs(nullptr, need, kin, kout, vin, vout, (int)total, (int)nbig,
+                                                         beg, end, st);
+    else e = hipcub::DeviceSegmentedSort::SortKeys(nullptr, need, kin, kout, (int)total, (int)nbig, beg, end, st);
+    if (e == hipSuccess && need > *L.tmp_bytes) {
+        if (*L.tmp) hipFree(*L.tmp);
+        *L.tmp = nullptr;
+        *L.tmp_bytes = 0;
+        e = hipMalloc(L.tmp, need);
+        if (e == hipSuccess) *L.tmp_bytes = need;
+    }
+    if (e == hipSuccess) {
+        size_t have = *L.tmp_bytes;
+        if (vin) e = hipcub::DeviceSegmentedSort::SortPairs(*L.tmp, have, kin, kout, vin, vout, (int)total, (int)nbig,
+                                                             beg, end, st);
+        else e = hipcub::DeviceSegmentedSort::SortKeys(*L.tmp, have, kin, kout, (int)total, (int)nbig, beg, end, st);
+    }
+    return e;
+}
+}  // namespace
+
+hipError_t mf_merge_row_index(GraphState& S, const MergeLists& L, int64_t ndel, int64_t ntail, const int64_t* nidx,
+                              const int32_t* nnl, int32_t* nsval, int32_t* nspos, int* d_flags) {
+    const DevGraph& g = S.g;
+    hipStream_t st = S.stream;
+    if (ndel > (int64_t)INT32_MAX || ntail > (int64_t)INT32_MAX) return hipErrorInvalidValue;
+    // rows with deletions are the compaction's tomb list
+    hipError_t e = sort_lists(S, L.doff, S.d_tomb_list, S.d_walk_ntasks, ndel, L.dval, nullptr, L.dsv, nullptr, L);
+    if (e == hipSuccess) e = sort_lists(S, L.toff, L.trows, L.ntrows, ntail, L.tval, L.tidx, L.tsv, L.tsi, L);
+    if (e == hipSuccess) {
+        OldRows o{g.nindex, g.nlist, g.sval, g.spos};
+        k_merge_base<<<S.blocks, 256, 0, st>>>(o, g.n, L, L.dsv, L.tsv, nidx, nsval, nspos);
+        if (ntail) k_merge_tail<<<S.blocks, 256, 0, st>>>(o, g.n, L, L.dsv, L.tsv, L.tsi, nidx, nsval, nspos);
+        k_check_row_index<<<S.blocks, 256, 0, st>>>(nidx, nnl, nsval, nspos, g.n, d_flags);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    return e;
 }
