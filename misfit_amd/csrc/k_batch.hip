@@ -1386,7 +1386,7 @@ void mf_launch_batch_prologue(const BP& P, const LaunchCfg& L) {
     LAUNCH_C(k_init, G_UPD, P);
     LAUNCH_C(k_claim, G_UPD, P);
     LAUNCH_C(k_resolve, G_UPD, P);
-    LAUNCH_C(k_append, G_UPD, P);
+    LAUNCH_C(k_append, G_WAVE_UPD, P);
     LAUNCH_C(k_partner, G_UPD, P);
     LAUNCH_C(k_kcount, G_WAVE_UPD, P);
     LAUNCH_C(k_pu_scan, G_SCAN, P);

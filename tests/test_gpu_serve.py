@@ -128,3 +128,39 @@ def test_admission_by_free_memory(oracle, engine_mod):
         assert st["updates"] == len(u)
         want, _, _ = H.oracle_replay(oracle, ni, nl, mis0, batches, completion=True, literal=False)
         assert np.array_equal(srv.get_mis(gid, len(ni) - 1), want[0])
+
+
+def test_concurrent_clients_and_graph_churn(oracle, engine_mod):
+    """Client threads submit to their own graphs at the same time while a
+    graph is removed and another added; every graph ends with the oracle's
+    MIS (one batch per graph in flight, four dispatcher threads)."""
+    import threading
+    work = [_workload(engine_mod, oracle, k, s, seed, nb=4, bs=2000)
+            for (k, s, seed) in [("er", 1 << 13, 51), ("rmat", 13, 52), ("rgg", 1 << 13, 53), ("er", 1 << 12, 54)]]
+    with engine_mod.Server(0, compact_fraction=0.0, max_inflight=1, workers=4) as srv:
+        gids = [srv.add_graph(ni, nl, mis0) for (ni, nl, mis0, _) in work[:3]]
+        errors = []
+
+        def client(gid, batches):
+            try:
+                reqs = [srv.submit(gid, engine_mod.make_updates(*b)) for b in batches]
+                for r in reqs:
+                    srv.wait(r)
+            except Exception as e:  # noqa: BLE001 - reported below
+                errors.append(e)
+
+        threads = [threading.Thread(target=client, args=(g, w[3])) for g, w in zip(gids, work[:3])]
+        for t in threads:
+            t.start()
+        # churn while the clients run: a fourth graph comes and goes, then comes back
+        g4 = srv.add_graph(*work[3][:3])
+        srv.remove_graph(g4)
+        g4 = srv.add_graph(*work[3][:3])
+        client(g4, work[3][3])
+        for t in threads:
+            t.join()
+        assert not errors, errors
+        for gid, (ni, nl, mis0, batches) in zip(gids + [g4], work):
+            want, _, _ = H.oracle_replay(oracle, ni, nl, mis0, batches, completion=True, literal=False)
+            assert np.array_equal(srv.get_mis(gid, len(ni) - 1), want[-1]), "graph %d" % gid
+            assert srv.check(gid) == (0, 0, 0)
