@@ -62,7 +62,6 @@ struct misfit_ctx {
     bool always_halt = false;     // every batch may halt (nothing is ever queued behind one)
     double window_factor = 16.0;  // MISFIT_WINDOW_FACTOR: window = this x expected edges per round
     int round_per_cu = 0;      // MISFIT_ROUND_WG_PER_CU: round grid (0: by occupancy)
-    int rounds_pos = 0;
     bool debug_rounds = false;
     uint32_t epoch = 0;
     uint32_t bytes_tag = 0;

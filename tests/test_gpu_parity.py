@@ -288,3 +288,55 @@ def test_unsorted_rows_parity(oracle, engine_mod, eng, mode):
     dup[ni[3] + 1] = dup[ni[3]]  # a row listing a neighbour twice is refused
     with pytest.raises(engine_mod.MisfitError):
         eng.load_csr(ni, dup)
+
+
+def _star_and_friends():
+    """(name, nindex, nlist, batches) of small corner-case graphs."""
+    cases = []
+    # no edges at all: a batch inserts some, the next deletes one again
+    ni = np.zeros(9, np.int64)
+    nl = np.zeros(0, np.int32)
+    cases.append(("empty graph", ni, nl, [
+        (np.array([0, 1, 2, 5], np.int32), np.array([1, 2, 3, 6], np.int32), np.ones(4, np.uint8)),
+        (np.array([1], np.int32), np.array([2], np.int32), np.zeros(1, np.uint8))]))
+    # one vertex: only an empty batch is possible
+    cases.append(("one vertex", np.zeros(2, np.int64), np.zeros(0, np.int32),
+                  [(np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0, np.uint8))]))
+    # a star (hub 0 with 1,200 leaves) plus isolated vertices 1201..1210:
+    # delete spokes, connect leaves to each other and to isolated vertices
+    n = 1211
+    rows = [list(range(1, 1201))] + [[0] for _ in range(1200)] + [[] for _ in range(10)]
+    ni = np.zeros(n + 1, np.int64)
+    for x in range(n):
+        ni[x + 1] = ni[x] + len(rows[x])
+    nl = np.array([y for r in rows for y in r], np.int32)
+    rng = np.random.default_rng(19)
+    dels = rng.choice(np.arange(1, 1201), 300, replace=False).astype(np.int32)
+    a = rng.integers(1, 1211, 400).astype(np.int32)
+    b = rng.integers(1, 1211, 400).astype(np.int32)
+    keep = a != b
+    a, b = a[keep], b[keep]
+    cases.append(("star", ni, nl, [
+        (np.zeros(300, np.int32), dels, np.zeros(300, np.uint8)),       # hub loses 300 spokes
+        (a, b, np.ones(len(a), np.uint8)),                               # leaf-leaf and isolated links
+        (np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0, np.uint8)),  # an empty batch
+        (np.zeros(50, np.int32), dels[:50], np.ones(50, np.uint8))]))    # 50 spokes come back
+    return cases
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("case", range(3))
+def test_corner_case_graphs(oracle, engine_mod, eng, mode, case):
+    """A graph without edges, a one-vertex graph, and a star with isolated
+    vertices, with empty batches among the others: the MIS after every batch
+    is the oracle's (flags as given: applied as insert-if-absent /
+    delete-if-present), and the full check passes."""
+    name, ni, nl, batches = _star_and_friends()[case]
+    mis0 = oracle.Graph.from_csr(ni, nl).greedy_mis()
+    want, _, _ = H.oracle_replay(oracle, ni, nl, mis0, batches)
+    got, stats, checks = H.engine_replay(engine_mod, eng, ni, nl, mis0, batches, mode=mode)
+    for k, (w, g) in enumerate(zip(want, got)):
+        assert np.array_equal(w, g), "%s: batch %d" % (name, k)
+    assert all(c == (0, 0, 0) for c in checks), name
+    for (u, _, _), st in zip(batches, stats):
+        assert st["updates"] == len(u)
