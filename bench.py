@@ -97,6 +97,10 @@ def committed_traffic(args):
     return None, None
 
 
+KERNEL_NOTE = ("queued batch (prologue, grid rounds, finisher): device time of the timed region by a pair of HIP "
+               "events on the engine's stream, divided by the batches in it")
+
+
 def pct(v, q):
     return float(np.percentile(np.asarray(v, np.float64), q)) if len(v) else 0.0
 
@@ -146,6 +150,7 @@ def run_served(M, eng, ni, nl, mis0, ups, W, K, batch, depth=2):
         eng.process_batch(ups[b * batch:(b + 1) * batch])
     eng.sync()
     starts, lat, dev, tickets, stats = {}, [], [], {}, []
+    eng.timer_begin()  # HIP event on the engine's stream: the timed region's device time
     t0 = time.perf_counter()
     nxt = W
     for b in range(W, W + K):
@@ -159,9 +164,10 @@ def run_served(M, eng, ni, nl, mis0, ups, W, K, batch, depth=2):
         stats.append(st)
     eng.sync()
     wall = time.perf_counter() - t0
+    region_ms = eng.timer_end()
     conf, unc, bad = eng.check()
     return {"lat_ms": lat, "dev_ms": dev, "stats": stats, "wall_s": wall, "mis": eng.get_mis(),
-            "check": (conf, unc, bad)}
+            "check": (conf, unc, bad), "region_ms": region_ms}
 
 
 def cpu_baseline(M, eng, ni, nl, mis0, ups, budget_s):
@@ -330,13 +336,14 @@ def single_graph_line(M, args, name, ni, nl, mis0, u, v, W, K, B, index_bytes, c
     served_ups = K * B / served["wall_s"]
     plain_ups = K * B / plain["wall_s"]
     tot_bytes = float(np.sum(plain["bytes"]))
-    ach = tot_bytes / (float(np.sum(served["dev_ms"])) / 1e3) / 1e9
+    ach = tot_bytes / (served["region_ms"] / 1e3) / 1e9
     roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None, "traffic_source": None,
-            "kernel": "queued batch (prologue, grid rounds, finisher) timed per batch with HIP events",
+            "kernel": KERNEL_NOTE,
             "algorithmic_bytes_per_batch": round(tot_bytes / K, 1),
             "algorithmic_bytes_per_update": round(tot_bytes / (K * B), 2),
-            "kernel_ms_mean": round(float(np.mean(served["dev_ms"])), 4),
+            "kernel_ms_mean": round(served["region_ms"] / K, 4),
+            "batch_device_clock_ms_mean": round(float(np.mean(served["dev_ms"])), 4),
             "stream_copy_gbs": copy["copy_gbs"] if copy else None,
             "frac_of_stream_copy": round(ach / copy["copy_gbs"], 5) if copy else None}
     cfg = {"workload": workload, "name": name, "graph_entries": int(len(nl)), "batch_updates": B,
@@ -707,15 +714,16 @@ def main():
     roof = None
     if plain is not None and plain["bytes"]:
         tot_bytes = float(np.sum(plain["bytes"]))
-        tot_s = float(np.sum(served["dev_ms"])) / 1e3
+        tot_s = served["region_ms"] / 1e3
         ach = tot_bytes / tot_s / 1e9
         traffic, traffic_src = committed_traffic(args)
         roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": traffic_src,
-                "kernel": "queued batch (prologue, grid rounds, finisher) timed per batch with HIP events",
+                "kernel": KERNEL_NOTE,
                 "algorithmic_bytes_per_batch": round(tot_bytes / K, 1),
                 "algorithmic_bytes_per_update": round(tot_bytes / (K * B), 2),
-                "kernel_ms_mean": round(float(np.mean(served["dev_ms"])), 4),
+                "kernel_ms_mean": round(served["region_ms"] / K, 4),
+                "batch_device_clock_ms_mean": round(float(np.mean(served["dev_ms"])), 4),
                 "stream_copy_gbs": copy["copy_gbs"] if copy else None,
                 "frac_of_stream_copy": round(ach / copy["copy_gbs"], 5) if copy else None}
     cfg = {

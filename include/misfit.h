@@ -61,7 +61,7 @@ typedef struct {
     int64_t admissible;       /* ... of which have a row */
     int64_t delta_ops;        /* effective overlay operations */
     int64_t overlay_entries;  /* pool slots in use after the batch */
-    double device_ms;         /* batch time on the device (events) */
+    double device_ms;         /* batch time on the device (device wall clock, first kernel to end of tail) */
     double prologue_ms;       /* ... of which the overlay update, may-uncover set and admissible index */
 } misfit_batch_stats;
 
@@ -138,6 +138,16 @@ int misfit_process_batch(misfit_ctx* ctx, const misfit_update* ups, int64_t coun
 /* Waits for all queued work.  Replaces synchronizeChannels
  * (src/misfit/misfit.cu:464-466). */
 int misfit_sync(misfit_ctx* ctx);
+
+/* Device time of a region of the context's stream, by a pair of HIP events:
+ * misfit_timer_begin records the first where the stream stands now;
+ * misfit_timer_end records the second, waits for it and returns the elapsed
+ * milliseconds.  A measurement aid in the place of the reference's per-stage
+ * event pairs (src/misfit/misfit.cu:612-650): per-batch events on the stream
+ * cost the device several microseconds each, so batch times come from the
+ * device clock (misfit_batch_stats.device_ms) and whole runs from these. */
+int misfit_timer_begin(misfit_ctx* ctx);
+int misfit_timer_end(misfit_ctx* ctx, double* ms);
 
 /* Full check over base plus overlay: conflicts = edges with both ends in the
  * MIS, uncovered = non-members without an MIS neighbour, count_errors =

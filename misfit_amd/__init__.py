@@ -91,7 +91,7 @@ EXPORTS = (
     "misfit_compact", "misfit_graph_size", "misfit_export_csr", "misfit_neighbors",
     "misfit_batch_bytes", "misfit_device_info", "misfit_egr_open", "misfit_egr_close", "misfit_write_egr",
     "misfit_gen_er", "misfit_gen_rmat", "misfit_gen_rgg", "misfit_greedy_mis", "misfit_gen_updates",
-    "misfit_mtx_to_egr", "misfit_egr_check", "misfit_poll",
+    "misfit_mtx_to_egr", "misfit_egr_check", "misfit_poll", "misfit_timer_begin", "misfit_timer_end",
     "misfit_server_create", "misfit_server_destroy", "misfit_server_last_error", "misfit_server_add_graph",
     "misfit_server_remove_graph", "misfit_server_submit", "misfit_server_wait", "misfit_server_get_mis",
     "misfit_server_check",
@@ -123,6 +123,8 @@ def lib():
         L.misfit_wait.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(BatchStats)]
         L.misfit_process_batch.argtypes = [vp, vp, ctypes.c_int64, ctypes.POINTER(BatchStats)]
         L.misfit_sync.argtypes = [vp]
+        L.misfit_timer_begin.argtypes = [vp]
+        L.misfit_timer_end.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
         L.misfit_check.argtypes = [vp, _i64p, _i64p, _i64p]
         L.misfit_compact.argtypes = [vp, _i64p]
         L.misfit_graph_size.argtypes = [vp, _i32p, _i64p, _i64p]
@@ -414,6 +416,17 @@ class Engine:
 
     def sync(self):
         self._ck(lib().misfit_sync(self.ctx))
+
+    def timer_begin(self):
+        """Records a HIP event where the engine's stream stands now."""
+        self._ck(lib().misfit_timer_begin(self.ctx))
+
+    def timer_end(self):
+        """Records a second event after all queued work, waits for it and
+        returns the milliseconds since timer_begin's event."""
+        ms = ctypes.c_double()
+        self._ck(lib().misfit_timer_end(self.ctx, ctypes.byref(ms)))
+        return ms.value
 
     def check(self):
         c, u, e = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()

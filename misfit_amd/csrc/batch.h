@@ -43,6 +43,7 @@ enum BatchCounter {
     C_KTASK,         // row chunks of members that may be evicted
     C_GUARD,         // a loop bound was hit or a list overflowed (engine bug): results invalid
     C_HALT,          // the tail stopped with many edges left: the host queues more rounds
+    C_T0, C_TP, C_T1,  // device wall clock: batch start, first round, tail end
     C_NCOUNTERS
 };
 
@@ -55,6 +56,11 @@ struct BP {
     const int32_t* bu;
     const int32_t* bv;
     const uint8_t* bins;
+    // the batch as handed over (pinned host memory, read by the first
+    // kernel into bu/bv/bins), or null when bu/bv/bins already hold it
+    const int32_t* hu;
+    const int32_t* hv;
+    const uint8_t* hins;
     int64_t U;
     uint32_t b;        // batch epoch, >= 1
     int completion;
@@ -67,6 +73,12 @@ struct BP {
     int32_t* bt_nxt;
     int32_t* bt_slot;
     u64 bt_mask;
+    // pre-batch adjacency of update i's pair, searched beside the claim:
+    // positions of hi in row lo and of lo in row hi (-1: not in the base),
+    // bit 0 = live overlay entry, bit 1 = base entry deleted
+    int64_t* bt_plo;
+    int64_t* bt_phi;
+    uint8_t* bt_fl;
     // new overlay entries
     int32_t* ne_x;
     int32_t* ne_y;
@@ -116,7 +128,7 @@ struct LaunchCfg {
 };
 void mf_launch_batch_prologue(const BP& P, const LaunchCfg& L);
 int mf_round_grid(const LaunchCfg& L);  // workgroups of the speculation/validation kernels
-void mf_launch_round(const BP& P, const LaunchCfg& L, int par);
+void mf_launch_round(const BP& P, const LaunchCfg& L, int par, bool first = false);  // first: stamps C_TP
 void mf_launch_finish(const BP& P, const LaunchCfg& L, int par);
 void mf_launch_completion_collect(const BP& P, const LaunchCfg& L);
 void mf_launch_stats(const BP& P, const LaunchCfg& L);

@@ -34,8 +34,10 @@ struct Slot {
     int32_t* d_v = nullptr;
     uint8_t* d_ins = nullptr;
     u64* h_ctr = nullptr;
-    hipEvent_t uploaded = nullptr, done = nullptr, t0 = nullptr, t1 = nullptr;
-    hipEvent_t tp = nullptr;  // end of the batch prologue
+    const int32_t* g_u = nullptr;  // the staging buffers as the device addresses them
+    const int32_t* g_v = nullptr;
+    const uint8_t* g_ins = nullptr;
+    hipEvent_t done = nullptr;
     int64_t count = 0;
     uint64_t ticket = 0;
     bool busy = false;
@@ -61,6 +63,8 @@ struct misfit_ctx {
     int hist_pos = 0;
     bool always_halt = false;     // every batch may halt (nothing is ever queued behind one)
     double window_factor = 16.0;  // MISFIT_WINDOW_FACTOR: window = this x expected edges per round
+    double clock_khz = 0.0;       // device wall clock, for per-batch device times
+    hipEvent_t timer0 = nullptr, timer1 = nullptr;  // misfit_timer_begin/end
     int round_per_cu = 0;      // MISFIT_ROUND_WG_PER_CU: round grid (0: by occupancy)
     bool debug_rounds = false;
     uint32_t epoch = 0;
@@ -174,16 +178,17 @@ extern "C" int misfit_ctx_create(int device, misfit_ctx** out) {
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
     c->S.blocks = std::max(64, cus * 8);
     c->S.stream = c->stream;
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+        c->clock_khz = (double)khz;
+    hipEventCreate(&c->timer0);
+    hipEventCreate(&c->timer1);
     if (hipHostMalloc((void**)&c->h_small, C_NCOUNTERS * MF_CS * sizeof(u64), hipHostMallocDefault) != hipSuccess) {
         delete c;
         return MISFIT_E_NOMEM;
     }
     for (auto& s : c->slots) {
-        hipEventCreateWithFlags(&s.uploaded, hipEventDisableTiming);
         hipEventCreateWithFlags(&s.done, hipEventDisableTiming);
-        hipEventCreate(&s.t0);
-        hipEventCreate(&s.t1);
-        hipEventCreate(&s.tp);
         // written by the batch's last kernel, read after its event
         hipHostMalloc((void**)&s.h_ctr, (C_NCOUNTERS + 1) * sizeof(u64), hipHostMallocCoherent);
     }
@@ -200,6 +205,7 @@ static void free_slots(misfit_ctx* c) {
         if (s.h_v) hipHostFree(s.h_v);
         if (s.h_ins) hipHostFree(s.h_ins);
         s.h_u = nullptr; s.h_v = nullptr; s.h_ins = nullptr;
+        s.g_u = nullptr; s.g_v = nullptr; s.g_ins = nullptr;
         s.d_u = nullptr; s.d_v = nullptr; s.d_ins = nullptr;
     }
 }
@@ -212,12 +218,10 @@ extern "C" int misfit_ctx_destroy(misfit_ctx* c) {
     free_slots(c);
     for (auto& s : c->slots) {
         if (s.h_ctr) hipHostFree(s.h_ctr);
-        hipEventDestroy(s.uploaded);
         hipEventDestroy(s.done);
-        hipEventDestroy(s.t0);
-        hipEventDestroy(s.t1);
-        hipEventDestroy(s.tp);
     }
+    hipEventDestroy(c->timer0);
+    hipEventDestroy(c->timer1);
     free_list(c->batch_allocs);
     free_list(c->graph_allocs);
     free_compact_scratch(c);
@@ -253,6 +257,25 @@ extern "C" int misfit_sync(misfit_ctx* c) {
     return MISFIT_OK;
 }
 
+extern "C" int misfit_timer_begin(misfit_ctx* c) {
+    if (!c) return MISFIT_E_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipEventRecord(c->timer0, c->stream));
+    return MISFIT_OK;
+}
+
+extern "C" int misfit_timer_end(misfit_ctx* c, double* ms) {
+    if (!c || !ms) return MISFIT_E_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (int rc_ = settle(c)) return rc_;
+    HIPCHK(c, hipEventRecord(c->timer1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->timer1));
+    float f = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&f, c->timer0, c->timer1));
+    *ms = (double)f;
+    return MISFIT_OK;
+}
+
 // ---------------------------------------------------------------- capacity
 // Per-batch scratch for batches of up to U updates.
 static int ensure_batch_capacity(misfit_ctx* c, int64_t U) {
@@ -269,6 +292,9 @@ static int ensure_batch_capacity(misfit_ctx* c, int64_t U) {
     HIPCHK(c, dalloc(A, &P.bt_head, (int64_t)c->bt_size));
     HIPCHK(c, dalloc(A, &P.bt_nxt, cap));
     HIPCHK(c, dalloc(A, &P.bt_slot, cap));
+    HIPCHK(c, dalloc(A, &P.bt_plo, cap));
+    HIPCHK(c, dalloc(A, &P.bt_phi, cap));
+    HIPCHK(c, dalloc(A, &P.bt_fl, cap));
     HIPCHK(c, dalloc(A, &P.ne_x, 2 * cap));
     HIPCHK(c, dalloc(A, &P.ne_y, 2 * cap));
     HIPCHK(c, dalloc(A, &P.ne_seq, 2 * cap));
@@ -299,6 +325,13 @@ static int ensure_batch_capacity(misfit_ctx* c, int64_t U) {
         HIPCHK(c, hipHostMalloc((void**)&s.h_u, cap * sizeof(int32_t), hipHostMallocDefault));
         HIPCHK(c, hipHostMalloc((void**)&s.h_v, cap * sizeof(int32_t), hipHostMallocDefault));
         HIPCHK(c, hipHostMalloc((void**)&s.h_ins, cap, hipHostMallocDefault));
+        void* dp = nullptr;
+        HIPCHK(c, hipHostGetDevicePointer(&dp, s.h_u, 0));
+        s.g_u = (const int32_t*)dp;
+        HIPCHK(c, hipHostGetDevicePointer(&dp, s.h_v, 0));
+        s.g_v = (const int32_t*)dp;
+        HIPCHK(c, hipHostGetDevicePointer(&dp, s.h_ins, 0));
+        s.g_ins = (const uint8_t*)dp;
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->U_cap = cap;
@@ -589,22 +622,23 @@ static void fill_stats(const misfit_ctx* c, const Slot& s, misfit_batch_stats* s
     st->admissible = (int64_t)k[C_ADM];
     st->delta_ops = (int64_t)k[C_DELTA_OPS];
     st->overlay_entries = (int64_t)k[C_NCOUNTERS];
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, s.t0, s.t1) == hipSuccess) st->device_ms = ms;
-    if (hipEventElapsedTime(&ms, s.t0, s.tp) == hipSuccess) st->prologue_ms = ms;
-    (void)c;
+    // device wall clock (kHz ticks per ms): first kernel start to the end of
+    // the tail, and to the start of the first round
+    if (c->clock_khz > 0 && k[C_T1] >= k[C_T0]) {
+        st->device_ms = (double)(k[C_T1] - k[C_T0]) / c->clock_khz;
+        st->prologue_ms = k[C_TP] >= k[C_T0] ? (double)(k[C_TP] - k[C_T0]) / c->clock_khz : st->device_ms;
+    }
 }
 
 // Host-driven rounds: one round = 4 kernels, then the host reads where the
 // committed prefix ends; the completion iterations likewise.
-static int drive_plain(misfit_ctx* c, BP& P, const LaunchCfg& L, hipEvent_t tp) {
+static int drive_plain(misfit_ctx* c, BP& P, const LaunchCfg& L) {
     mf_launch_batch_prologue(P, L);
-    HIPCHK(c, hipEventRecord(tp, L.stream));
     for (uint32_t r = 0;; r++) {
         if ((int64_t)r > P.U + 1) return fail(c, MISFIT_E_CHECK, "speculative rounds made no progress");
         int par = (int)(r & 1u);
         if (P.U == 0) break;
-        mf_launch_round(P, L, par);
+        mf_launch_round(P, L, par, r == 0);
         HIPCHK(c, hipMemcpyAsync(c->h_small, P.ctr, C_NCOUNTERS * MF_CS * sizeof(u64), hipMemcpyDeviceToHost,
                                  c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -636,10 +670,9 @@ static int drive_plain(misfit_ctx* c, BP& P, const LaunchCfg& L, hipEvent_t tp) 
 
 // The queued sequence of one batch: prologue, grid-wide rounds, then the
 // one-workgroup tail (leftover rounds, counters, completion, publication).
-static void launch_queued(const BP& P, const LaunchCfg& L, int k, hipEvent_t tp) {
+static void launch_queued(const BP& P, const LaunchCfg& L, int k) {
     mf_launch_batch_prologue(P, L);
-    hipEventRecord(tp, L.stream);
-    for (int r = 0; r < k; r++) mf_launch_round(P, L, r & 1);
+    for (int r = 0; r < k; r++) mf_launch_round(P, L, r & 1, r == 0);
     mf_launch_finish(P, L, k & 1);
 }
 
@@ -721,8 +754,8 @@ static RoundPlan plan_rounds(misfit_ctx* c, int64_t count, int64_t full_window) 
 }
 
 // Whole batch enqueued with no host synchronisation inside it.
-static int drive_queued(misfit_ctx* c, BP& P, const LaunchCfg& L, int k, hipEvent_t tp) {
-    launch_queued(P, L, k, tp);
+static int drive_queued(misfit_ctx* c, BP& P, const LaunchCfg& L, int k) {
+    launch_queued(P, L, k);
     HIPCHK(c, hipGetLastError());
     return MISFIT_OK;
 }
@@ -734,7 +767,7 @@ static int drive_queued(misfit_ctx* c, BP& P, const LaunchCfg& L, int k, hipEven
 // few rounds is noisy, and doubling bounds the empty rounds to half.  The
 // window follows the batch's measured edges per round like a planned one
 // (rounds are independent of each other's window).  Then another tail, and
-// the batch's end events are recorded again.  Nothing else of this graph
+// the batch's end event is recorded again.  Nothing else of this graph
 // was queued behind it.
 static int continue_batch(misfit_ctx* c, Slot& s) {
     const u64* h = s.h_ctr;
@@ -759,7 +792,6 @@ static int continue_batch(misfit_ctx* c, Slot& s) {
     s.par_next = (s.par_next + K) & 1;
     mf_launch_finish(s.P, s.L, s.par_next);
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipEventRecord(s.t1, c->stream));
     HIPCHK(c, hipEventRecord(s.done, c->stream));
     return MISFIT_OK;
 }
@@ -864,13 +896,6 @@ extern "C" int misfit_submit_batch(misfit_ctx* c, const misfit_update* ups, int6
     s.count = count;
     s.ticket = t;
     s.busy = true;
-    if (count) {
-        HIPCHK(c, hipMemcpyAsync(s.d_u, s.h_u, count * 4, hipMemcpyHostToDevice, c->copy_stream));
-        HIPCHK(c, hipMemcpyAsync(s.d_v, s.h_v, count * 4, hipMemcpyHostToDevice, c->copy_stream));
-        HIPCHK(c, hipMemcpyAsync(s.d_ins, s.h_ins, count, hipMemcpyHostToDevice, c->copy_stream));
-    }
-    HIPCHK(c, hipEventRecord(s.uploaded, c->copy_stream));
-    HIPCHK(c, hipStreamWaitEvent(c->stream, s.uploaded, 0));
     c->ins_bound += nins;
     c->pool_bound += 8 * nins;
     c->epoch++;
@@ -879,6 +904,11 @@ extern "C" int misfit_submit_batch(misfit_ctx* c, const misfit_update* ups, int6
     P.bu = s.d_u;
     P.bv = s.d_v;
     P.bins = s.d_ins;
+    // the first kernel reads the staged batch (the slot is not written again
+    // before this batch's end event)
+    P.hu = count ? s.g_u : nullptr;
+    P.hv = count ? s.g_v : nullptr;
+    P.hins = count ? s.g_ins : nullptr;
     P.U = count;
     P.b = c->epoch;
     P.completion = c->completion;
@@ -903,12 +933,11 @@ extern "C" int misfit_submit_batch(misfit_ctx* c, const misfit_update* ups, int6
     s.par_next = k & 1;
     s.may_halt = !sure;
     c->last_U = count;
-    HIPCHK(c, hipEventRecord(s.t0, c->stream));
-    rc = queued ? drive_queued(c, P, L, k, s.tp) : drive_plain(c, P, L, s.tp);
+    rc = queued ? drive_queued(c, P, L, k) : drive_plain(c, P, L);
     if (rc) return rc;
-    // the counters and the pool top reached s.h_ctr from the batch's last
-    // kernel (the finisher, or the publish kernel of the plain driver)
-    HIPCHK(c, hipEventRecord(s.t1, c->stream));
+    // the counters, the pool top and the device clock stamps reached s.h_ctr
+    // from the batch's last kernel (the finisher, or the publish kernel of
+    // the plain driver)
     HIPCHK(c, hipEventRecord(s.done, c->stream));
     if (ticket) *ticket = t;
     return MISFIT_OK;

@@ -184,6 +184,45 @@ __device__ __forceinline__ int64_t base_find(const DevGraph& g, int32_t x, int32
     return -1;
 }
 
+// base_find by a group of G lanes of one wave (lanes gbase .. gbase+G-1,
+// `gl` = lane - gbase; the whole group calls it together and gets the same
+// result): a (G+1)-ary search that probes G positions per step, so a hub row
+// of 2^17 entries takes 6 dependent loads at G = 8 instead of 17.
+template <int G>
+__device__ __forceinline__ int64_t group_find(const DevGraph& g, int32_t x, int32_t y, int gl, int gbase) {
+    const u64 gmask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
+    const int64_t b = g.nindex[x], e = g.nindex[x + 1];
+    if (!g.sorted && !g.sval) {
+        for (int64_t p0 = b; p0 < e; p0 += G) {
+            const int64_t p = p0 + gl;
+            const bool hit = p < e && decode(g.nlist[p]) == y;
+            const u64 m = (__ballot(hit) >> gbase) & gmask;
+            if (m) return p0 + __ffsll((long long)m) - 1;
+        }
+        return -1;
+    }
+    // the first index of the sorted view holding a value >= y lies in [lo, hi]
+    // (hi = e: none)
+    int64_t lo = b, hi = e;
+    while (hi - lo >= G) {
+        const int64_t span = hi - lo;
+        const int64_t p = lo + span * (gl + 1) / (G + 1);  // lo <= p < hi
+        const int32_t d = g.sorted ? decode(g.nlist[p]) : g.sval[p];
+        const u64 m = (__ballot(d < y) >> gbase) & gmask;  // probes ascend: a prefix is below y
+        const int cnt = __popcll(m);
+        const int64_t nlo = cnt == 0 ? lo : lo + span * cnt / (G + 1) + 1;
+        const int64_t nhi = cnt == G ? hi : lo + span * (cnt + 1) / (G + 1);
+        lo = nlo;
+        hi = nhi;
+    }
+    const int64_t p = lo + gl;  // at most G candidates: lo .. hi, below e
+    const bool hit = p <= hi && p < e && (g.sorted ? decode(g.nlist[p]) : g.sval[p]) == y;
+    const u64 m = (__ballot(hit) >> gbase) & gmask;
+    if (!m) return -1;
+    const int64_t q = lo + __ffsll((long long)m) - 1;
+    return g.sorted ? q : b + g.spos[q];
+}
+
 // Position of y inside base row x and of x inside base row y (or -1): the
 // two binary searches advance in lockstep so their dependent load chains
 // overlap.

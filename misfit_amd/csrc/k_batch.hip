@@ -184,9 +184,46 @@ __device__ __forceinline__ void round_count_add(u64* w, uint32_t R) {
 }
 
 // ---------------------------------------------------------------- prologue
-// P1: claim pairs, tag endpoints and possible evictees.
+// P1b: the pair's adjacency before the batch (base positions, overlay
+// state), for P2.  Nothing of the graph changes before P2, so these
+// dependent searches run in waves of their own beside the claims instead of
+// after them: 16 lanes per update, 8 searching each row (group_find).
+#define MF_PAIR_LANES 16
+__device__ __forceinline__ void search_pair(const BP& P, int64_t i, int gl) {
+    const DevGraph& g = P.g;
+    const int32_t a = P.bu[i], bb = P.bv[i];
+    const int32_t lo = a < bb ? a : bb, hi = a < bb ? bb : a;
+    const bool first = gl < MF_PAIR_LANES / 2;
+    const int gbase = lane_id() & ~(MF_PAIR_LANES / 2 - 1);
+    const int64_t p = group_find<MF_PAIR_LANES / 2>(g, first ? lo : hi, first ? hi : lo, gl & (MF_PAIR_LANES / 2 - 1), gbase);
+    const int64_t phi = __shfl((long long)p, lane_id() + MF_PAIR_LANES / 2, MF_WAVE);
+    if (gl != 0) return;
+    const int64_t plo = p;
+    uint8_t fl = 0;
+    if (plo >= 0) fl = g.nlist[plo] < 0 ? 2 : 0;
+    else fl = ph_lookup(g, pair_key(lo, hi)) == 1 ? 1 : 0;
+    P.bt_plo[i] = plo;
+    P.bt_phi[i] = phi;
+    P.bt_fl[i] = fl;
+}
+
+// Threads of the claim kernel: [0, U) claim, the searches start at the next
+// wave boundary (MF_PAIR_LANES per update).
+__host__ __device__ __forceinline__ int64_t claim_search_base(int64_t U) { return (U + MF_WAVE - 1) / MF_WAVE * MF_WAVE; }
+__host__ __device__ __forceinline__ int64_t claim_threads(int64_t U) { return claim_search_base(U) + MF_PAIR_LANES * U; }
+
+// P1: claim pairs, tag endpoints and possible evictees, and search every
+// update's pair.  A search group's lanes share one wave and one loop
+// iteration: thread t is lane t mod 64 of its wave (blocks are whole waves).
 __device__ void ph_claim(const BP& P, const GC& c) {
-    for (int64_t i = c.tid; i < P.U; i += c.nthreads) {
+    const int64_t s0 = claim_search_base(P.U), end = claim_threads(P.U);
+    for (int64_t t = c.tid; t < end; t += c.nthreads) {
+        if (t >= s0) {
+            search_pair(P, (t - s0) / MF_PAIR_LANES, (int)((t - s0) % MF_PAIR_LANES));
+            continue;
+        }
+        if (t >= P.U) continue;
+        const int64_t i = t;
         int32_t a = P.bu[i], bb = P.bv[i];
         u64 key = pair_key(a, bb);
         u64 s = mix64(key) & P.bt_mask;
@@ -242,12 +279,12 @@ __device__ void ph_resolve(const BP& P, const GC& c) {
         if (P.bt_head[slot] != (int32_t)i) continue;
         int32_t a = P.bu[i], bb = P.bv[i];
         int32_t lo = a < bb ? a : bb, hi = a < bb ? bb : a;
-        int64_t plo, phi;
-        base_find_pair(g, lo, hi, plo, phi);
+        const int64_t plo = P.bt_plo[i], phi = P.bt_phi[i];
+        const uint8_t fl = P.bt_fl[i];
         bool in_base = plo >= 0;
-        bool tomb0 = in_base && g.nlist[plo] < 0;
+        bool tomb0 = (fl & 2) != 0;
         u64 key = pair_key(lo, hi);
-        bool alive0 = !in_base && ph_lookup(g, key) == 1;
+        bool alive0 = (fl & 1) != 0;
         bool adj0 = (in_base && !tomb0) || alive0;
         bool tomb = tomb0, alive = alive0, adj = adj0, old_killed = false;
         int32_t last_ins = -1;
@@ -314,16 +351,31 @@ __device__ void ph_resolve(const BP& P, const GC& c) {
 __device__ void ph_append(const BP& P, const GC& c) {
     const DevGraph& g = P.g;
     int64_t ne = 2 * P.U;
-    for (int64_t k = c.tid; k < ne; k += c.nthreads) {
-        int32_t x = P.ne_x[k];
-        if (x < 0 || g.ne_head[x] != (int32_t)k) continue;
-        int32_t cnt = 0;
-        for (int32_t j = (int32_t)k; j != -1; j = P.ne_nxt[j]) cnt++;
-        int32_t len = g.ov_len[x], cap = g.ov_cap[x];
-        if (len + cnt > cap) {
-            int32_t nc = 2;
-            while (nc < len + cnt) nc <<= 1;
-            u64 st = atomicAdd(g.pool_top, (u64)nc);
+    // whole waves step together so the pool reservation below is one atomic
+    // per wave (most appending vertices need a new run, and they all
+    // reserve from one counter)
+    for (int64_t k0 = c.tid - c.lane; k0 < ne; k0 += c.nthreads) {
+        const int64_t k = k0 + c.lane;
+        int32_t x = k < ne ? P.ne_x[k] : -1;
+        const bool head = x >= 0 && g.ne_head[x] == (int32_t)k;
+        int32_t cnt = 0, len = 0, cap = 0, nc = 0;
+        if (head) {
+            for (int32_t j = (int32_t)k; j != -1; j = P.ne_nxt[j]) cnt++;
+            len = g.ov_len[x];
+            cap = g.ov_cap[x];
+            if (len + cnt > cap) {
+                nc = 2;
+                while (nc < len + cnt) nc <<= 1;
+            }
+        }
+        const int64_t incl = wave_incl_scan(nc);
+        const int64_t total = __shfl(incl, MF_WAVE - 1, MF_WAVE);
+        u64 base = 0;
+        if (c.lane == 0 && total) base = atomicAdd(g.pool_top, (u64)total);
+        base = (u64)__shfl((long long)base, 0, MF_WAVE);
+        if (!head) continue;
+        if (nc) {
+            const u64 st = base + (u64)(incl - nc);
             if ((int64_t)(st + nc) > g.pool_cap) {
                 atomicExch(&P.ctr[MF_CS * (C_POOL_OVF)], 1ull);
                 g.ne_head[x] = -1;
@@ -1150,11 +1202,27 @@ __device__ void ph_stats(const BP& P, const GC& c) {
 }
 
 // ---------------------------------------------------------------- kernels
+// Counters, the batch start on the device clock, and the batch itself from
+// the pinned staging buffers (9 bytes per update over PCIe, read here rather
+// than copied on another stream: a cross-stream wait costs more than these
+// reads).
 __device__ void ph_init(const BP& P, const GC& c) {
-    for (int64_t i = c.tid; i < C_NCOUNTERS; i += c.nthreads) P.ctr[MF_CS * (i)] = 0;
+    for (int64_t i = c.tid; i < C_NCOUNTERS; i += c.nthreads)
+        if (i != C_T0) P.ctr[MF_CS * (i)] = 0;
     if (c.tid == 0) {
+        P.ctr[MF_CS * (C_T0)] = wall_clock64();
         P.ctr[MF_CS * (C_CONF0)] = ~0ull;
         P.ctr[MF_CS * (C_CONF1)] = ~0ull;
+    }
+    if (P.hu) {
+        int32_t* du = const_cast<int32_t*>(P.bu);
+        int32_t* dv = const_cast<int32_t*>(P.bv);
+        uint8_t* di = const_cast<uint8_t*>(P.bins);
+        for (int64_t i = c.tid; i < P.U; i += c.nthreads) {
+            du[i] = P.hu[i];
+            dv[i] = P.hv[i];
+            di[i] = P.hins[i];
+        }
     }
 }
 
@@ -1188,9 +1256,14 @@ MF_KERNEL(k_comp_collect, ph_comp_collect(P, c))
 MF_KERNEL(k_comp_decide, ph_comp_decide(P, c))
 MF_KERNEL(k_stats, ph_stats(P, c))
 
+// par bit 1: the batch's first round, which stamps the end of the prologue.
 __global__ __launch_bounds__(256) void k_spec(BP P, int par) {
     __shared__ int cur[256 / MF_WAVE];
     GC c = grid_coords();
+    if (par & 2) {
+        if (c.tid == 0) P.ctr[MF_CS * (C_TP)] = wall_clock64();
+        par &= 1;
+    }
     RoundView v = round_view(P, par, P.window);
     if (v.live) ph_spec_w(P, c, v, cur);
 }
@@ -1224,10 +1297,11 @@ __device__ __forceinline__ void phase_barrier() {
     asm volatile("buffer_inv sc0" ::: "memory");  // drop this CU's L1 lines
 }
 
-// Copies the counters and the pool top to the pinned host block of the
-// batch (one workgroup), so the host reads the batch's results without a
-// copy on the stream.
+// Stamps the end of the batch on the device clock, then copies the counters
+// and the pool top to the pinned host block of the batch (one workgroup), so
+// the host reads the batch's results without a copy on the stream.
 __device__ void publish(const BP& P, const GC& c) {
+    if (c.tid == 0) P.ctr[MF_CS * (C_T1)] = wall_clock64();
     phase_barrier();
     for (int64_t i = c.tid; i < C_NCOUNTERS; i += c.nthreads) P.h_out[i] = P.ctr[MF_CS * (i)];
     if (c.tid == 0) P.h_out[C_NCOUNTERS] = *P.g.pool_top;
@@ -1350,14 +1424,17 @@ int grid_for(const void* fn, const LaunchCfg& L) {
 }
 }  // namespace
 
-enum GridClass { G_FULL, G_UPD, G_WAVE_UPD, G_SCAN };
+enum GridClass { G_FULL, G_UPD, G_CLAIM, G_WAVE_UPD, G_SCAN };
 
 // The grid of a phase for this batch: what its work needs (see LaunchCfg),
 // at least 8 workgroups, at most what the GPU holds at once.
 int grid_cls(const void* fn, const LaunchCfg& L, GridClass cls) {
     int full = grid_for(fn, L);
     if (L.U < 0 || cls == G_FULL) return full;
-    int64_t want = cls == G_UPD ? (L.U + 255) / 256 : cls == G_WAVE_UPD ? (L.U + 3) / 4 : (L.n + 1023) / 1024;
+    int64_t want = cls == G_UPD        ? (L.U + 255) / 256
+                   : cls == G_CLAIM    ? (claim_threads(L.U) + 255) / 256
+                   : cls == G_WAVE_UPD ? (L.U + 3) / 4
+                                       : (L.n + 1023) / 1024;
     if (want < 8) want = 8;
     return want < full ? (int)want : full;
 }
@@ -1384,7 +1461,7 @@ int mf_round_grid(const LaunchCfg& L) {
 
 void mf_launch_batch_prologue(const BP& P, const LaunchCfg& L) {
     LAUNCH_C(k_init, G_UPD, P);
-    LAUNCH_C(k_claim, G_UPD, P);
+    LAUNCH_C(k_claim, G_CLAIM, P);
     LAUNCH_C(k_resolve, G_UPD, P);
     LAUNCH_C(k_append, G_WAVE_UPD, P);
     LAUNCH_C(k_partner, G_UPD, P);
@@ -1395,8 +1472,8 @@ void mf_launch_batch_prologue(const BP& P, const LaunchCfg& L) {
     LAUNCH_C(k_fa_fill, G_WAVE_UPD, P);
 }
 
-void mf_launch_round(const BP& P, const LaunchCfg& L, int par) {
-    LAUNCH_ROUND(k_spec, P, par);
+void mf_launch_round(const BP& P, const LaunchCfg& L, int par, bool first) {
+    LAUNCH_ROUND(k_spec, P, par | (first ? 2 : 0));
     LAUNCH_ROUND(k_validate, P, par);
     if (L.round_grid > 0) LAUNCH_ROUND(k_commit, P, par);
     else LAUNCH(k_commit, P, par);
