@@ -67,6 +67,7 @@ struct BP {
     int64_t window;    // max edges speculated per round
     int64_t n_ins;     // insertion updates in the batch (bounds evictions per round)
     int may_halt;      // the tail may stop with edges left (nothing of this graph is queued behind)
+    int pu_list;       // the MIS was maximal before the batch: the PU scan tests only tagged vertices
     // pair table of this batch
     u64* bt_key;
     int32_t* bt_head;
@@ -75,7 +76,8 @@ struct BP {
     u64 bt_mask;
     // pre-batch adjacency of update i's pair, searched beside the claim:
     // positions of hi in row lo and of lo in row hi (-1: not in the base),
-    // bit 0 = live overlay entry, bit 1 = base entry deleted
+    // bit 0 = live overlay entry, bit 1 = base entry deleted; P2 adds
+    // bit 2 = adjacent after the batch
     int64_t* bt_plo;
     int64_t* bt_phi;
     uint8_t* bt_fl;
@@ -121,7 +123,8 @@ struct LaunchCfg {
     // Per-batch sizing (U < 0: every phase gets the whole GPU).  Phases get
     // as many workgroups as their work needs, so a small batch leaves the
     // GPU to other graphs' batches: a thread per update, a wave per update,
-    // a thread per four vertices; the rounds get round_grid workgroups.
+    // a thread per four vertices (or per 32 when only tagged ones are
+    // tested); the rounds get round_grid workgroups.
     int64_t U = -1;
     int64_t n = 0;
     int round_grid = 0;

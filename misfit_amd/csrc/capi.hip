@@ -55,6 +55,8 @@ struct misfit_ctx {
     GraphState S{};
     bool has_graph = false;
     int completion = 1;
+    bool maximal = false;  // the MIS has no uncovered non-member (checked at set_mis, kept by the completion)
+    bool force_pu_scan = false;  // MISFIT_PU_SCAN=full: always scan every vertex for PU
     int mode = MISFIT_MODE_QUEUED;
     int grid_rounds = 0;       // MISFIT_GRID_ROUNDS: fixed number of queued grid rounds
     bool grid_rounds_fixed = false;
@@ -169,6 +171,8 @@ extern "C" int misfit_ctx_create(int device, misfit_ctx** out) {
     if (rw) c->round_per_cu = std::max(0, std::min(8, atoi(rw)));
     const char* wf = getenv("MISFIT_WINDOW_FACTOR");
     if (wf && atof(wf) >= 1.0) c->window_factor = atof(wf);
+    const char* ps = getenv("MISFIT_PU_SCAN");
+    c->force_pu_scan = ps && strcmp(ps, "full") == 0;
     const char* gr = getenv("MISFIT_GRID_ROUNDS");
     if (gr) {
         c->grid_rounds = std::max(0, atoi(gr));
@@ -424,6 +428,7 @@ extern "C" int misfit_load_csr(misfit_ctx* c, int32_t n, int64_t m, const void* 
     free_list(c->graph_allocs);
     free_compact_scratch(c);
     c->has_graph = false;
+    c->maximal = false;
     c->U_cap = 0;
     free_list(c->batch_allocs);
     free_slots(c);
@@ -467,7 +472,7 @@ extern "C" int misfit_load_csr(misfit_ctx* c, int32_t n, int64_t m, const void* 
     HIPCHK(c, dalloc(A, &g.evb, n));
     HIPCHK(c, dalloc(A, &g.pfb, n));
     HIPCHK(c, dalloc(A, &g.kw, n));
-    HIPCHK(c, dalloc(A, &g.tagb, n));
+    HIPCHK(c, dalloc(A, &g.tagw, ((int64_t)n + 31) / 32));
     HIPCHK(c, dalloc(A, &g.nevb, n));
     HIPCHK(c, dalloc(A, &g.cstat, n));
     HIPCHK(c, dalloc(A, &g.mmark, n));
@@ -507,7 +512,7 @@ extern "C" int misfit_load_csr(misfit_ctx* c, int32_t n, int64_t m, const void* 
     mf_fill_u32(g.evb, 0u, n, st);
     mf_fill_u32(g.pfb, 0u, n, st);
     mf_fill_u64(g.kw, 0ull, n, st);
-    mf_fill_u32(g.tagb, 0u, n, st);
+    mf_fill_u32(g.tagw, 0u, ((int64_t)n + 31) / 32, st);
     mf_fill_u32(g.nevb, 0u, n, st);
     mf_fill_u32(g.cstat, 0u, n, st);
     mf_fill_u64(g.mmark, 0ull, n, st);
@@ -558,6 +563,15 @@ extern "C" int misfit_set_mis(misfit_ctx* c, const uint8_t* mis) {
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     hipFree(d);
     HIPCHK(c, e);
+    // a maximal start stays maximal while the completion runs (a vertex can
+    // only lose its last MIS neighbour next to an eviction or at a deleted
+    // edge, both in the completion's candidate set), and then the PU scan
+    // needs only the vertices the k-count tags
+    c->maximal = false;
+    HIPCHK(c, mf_check(c->S, c->d_acc));
+    HIPCHK(c, hipMemcpyAsync(c->h_small, c->d_acc, 4 * sizeof(u64), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->maximal = c->h_small[1] == 0;
     return MISFIT_OK;
 }
 
@@ -912,6 +926,8 @@ extern "C" int misfit_submit_batch(misfit_ctx* c, const misfit_update* ups, int6
     P.U = count;
     P.b = c->epoch;
     P.completion = c->completion;
+    P.pu_list = c->maximal && !c->force_pu_scan ? 1 : 0;
+    if (!c->completion) c->maximal = false;
     // grids sized to the batch; one edge per wave of the round grid and round
     LaunchCfg L{c->S.blocks, c->stream, c->round_per_cu};
     L.U = count;

@@ -32,7 +32,8 @@
 //     kw      = b<<32 | k   k = MIS neighbours of the vertex that may be
 //                   evicted in batch b (counted for non-members next to
 //                   members of W)
-//     tagb    = b   the vertex has a count in kw for batch b
+//     tagw    bitmap (not epoch words): the vertex has a count in kw for
+//                   the current batch; the PU scan clears it as it reads it
 //     evb     = b   vertex was evicted in batch b
 //     nevb    = b   vertex is a neighbour of one evicted in batch b
 //     pfb     = b   vertex is max(u,v) of an insertion of batch b whose other
@@ -79,7 +80,7 @@ struct DevGraph {
     uint32_t* pub;
     uint32_t* evb;
     u64* kw;
-    uint32_t* tagb;
+    uint32_t* tagw;  // bitmap of the vertices the k-count tagged (cleared by the PU scan)
     uint32_t* pfb;   // = b: an insertion of batch b may let this vertex become uncovered after its eviction
     uint32_t* nevb;  // = b: neighbour of a vertex evicted in batch b
     uint32_t* cstat;

@@ -317,6 +317,8 @@ __device__ void ph_resolve(const BP& P, const GC& c) {
         P.bt_key[slot] = MF_EMPTY_KEY;
         P.bt_head[slot] = -1;
         my_ops += (u64)ops;
+        if (adj)  // the pair's adjacency after the batch, for P5a
+            for (int32_t j = (int32_t)i; j != -1; j = P.bt_nxt[j]) P.bt_fl[j] |= 4;
         if (in_base && tomb != tomb0) {
             set_base_deleted(g, lo, plo, tomb);
             if (phi >= 0) set_base_deleted(g, hi, phi, tomb);
@@ -413,7 +415,7 @@ __device__ void ph_partner(const BP& P, const GC& c) {
         int32_t a = P.bu[i], bb = P.bv[i];
         int32_t w = a > bb ? a : bb, o = a > bb ? bb : a;
         if (P.g.pfb[w] == P.b) continue;
-        if (P.g.wpotb[o] == P.b || !adjacent(P.g, w, o)) P.g.pfb[w] = P.b;
+        if (P.g.wpotb[o] == P.b || !(P.bt_fl[i] & 4)) P.g.pfb[w] = P.b;  // bit 2: adjacent after the batch (P2)
     }
     int64_t nw = (int64_t)P.ctr[MF_CS * (C_WPOT)];
     // whole waves step together so the reservation below is wave-wide
@@ -440,7 +442,7 @@ __device__ void ph_partner(const BP& P, const GC& c) {
 // next to members of W∩MIS (one wave per 64-entry chunk of such a row).  The
 // count word is raised to the batch epoch and incremented with two no-return
 // atomics (neighbours are mostly distinct, and nothing waits on the result);
-// a plain store tags z for the scan below.  Flagged members of W, and
+// a no-return atomic OR tags z in the scan's bitmap.  Flagged members of W, and
 // endpoints whose deleted edge took their last MIS neighbour, join PU here.
 __device__ void ph_kcount(const BP& P, const GC& c) {
     const DevGraph& g = P.g;
@@ -470,33 +472,59 @@ __device__ void ph_kcount(const BP& P, const GC& c) {
         if ((mz & MF_MISBIT) || (u64)(mz & MF_CMASK) > (u64)nw) continue;
         atomicMax(&g.kw[z], ep);
         atomicAdd(&g.kw[z], 1ull);
-        g.tagb[z] = P.b;
+        atomicOr(&g.tagw[z >> 5], 1u << (z & 31));
     }
 }
 
-// P6: PU from one pass over all vertices, four per thread: a non-member
-// with no MIS neighbour, or a tagged one all of whose MIS neighbours may be
-// evicted (c(z) = k(z)).
-__device__ __forceinline__ void pu_scan_one(const BP& P, int32_t x, uint32_t m, uint32_t t) {
+// P6: PU: a non-member with no MIS neighbour, or a tagged one all of whose
+// MIS neighbours may be evicted (c(z) = k(z)).  The tags are a bitmap that
+// this pass clears as it reads it, so it is zero at every batch start.
+__device__ __forceinline__ void pu_scan_one(const BP& P, int32_t x, uint32_t m, bool tagged) {
     bool p = m == 0;
-    if (!p && t == P.b && !(m & MF_MISBIT)) p = (m & MF_CMASK) == (uint32_t)(P.g.kw[x] & 0xFFFFFFFFull);
+    if (!p && tagged && !(m & MF_MISBIT)) p = (m & MF_CMASK) == (uint32_t)(P.g.kw[x] & 0xFFFFFFFFull);
     pu_add(P, x, p);
 }
 
+// With a maximal MIS before the batch (P.pu_list), a vertex outside the MIS
+// with no MIS neighbour after the overlay update is an endpoint of a deleted
+// edge (the seeds, added in P5), so only the tagged vertices are tested: a
+// thread per bitmap word.  Otherwise every vertex, four per thread; the
+// eight lanes that share a bitmap word sit in one wave, which reads the word
+// before its first lane clears it.
 __device__ void ph_pu_scan(const BP& P, const GC& c) {
     const DevGraph& g = P.g;
-    const int64_t n4 = (int64_t)g.n / 4;
-    const uint4* mc4 = reinterpret_cast<const uint4*>(g.mc);
-    const uint4* tg4 = reinterpret_cast<const uint4*>(g.tagb);
-    for (int64_t q = c.tid; q < n4; q += c.nthreads) {
-        uint4 m = mc4[q], t = tg4[q];
-        int32_t x = (int32_t)(4 * q);
-        pu_scan_one(P, x, m.x, t.x);
-        pu_scan_one(P, x + 1, m.y, t.y);
-        pu_scan_one(P, x + 2, m.z, t.z);
-        pu_scan_one(P, x + 3, m.w, t.w);
+    const int64_t nwords = ((int64_t)g.n + 31) / 32;
+    if (P.pu_list) {
+        for (int64_t q = c.tid; q < nwords; q += c.nthreads) {
+            uint32_t w = g.tagw[q];
+            if (!w) continue;
+            g.tagw[q] = 0;
+            while (w) {
+                const int32_t x = (int32_t)(32 * q + __ffs(w) - 1);
+                w &= w - 1;
+                pu_scan_one(P, x, g.mc[x], true);
+            }
+        }
+        return;
     }
-    for (int64_t x = 4 * n4 + c.tid; x < g.n; x += c.nthreads) pu_scan_one(P, (int32_t)x, g.mc[x], g.tagb[x]);
+    const int64_t full = (int64_t)g.n / 32;  // whole words
+    const uint4* mc4 = reinterpret_cast<const uint4*>(g.mc);
+    for (int64_t q = c.tid; q < 8 * full; q += c.nthreads) {
+        const uint4 m = mc4[q];
+        const uint32_t w = g.tagw[q >> 3];
+        const uint32_t t = w >> (4 * (q & 7));
+        const int32_t x = (int32_t)(4 * q);
+        pu_scan_one(P, x, m.x, t & 1u);
+        pu_scan_one(P, x + 1, m.y, t & 2u);
+        pu_scan_one(P, x + 2, m.z, t & 4u);
+        pu_scan_one(P, x + 3, m.w, t & 8u);
+        if ((q & 7) == 0 && w) g.tagw[q >> 3] = 0;
+    }
+    if (c.tid == 0 && full < nwords) {  // the last, partial word
+        const uint32_t w = g.tagw[full];
+        for (int64_t x = 32 * full; x < g.n; x++) pu_scan_one(P, (int32_t)x, g.mc[x], (w >> (x & 31)) & 1u);
+        if (w) g.tagw[full] = 0;
+    }
 }
 
 // P7: a PU member is admissible (ADM) when it has a row: it is an endpoint
@@ -1424,7 +1452,7 @@ int grid_for(const void* fn, const LaunchCfg& L) {
 }
 }  // namespace
 
-enum GridClass { G_FULL, G_UPD, G_CLAIM, G_WAVE_UPD, G_SCAN };
+enum GridClass { G_FULL, G_UPD, G_CLAIM, G_WAVE_UPD, G_SCAN, G_SCAN32 };
 
 // The grid of a phase for this batch: what its work needs (see LaunchCfg),
 // at least 8 workgroups, at most what the GPU holds at once.
@@ -1434,7 +1462,8 @@ int grid_cls(const void* fn, const LaunchCfg& L, GridClass cls) {
     int64_t want = cls == G_UPD        ? (L.U + 255) / 256
                    : cls == G_CLAIM    ? (claim_threads(L.U) + 255) / 256
                    : cls == G_WAVE_UPD ? (L.U + 3) / 4
-                                       : (L.n + 1023) / 1024;
+                   : cls == G_SCAN     ? (L.n + 1023) / 1024
+                                       : ((L.n + 31) / 32 + 255) / 256;
     if (want < 8) want = 8;
     return want < full ? (int)want : full;
 }
@@ -1466,7 +1495,7 @@ void mf_launch_batch_prologue(const BP& P, const LaunchCfg& L) {
     LAUNCH_C(k_append, G_WAVE_UPD, P);
     LAUNCH_C(k_partner, G_UPD, P);
     LAUNCH_C(k_kcount, G_WAVE_UPD, P);
-    LAUNCH_C(k_pu_scan, G_SCAN, P);
+    LAUNCH_C(k_pu_scan, P.pu_list ? G_SCAN32 : G_SCAN, P);
     LAUNCH_C(k_fa_count, G_WAVE_UPD, P);
     LAUNCH_C(k_fa_alloc, G_UPD, P);
     LAUNCH_C(k_fa_fill, G_WAVE_UPD, P);
