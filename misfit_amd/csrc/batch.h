@@ -105,6 +105,8 @@ struct BP {
     int32_t* res_x;
     uint4* rlog;       // guarded reads of a round, MF_RLOG per wave of the grid
     int32_t* rlog_cnt; // records per wave
+    int4* cm;          // per wave of the round grid: its first edge's kind, evicted
+                       // and admitted vertex and index, left by the speculation for the commit
     uint32_t* relog;   // per edge: = round when its read log overflowed
     // completion
     int32_t* cand;

@@ -314,6 +314,7 @@ static int ensure_batch_capacity(misfit_ctx* c, int64_t U) {
     const int64_t nwaves = (int64_t)c->S.blocks * (256 / MF_WAVE);
     HIPCHK(c, dalloc(A, &P.rlog, nwaves * MF_RLOG));
     HIPCHK(c, dalloc(A, &P.rlog_cnt, nwaves));
+    HIPCHK(c, dalloc(A, &P.cm, nwaves));
     c->fa_cap = std::max<int64_t>(1 << 22, 16 * cap);
     HIPCHK(c, dalloc(A, &P.fa_z, c->fa_cap));
     HIPCHK(c, dalloc(A, &P.fa_pos, c->fa_cap));

@@ -912,9 +912,9 @@ __device__ __forceinline__ RoundView round_view(const BP& P, int par, int64_t wi
     return v;
 }
 
-__device__ __forceinline__ uint8_t classify_edge(const BP& P, int64_t e, int32_t a, int32_t bb) {
+__device__ __forceinline__ uint8_t classify_edge(const BP& P, bool ins, int32_t a, int32_t bb) {
     bool ma = (P.g.mc[a] & MF_MISBIT) != 0, mb = (P.g.mc[bb] & MF_MISBIT) != 0;
-    if (P.bins[e]) return (ma && mb) ? 1 : 0;
+    if (ins) return (ma && mb) ? 1 : 0;
     return (ma != mb) ? 2 : 0;
 }
 
@@ -941,7 +941,7 @@ __device__ __forceinline__ void mark_vertex(const BP& P, int32_t z, u64 mv, bool
 // Speculative evaluation of active edge e (kind k) by one wave; the reads
 // its outcome rests on go to `rc` (nowhere, or the wave's read log).
 __device__ __forceinline__ void spec_edge(const BP& P, int32_t e, uint8_t k, uint32_t R, u64 bound, int lane,
-                                          Recorder& rc) {
+                                          Recorder& rc, int4* cmw) {
     int32_t a = P.bu[e], bb = P.bv[e];
     rc.R = R;
     rc.e = e;
@@ -949,14 +949,22 @@ __device__ __forceinline__ void spec_edge(const BP& P, int32_t e, uint8_t k, uin
     if (k == 1) {
         int32_t w = a > bb ? a : bb;
         int32_t x = eval_insertion(P, w, rc);
-        if (lane == 0) { P.res_w[e] = w; P.res_x[e] = x; }
+        if (lane == 0) {
+            P.res_w[e] = w;
+            P.res_x[e] = x;
+            if (cmw) *cmw = make_int4(1, w, x, e);
+        }
         mark_vertex(P, w, mv, true, R, bound, lane);
         if (x >= 0) mark_vertex(P, x, mv, false, R, bound, lane);
     } else {
         bool ma = (P.g.mc[a] & MF_MISBIT) != 0;
         int32_t tgt = ma ? bb : a, other = ma ? a : bb;
         bool ok = eval_deletion(P, tgt, other, rc);
-        if (lane == 0) { P.res_w[e] = -1; P.res_x[e] = ok ? tgt : -1; }
+        if (lane == 0) {
+            P.res_w[e] = -1;
+            P.res_x[e] = ok ? tgt : -1;
+            if (cmw) *cmw = make_int4(2, -1, ok ? tgt : -1, e);
+        }
         if (ok) mark_vertex(P, tgt, mv, false, R, bound, lane);
     }
 }
@@ -999,14 +1007,16 @@ __device__ __forceinline__ void apply_flip(const BP& P, int32_t z, bool join, in
     }
 }
 
-__device__ __forceinline__ void commit_edge(const BP& P, int32_t e, uint8_t k, int lane) {
-    int32_t x = P.res_x[e];
+__device__ __forceinline__ void commit_flips(const BP& P, uint8_t k, int32_t w, int32_t x, int lane) {
     if (k == 1) {
-        int32_t w = P.res_w[e];
         apply_flip(P, w, false, lane);
         if (lane == 0) P.g.evb[w] = P.b;
     }
     if (x >= 0) apply_flip(P, x, true, lane);
+}
+
+__device__ __forceinline__ void commit_edge(const BP& P, int32_t e, uint8_t k, int lane) {
+    commit_flips(P, k, k == 1 ? P.res_w[e] : -1, P.res_x[e], lane);
 }
 
 // First conflicting edge of the round (INT64_MAX when none): the counter
@@ -1044,9 +1054,15 @@ __device__ void ph_spec_w(const BP& P, const GC& c, const RoundView& v, int* cur
     Recorder rc{REC_LOG, ReadLog{P.rlog + c.wave * MF_RLOG, cur + wiw, P.relog}};
     for (int64_t e = v.start + c.wave; e < v.end; e += c.nwaves) {
         int32_t a = P.bu[e], bb = P.bv[e];
-        uint8_t k = classify_edge(P, e, a, bb);
-        if (c.lane == 0) P.kind[e] = k;
-        if (k) spec_edge(P, (int32_t)e, k, v.R, (u64)P.n_ins, c.lane, rc);
+        uint8_t k = classify_edge(P, P.bins[e] != 0, a, bb);
+        // the commit's record of the wave's first edge (wave w of the commit
+        // commits edge start + w)
+        int4* cmw = e == v.start + c.wave ? P.cm + c.wave : nullptr;
+        if (c.lane == 0) {
+            P.kind[e] = k;
+            if (cmw && !k) *cmw = make_int4(0, -1, -1, (int32_t)e);
+        }
+        if (k) spec_edge(P, (int32_t)e, k, v.R, (u64)P.n_ins, c.lane, rc, cmw);
     }
     if (c.lane == 0) {
         int n = cur[wiw];
@@ -1054,43 +1070,63 @@ __device__ void ph_spec_w(const BP& P, const GC& c, const RoundView& v, int* cur
     }
 }
 
-// Validation: every logged read is checked on its own (one thread per log
-// slot: a few independent loads instead of the evaluation's dependent
-// chain); then every edge of the window checks its endpoints' membership
-// (its classification read it), and an edge whose log overflowed is
-// re-walked.  Same grid as the speculation kernel, so wave w's log segment
-// is rlog[w * MF_RLOG ...].
+// Validation: wave w checks what wave w of the speculation (same grid) did:
+// the reads logged in its segment rlog[w * MF_RLOG ...] and its edge's
+// endpoints (the classification read them), with a re-walk when the log
+// overflowed.  The segment's records are loaded before the count says which
+// are filled, together with the edge, so a check is two or three dependent
+// loads deep instead of a chain through the count, the record and the edge.
+static_assert(MF_RLOG == 2 * MF_WAVE, "two log records per lane");
+__device__ __forceinline__ bool endpoint_conflict(const BP& P, int64_t e, uint8_t k, int32_t a, int32_t bb,
+                                                  uint32_t R) {
+    if (k && P.relog[e] == R) return validate_edge(P, (int32_t)e, k, R);
+    return mk_before(P.g.mmark, a, R, (int32_t)e) || mk_before(P.g.mmark, bb, R, (int32_t)e);
+}
+
 __device__ void ph_validate_w(const BP& P, const GC& c, const RoundView& v, int par) {
+    const uint4* seg = P.rlog + c.wave * MF_RLOG;
+    const int64_t e0 = v.start + c.wave;
+    const bool has = e0 < v.end;
+    const int n = P.rlog_cnt[c.wave];
+    const uint4 r0 = seg[c.lane], r1 = seg[c.lane + MF_WAVE];
+    uint8_t k0 = 0;
+    int32_t a0 = 0, b0 = 0;
+    if (has) {
+        k0 = P.kind[e0];
+        a0 = P.bu[e0];
+        b0 = P.bv[e0];
+    }
+    bool c0 = false, c1 = false;
+    if (c.lane < n) vread(P, (int32_t)r0.x, r0.y, r0.z, v.R, (int32_t)r0.w, c0);
+    if (c.lane + MF_WAVE < n) vread(P, (int32_t)r1.x, r1.y, r1.z, v.R, (int32_t)r1.w, c1);
     u64 mine = ~0ull;
-    const int64_t nslots = c.nwaves * MF_RLOG;
-    for (int64_t s = c.tid; s < nslots; s += c.nthreads) {
-        if ((int32_t)(s % MF_RLOG) >= P.rlog_cnt[s / MF_RLOG]) continue;
-        uint4 r = P.rlog[s];
-        bool conf = false;
-        vread(P, (int32_t)r.x, r.y, r.z, v.R, (int32_t)r.w, conf);
-        if (conf && (u64)r.w < mine) mine = (u64)r.w;
-    }
-    mine = wave_min_u64(mine);  // wave-uniform from here on
-    for (int64_t e = v.start + c.wave; e < v.end; e += c.nwaves) {
-        // an edge past a conflict already found cannot move the minimum
-        if ((u64)e > P.ctr[MF_CS * (C_CONF0 + par)] || (u64)e >= mine) break;
-        uint8_t k = P.kind[e];
-        bool conf;
-        if (k && P.relog[e] == v.R) {
-            conf = validate_edge(P, (int32_t)e, k, v.R);
-        } else {
-            int32_t a = P.bu[e], bb = P.bv[e];
-            conf = mk_before(P.g.mmark, a, v.R, (int32_t)e) || mk_before(P.g.mmark, bb, v.R, (int32_t)e);
-        }
-        if (conf && (u64)e < mine) mine = (u64)e;
-    }
+    if (c0) mine = (u64)r0.w;
+    if (c1 && (u64)r1.w < mine) mine = (u64)r1.w;
+    if (has && endpoint_conflict(P, e0, k0, a0, b0, v.R) && (u64)e0 < mine) mine = (u64)e0;  // wave-uniform call
+    // a window wider than the grid gives a wave further edges
+    for (int64_t e = e0 + c.nwaves; e < v.end; e += c.nwaves)
+        if (endpoint_conflict(P, e, P.kind[e], P.bu[e], P.bv[e], v.R) && (u64)e < mine) mine = (u64)e;
+    mine = wave_min_u64(mine);
     if (mine != ~0ull && c.lane == 0) atomicMin(&P.ctr[MF_CS * (C_CONF0 + par)], mine);
 }
 
+// Wave w commits edge start + w from the record wave w of the speculation
+// left (loaded beside the conflict index, not after it), further edges of a
+// wider window from the arrays.
 __device__ void ph_commit_w(const BP& P, const GC& c, const RoundView& v, int par) {
+    const int64_t e0 = v.start + c.wave;
+    const int4 m = e0 < v.end ? P.cm[c.wave] : make_int4(0, -1, -1, -1);
     const int64_t conflict = round_conflict(P, par);
     const int64_t lim = conflict < v.end ? conflict : v.end;
-    for (int64_t e = v.start + c.wave; e < lim; e += c.nwaves) {
+    if (e0 < lim) {
+        if (m.w == (int32_t)e0) {
+            if (m.x) commit_flips(P, (uint8_t)m.x, m.y, m.z, c.lane);
+        } else {  // no record for this edge (not expected: both kernels use the round grid)
+            const uint8_t k = P.kind[e0];
+            if (k) commit_edge(P, (int32_t)e0, k, c.lane);
+        }
+    }
+    for (int64_t e = e0 + c.nwaves; e < lim; e += c.nwaves) {
         uint8_t k = P.kind[e];
         if (k) commit_edge(P, (int32_t)e, k, c.lane);
     }
@@ -1102,7 +1138,7 @@ __device__ void ph_commit_w(const BP& P, const GC& c, const RoundView& v, int pa
 __device__ void ph_classify_l(const BP& P, const GC& c, const RoundView& v, int par) {
     for (int64_t e = v.start + c.tid; e < v.end; e += c.nthreads) {
         int32_t a = P.bu[e], bb = P.bv[e];
-        uint8_t k = classify_edge(P, e, a, bb);
+        uint8_t k = classify_edge(P, P.bins[e] != 0, a, bb);
         P.kind[e] = k;
         wave_push(P.active, &P.ctr[MF_CS * (C_NACT0 + par)], (int32_t)e, k != 0);
     }
@@ -1113,7 +1149,7 @@ __device__ void ph_spec_l(const BP& P, const GC& c, int par, uint32_t R) {
     for (int64_t i = c.wave; i < na; i += c.nwaves) {
         int32_t e = P.active[i];
         Recorder none{REC_NONE};
-        spec_edge(P, e, P.kind[e], R, (u64)na, c.lane, none);
+        spec_edge(P, e, P.kind[e], R, (u64)na, c.lane, none, nullptr);
     }
 }
 
