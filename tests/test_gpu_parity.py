@@ -340,3 +340,56 @@ def test_corner_case_graphs(oracle, engine_mod, eng, mode, case):
     assert all(c == (0, 0, 0) for c in checks), name
     for (u, _, _), st in zip(batches, stats):
         assert st["updates"] == len(u)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_tagged_scan_equals_full_scan(oracle, engine_mod, mode, monkeypatch):
+    """With a maximal MIS before every batch the may-uncover scan tests only
+    the vertices the k-count tagged; forcing the scan over every vertex
+    (MISFIT_PU_SCAN=full, read when a context is created) must give the same
+    MIS after every batch, and both the oracle's."""
+    ni, nl = engine_mod.gen_rmat(15, 16, 11)
+    mis0 = engine_mod.greedy_mis(ni, nl)
+    u, v = engine_mod.gen_updates(engine_mod.MIXED_EXISTING, ni, nl, 12000, 11)
+    g = oracle.Graph.from_csr(ni, nl)
+    ins = g.flags(u, v)
+    batches = [(u[i:i + 3000], v[i:i + 3000], ins[i:i + 3000]) for i in range(0, 12000, 3000)]
+    runs = []
+    for scan in ("", "full"):
+        if scan:
+            monkeypatch.setenv("MISFIT_PU_SCAN", scan)
+        else:
+            monkeypatch.delenv("MISFIT_PU_SCAN", raising=False)
+        with engine_mod.Engine(0) as e:
+            got, stats, checks = H.engine_replay(engine_mod, e, ni, nl, mis0, batches, mode=mode)
+        assert all(c == (0, 0, 0) for c in checks)
+        runs.append((got, [s["maybe_uncovered"] for s in stats]))
+    for a, b in zip(runs[0][0], runs[1][0]):
+        assert np.array_equal(a, b)
+    assert runs[0][1] == runs[1][1]  # the same PU set size both ways
+    want, _, _ = H.oracle_replay(oracle, ni, nl, mis0, batches, completion=True, literal=False)
+    for a, b in zip(want, runs[0][0]):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_device_times_and_timer(engine_mod, eng, mode):
+    """Per-batch device times come from the device clock (the prologue is part
+    of the batch), and the event timer brackets queued work."""
+    ni, nl = engine_mod.gen_rmat(14, 16, 3)
+    mis0 = engine_mod.greedy_mis(ni, nl)
+    u, v = engine_mod.gen_updates(engine_mod.MIXED_EXISTING, ni, nl, 6000, 3)
+    eng.load_csr(ni, nl)
+    ups = engine_mod.make_updates(u, v, eng.flags(u, v))
+    eng.set_mis(mis0)
+    eng.set_mode(mode)
+    eng.sync()
+    eng.timer_begin()
+    tickets = [eng.submit(ups[i:i + 2000]) for i in range(0, 6000, 2000)]
+    stats = [eng.wait(t) for t in tickets]
+    region = eng.timer_end()
+    for s in stats:
+        assert 0.0 < s["prologue_ms"] <= s["device_ms"] < 1000.0
+    # the region holds every batch's device time
+    assert region >= 0.9 * sum(s["device_ms"] for s in stats)
+    assert eng.check() == (0, 0, 0)
