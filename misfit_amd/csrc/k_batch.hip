@@ -576,27 +576,54 @@ __device__ void ph_fa_alloc(const BP& P, const GC& c) {
     }
 }
 
-// P10: fill the index: (z, position of z in row(x1)).
+// P10: fill the index: (z, position of z in row(x1)).  ADM members are few
+// and the grid has waves to spare, so a wave takes eight neighbours x1 of
+// one ADM member z at a time and each x1 is searched by a group of 8 lanes
+// (group_find: 6 dependent loads for a row of 2^17 instead of the 17 of one
+// lane's binary search), then x1's overlay run when the base entry is absent
+// or deleted (the flat position of row_pos).
+#define MF_FA_PARTS 16  // waves per ADM member, eight neighbours each per step
 __device__ void ph_fa_fill(const BP& P, const GC& c) {
     const DevGraph& g = P.g;
+    constexpr int G = 8;
+    const int grp = c.lane / G, gl = c.lane % G, gbase = grp * G;
     int64_t np = (int64_t)P.ctr[MF_CS * (C_PU)];
     if (np > P.pu_cap) np = P.pu_cap;
-    for (int64_t i = c.wave; i < np; i += c.nwaves) {
-        int32_t z = P.pu[i];
+    for (int64_t t = c.wave; t < np * MF_FA_PARTS; t += c.nwaves) {
+        const int32_t z = P.pu[t / MF_FA_PARTS];
         if (!is_adm(P, z)) continue;
         Row r = get_row(g, z);
-        int64_t L = r.len();
-        for (int64_t j = c.lane; j < L; j += MF_WAVE) {
-            int32_t x1 = r.at(j);
-            if (x1 < 0 || x1 == z) continue;
-            if (row_len(g, x1) <= MF_FA_T) continue;
-            int64_t st = g.fa_start[x1];
-            if (st < 0) continue;
-            int64_t pos = row_pos(g, x1, z);
-            if (pos < 0) continue;
-            int32_t slot = atomicAdd(&g.fa_fill[x1], 1);
-            P.fa_z[st + slot] = z;
-            P.fa_pos[st + slot] = (int32_t)pos;
+        const int64_t L = r.len();
+        for (int64_t j0 = (t % MF_FA_PARTS) * G; j0 < L; j0 += (int64_t)MF_FA_PARTS * G) {
+            const int64_t j = j0 + grp;  // this group's neighbour: the 8 lanes agree
+            int32_t x1 = -1;
+            if (j < L) {
+                const int32_t y = r.at(j);
+                if (y >= 0 && y != z && row_len(g, y) > MF_FA_T && g.fa_start[y] >= 0) x1 = y;
+            }
+            if (x1 < 0) continue;
+            const int64_t p = group_find<G>(g, x1, z, gl, gbase);
+            int64_t pos = -1;
+            if (p >= 0 && g.nlist[p] >= 0) {
+                pos = p - g.nindex[x1];
+            } else {
+                const int32_t al = g.ov_len[x1];
+                const int32_t* run = g.pool + g.ov_start[x1];
+                for (int32_t t0 = 0; t0 < al; t0 += G) {
+                    const int32_t q = t0 + gl;
+                    const u64 hit = (__ballot(q < al && run[q] == z) >> gbase) & ((1ull << G) - 1ull);
+                    if (hit) {
+                        pos = (g.nindex[x1 + 1] - g.nindex[x1]) + t0 + __ffsll((long long)hit) - 1;
+                        break;
+                    }
+                }
+            }
+            if (gl == 0 && pos >= 0) {
+                const int64_t st = g.fa_start[x1];
+                const int32_t slot = atomicAdd(&g.fa_fill[x1], 1);
+                P.fa_z[st + slot] = z;
+                P.fa_pos[st + slot] = (int32_t)pos;
+            }
         }
     }
 }
