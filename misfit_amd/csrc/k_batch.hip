@@ -358,13 +358,29 @@ __device__ void ph_append(const BP& P, const GC& c) {
     // reserve from one counter)
     for (int64_t k0 = c.tid - c.lane; k0 < ne; k0 += c.nthreads) {
         const int64_t k = k0 + c.lane;
-        int32_t x = k < ne ? P.ne_x[k] : -1;
-        const bool head = x >= 0 && g.ne_head[x] == (int32_t)k;
-        int32_t cnt = 0, len = 0, cap = 0, nc = 0;
-        if (head) {
-            for (int32_t j = (int32_t)k; j != -1; j = P.ne_nxt[j]) cnt++;
+        // everything that depends on k alone first, then everything that
+        // depends on x: a vertex with one new entry (the usual case) needs
+        // no chain walk, so its append is two loads and the pool
+        // reservation deep
+        int32_t x = -1, nxt = -1, y0 = 0;
+        if (k < ne) {
+            x = P.ne_x[k];
+            nxt = P.ne_nxt[k];
+            y0 = P.ne_y[k];
+        }
+        int32_t hd = -1, len = 0, cap = 0;
+        int64_t ost = 0;
+        if (x >= 0) {
+            hd = g.ne_head[x];
             len = g.ov_len[x];
             cap = g.ov_cap[x];
+            ost = g.ov_start[x];
+        }
+        const bool head = x >= 0 && hd == (int32_t)k;
+        int32_t cnt = 0, nc = 0;
+        if (head) {
+            cnt = 1;
+            for (int32_t j = nxt; j != -1; j = P.ne_nxt[j]) cnt++;
             if (len + cnt > cap) {
                 nc = 2;
                 while (nc < len + cnt) nc <<= 1;
@@ -384,22 +400,27 @@ __device__ void ph_append(const BP& P, const GC& c) {
                 continue;
             }
             if (len) {
-                const int32_t* src = g.pool + g.ov_start[x];
+                const int32_t* src = g.pool + ost;
                 for (int32_t t = 0; t < len; t++) g.pool[st + t] = src[t];
             }
-            g.ov_start[x] = (int64_t)st;
+            ost = (int64_t)st;
+            g.ov_start[x] = ost;
             g.ov_cap[x] = nc;
         }
-        int32_t* run = g.pool + g.ov_start[x];
-        int32_t prev_seq = -1;
-        for (int32_t t = 0; t < cnt; t++) {
-            int32_t best = -1, bseq = INT32_MAX;
-            for (int32_t j = (int32_t)k; j != -1; j = P.ne_nxt[j]) {
-                int32_t s = P.ne_seq[j];
-                if (s > prev_seq && s < bseq) { bseq = s; best = j; }
+        int32_t* run = g.pool + ost;
+        if (cnt == 1) {
+            run[len] = y0;
+        } else {
+            int32_t prev_seq = -1;
+            for (int32_t t = 0; t < cnt; t++) {
+                int32_t best = -1, bseq = INT32_MAX;
+                for (int32_t j = (int32_t)k; j != -1; j = P.ne_nxt[j]) {
+                    int32_t sq = P.ne_seq[j];
+                    if (sq > prev_seq && sq < bseq) { bseq = sq; best = j; }
+                }
+                run[len + t] = P.ne_y[best];
+                prev_seq = bseq;
             }
-            run[len + t] = P.ne_y[best];
-            prev_seq = bseq;
         }
         g.ov_len[x] = len + cnt;
         atomicAdd(&g.ov_alive[x], cnt);
