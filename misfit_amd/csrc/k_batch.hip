@@ -1301,7 +1301,19 @@ __device__ void ph_comp_apply(const BP& P, const GC& c, int par) {
 // per batch instead of with per-edge atomics.
 __device__ void ph_stats(const BP& P, const GC& c) {
     u64 ai = 0, ti = 0, ad = 0, td = 0;
-    for (int64_t e = c.tid; e < P.U; e += c.nthreads) {
+    // four kinds per 32-bit load (kind is a whole allocation)
+    const int64_t nq = P.U / 4;
+    const uint32_t* k4 = reinterpret_cast<const uint32_t*>(P.kind);
+    for (int64_t q = c.tid; q < nq; q += c.nthreads) {
+        const uint32_t w = k4[q];
+        if (!w) continue;
+        for (int j = 0; j < 4; j++) {
+            const uint32_t k = (w >> (8 * j)) & 0xFFu;
+            if (k == 1) { ai++; ti += P.res_x[4 * q + j] >= 0; }
+            else if (k == 2) { ad++; td += P.res_x[4 * q + j] >= 0; }
+        }
+    }
+    for (int64_t e = 4 * nq + c.tid; e < P.U; e += c.nthreads) {
         uint8_t k = P.kind[e];
         if (k == 1) { ai++; ti += P.res_x[e] >= 0; }
         else if (k == 2) { ad++; td += P.res_x[e] >= 0; }
