@@ -65,6 +65,7 @@ struct misfit_ctx {
     int hist_pos = 0;
     bool always_halt = false;     // every batch may halt (nothing is ever queued behind one)
     double window_factor = 16.0;  // MISFIT_WINDOW_FACTOR: window = this x expected edges per round
+    int round_margin = 3;         // MISFIT_ROUND_MARGIN: rounds queued beyond the planned rate
     double clock_khz = 0.0;       // device wall clock, for per-batch device times
     hipEvent_t timer0 = nullptr, timer1 = nullptr;  // misfit_timer_begin/end
     int round_per_cu = 0;      // MISFIT_ROUND_WG_PER_CU: round grid (0: by occupancy)
@@ -171,6 +172,8 @@ extern "C" int misfit_ctx_create(int device, misfit_ctx** out) {
     if (rw) c->round_per_cu = std::max(0, std::min(8, atoi(rw)));
     const char* wf = getenv("MISFIT_WINDOW_FACTOR");
     if (wf && atof(wf) >= 1.0) c->window_factor = atof(wf);
+    const char* rm = getenv("MISFIT_ROUND_MARGIN");
+    if (rm) c->round_margin = std::max(0, std::min(16, atoi(rm)));
     const char* ps = getenv("MISFIT_PU_SCAN");
     c->force_pu_scan = ps && strcmp(ps, "full") == 0;
     const char* gr = getenv("MISFIT_GRID_ROUNDS");
@@ -761,7 +764,8 @@ static RoundPlan plan_rounds(misfit_ctx* c, int64_t count, int64_t full_window) 
     // with enough history the second largest rate plus two rounds (about
     // the 90th percentile of 16), else the largest with a margin: a round
     // left to the one-workgroup tail costs dozens of empty grid rounds
-    double k = eligible >= 6 ? std::ceil((double)count * rpu2) + 3.0 : std::ceil((double)count * rpu * 1.15) + 3.0;
+    const double mg = (double)c->round_margin;
+    double k = eligible >= 6 ? std::ceil((double)count * rpu2) + mg : std::ceil((double)count * rpu * 1.15) + mg;
     double per_window = std::ceil((double)count / (double)std::max<int64_t>(p.window, 1)) + 2.0;
     k = std::max(k, per_window);
     p.k = (int)std::min(k, 65536.0);

@@ -53,7 +53,6 @@
 #include "batch.h"
 
 // ---------------------------------------------------------------- helpers
-__device__ __forceinline__ bool in_pu(const BP& P, int32_t x) { return (P.g.pub[x] >> 2) == P.b; }
 __device__ __forceinline__ bool is_adm(const BP& P, int32_t x) { return P.g.pub[x] == 4u * P.b + 3u; }
 
 // z joins PU when pred holds (first time in this batch: appended to the PU
@@ -661,10 +660,13 @@ __device__ void ph_fa_fill(const BP& P, const GC& c) {
 // Is x (an ADM vertex, x != w) admissible right after w's eviction?  xadj:
 // x is a neighbour of w (w is one of the MIS neighbours x counts).  On false,
 // `wit` is the blocking read; with ALL, every read is recorded.
+//
+// mx = mc[x] and sx = seedb[x], loaded by the caller together with x's PU
+// tag (independent loads issued at once shorten the evaluation's chain).
 template <bool ALL>
-__device__ bool cand_ok(const BP& P, int32_t x, int32_t w, bool xadj, Rd& wit, Recorder& rc) {
+__device__ bool cand_ok_v(const BP& P, int32_t x, int32_t w, bool xadj, uint32_t mx, uint32_t sx, Rd& wit,
+                          Recorder& rc) {
     const DevGraph& g = P.g;
-    uint32_t mx = g.mc[x];
     if (mx & MF_MISBIT) {
         wit = rd(x, mx, MF_T_MIS);
         return false;
@@ -676,29 +678,28 @@ __device__ bool cand_ok(const BP& P, int32_t x, int32_t w, bool xadj, Rd& wit, R
         return false;
     }
     if (ALL) record(P, rc, rd(x, mx, 0u));
-    if (g.seedb[x] != P.b) return true;  // candidate row = adj(x)
+    if (sx != P.b) return true;  // candidate row = adj(x)
     // seed row {x} ∪ N1(x) ∪ N2(x) must hold no MIS member
     Row r = get_row(g, x);
     int64_t L = r.len();
     for (int64_t j = 0; j < L; j++) {
         int32_t y = r.at(j);
         if (y < 0 || y == x) continue;
+        const uint32_t py = g.pub[y], my = g.mc[y], wy = g.wpotb[y];
         if (y == w) {  // w is outside the MIS now; its own count is unchanged
-            uint32_t mw = g.mc[w];
-            if (mw & MF_CMASK) {
-                wit = rd(w, mw, 1u | MF_T_NOFLIP);
+            if (my & MF_CMASK) {
+                wit = rd(w, my, 1u | MF_T_NOFLIP);
                 return false;
             }
-            if (ALL) record(P, rc, rd(w, mw, 0u));
+            if (ALL) record(P, rc, rd(w, my, 0u));
             continue;
         }
-        if (!in_pu(P, y)) {
+        if ((py >> 2) != P.b) {  // outside PU
             wit = rd_none();
             return false;
         }
-        uint32_t my = g.mc[y];
         if (my & MF_MISBIT) {
-            wit = g.wpotb[y] == P.b ? rd(y, my, MF_T_MIS) : rd_none();
+            wit = wy == P.b ? rd(y, my, MF_T_MIS) : rd_none();
             return false;
         }
         uint32_t cy = my & MF_CMASK;
@@ -715,21 +716,29 @@ __device__ bool cand_ok(const BP& P, int32_t x, int32_t w, bool xadj, Rd& wit, R
     return true;
 }
 
+template <bool ALL>
+__device__ __forceinline__ bool cand_ok(const BP& P, int32_t x, int32_t w, bool xadj, Rd& wit, Recorder& rc) {
+    return cand_ok_v<ALL>(P, x, w, xadj, P.g.mc[x], P.g.seedb[x], wit, rc);
+}
+
 // Insertion with both ends in the MIS, w evicted: first admissible vertex of
 // row(w) in BFS order, or -1.  Wave-cooperative; all lanes return the same.
 __device__ int32_t eval_insertion(const BP& P, int32_t w, Recorder& rc) {
     const DevGraph& g = P.g;
     const int lane = lane_id();
     Recorder none{REC_NONE};
+    // loads that do not depend on each other first
+    const uint32_t pw = g.pub[w], mw = g.mc[w];
+    const Row rw = get_row(g, w);
+    const int64_t L = rw.len();
+    const bool fa_ok = P.ctr[MF_CS * (C_FA_OVF)] == 0;
     // position 0: w itself.  Outside PU, w keeps an MIS neighbour whenever it
     // is outside the MIS (its insertion partner), so it is blocked for sure.
-    if (in_pu(P, w)) {
-        uint32_t mw = g.mc[w];
+    if ((pw >> 2) == P.b) {
         if (mw & MF_CMASK) {
             if (lane == 0) record(P, rc, rd(w, mw, 1u | MF_T_NOFLIP));
         } else {
-            Row r = get_row(g, w);
-            int64_t L = r.len();
+            const Row& r = rw;
             bool blocked = false;
             for (int64_t j0 = 0; j0 < L && !blocked; j0 += MF_WAVE) {
                 int64_t j = j0 + lane;
@@ -738,12 +747,12 @@ __device__ int32_t eval_insertion(const BP& P, int32_t w, Recorder& rc) {
                 if (j < L) {
                     int32_t y = r.at(j);
                     if (y >= 0 && y != w) {
-                        if (!in_pu(P, y)) bl = fixed = true;
+                        const uint32_t py = g.pub[y], my = g.mc[y], wy = g.wpotb[y];
+                        if ((py >> 2) != P.b) bl = fixed = true;  // outside PU
                         else {
-                            uint32_t my = g.mc[y];
                             if (my & MF_MISBIT) {
                                 bl = true;
-                                fixed = g.wpotb[y] != P.b;  // a member outside W stays one
+                                fixed = wy != P.b;  // a member outside W stays one
                                 wt = rd(y, my, MF_T_MIS);
                             } else if ((my & MF_CMASK) >= 2) {  // w was one of them
                                 bl = true;
@@ -768,8 +777,6 @@ __device__ int32_t eval_insertion(const BP& P, int32_t w, Recorder& rc) {
             }
         }
     }
-    Row rw = get_row(g, w);
-    const int64_t L = rw.len();
     // part 2: N1(w) in adjacency order
     for (int64_t j0 = 0; j0 < L; j0 += MF_WAVE) {
         int64_t j = j0 + lane;
@@ -778,9 +785,12 @@ __device__ int32_t eval_insertion(const BP& P, int32_t w, Recorder& rc) {
         Rd wt = rd_none();
         if (j < L) {
             x = rw.at(j);
-            if (x >= 0 && x != w && is_adm(P, x)) {
-                cnd = true;
-                ok = cand_ok<false>(P, x, w, true, wt, none);
+            if (x >= 0 && x != w) {
+                const uint32_t px = g.pub[x], mx = g.mc[x], sx = g.seedb[x];
+                if (px == 4u * P.b + 3u) {  // admissible
+                    cnd = true;
+                    ok = cand_ok_v<false>(P, x, w, true, mx, sx, wt, none);
+                }
             }
         }
         u64 bal = __ballot(ok);
@@ -792,7 +802,6 @@ __device__ int32_t eval_insertion(const BP& P, int32_t w, Recorder& rc) {
         }
     }
     // part 3: concat over x1 in adj(w) of adj(x1); long rows through the index
-    const bool fa_ok = P.ctr[MF_CS * (C_FA_OVF)] == 0;
     for (int64_t j0 = 0; j0 < L; j0 += MF_WAVE) {
         int64_t j = j0 + lane;
         int kindl = 0;  // 0 empty, 1 row scan, 2 index
@@ -852,9 +861,15 @@ __device__ int32_t eval_insertion(const BP& P, int32_t w, Recorder& rc) {
                     x = off < oblen ? obase[off] : oadd[off - oblen];
                     pos = off;
                 }
-                if (x >= 0 && x != w && is_adm(P, x)) {
+                uint32_t px = 0, mx = 0, sx = 0;
+                if (x >= 0 && x != w) {
+                    px = g.pub[x];
+                    mx = g.mc[x];
+                    sx = g.seedb[x];
+                }
+                if (x >= 0 && x != w && px == 4u * P.b + 3u) {  // admissible
                     Rd wt = rd_none();
-                    if (cand_ok<false>(P, x, w, false, wt, none))
+                    if (cand_ok_v<false>(P, x, w, false, mx, sx, wt, none))
                         key = ((u64)(uint32_t)owner << 32) | (u64)(uint32_t)pos;
                     else
                         record(P, rc, wt);  // (also for candidates after the result: harmless)
@@ -898,13 +913,13 @@ __device__ bool eval_deletion(const BP& P, int32_t tgt, int32_t other, Recorder&
         if (j < L) {
             int32_t y = r.at(j);
             if (y >= 0 && y != tgt) {
-                uint32_t my = g.mc[y];
+                const uint32_t my = g.mc[y], wy = g.wpotb[y];
                 uint32_t cy = my & MF_CMASK;
                 if (y == other) {  // its MIS neighbours sit two hops from tgt
                     if (cy) { bl = true; wt = rd(y, my, 1u | MF_T_NOFLIP); }
                 } else if (my & MF_MISBIT) {
                     bl = true;
-                    fixed = g.wpotb[y] != P.b;  // a member outside W stays one
+                    fixed = wy != P.b;  // a member outside W stays one
                     wt = rd(y, my, MF_T_MIS);
                 } else if (cy >= 2) {
                     bl = true;
