@@ -721,9 +721,20 @@ __device__ __forceinline__ bool cand_ok(const BP& P, int32_t x, int32_t w, bool 
     return cand_ok_v<ALL>(P, x, w, xadj, P.g.mc[x], P.g.seedb[x], wit, rc);
 }
 
+// The eviction marks of w's neighbours (mark_vertex), issued by the
+// evaluation's pass over adj(w) from the counts it loads anyway; `done`: the
+// prefix of the row covered.
+struct EvictMarks {
+    u64 mv;
+    uint32_t R;
+    u64 bound;
+    int64_t done;
+};
+
 // Insertion with both ends in the MIS, w evicted: first admissible vertex of
 // row(w) in BFS order, or -1.  Wave-cooperative; all lanes return the same.
-__device__ int32_t eval_insertion(const BP& P, int32_t w, Recorder& rc) {
+// With `em`, w's neighbours are marked as part 2 passes them.
+__device__ int32_t eval_insertion(const BP& P, int32_t w, Recorder& rc, EvictMarks* em = nullptr) {
     const DevGraph& g = P.g;
     const int lane = lane_id();
     Recorder none{REC_NONE};
@@ -787,12 +798,17 @@ __device__ int32_t eval_insertion(const BP& P, int32_t w, Recorder& rc) {
             x = rw.at(j);
             if (x >= 0 && x != w) {
                 const uint32_t px = g.pub[x], mx = g.mc[x], sx = g.seedb[x];
+                if (em && (u64)(mx & MF_CMASK) < 2ull + em->bound) {
+                    atomicMax(&g.cmark[x], em->mv);
+                    round_count_add(&g.cdec[x], em->R);
+                }
                 if (px == 4u * P.b + 3u) {  // admissible
                     cnd = true;
                     ok = cand_ok_v<false>(P, x, w, true, mx, sx, wt, none);
                 }
             }
         }
+        if (em) em->done = j0 + MF_WAVE < L ? j0 + MF_WAVE : L;
         u64 bal = __ballot(ok);
         const int f = bal ? first_lane(bal) : MF_WAVE;
         if (cnd && !ok && lane < f) record(P, rc, wt);  // earlier candidates stay blocked
@@ -986,12 +1002,14 @@ __device__ __forceinline__ uint8_t classify_edge(const BP& P, bool ins, int32_t 
 // bound for this round).  A neighbour whose count is at least 2 + `bound`
 // (bound >= evictions this round) is left unmarked: no read of it can
 // change its outcome this round (every count threshold is at most 2).
+// Entries before `from` were marked already (by the evaluation's first pass
+// over an evicted vertex's row, EvictMarks).
 __device__ __forceinline__ void mark_vertex(const BP& P, int32_t z, u64 mv, bool evict, uint32_t R, u64 bound,
-                                            int lane) {
+                                            int lane, int64_t from = 0) {
     if (lane == 0) atomicMax(&P.g.mmark[z], mv);
     Row r = get_row(P.g, z);
     int64_t L = r.len();
-    for (int64_t j = lane; j < L; j += MF_WAVE) {
+    for (int64_t j = from + lane; j < L; j += MF_WAVE) {
         int32_t y = r.at(j);
         if (y >= 0 && y != z) {
             if ((u64)(P.g.mc[y] & MF_CMASK) >= 2ull + bound) continue;
@@ -1011,13 +1029,14 @@ __device__ __forceinline__ void spec_edge(const BP& P, int32_t e, uint8_t k, uin
     u64 mv = ((u64)R << 32) | (u64)(0xFFFFFFFFu - (uint32_t)e);
     if (k == 1) {
         int32_t w = a > bb ? a : bb;
-        int32_t x = eval_insertion(P, w, rc);
+        EvictMarks em{mv, R, bound, 0};
+        int32_t x = eval_insertion(P, w, rc, &em);
         if (lane == 0) {
             P.res_w[e] = w;
             P.res_x[e] = x;
             if (cmw) *cmw = make_int4(1, w, x, e);
         }
-        mark_vertex(P, w, mv, true, R, bound, lane);
+        mark_vertex(P, w, mv, true, R, bound, lane, em.done);
         if (x >= 0) mark_vertex(P, x, mv, false, R, bound, lane);
     } else {
         bool ma = (P.g.mc[a] & MF_MISBIT) != 0;
