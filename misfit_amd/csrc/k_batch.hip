@@ -721,10 +721,11 @@ __device__ __forceinline__ bool cand_ok(const BP& P, int32_t x, int32_t w, bool 
     return cand_ok_v<ALL>(P, x, w, xadj, P.g.mc[x], P.g.seedb[x], wit, rc);
 }
 
-// The eviction marks of w's neighbours (mark_vertex), issued by the
-// evaluation's pass over adj(w) from the counts it loads anyway; `done`: the
-// prefix of the row covered.
-struct EvictMarks {
+// The marks of a speculated flip's neighbours (mark_vertex), issued by an
+// evaluation's pass over the flipped vertex's row from the counts it loads
+// anyway: an evicted w's in part 2 of eval_insertion (`done`: the prefix of
+// the row covered), an admitted deletion endpoint's in eval_deletion.
+struct SpecMarks {
     u64 mv;
     uint32_t R;
     u64 bound;
@@ -734,7 +735,7 @@ struct EvictMarks {
 // Insertion with both ends in the MIS, w evicted: first admissible vertex of
 // row(w) in BFS order, or -1.  Wave-cooperative; all lanes return the same.
 // With `em`, w's neighbours are marked as part 2 passes them.
-__device__ int32_t eval_insertion(const BP& P, int32_t w, Recorder& rc, EvictMarks* em = nullptr) {
+__device__ int32_t eval_insertion(const BP& P, int32_t w, Recorder& rc, SpecMarks* em = nullptr) {
     const DevGraph& g = P.g;
     const int lane = lane_id();
     Recorder none{REC_NONE};
@@ -917,7 +918,9 @@ __device__ int32_t eval_insertion(const BP& P, int32_t w, Recorder& rc, EvictMar
 
 // Deletion, exactly one end `other` in the MIS: may tgt join?  row(tgt) =
 // {tgt} ∪ N1 ∪ N2 must hold no MIS member other than `other`.
-__device__ bool eval_deletion(const BP& P, int32_t tgt, int32_t other, Recorder& rc) {
+// With `em` (its `done` unused), the admission marks of tgt's neighbours
+// (mark_vertex) are issued by the final pass that records the reads.
+__device__ bool eval_deletion(const BP& P, int32_t tgt, int32_t other, Recorder& rc, const SpecMarks* em = nullptr) {
     const DevGraph& g = P.g;
     const int lane = lane_id();
     Row r = get_row(g, tgt);
@@ -955,7 +958,11 @@ __device__ bool eval_deletion(const BP& P, int32_t tgt, int32_t other, Recorder&
     // tgt joins: every read of the scan decides it
     for (int64_t j = lane; j < L; j += MF_WAVE) {
         int32_t y = r.at(j);
-        if (y >= 0 && y != tgt) record(P, rc, rd(y, g.mc[y], 0u));
+        if (y >= 0 && y != tgt) {
+            const uint32_t my = g.mc[y];
+            record(P, rc, rd(y, my, 0u));
+            if (em && (u64)(my & MF_CMASK) < 2ull + em->bound) atomicMax(&g.cmark[y], em->mv);
+        }
     }
     return true;
 }
@@ -1003,7 +1010,7 @@ __device__ __forceinline__ uint8_t classify_edge(const BP& P, bool ins, int32_t 
 // (bound >= evictions this round) is left unmarked: no read of it can
 // change its outcome this round (every count threshold is at most 2).
 // Entries before `from` were marked already (by the evaluation's first pass
-// over an evicted vertex's row, EvictMarks).
+// over an evicted vertex's row, SpecMarks).
 __device__ __forceinline__ void mark_vertex(const BP& P, int32_t z, u64 mv, bool evict, uint32_t R, u64 bound,
                                             int lane, int64_t from = 0) {
     if (lane == 0) atomicMax(&P.g.mmark[z], mv);
@@ -1029,7 +1036,7 @@ __device__ __forceinline__ void spec_edge(const BP& P, int32_t e, uint8_t k, uin
     u64 mv = ((u64)R << 32) | (u64)(0xFFFFFFFFu - (uint32_t)e);
     if (k == 1) {
         int32_t w = a > bb ? a : bb;
-        EvictMarks em{mv, R, bound, 0};
+        SpecMarks em{mv, R, bound, 0};
         int32_t x = eval_insertion(P, w, rc, &em);
         if (lane == 0) {
             P.res_w[e] = w;
@@ -1041,13 +1048,14 @@ __device__ __forceinline__ void spec_edge(const BP& P, int32_t e, uint8_t k, uin
     } else {
         bool ma = (P.g.mc[a] & MF_MISBIT) != 0;
         int32_t tgt = ma ? bb : a, other = ma ? a : bb;
-        bool ok = eval_deletion(P, tgt, other, rc);
+        const SpecMarks em{mv, R, bound, 0};
+        bool ok = eval_deletion(P, tgt, other, rc, &em);
         if (lane == 0) {
             P.res_w[e] = -1;
             P.res_x[e] = ok ? tgt : -1;
             if (cmw) *cmw = make_int4(2, -1, ok ? tgt : -1, e);
+            if (ok) atomicMax(&P.g.mmark[tgt], mv);  // its neighbours were marked by the evaluation
         }
-        if (ok) mark_vertex(P, tgt, mv, false, R, bound, lane);
     }
 }
 
