@@ -1163,9 +1163,10 @@ __device__ void ph_spec_w(const BP& P, const GC& c, const RoundView& v, int* cur
 // Validation: wave w checks what wave w of the speculation (same grid) did:
 // the reads logged in its segment rlog[w * MF_RLOG ...] and its edge's
 // endpoints (the classification read them), with a re-walk when the log
-// overflowed.  The segment's records are loaded before the count says which
-// are filled, together with the edge, so a check is two or three dependent
-// loads deep instead of a chain through the count, the record and the edge.
+// overflowed.  The segment's first records are loaded before the count says
+// which are filled, together with the edge, so a check is two or three
+// dependent loads deep instead of a chain through the count, the record and
+// the edge.
 static_assert(MF_RLOG == 2 * MF_WAVE, "two log records per lane");
 __device__ __forceinline__ bool endpoint_conflict(const BP& P, int64_t e, uint8_t k, int32_t a, int32_t bb,
                                                   uint32_t R) {
@@ -1178,7 +1179,12 @@ __device__ void ph_validate_w(const BP& P, const GC& c, const RoundView& v, int 
     const int64_t e0 = v.start + c.wave;
     const bool has = e0 < v.end;
     const int n = P.rlog_cnt[c.wave];
-    const uint4 r0 = seg[c.lane], r1 = seg[c.lane + MF_WAVE];
+    // the first record of each lane before the count is known (most
+    // segments hold at most 64), the second only when the count says it is
+    // filled
+    const uint4 r0 = seg[c.lane];
+    uint4 r1 = make_uint4(0u, 0u, 0u, 0u);
+    if (c.lane + MF_WAVE < n) r1 = seg[c.lane + MF_WAVE];
     uint8_t k0 = 0;
     int32_t a0 = 0, b0 = 0;
     if (has) {
