@@ -45,7 +45,7 @@ typedef struct {
     uint8_t pad[3];
 } misfit_update;
 
-/* Per-batch statistics (counts from the device, times from HIP events). */
+/* Per-batch statistics (counts from the device, times from the device clock). */
 typedef struct {
     int64_t updates;          /* updates in the batch */
     int64_t seeds;            /* distinct endpoints */

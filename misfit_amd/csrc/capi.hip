@@ -50,7 +50,6 @@ struct Slot {
 struct misfit_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipStream_t copy_stream = nullptr;
     std::string err;
     GraphState S{};
     bool has_graph = false;
@@ -161,8 +160,7 @@ extern "C" int misfit_ctx_create(int device, misfit_ctx** out) {
     if (device < 0 || device >= ndev) return MISFIT_E_ARG;
     misfit_ctx* c = new misfit_ctx();
     c->device = device;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return MISFIT_E_HIP;
     }
@@ -203,9 +201,9 @@ extern "C" int misfit_ctx_create(int device, misfit_ctx** out) {
     return MISFIT_OK;
 }
 
-// Staging buffers only: a slot's ticket, counters and events stay, so a
+// Staging buffers only: a slot's ticket, counters and event stay, so a
 // batch submitted before the buffers were regrown can still be waited on
-// (the regrowth synchronises the streams first).
+// (the regrowth synchronises the stream first).
 static void free_slots(misfit_ctx* c) {
     for (auto& s : c->slots) {
         if (s.h_u) hipHostFree(s.h_u);
@@ -221,7 +219,6 @@ extern "C" int misfit_ctx_destroy(misfit_ctx* c) {
     if (!c) return MISFIT_E_ARG;
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
-    hipStreamSynchronize(c->copy_stream);
     free_slots(c);
     for (auto& s : c->slots) {
         if (s.h_ctr) hipHostFree(s.h_ctr);
@@ -237,7 +234,6 @@ extern "C" int misfit_ctx_destroy(misfit_ctx* c) {
     if (c->d_spos) hipFree(c->d_spos);
     if (c->h_small) hipHostFree(c->h_small);
     hipStreamDestroy(c->stream);
-    hipStreamDestroy(c->copy_stream);
     delete c;
     return MISFIT_OK;
 }
@@ -259,7 +255,6 @@ extern "C" int misfit_sync(misfit_ctx* c) {
     if (!c) return MISFIT_E_ARG;
     HIPCHK(c, hipSetDevice(c->device));
     if (int rc_ = settle(c)) return rc_;
-    HIPCHK(c, hipStreamSynchronize(c->copy_stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return MISFIT_OK;
 }
@@ -287,7 +282,6 @@ extern "C" int misfit_timer_end(misfit_ctx* c, double* ms) {
 // Per-batch scratch for batches of up to U updates.
 static int ensure_batch_capacity(misfit_ctx* c, int64_t U) {
     if (U <= c->U_cap && c->P.ctr) return MISFIT_OK;
-    HIPCHK(c, hipStreamSynchronize(c->copy_stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     int64_t cap = std::max<int64_t>(std::max<int64_t>(U, 1 << 14), c->U_cap * 2);
     free_list(c->batch_allocs);
@@ -352,7 +346,6 @@ static int ensure_overlay_capacity(misfit_ctx* c, int64_t extra_ins) {
     int64_t need_ph = 2 * (c->ins_bound + extra_ins) + 1024;
     int64_t need_pool = c->pool_bound + 8 * extra_ins + 16;
     if (need_ph <= c->ph_cap && need_pool <= g.pool_cap) return MISFIT_OK;
-    HIPCHK(c, hipStreamSynchronize(c->copy_stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (need_pool > g.pool_cap) {
         int64_t nc = std::max<int64_t>(need_pool * 2, g.pool_cap * 2);
@@ -426,7 +419,6 @@ extern "C" int misfit_load_csr(misfit_ctx* c, int32_t n, int64_t m, const void* 
     if (n < 1 || m < 0 || !nindex || (m > 0 && !nlist)) return fail(c, MISFIT_E_ARG, "invalid graph arguments");
     HIPCHK(c, hipSetDevice(c->device));
     if (int rc_ = settle(c)) return rc_;
-    HIPCHK(c, hipStreamSynchronize(c->copy_stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     // the batch scratch keeps per-batch pointers only; graph arrays are new
     free_list(c->graph_allocs);
@@ -1124,7 +1116,6 @@ extern "C" int misfit_compact(misfit_ctx* c, int64_t* new_m) {
     if (!c->has_graph) return fail(c, MISFIT_E_STATE, "no graph loaded");
     HIPCHK(c, hipSetDevice(c->device));
     if (int rc_ = settle(c)) return rc_;
-    HIPCHK(c, hipStreamSynchronize(c->copy_stream));
     DevGraph& g = c->P.g;
     hipStream_t st = c->stream;
     // every device buffer made here is freed unless it becomes the new graph;

@@ -245,9 +245,9 @@ inline double percentile(std::vector<double> v, double q) {
 // (overlay update, may-uncover set, admissible index); 4 = the exact rule's
 // speculative rounds, the tail and the completion; 7 = the rest of the
 // host's time for the batch; 8 = speculative rounds.  Stages this engine
-// does not have (the upload is overlapped on the copy stream, results come
-// back through pinned memory, the overlay lives on the device, nothing is
-// clustered) print 0.000.  This engine's own detail follows on lines of
+// does not have (the first kernel reads the batch from pinned memory,
+// results come back through pinned memory, the overlay lives on the device,
+// nothing is clustered) print 0.000.  This engine's own detail follows on lines of
 // its own after "Finished MIS".
 struct BatchTimes {
     double total_ms = 0;   // from the start of its submission to the end of its collection
