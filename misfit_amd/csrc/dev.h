@@ -134,15 +134,19 @@ __device__ __forceinline__ u64 mix64(u64 z) {
     return z ^ (z >> 31);
 }
 
-// Value of an inserted pair: 1 present, 0 removed or never inserted.
-__device__ __forceinline__ int32_t ph_lookup(const DevGraph& g, u64 key) {
-    u64 s = mix64(key) & g.ph_mask;
+// Value of an inserted pair from probe slot s on: 1 present, 0 removed or
+// never inserted.
+__device__ __forceinline__ int32_t ph_lookup_from(const DevGraph& g, u64 key, u64 s) {
     while (true) {
         u64 k = g.ph_key[s];
         if (k == key) return g.ph_val[s];
         if (k == MF_EMPTY_KEY) return 0;
         s = (s + 1) & g.ph_mask;
     }
+}
+
+__device__ __forceinline__ int32_t ph_lookup(const DevGraph& g, u64 key) {
+    return ph_lookup_from(g, key, mix64(key) & g.ph_mask);
 }
 
 // Slot of `key`, inserting it when absent.  The host keeps the load factor
@@ -296,6 +300,22 @@ __device__ __forceinline__ void wave_push(int32_t* list, u64* ctr, int32_t v, bo
     if (lane == leader) base = atomicAdd(ctr, (u64)__popcll(bal));
     base = (u64)__shfl((long long)base, leader, MF_WAVE);
     if (pred) list[base + __popcll(bal & ((1ull << lane) - 1ull))] = v;
+}
+
+// wave_push of up to two values per lane (v1 when p1, v2 when p2), with one
+// atomic per wave.
+__device__ __forceinline__ void wave_push2(int32_t* list, u64* ctr, int32_t v1, bool p1, int32_t v2, bool p2) {
+    const u64 b1 = __ballot(p1), b2 = __ballot(p2);
+    if (!(b1 | b2)) return;
+    const int lane = lane_id();
+    const u64 below = (1ull << lane) - 1ull;
+    const int leader = first_lane(__ballot(1));
+    u64 base = 0;
+    if (lane == leader) base = atomicAdd(ctr, (u64)(__popcll(b1) + __popcll(b2)));
+    base = (u64)__shfl((long long)base, leader, MF_WAVE);
+    const u64 at = base + (u64)(__popcll(b1 & below) + __popcll(b2 & below));
+    if (p1) list[at] = v1;
+    if (p2) list[at + (p1 ? 1 : 0)] = v2;
 }
 
 // Wave-wide sum; every lane of the wave must be active.

@@ -194,13 +194,24 @@ __device__ __forceinline__ void search_pair(const BP& P, int64_t i, int gl) {
     const int32_t lo = a < bb ? a : bb, hi = a < bb ? bb : a;
     const bool first = gl < MF_PAIR_LANES / 2;
     const int gbase = lane_id() & ~(MF_PAIR_LANES / 2 - 1);
+    // the overlay hash's first probe, loaded before the searches and used
+    // after them when the pair is not in the base
+    const u64 key = pair_key(lo, hi);
+    const u64 h0 = mix64(key) & g.ph_mask;
+    u64 k0 = 0;
+    int32_t v0 = 0;
+    if (gl == 0) {
+        k0 = g.ph_key[h0];
+        v0 = g.ph_val[h0];
+    }
     const int64_t p = group_find<MF_PAIR_LANES / 2>(g, first ? lo : hi, first ? hi : lo, gl & (MF_PAIR_LANES / 2 - 1), gbase);
     const int64_t phi = __shfl((long long)p, lane_id() + MF_PAIR_LANES / 2, MF_WAVE);
     if (gl != 0) return;
     const int64_t plo = p;
     uint8_t fl = 0;
     if (plo >= 0) fl = g.nlist[plo] < 0 ? 2 : 0;
-    else fl = ph_lookup(g, pair_key(lo, hi)) == 1 ? 1 : 0;
+    else if (k0 == key) fl = v0 == 1 ? 1 : 0;
+    else if (k0 != MF_EMPTY_KEY) fl = ph_lookup_from(g, key, (h0 + 1) & g.ph_mask) == 1 ? 1 : 0;
     P.bt_plo[i] = plo;
     P.bt_phi[i] = phi;
     P.bt_fl[i] = fl;
@@ -224,6 +235,13 @@ __device__ void ph_claim(const BP& P, const GC& c) {
         if (t >= P.U) continue;
         const int64_t i = t;
         int32_t a = P.bu[i], bb = P.bv[i];
+        const bool ins = P.bins[i] != 0;
+        const int32_t w = a > bb ? a : bb;
+        // the tag exchanges are independent of the pair table: all in flight
+        // together with its insertion
+        const bool fa = atomicExch(&P.g.seedb[a], P.b) != P.b;
+        const bool fb = atomicExch(&P.g.seedb[bb], P.b) != P.b;
+        const bool fw = ins && atomicExch(&P.g.wpotb[w], P.b) != P.b;
         u64 key = pair_key(a, bb);
         u64 s = mix64(key) & P.bt_mask;
         while (true) {
@@ -234,10 +252,8 @@ __device__ void ph_claim(const BP& P, const GC& c) {
         int32_t prevh = atomicExch(&P.bt_head[s], (int32_t)i);
         P.bt_nxt[i] = prevh;
         P.bt_slot[i] = (int32_t)s;
-        wave_push(P.seeds, &P.ctr[MF_CS * (C_SEEDS)], a, atomicExch(&P.g.seedb[a], P.b) != P.b);
-        wave_push(P.seeds, &P.ctr[MF_CS * (C_SEEDS)], bb, atomicExch(&P.g.seedb[bb], P.b) != P.b);
-        int32_t w = a > bb ? a : bb;
-        wave_push(P.wpot, &P.ctr[MF_CS * (C_WPOT)], w, P.bins[i] && atomicExch(&P.g.wpotb[w], P.b) != P.b);
+        wave_push2(P.seeds, &P.ctr[MF_CS * (C_SEEDS)], a, fa, bb, fb);
+        wave_push(P.wpot, &P.ctr[MF_CS * (C_WPOT)], w, fw);
     }
 }
 
