@@ -290,12 +290,15 @@ __device__ void ph_resolve(const BP& P, const GC& c) {
         // new overlay entries of the pair headed by update i go to 2i, 2i+1
         P.ne_x[2 * i] = -1;
         P.ne_x[2 * i + 1] = -1;
-        int32_t slot = P.bt_slot[i];
-        if (P.bt_head[slot] != (int32_t)i) continue;
-        int32_t a = P.bu[i], bb = P.bv[i];
-        int32_t lo = a < bb ? a : bb, hi = a < bb ? bb : a;
+        // everything update i's own index gives is loaded beside its slot,
+        // before the head test (and the endpoints' count words with it)
+        const int32_t slot = P.bt_slot[i];
+        const int32_t a = P.bu[i], bb = P.bv[i];
         const int64_t plo = P.bt_plo[i], phi = P.bt_phi[i];
         const uint8_t fl = P.bt_fl[i];
+        const uint32_t ma = g.mc[a], mb = g.mc[bb];
+        if (P.bt_head[slot] != (int32_t)i) continue;
+        int32_t lo = a < bb ? a : bb, hi = a < bb ? bb : a;
         bool in_base = plo >= 0;
         bool tomb0 = (fl & 2) != 0;
         u64 key = pair_key(lo, hi);
@@ -355,7 +358,7 @@ __device__ void ph_resolve(const BP& P, const GC& c) {
         }
         if (adj != adj0) {
             uint32_t d = adj ? 1u : 0xFFFFFFFFu;  // +1 or -1 on the count bits
-            uint32_t mlo = g.mc[lo], mhi = g.mc[hi];
+            const uint32_t mlo = a < bb ? ma : mb, mhi = a < bb ? mb : ma;
             if (mhi & MF_MISBIT) atomicAdd(&g.mc[lo], d);
             if (mlo & MF_MISBIT) atomicAdd(&g.mc[hi], d);
         }
