@@ -345,16 +345,25 @@ __device__ void ph_resolve(const BP& P, const GC& c) {
             kill_run_entry(g, lo, hi);
             kill_run_entry(g, hi, lo);
         }
+        // new entries, chained per vertex: the head exchanges are issued
+        // before the hash slot's claim (independent), their results stored
+        // after it (nothing in this kernel reads the chains)
+        const bool fresh = alive && last_ins >= 0;
+        const int64_t k = 2 * i;
+        int32_t nlo = -1, nhi = -1;
+        if (fresh) {
+            nlo = atomicExch(&g.ne_head[lo], (int32_t)k);
+            nhi = atomicExch(&g.ne_head[hi], (int32_t)(k + 1));
+        }
         if (!in_base && (alive != alive0 || last_ins >= 0)) {
             u64 s = ph_slot(g, key);
             g.ph_val[s] = alive ? 1 : 0;
         }
-        if (alive && last_ins >= 0) {  // new entries, chained per vertex
-            int64_t k = 2 * i;
+        if (fresh) {
             P.ne_x[k] = lo; P.ne_y[k] = hi; P.ne_seq[k] = last_ins;
             P.ne_x[k + 1] = hi; P.ne_y[k + 1] = lo; P.ne_seq[k + 1] = last_ins;
-            P.ne_nxt[k] = atomicExch(&g.ne_head[lo], (int32_t)k);
-            P.ne_nxt[k + 1] = atomicExch(&g.ne_head[hi], (int32_t)(k + 1));
+            P.ne_nxt[k] = nlo;
+            P.ne_nxt[k + 1] = nhi;
         }
         if (adj != adj0) {
             uint32_t d = adj ? 1u : 0xFFFFFFFFu;  // +1 or -1 on the count bits
