@@ -475,7 +475,8 @@ extern "C" int misfit_load_csr(misfit_ctx* c, int32_t n, int64_t m, const void* 
     HIPCHK(c, dalloc(A, &g.cmark, n));
     HIPCHK(c, dalloc(A, &g.cdec, n));
     HIPCHK(c, dalloc(A, &g.ne_head, n));
-    HIPCHK(c, dalloc(A, &g.fa_cnt, n));
+    HIPCHK(c, dalloc(A, &g.fa_tag, n));
+    HIPCHK(c, dalloc(A, &g.fa_n, n));
     HIPCHK(c, dalloc(A, &g.fa_start, n));
     HIPCHK(c, dalloc(A, &g.fa_fill, n));
     BP& P = c->P;
@@ -515,7 +516,8 @@ extern "C" int misfit_load_csr(misfit_ctx* c, int32_t n, int64_t m, const void* 
     mf_fill_u64(g.cmark, 0ull, n, st);
     mf_fill_u64(g.cdec, 0ull, n, st);
     mf_fill_i32(g.ne_head, -1, n, st);
-    mf_fill_u64(g.fa_cnt, 0ull, n, st);
+    mf_fill_u32(g.fa_tag, 0u, n, st);
+    mf_fill_u32(g.fa_n, 0u, n, st);
     mf_fill_i64(g.fa_start, -1, n, st);
     mf_fill_i32(g.fa_fill, 0, n, st);
     mf_fill_u32(S.d_bmark, 0u, n, st);

@@ -88,7 +88,8 @@ struct DevGraph {
     u64* cmark;
     u64* cdec;       // round<<32 | evictions next to the vertex speculated this round
     int32_t* ne_head;
-    u64* fa_cnt;     // b<<32 | count
+    uint32_t* fa_tag;  // = b: the row has admissible-index entries in batch b
+    uint32_t* fa_n;    // entries counted for the index (reset by the allocation)
     int64_t* fa_start;
     int32_t* fa_fill;
     int sorted;      // every base row strictly ascending

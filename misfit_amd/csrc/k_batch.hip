@@ -66,22 +66,6 @@ __device__ __forceinline__ void pu_add(const BP& P, int32_t z, bool pred) {
     wave_push(P.pu, &P.ctr[MF_CS * (C_PU)], z, first);
 }
 
-// Epoch-tagged counter word b<<32 | count.  Returns true on the first touch
-// in this batch.
-__device__ __forceinline__ bool epoch_count_add(u64* w, uint32_t b) {
-    u64 hi = (u64)b << 32;
-    u64 old = *w;
-    while (true) {
-        if ((old & 0xFFFFFFFF00000000ull) == hi) {
-            atomicAdd(w, 1ull);
-            return false;
-        }
-        u64 prev = atomicCAS(w, old, hi | 1ull);
-        if (prev == old) return true;
-        old = prev;
-    }
-}
-
 __device__ __forceinline__ bool mk_before(const u64* mk, int32_t x, uint32_t R, int32_t e) {
     u64 m = mk[x];
     return (uint32_t)(m >> 32) == R && (int64_t)(0xFFFFFFFFu - (uint32_t)m) < (int64_t)e;
@@ -604,8 +588,12 @@ __device__ void ph_fa_count(const BP& P, const GC& c) {
         for (int64_t j = c.lane; j < L; j += MF_WAVE) {
             int32_t x1 = r.at(j);
             if (x1 < 0 || x1 == z) continue;
-            if (row_len(g, x1) <= MF_FA_T) continue;
-            wave_push(P.big, &P.ctr[MF_CS * (C_BIG)], x1, epoch_count_add(&g.fa_cnt[x1], P.b));
+            if (MF_FA_T > 0 && row_len(g, x1) <= MF_FA_T) continue;
+            // one returning exchange tells the first toucher of the batch;
+            // the count is a no-return add (the allocation resets it)
+            const bool first = atomicExch(&g.fa_tag[x1], P.b) != P.b;
+            atomicAdd(&g.fa_n[x1], 1u);
+            wave_push(P.big, &P.ctr[MF_CS * (C_BIG)], x1, first);
         }
     }
 }
@@ -617,7 +605,8 @@ __device__ void ph_fa_alloc(const BP& P, const GC& c) {
     if (ovf) nb = P.big_cap;
     for (int64_t i = c.tid; i < nb; i += c.nthreads) {
         int32_t x1 = P.big[i];
-        u64 cnt = P.g.fa_cnt[x1] & 0xFFFFFFFFull;
+        const u64 cnt = P.g.fa_n[x1];
+        P.g.fa_n[x1] = 0;  // zero again for the next batch's count
         u64 s = atomicAdd(&P.ctr[MF_CS * (C_FA_TOP)], cnt);
         P.g.fa_start[x1] = (ovf || (int64_t)(s + cnt) > P.fa_cap) ? -1 : (int64_t)s;
         P.g.fa_fill[x1] = 0;
@@ -647,7 +636,7 @@ __device__ void ph_fa_fill(const BP& P, const GC& c) {
             int32_t x1 = -1;
             if (j < L) {
                 const int32_t y = r.at(j);
-                if (y >= 0 && y != z && row_len(g, y) > MF_FA_T && g.fa_start[y] >= 0) x1 = y;
+                if (y >= 0 && y != z && (MF_FA_T == 0 || row_len(g, y) > MF_FA_T) && g.fa_start[y] >= 0) x1 = y;
             }
             if (x1 < 0) continue;
             const int64_t p = group_find<G>(g, x1, z, gl, gbase);
@@ -858,7 +847,7 @@ __device__ int32_t eval_insertion(const BP& P, int32_t w, Recorder& rc, SpecMark
             if (x1 >= 0 && x1 != w) {
                 // every row with an admissible neighbour has index entries
                 // this batch; only an overflowed index falls back to the row
-                bool indexed = (uint32_t)(g.fa_cnt[x1] >> 32) == P.b;
+                bool indexed = g.fa_tag[x1] == P.b;
                 int64_t st = indexed && fa_ok ? g.fa_start[x1] : -1;
                 if (st >= 0) {
                     kindl = 2;
