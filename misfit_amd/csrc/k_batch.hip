@@ -1498,14 +1498,17 @@ __device__ void publish(const BP& P, const GC& c) {
 }
 
 // Remaining speculative rounds in one workgroup; returns false when a loop
-// bound was hit.
-__device__ bool finish_rounds(const BP& P, const GC& c, int par) {
+// bound was hit.  `ran`: some round ran (nothing before the first needs a
+// barrier: the kernel's start sees what earlier kernels wrote).
+__device__ bool finish_rounds(const BP& P, const GC& c, int par, bool& ran) {
     uint32_t guard = 0;
+    ran = false;
     while (true) {
-        phase_barrier();
+        if (ran) phase_barrier();
         const int64_t win = P.window < MF_FIN_WINDOW ? P.window : MF_FIN_WINDOW;
         RoundView v = round_view(P, par, win);
         if (!v.live) return true;
+        ran = true;
         if ((int64_t)guard++ > P.U + 1) {  // every round commits its first edge
             if (c.tid == 0) P.ctr[MF_CS * (C_GUARD)] = 1;
             return false;
@@ -1561,8 +1564,9 @@ __global__ __launch_bounds__(1024) void k_finish(BP P, int par) {
             return;
         }
     }
-    if (finish_rounds(P, c, par)) {
-        phase_barrier();
+    bool ran = false;
+    if (finish_rounds(P, c, par, ran)) {
+        if (ran) phase_barrier();
         ph_stats(P, c);
         if (P.completion) {
             ph_comp_collect(P, c);
