@@ -64,7 +64,7 @@ struct misfit_ctx {
     int hist_pos = 0;
     bool always_halt = false;     // every batch may halt (nothing is ever queued behind one)
     double window_factor = 16.0;  // MISFIT_WINDOW_FACTOR: window = this x expected edges per round
-    int round_margin = 3;         // MISFIT_ROUND_MARGIN: rounds queued beyond the planned rate
+    int round_margin = -1;        // MISFIT_ROUND_MARGIN: rounds queued beyond the planned rate (-1: by the plan)
     double clock_khz = 0.0;       // device wall clock, for per-batch device times
     hipEvent_t timer0 = nullptr, timer1 = nullptr;  // misfit_timer_begin/end
     int round_per_cu = 0;      // MISFIT_ROUND_WG_PER_CU: round grid (0: by occupancy)
@@ -691,7 +691,8 @@ static void launch_queued(const BP& P, const LaunchCfg& L, int k) {
 // The rounds of a batch of `count` updates.  Their number: one round plus
 // a high rate of further rounds per update seen on recent batches at least
 // a quarter as large (conflicts per update grow with the batch, so larger
-// batches bound smaller ones), plus two, and at least one round per window;
+// batches bound smaller ones), plus a margin, and at least one round per
+// window;
 // rounds past the end of the batch cost one empty launch each.
 // Their window (edges speculated per round, one per wave of the round
 // grid): the whole GPU, or 16 times the edges a round is expected to commit
@@ -755,11 +756,15 @@ static RoundPlan plan_rounds(misfit_ctx* c, int64_t count, int64_t full_window) 
         p.sure = true;
         return p;
     }
-    // with enough history the second largest rate plus two rounds (about
-    // the 90th percentile of 16), else the largest with a margin: a round
-    // left to the one-workgroup tail costs dozens of empty grid rounds
-    const double mg = (double)c->round_margin;
-    double k = eligible >= 6 ? std::ceil((double)count * rpu2) + mg : std::ceil((double)count * rpu * 1.15) + mg;
+    // with enough history the second largest rate (about the 90th
+    // percentile of 16), else the largest with 15% more, then a margin that
+    // grows with the planned rounds (a fifth of them, 1 to 3): a round
+    // queued past the end costs one empty round of three kernels, a round
+    // missing costs a one-workgroup round in the tail, far slower on large
+    // batches
+    const double base = eligible >= 6 ? std::ceil((double)count * rpu2) : std::ceil((double)count * rpu * 1.15);
+    const double mg = c->round_margin >= 0 ? (double)c->round_margin : std::min(3.0, std::max(1.0, std::ceil(0.2 * base)));
+    double k = base + mg;
     double per_window = std::ceil((double)count / (double)std::max<int64_t>(p.window, 1)) + 2.0;
     k = std::max(k, per_window);
     p.k = (int)std::min(k, 65536.0);
