@@ -443,11 +443,13 @@ __device__ void ph_append(const BP& P, const GC& c) {
 // cut the rows of W's MIS members into chunks for the counting pass.
 __device__ void ph_partner(const BP& P, const GC& c) {
     for (int64_t i = c.tid; i < P.U; i += c.nthreads) {
-        if (!P.bins[i]) continue;
-        int32_t a = P.bu[i], bb = P.bv[i];
+        const bool ins = P.bins[i] != 0;
+        const int32_t a = P.bu[i], bb = P.bv[i];
+        const uint8_t fl = P.bt_fl[i];
+        if (!ins) continue;
         int32_t w = a > bb ? a : bb, o = a > bb ? bb : a;
-        if (P.g.pfb[w] == P.b) continue;
-        if (P.g.wpotb[o] == P.b || !(P.bt_fl[i] & 4)) P.g.pfb[w] = P.b;  // bit 2: adjacent after the batch (P2)
+        // (the store is idempotent: no test of pfb[w] first)
+        if (P.g.wpotb[o] == P.b || !(fl & 4)) P.g.pfb[w] = P.b;  // bit 2: adjacent after the batch (P2)
     }
     int64_t nw = (int64_t)P.ctr[MF_CS * (C_WPOT)];
     // whole waves step together so the reservation below is wave-wide
@@ -517,6 +519,13 @@ __device__ __forceinline__ void pu_scan_one(const BP& P, int32_t x, uint32_t m, 
     pu_add(P, x, p);
 }
 
+// A tagged vertex: its count word and k-count loaded together.
+__device__ __forceinline__ void pu_scan_tagged(const BP& P, int32_t x) {
+    const uint32_t m = P.g.mc[x];
+    const uint32_t k = (uint32_t)(P.g.kw[x] & 0xFFFFFFFFull);
+    pu_add(P, x, m == 0 || (!(m & MF_MISBIT) && (m & MF_CMASK) == k));
+}
+
 // With a maximal MIS before the batch (P.pu_list), a vertex outside the MIS
 // with no MIS neighbour after the overlay update is an endpoint of a deleted
 // edge (the seeds, added in P5), so only the tagged vertices are tested: a
@@ -534,7 +543,7 @@ __device__ void ph_pu_scan(const BP& P, const GC& c) {
             while (w) {
                 const int32_t x = (int32_t)(32 * q + __ffs(w) - 1);
                 w &= w - 1;
-                pu_scan_one(P, x, g.mc[x], true);
+                pu_scan_tagged(P, x);
             }
         }
         return;
