@@ -324,9 +324,12 @@ static int ensure_batch_capacity(misfit_ctx* c, int64_t U) {
         HIPCHK(c, dalloc(A, &s.d_u, cap));
         HIPCHK(c, dalloc(A, &s.d_v, cap));
         HIPCHK(c, dalloc(A, &s.d_ins, cap));
-        HIPCHK(c, hipHostMalloc((void**)&s.h_u, cap * sizeof(int32_t), hipHostMallocDefault));
-        HIPCHK(c, hipHostMalloc((void**)&s.h_v, cap * sizeof(int32_t), hipHostMallocDefault));
-        HIPCHK(c, hipHostMalloc((void**)&s.h_ins, cap, hipHostMallocDefault));
+        // read by the batch's first kernel over PCIe, and written again for
+        // a later batch of the slot: coherent, so no device cache can hold a
+        // previous batch's lines
+        HIPCHK(c, hipHostMalloc((void**)&s.h_u, cap * sizeof(int32_t), hipHostMallocCoherent));
+        HIPCHK(c, hipHostMalloc((void**)&s.h_v, cap * sizeof(int32_t), hipHostMallocCoherent));
+        HIPCHK(c, hipHostMalloc((void**)&s.h_ins, cap, hipHostMallocCoherent));
         void* dp = nullptr;
         HIPCHK(c, hipHostGetDevicePointer(&dp, s.h_u, 0));
         s.g_u = (const int32_t*)dp;
