@@ -33,6 +33,7 @@ extern "C" {
 #define MISFIT_E_CHECK (-7)      /* an internal consistency check failed */
 
 typedef struct misfit_ctx misfit_ctx;
+typedef struct misfit_staged misfit_staged;
 
 /* One edge update.  Vertices are 0-indexed.  is_insertion follows the
  * reference's flag: computed once against the graph as loaded
@@ -118,6 +119,20 @@ int misfit_flags(misfit_ctx* ctx, const int32_t* u, const int32_t* v, int64_t co
  * (src/misfit_async/misfit_async.cu:484-675) together with the overlay
  * update of launchBatchGPU (:718-740). */
 int misfit_submit_batch(misfit_ctx* ctx, const misfit_update* ups, int64_t count, uint64_t* ticket);
+
+/* The host-to-device copy of a batch, taken out of the submission: the
+ * reference stages every batch from pinned host memory to the device before
+ * its kernels run (src/misfit/misfit.cu:129-149,612-619); here
+ * misfit_stage_batch checks the updates as misfit_submit_batch does and
+ * uploads them to device memory owned by the context, and
+ * misfit_submit_staged queues such a batch with no host copy (the same
+ * results as misfit_submit_batch with the same updates).  A staged batch may
+ * be submitted any number of times, on its own context only.
+ * misfit_release_staged waits for the context's stream and frees it; what
+ * is not released is freed by misfit_ctx_destroy. */
+int misfit_stage_batch(misfit_ctx* ctx, const misfit_update* ups, int64_t count, misfit_staged** out);
+int misfit_submit_staged(misfit_ctx* ctx, const misfit_staged* batch, uint64_t* ticket);
+int misfit_release_staged(misfit_ctx* ctx, misfit_staged* batch);
 
 /* Waits for a submitted batch and returns its statistics (st may be NULL).
  * Replaces syncAndCollect (src/misfit_async/misfit_async.cu:677-701). */

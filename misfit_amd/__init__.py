@@ -92,6 +92,7 @@ EXPORTS = (
     "misfit_batch_bytes", "misfit_device_info", "misfit_egr_open", "misfit_egr_close", "misfit_write_egr",
     "misfit_gen_er", "misfit_gen_rmat", "misfit_gen_rgg", "misfit_greedy_mis", "misfit_gen_updates",
     "misfit_mtx_to_egr", "misfit_egr_check", "misfit_poll", "misfit_timer_begin", "misfit_timer_end",
+    "misfit_stage_batch", "misfit_submit_staged", "misfit_release_staged",
     "misfit_server_create", "misfit_server_destroy", "misfit_server_last_error", "misfit_server_add_graph",
     "misfit_server_remove_graph", "misfit_server_submit", "misfit_server_wait", "misfit_server_get_mis",
     "misfit_server_check",
@@ -120,6 +121,9 @@ def lib():
         L.misfit_set_mode.argtypes = [vp, ctypes.c_int]
         L.misfit_flags.argtypes = [vp, _i32p, _i32p, ctypes.c_int64, _u8p]
         L.misfit_submit_batch.argtypes = [vp, vp, ctypes.c_int64, _u64p]
+        L.misfit_stage_batch.argtypes = [vp, vp, ctypes.c_int64, ctypes.POINTER(vp)]
+        L.misfit_submit_staged.argtypes = [vp, vp, _u64p]
+        L.misfit_release_staged.argtypes = [vp, vp]
         L.misfit_wait.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(BatchStats)]
         L.misfit_process_batch.argtypes = [vp, vp, ctypes.c_int64, ctypes.POINTER(BatchStats)]
         L.misfit_sync.argtypes = [vp]
@@ -394,6 +398,25 @@ class Engine:
         self._ck(lib().misfit_submit_batch(self.ctx, updates.ctypes.data_as(ctypes.c_void_p), len(updates),
                                            ctypes.byref(t)))
         return t.value
+
+    def stage(self, updates):
+        """Checks a batch and uploads it to device memory; returns a handle
+        for submit_staged (freed by release_staged or with the engine)."""
+        updates = np.ascontiguousarray(updates, UPDATE_DTYPE)
+        h = ctypes.c_void_p()
+        self._ck(lib().misfit_stage_batch(self.ctx, updates.ctypes.data_as(ctypes.c_void_p), len(updates),
+                                          ctypes.byref(h)))
+        return h
+
+    def submit_staged(self, handle):
+        """Queues a staged batch (no host copy); returns its ticket."""
+        t = ctypes.c_uint64()
+        self._ck(lib().misfit_submit_staged(self.ctx, handle, ctypes.byref(t)))
+        return t.value
+
+    def release_staged(self, handle):
+        """Waits for the engine's stream and frees a staged batch."""
+        self._ck(lib().misfit_release_staged(self.ctx, handle))
 
     def wait(self, ticket):
         st = BatchStats()

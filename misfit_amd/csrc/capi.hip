@@ -108,6 +108,17 @@ struct misfit_ctx {
     u64* h_small = nullptr;    // pinned scratch for small readbacks
     Slot slots[MF_NSLOT];
     uint64_t next_ticket = 1;
+    std::vector<misfit_staged*> staged;  // batches uploaded ahead of submission, freed on destroy
+};
+
+// A batch uploaded to device memory ahead of its submission.
+struct misfit_staged {
+    int32_t* u = nullptr;
+    int32_t* v = nullptr;
+    uint8_t* ins = nullptr;
+    int64_t count = 0;
+    int64_t nins = 0;
+    int32_t n = 0;  // the vertex count the endpoints were checked against
 };
 
 // ---------------------------------------------------------------- errors
@@ -229,6 +240,10 @@ extern "C" int misfit_ctx_destroy(misfit_ctx* c) {
     free_list(c->batch_allocs);
     free_list(c->graph_allocs);
     free_compact_scratch(c);
+    for (misfit_staged* b : c->staged) {
+        hipFree(b->u);
+        delete b;
+    }
     if (c->d_ktask) hipFree(c->d_ktask);
     if (c->d_sval) hipFree(c->d_sval);
     if (c->d_spos) hipFree(c->d_spos);
@@ -886,8 +901,7 @@ static int ensure_ktask_capacity(misfit_ctx* c, int64_t U) {
     return MISFIT_OK;
 }
 
-extern "C" int misfit_submit_batch(misfit_ctx* c, const misfit_update* ups, int64_t count, uint64_t* ticket) {
-    if (!c || count < 0 || (count && !ups)) return MISFIT_E_ARG;
+static int check_updates(misfit_ctx* c, const misfit_update* ups, int64_t count, int64_t* nins_out) {
     if (!c->has_graph) return fail(c, MISFIT_E_STATE, "no graph loaded");
     if (count > INT32_MAX / 4) return fail(c, MISFIT_E_ARG, "batch too large");
     int32_t n = c->P.g.n;
@@ -897,6 +911,14 @@ extern "C" int misfit_submit_batch(misfit_ctx* c, const misfit_update* ups, int6
             return fail(c, MISFIT_E_ARG, "update " + std::to_string(i) + ": endpoints out of range or equal");
         nins += ups[i].is_insertion ? 1 : 0;
     }
+    *nins_out = nins;
+    return MISFIT_OK;
+}
+
+// Queues one batch: from the caller's host array (ups, copied into the
+// slot's pinned staging) or from a batch already in device memory (st).
+static int submit(misfit_ctx* c, const misfit_update* ups, const misfit_staged* st, int64_t count,
+                  int64_t nins, uint64_t* ticket) {
     HIPCHK(c, hipSetDevice(c->device));
     int rc = settle(c);
     if (rc) return rc;
@@ -909,10 +931,12 @@ extern "C" int misfit_submit_batch(misfit_ctx* c, const misfit_update* ups, int6
     uint64_t t = c->next_ticket++;
     Slot& s = c->slots[t % MF_NSLOT];
     if (s.busy) HIPCHK(c, hipEventSynchronize(s.done));  // staging still owned by an older batch
-    for (int64_t i = 0; i < count; i++) {
-        s.h_u[i] = ups[i].u;
-        s.h_v[i] = ups[i].v;
-        s.h_ins[i] = ups[i].is_insertion ? 1 : 0;
+    if (!st) {
+        for (int64_t i = 0; i < count; i++) {
+            s.h_u[i] = ups[i].u;
+            s.h_v[i] = ups[i].v;
+            s.h_ins[i] = ups[i].is_insertion ? 1 : 0;
+        }
     }
     s.count = count;
     s.ticket = t;
@@ -925,11 +949,11 @@ extern "C" int misfit_submit_batch(misfit_ctx* c, const misfit_update* ups, int6
     P.bu = s.d_u;
     P.bv = s.d_v;
     P.bins = s.d_ins;
-    // the first kernel reads the staged batch (the slot is not written again
-    // before this batch's end event)
-    P.hu = count ? s.g_u : nullptr;
-    P.hv = count ? s.g_v : nullptr;
-    P.hins = count ? s.g_ins : nullptr;
+    // the first kernel reads the batch: the slot's pinned staging (not
+    // written again before this batch's end event) or the device copy
+    P.hu = !count ? nullptr : st ? st->u : s.g_u;
+    P.hv = !count ? nullptr : st ? st->v : s.g_v;
+    P.hins = !count ? nullptr : st ? st->ins : s.g_ins;
     P.U = count;
     P.b = c->epoch;
     P.completion = c->completion;
@@ -963,6 +987,72 @@ extern "C" int misfit_submit_batch(misfit_ctx* c, const misfit_update* ups, int6
     // the plain driver)
     HIPCHK(c, hipEventRecord(s.done, c->stream));
     if (ticket) *ticket = t;
+    return MISFIT_OK;
+}
+
+extern "C" int misfit_submit_batch(misfit_ctx* c, const misfit_update* ups, int64_t count, uint64_t* ticket) {
+    if (!c || count < 0 || (count && !ups)) return MISFIT_E_ARG;
+    int64_t nins = 0;
+    if (int rc = check_updates(c, ups, count, &nins)) return rc;
+    return submit(c, ups, nullptr, count, nins, ticket);
+}
+
+extern "C" int misfit_stage_batch(misfit_ctx* c, const misfit_update* ups, int64_t count, misfit_staged** out) {
+    if (!c || count < 0 || (count && !ups) || !out) return MISFIT_E_ARG;
+    int64_t nins = 0;
+    if (int rc = check_updates(c, ups, count, &nins)) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    // one allocation: u, v (int32 each), then the flags
+    const size_t nb = (size_t)std::max<int64_t>(count, 1);
+    std::vector<uint8_t> h(nb * 9);
+    int32_t* hu = (int32_t*)h.data();
+    int32_t* hv = hu + nb;
+    uint8_t* hi = (uint8_t*)(hv + nb);
+    for (int64_t i = 0; i < count; i++) {
+        hu[i] = ups[i].u;
+        hv[i] = ups[i].v;
+        hi[i] = ups[i].is_insertion ? 1 : 0;
+    }
+    void* d = nullptr;
+    HIPCHK(c, hipMalloc(&d, h.size()));
+    hipError_t e = hipMemcpy(d, h.data(), h.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipFree(d);
+        return fail(c, MISFIT_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    }
+    misfit_staged* b = new misfit_staged;
+    b->u = (int32_t*)d;
+    b->v = b->u + nb;
+    b->ins = (uint8_t*)(b->v + nb);
+    b->count = count;
+    b->nins = nins;
+    b->n = c->P.g.n;
+    c->staged.push_back(b);
+    *out = b;
+    return MISFIT_OK;
+}
+
+static bool owns_staged(const misfit_ctx* c, const misfit_staged* b) {
+    return std::find(c->staged.begin(), c->staged.end(), b) != c->staged.end();
+}
+
+extern "C" int misfit_submit_staged(misfit_ctx* c, const misfit_staged* b, uint64_t* ticket) {
+    if (!c || !b) return MISFIT_E_ARG;
+    if (!owns_staged(c, b)) return fail(c, MISFIT_E_ARG, "staged batch does not belong to this context");
+    if (!c->has_graph) return fail(c, MISFIT_E_STATE, "no graph loaded");
+    if (b->n > c->P.g.n) return fail(c, MISFIT_E_STATE, "staged batch was checked against a larger graph");
+    return submit(c, nullptr, b, b->count, b->nins, ticket);
+}
+
+extern "C" int misfit_release_staged(misfit_ctx* c, misfit_staged* b) {
+    if (!c || !b) return MISFIT_E_ARG;
+    if (!owns_staged(c, b)) return fail(c, MISFIT_E_ARG, "staged batch does not belong to this context");
+    HIPCHK(c, hipSetDevice(c->device));
+    // batches that read it may still be queued
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->staged.erase(std::find(c->staged.begin(), c->staged.end(), b));
+    hipFree(b->u);
+    delete b;
     return MISFIT_OK;
 }
 

@@ -393,3 +393,47 @@ def test_device_times_and_timer(engine_mod, eng, mode):
     # the region holds every batch's device time
     assert region >= 0.9 * sum(s["device_ms"] for s in stats)
     assert eng.check() == (0, 0, 0)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_staged_batches_parity(oracle, engine_mod, eng, mode):
+    """Batches uploaded ahead of submission give the oracle's MIS after every
+    batch, two in flight; an empty batch stages; a staged batch is refused by
+    another context; release frees it after the work that reads it."""
+    rng = np.random.default_rng(31)
+    n = 500
+    ni, nl = H.random_graph(rng, n, 0.02)
+    mis0 = oracle.Graph.from_csr(ni, nl).greedy_mis()
+    batches = H.random_batches(rng, oracle, ni, nl, 6, 120, general=True)
+    want, _, _ = H.oracle_replay(oracle, ni, nl, mis0, batches, completion=True, literal=True)
+    eng.load_csr(ni, nl)
+    eng.set_mis(mis0)
+    eng.set_rule(True)
+    eng.set_mode(mode)
+    hs = [eng.stage(engine_mod.make_updates(*b)) for b in batches]
+    empty = eng.stage(engine_mod.make_updates(np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0, np.uint8)))
+    tickets = [eng.submit_staged(hs[0])]
+    for b in range(len(batches)):
+        if b + 1 < len(batches):
+            tickets.append(eng.submit_staged(hs[b + 1]))
+        st = eng.wait(tickets[b])
+        assert st["updates"] == len(batches[b][0])
+        if b + 1 == len(batches):
+            assert np.array_equal(eng.get_mis(), want[b])
+    assert eng.wait(eng.submit_staged(empty))["updates"] == 0
+    assert np.array_equal(eng.get_mis(), want[-1])
+    assert eng.check() == (0, 0, 0)
+    with engine_mod.Engine(0) as other:
+        other.load_csr(ni, nl)
+        with pytest.raises(engine_mod.MisfitError):
+            other.submit_staged(hs[0])
+    # one at a time: the MIS after every batch
+    eng.load_csr(ni, nl)
+    eng.set_mis(mis0)
+    for b in range(len(batches)):
+        eng.wait(eng.submit_staged(hs[b]))
+        assert np.array_equal(eng.get_mis(), want[b]), "batch %d differs" % b
+    for h in hs + [empty]:
+        eng.release_staged(h)
+    with pytest.raises(engine_mod.MisfitError):
+        eng.submit_staged(hs[0])
