@@ -667,7 +667,7 @@ def main():
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--batch", type=int, default=10000)
     ap.add_argument("--seed", type=int, default=7)
-    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--served-only", action="store_true", help="skip the plain driver (profiling runs)")
     ap.add_argument("--no-stream-copy", action="store_true", help="skip the copy-bandwidth calibration")
