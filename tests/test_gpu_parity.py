@@ -437,3 +437,38 @@ def test_staged_batches_parity(oracle, engine_mod, eng, mode):
         eng.release_staged(h)
     with pytest.raises(engine_mod.MisfitError):
         eng.submit_staged(hs[0])
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_invalid_updates_refused_without_effect(oracle, engine_mod, eng, mode):
+    """Endpoints out of range, negative or equal are refused by submit, stage
+    and process_batch with MISFIT_E_ARG naming the update, before anything
+    reaches the device: the MIS and the graph stay as they were, and the
+    next valid batch gives the oracle's result."""
+    rng = np.random.default_rng(41)
+    n = 200
+    ni, nl = H.random_graph(rng, n, 0.04)
+    mis0 = oracle.Graph.from_csr(ni, nl).greedy_mis()
+    batches = H.random_batches(rng, oracle, ni, nl, 2, 60, general=True)
+    want, _, _ = H.oracle_replay(oracle, ni, nl, mis0, batches, completion=True, literal=True)
+    eng.load_csr(ni, nl)
+    eng.set_mis(mis0)
+    eng.set_rule(True)
+    eng.set_mode(mode)
+    eng.wait(eng.submit(engine_mod.make_updates(*batches[0])))
+    assert np.array_equal(eng.get_mis(), want[0])
+    size0 = eng.graph_size()
+    u, v, ins = (np.array(a) for a in batches[1])
+    for bad_u, bad_v in ((n, 0), (-1, 3), (7, 7), (0, n + 5)):
+        bu, bv = u.copy(), v.copy()
+        bu[17], bv[17] = bad_u, bad_v
+        ups = engine_mod.make_updates(bu, bv, ins)
+        for call in (eng.submit, eng.stage, eng.process_batch):
+            with pytest.raises(engine_mod.MisfitError) as ei:
+                call(ups)
+            assert ei.value.code == -1 and "update 17" in str(ei.value)
+    assert np.array_equal(eng.get_mis(), want[0])
+    assert eng.graph_size() == size0
+    eng.wait(eng.submit(engine_mod.make_updates(u, v, ins)))
+    assert np.array_equal(eng.get_mis(), want[1])
+    assert eng.check() == (0, 0, 0)
