@@ -103,6 +103,20 @@ KERNEL_NOTE = ("queued batch (prologue, grid rounds, finisher): device time of t
                "events on the engine's stream, divided by the batches in it")
 
 
+def host_info():
+    """The host the CPU baseline ran on: CPU model and logical CPUs."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count()}
+
+
 def pct(v, q):
     return float(np.percentile(np.asarray(v, np.float64), q)) if len(v) else 0.0
 
@@ -219,7 +233,7 @@ def cpu_baseline(M, eng, ni, nl, mis0, ups, budget_s):
     eng.process_batch(ups[:n_up])
     same = bool(np.array_equal(eng.get_mis(), m_cpu))
     del g0
-    return {"value": n_up / dt, "unit": "edge updates/s", "cores": threads, "kind": "port",
+    return {"value": n_up / dt, "unit": "edge updates/s", "cores": threads, "kind": "port", **host_info(),
             "sample": "first %d updates of the stream as one batch on the loaded graph, literal 2-hop rows "
                       "(OpenMP, %d threads) + sequential rule replay + completion; %.2f s" % (n_up, threads, dt),
             "bit_exact_vs_gpu": same}
@@ -320,7 +334,7 @@ def cpu_stream(M, eng, ni, nl, mis0, ups, batch, nb, budget_s):
         eng.process_batch(ups[b * batch:(b + 1) * batch])
     same = bool(np.array_equal(eng.get_mis(), m))
     del g
-    return {"value": done * batch / sum(lat), "unit": "edge updates/s", "cores": threads, "kind": "port",
+    return {"value": done * batch / sum(lat), "unit": "edge updates/s", "cores": threads, "kind": "port", **host_info(),
             "sample": "first %d of %d batches of the stream, in order, literal 2-hop rows (OpenMP, %d threads) "
                       "+ sequential rule replay + completion; %.2f s" % (done, nb, threads, sum(lat)),
             "p50_ms": round(pct([x * 1e3 for x in lat], 50), 3), "p99_ms": round(pct([x * 1e3 for x in lat], 99), 3),
@@ -646,7 +660,7 @@ def cpu_many(M, graphs, budget_s):
             eng.set_mode(M.MODE_PLAIN)
             eng.process_batch(g["batches"][0])
             same = same and bool(np.array_equal(eng.get_mis(), m))
-    return {"value": ups_done / spent, "unit": "edge updates/s", "cores": threads, "kind": "port",
+    return {"value": ups_done / spent, "unit": "edge updates/s", "cores": threads, "kind": "port", **host_info(),
             "sample": "first batch of %d of the %d graphs, smallest first (%d updates), literal 2-hop rows "
                       "(OpenMP, %d threads) + sequential rule replay + completion; %.2f s"
                       % (len(done), len(graphs), ups_done, threads, spent),
