@@ -607,17 +607,31 @@ __device__ void ph_fa_count(const BP& P, const GC& c) {
     }
 }
 
-// P9: reserve index space per long row (no space: the row is scanned).
+// P9: reserve index space per long row (no space: the row is scanned).  One
+// reservation per wave on the pool top: returning atomics on one address
+// serialise.
 __device__ void ph_fa_alloc(const BP& P, const GC& c) {
     int64_t nb = (int64_t)P.ctr[MF_CS * (C_BIG)];
     bool ovf = nb > P.big_cap;
     if (ovf) nb = P.big_cap;
-    for (int64_t i = c.tid; i < nb; i += c.nthreads) {
-        int32_t x1 = P.big[i];
-        const u64 cnt = P.g.fa_n[x1];
+    // whole waves step together so the reservation below is wave-wide
+    for (int64_t i0 = c.tid - c.lane; i0 < nb; i0 += c.nthreads) {
+        const int64_t i = i0 + c.lane;
+        int32_t x1 = -1;
+        int64_t cnt = 0;
+        if (i < nb) {
+            x1 = P.big[i];
+            cnt = (int64_t)P.g.fa_n[x1];
+        }
+        const int64_t incl = wave_incl_scan(cnt);
+        const int64_t total = __shfl(incl, MF_WAVE - 1, MF_WAVE);
+        u64 base = 0;
+        if (c.lane == 0 && total) base = atomicAdd(&P.ctr[MF_CS * (C_FA_TOP)], (u64)total);
+        base = (u64)__shfl((long long)base, 0, MF_WAVE);
+        if (x1 < 0) continue;
+        const int64_t s = (int64_t)base + (incl - cnt);
         P.g.fa_n[x1] = 0;  // zero again for the next batch's count
-        u64 s = atomicAdd(&P.ctr[MF_CS * (C_FA_TOP)], cnt);
-        P.g.fa_start[x1] = (ovf || (int64_t)(s + cnt) > P.fa_cap) ? -1 : (int64_t)s;
+        P.g.fa_start[x1] = (ovf || s + cnt > P.fa_cap) ? -1 : s;
         P.g.fa_fill[x1] = 0;
     }
 }
