@@ -164,3 +164,40 @@ def test_concurrent_clients_and_graph_churn(oracle, engine_mod):
             want, _, _ = H.oracle_replay(oracle, ni, nl, mis0, batches, completion=True, literal=False)
             assert np.array_equal(srv.get_mis(gid, len(ni) - 1), want[-1]), "graph %d" % gid
             assert srv.check(gid) == (0, 0, 0)
+
+
+def test_tuner_on_the_device(engine_mod, tmp_path, monkeypatch):
+    """The autotuner times the queued driver under each knob setting on a
+    small R-MAT stream: every trial ends with the MIS of the defaults (the
+    tuner checks it), which is also the plain driver's, and the tuned
+    settings load back into the environment."""
+    from misfit_amd import tune as T
+    for k in T.SPACE:
+        monkeypatch.delenv(k, raising=False)
+    ni, nl = engine_mod.gen_rmat(14, 16, 5)
+    mis0 = engine_mod.greedy_mis(ni, nl)
+    B, W, K = 2000, 2, 6
+    u, v = engine_mod.gen_updates(engine_mod.MIXED_EXISTING, ni, nl, (W + K) * B, 5)
+    with engine_mod.Engine(0) as eng:
+        eng.load_csr(ni, nl)
+        ups = engine_mod.make_updates(u, v, eng.flags(u, v))
+        eng.set_mis(mis0)
+        eng.set_mode(engine_mod.MODE_PLAIN)
+        for b in range(W + K):
+            eng.process_batch(ups[b * B:(b + 1) * B])
+        want = eng.get_mis()
+    space = {"MISFIT_ROUND_MARGIN": ["", "1", "4"], "MISFIT_WINDOW_FACTOR": ["", "8"],
+             "MISFIT_ROUND_WG_PER_CU": ["", "2"]}
+    seen = []
+
+    def fn(env):
+        r, mis = T.measure(engine_mod, ni, nl, mis0, ups, B, W, K, env)
+        seen.append(mis)
+        return r, mis
+    res = T.tune(fn, space=space, repeats=1)
+    assert len(res["trials"]) == 1 + 2 + 1 + 1
+    assert all(np.array_equal(m, want) for m in seen)
+    assert res["updates_per_s"] >= res["default_updates_per_s"] > 0
+    p = str(tmp_path / "tune.json")
+    T.save(res, p)
+    assert T.apply(p) == res["env"]
