@@ -275,9 +275,10 @@ def open_loop(M, eng, ni, nl, mis0, ups, batch, nb, rate, mode, depth=2, seed=1)
     return {"p50_ms": round(pct(lat, 50), 3), "p99_ms": round(pct(lat, 99), 3), "mis": eng.get_mis()}
 
 
-def latency_at_load(M, eng, ni, nl, mis0, ups, batch, nb, plain_ups, fracs=(0.5, 0.8)):
+def latency_at_load(M, eng, ni, nl, mis0, ups, batch, nb, plain_ups, fracs=(0.5, 0.8, 1.2)):
     """Open-loop p50/p99 of both drivers at the same offered loads (fractions
-    of the plain driver's measured throughput); both must end with the same
+    of the plain driver's measured throughput; above 1 the plain driver's
+    queue grows for as long as batches arrive); both must end with the same
     MIS."""
     out = []
     for f in fracs:
