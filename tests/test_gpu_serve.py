@@ -173,7 +173,9 @@ def test_tuner_on_the_device(engine_mod, tmp_path, monkeypatch):
     settings load back into the environment."""
     from misfit_amd import tune as T
     for k in T.SPACE:
-        monkeypatch.delenv(k, raising=False)
+        # set, then delete: the knobs the tuner's apply() sets are undone after the test
+        monkeypatch.setenv(k, "0")
+        monkeypatch.delenv(k)
     ni, nl = engine_mod.gen_rmat(14, 16, 5)
     mis0 = engine_mod.greedy_mis(ni, nl)
     B, W, K = 2000, 2, 6

@@ -68,7 +68,9 @@ def test_repeats_take_the_best_and_must_agree():
 
 def test_file_apply_and_exports(tmp_path, monkeypatch):
     for k in T.SPACE:
-        monkeypatch.delenv(k, raising=False)
+        # set, then delete: the knobs the tuner's apply() sets are undone after the test
+        monkeypatch.setenv(k, "0")
+        monkeypatch.delenv(k)
     monkeypatch.setenv("MISFIT_ROUND_WG_PER_CU", "7")  # not in the file: unset by apply
     res = {"env": {"MISFIT_ROUND_MARGIN": "2"}, "updates_per_s": 1.0, "default_updates_per_s": 1.0,
            "trials": []}
