@@ -235,15 +235,28 @@ typedef struct {
 int misfit_server_create(int device, const misfit_server_opts* opts, misfit_server** out);
 int misfit_server_destroy(misfit_server* srv);
 const char* misfit_server_last_error(const misfit_server* srv);
-/* MISFIT_E_NOMEM (nothing allocated) when the graph does not fit. */
+/* MISFIT_E_NOMEM (nothing allocated) when the graph does not fit: its
+ * resident bytes (base, sorted row copy, overlay, per-vertex state) plus the
+ * headroom of the largest compaction a resident graph may run (compactions
+ * run one at a time) must fit in free device memory minus hbm_reserve. */
 int misfit_server_add_graph(misfit_server* srv, int32_t n, int64_t m, const void* nindex, int nindex_is_64,
                             const int32_t* nlist, const uint8_t* mis, int32_t* graph);
 int misfit_server_remove_graph(misfit_server* srv, int32_t graph);
+/* Changes the device bytes kept free (hbm_reserve) for every later
+ * admission of graphs, batches and compactions, e.g. to make room for
+ * another tenant of the GPU.  No reference counterpart: the reference
+ * serves one graph and sizes nothing by free memory. */
+int misfit_server_set_reserve(misfit_server* srv, int64_t hbm_reserve);
 /* Returns at once; the updates are copied. */
 int misfit_server_submit(misfit_server* srv, int32_t graph, const misfit_update* ups, int64_t count,
                          uint64_t* request);
 /* Blocks until the request has finished; its result code (MISFIT_E_NOMEM
- * when the batch was not admitted; the graph is then unchanged). */
+ * when the batch was not admitted; the graph is then unchanged).  A graph
+ * whose compaction could not run (its headroom does not fit in free device
+ * memory minus the reserve, or the compaction failed) stays as it was, and
+ * the next of its requests to start fails with that error without being
+ * applied; the compaction is tried again once the overlay has grown by
+ * another compact_fraction of the base. */
 int misfit_server_wait(misfit_server* srv, uint64_t request, misfit_request_stats* st);
 /* After waiting for a graph's requests: its MIS, and the full check. */
 int misfit_server_get_mis(misfit_server* srv, int32_t graph, uint8_t* mis);

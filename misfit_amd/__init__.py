@@ -95,7 +95,7 @@ EXPORTS = (
     "misfit_stage_batch", "misfit_submit_staged", "misfit_release_staged",
     "misfit_server_create", "misfit_server_destroy", "misfit_server_last_error", "misfit_server_add_graph",
     "misfit_server_remove_graph", "misfit_server_submit", "misfit_server_wait", "misfit_server_get_mis",
-    "misfit_server_check",
+    "misfit_server_check", "misfit_server_set_reserve",
 )
 
 _lib = None
@@ -152,6 +152,7 @@ def lib():
         L.misfit_server_wait.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(RequestStats)]
         L.misfit_server_get_mis.argtypes = [vp, ctypes.c_int32, _u8p]
         L.misfit_server_check.argtypes = [vp, ctypes.c_int32, _i64p, _i64p, _i64p]
+        L.misfit_server_set_reserve.argtypes = [vp, ctypes.c_int64]
         L.misfit_mtx_to_egr.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _i32p, _i64p, ctypes.c_char_p,
                                         ctypes.c_int]
         L.misfit_egr_check.argtypes = [ctypes.c_char_p, ctypes.POINTER(EgrReport), ctypes.c_char_p, ctypes.c_int]
@@ -264,13 +265,14 @@ def gen_er(n, M, seed):
 def gen_rmat(scale, edgefactor, seed, a=0.57, b=0.19, c=0.19):
     n = 1 << scale
     ni = np.zeros(n + 1, np.int64)
-    # one pass sizes and fills (the generator is deterministic)
-    m = _csr_from(lib().misfit_gen_rmat, scale, edgefactor, a, b, c, seed)
-    nl = np.zeros(max(m, 1), np.int32)
+    # one call: the entries (at most both directions of every generated
+    # edge) go to a buffer of that bound, cut to the count after
+    nl = np.empty(2 * edgefactor * n, np.int32)
     mm = ctypes.c_int64()
     _ck(lib().misfit_gen_rmat(scale, edgefactor, a, b, c, seed, _p(ni, _i64p), _p(nl, _i32p),
                               ctypes.byref(mm)))
-    return ni, nl[:m]
+    m = mm.value
+    return ni, (nl[:m].copy() if m < len(nl) // 2 else nl[:m])
 
 
 def gen_rgg(n, avg_degree, seed):
@@ -529,6 +531,10 @@ class Server:
 
     def remove_graph(self, gid):
         self._ck(lib().misfit_server_remove_graph(self.srv, gid))
+
+    def set_reserve(self, hbm_reserve):
+        """Device bytes kept free for every later admission (graphs, batches, compactions)."""
+        self._ck(lib().misfit_server_set_reserve(self.srv, int(hbm_reserve)))
 
     def submit(self, gid, updates):
         updates = np.ascontiguousarray(updates, UPDATE_DTYPE)

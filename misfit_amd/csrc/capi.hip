@@ -294,66 +294,98 @@ extern "C" int misfit_timer_end(misfit_ctx* c, double* ms) {
 }
 
 // ---------------------------------------------------------------- capacity
-// Per-batch scratch for batches of up to U updates.
+// Per-batch scratch for batches of up to U updates.  The new buffers are
+// allocated in full before any old one is freed: a failed growth leaves the
+// context exactly as it was (the old capacity still serves smaller batches).
 static int ensure_batch_capacity(misfit_ctx* c, int64_t U) {
     if (U <= c->U_cap && c->P.ctr) return MISFIT_OK;
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    int64_t cap = std::max<int64_t>(std::max<int64_t>(U, 1 << 14), c->U_cap * 2);
-    free_list(c->batch_allocs);
-    free_slots(c);
-    auto& A = c->batch_allocs;
-    BP& P = c->P;
-    c->bt_size = pow2_at_least((u64)(2 * cap));
-    HIPCHK(c, dalloc(A, &P.bt_key, (int64_t)c->bt_size));
-    HIPCHK(c, dalloc(A, &P.bt_head, (int64_t)c->bt_size));
-    HIPCHK(c, dalloc(A, &P.bt_nxt, cap));
-    HIPCHK(c, dalloc(A, &P.bt_slot, cap));
-    HIPCHK(c, dalloc(A, &P.bt_plo, cap));
-    HIPCHK(c, dalloc(A, &P.bt_phi, cap));
-    HIPCHK(c, dalloc(A, &P.bt_fl, cap));
-    HIPCHK(c, dalloc(A, &P.ne_x, 2 * cap));
-    HIPCHK(c, dalloc(A, &P.ne_y, 2 * cap));
-    HIPCHK(c, dalloc(A, &P.ne_seq, 2 * cap));
-    HIPCHK(c, dalloc(A, &P.ne_nxt, 2 * cap));
-    HIPCHK(c, dalloc(A, &P.seeds, 2 * cap));
-    HIPCHK(c, dalloc(A, &P.wpot, cap));
-    HIPCHK(c, dalloc(A, &P.active, cap));
-    HIPCHK(c, dalloc(A, &P.kind, cap));
-    HIPCHK(c, dalloc(A, &P.res_w, cap));
-    HIPCHK(c, dalloc(A, &P.res_x, cap));
-    HIPCHK(c, dalloc(A, &P.relog, cap));
+    const int64_t cap = std::max<int64_t>(std::max<int64_t>(U, 1 << 14), c->U_cap * 2);
+    std::vector<void*> A;       // new device buffers
+    std::vector<void*> pinned;  // new pinned host buffers
+    BP N = c->P;                // graph-level fields stay, batch-level ones are replaced
+    Slot ns[MF_NSLOT];
+    const u64 bt_size = pow2_at_least((u64)(2 * cap));
     // read logs: one segment per wave of the round grid (256-thread workgroups)
     const int64_t nwaves = (int64_t)c->S.blocks * (256 / MF_WAVE);
-    HIPCHK(c, dalloc(A, &P.rlog, nwaves * MF_RLOG));
-    HIPCHK(c, dalloc(A, &P.rlog_cnt, nwaves));
-    HIPCHK(c, dalloc(A, &P.cm, nwaves));
-    c->fa_cap = std::max<int64_t>(1 << 22, 16 * cap);
-    HIPCHK(c, dalloc(A, &P.fa_z, c->fa_cap));
-    HIPCHK(c, dalloc(A, &P.fa_pos, c->fa_cap));
-    P.bt_mask = c->bt_size - 1;
-    P.fa_cap = c->fa_cap;
-    mf_fill_u64(P.bt_key, MF_EMPTY_KEY, (int64_t)c->bt_size, c->stream);
-    mf_fill_u32(P.relog, 0u, cap, c->stream);
-    mf_fill_i32(P.bt_head, -1, (int64_t)c->bt_size, c->stream);
-    for (auto& s : c->slots) {
-        HIPCHK(c, dalloc(A, &s.d_u, cap));
-        HIPCHK(c, dalloc(A, &s.d_v, cap));
-        HIPCHK(c, dalloc(A, &s.d_ins, cap));
+    const int64_t fa_cap = std::max<int64_t>(1 << 22, 16 * cap);
+    hipError_t e = hipSuccess;
+    auto dev = [&](auto** p, int64_t count) {
+        if (e == hipSuccess) e = dalloc(A, p, count);
+    };
+    auto host = [&](auto** p, int64_t bytes) {
+        if (e != hipSuccess) return;
         // read by the batch's first kernel over PCIe, and written again for
         // a later batch of the slot: coherent, so no device cache can hold a
         // previous batch's lines
-        HIPCHK(c, hipHostMalloc((void**)&s.h_u, cap * sizeof(int32_t), hipHostMallocCoherent));
-        HIPCHK(c, hipHostMalloc((void**)&s.h_v, cap * sizeof(int32_t), hipHostMallocCoherent));
-        HIPCHK(c, hipHostMalloc((void**)&s.h_ins, cap, hipHostMallocCoherent));
+        e = hipHostMalloc((void**)p, (size_t)bytes, hipHostMallocCoherent);
+        if (e == hipSuccess) pinned.push_back((void*)*p);
+    };
+    dev(&N.bt_key, (int64_t)bt_size);
+    dev(&N.bt_head, (int64_t)bt_size);
+    dev(&N.bt_nxt, cap);
+    dev(&N.bt_slot, cap);
+    dev(&N.bt_plo, cap);
+    dev(&N.bt_phi, cap);
+    dev(&N.bt_fl, cap);
+    dev(&N.ne_x, 2 * cap);
+    dev(&N.ne_y, 2 * cap);
+    dev(&N.ne_seq, 2 * cap);
+    dev(&N.ne_nxt, 2 * cap);
+    dev(&N.seeds, 2 * cap);
+    dev(&N.wpot, cap);
+    dev(&N.active, cap);
+    dev(&N.kind, cap);
+    dev(&N.res_w, cap);
+    dev(&N.res_x, cap);
+    dev(&N.relog, cap);
+    dev(&N.rlog, nwaves * MF_RLOG);
+    dev(&N.rlog_cnt, nwaves);
+    dev(&N.cm, nwaves);
+    dev(&N.fa_z, fa_cap);
+    dev(&N.fa_pos, fa_cap);
+    for (auto& s : ns) {
+        dev(&s.d_u, cap);
+        dev(&s.d_v, cap);
+        dev(&s.d_ins, cap);
+        host(&s.h_u, cap * (int64_t)sizeof(int32_t));
+        host(&s.h_v, cap * (int64_t)sizeof(int32_t));
+        host(&s.h_ins, cap);
         void* dp = nullptr;
-        HIPCHK(c, hipHostGetDevicePointer(&dp, s.h_u, 0));
+        if (e == hipSuccess) e = hipHostGetDevicePointer(&dp, s.h_u, 0);
         s.g_u = (const int32_t*)dp;
-        HIPCHK(c, hipHostGetDevicePointer(&dp, s.h_v, 0));
+        if (e == hipSuccess) e = hipHostGetDevicePointer(&dp, s.h_v, 0);
         s.g_v = (const int32_t*)dp;
-        HIPCHK(c, hipHostGetDevicePointer(&dp, s.h_ins, 0));
+        if (e == hipSuccess) e = hipHostGetDevicePointer(&dp, s.h_ins, 0);
         s.g_ins = (const uint8_t*)dp;
     }
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (e == hipSuccess) {
+        N.bt_mask = bt_size - 1;
+        N.fa_cap = fa_cap;
+        mf_fill_u64(N.bt_key, MF_EMPTY_KEY, (int64_t)bt_size, c->stream);
+        mf_fill_u32(N.relog, 0u, cap, c->stream);
+        mf_fill_i32(N.bt_head, -1, (int64_t)bt_size, c->stream);
+        e = hipStreamSynchronize(c->stream);
+    }
+    if (e != hipSuccess) {
+        free_list(A);
+        for (void* p : pinned) hipHostFree(p);
+        const int code = e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation ? MISFIT_E_NOMEM : MISFIT_E_HIP;
+        return fail(c, code, std::string("batch buffers for ") + std::to_string(cap) + " updates: " + hipGetErrorString(e));
+    }
+    // every allocation succeeded: the old buffers go, the new ones take over
+    free_list(c->batch_allocs);
+    free_slots(c);
+    c->batch_allocs = std::move(A);
+    c->P = N;
+    for (int i = 0; i < MF_NSLOT; i++) {
+        Slot& s = c->slots[i];
+        s.h_u = ns[i].h_u; s.h_v = ns[i].h_v; s.h_ins = ns[i].h_ins;
+        s.g_u = ns[i].g_u; s.g_v = ns[i].g_v; s.g_ins = ns[i].g_ins;
+        s.d_u = ns[i].d_u; s.d_v = ns[i].d_v; s.d_ins = ns[i].d_ins;
+    }
+    c->bt_size = bt_size;
+    c->fa_cap = fa_cap;
     c->U_cap = cap;
     return MISFIT_OK;
 }
@@ -367,11 +399,18 @@ static int ensure_overlay_capacity(misfit_ctx* c, int64_t extra_ins) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (need_pool > g.pool_cap) {
         int64_t nc = std::max<int64_t>(need_pool * 2, g.pool_cap * 2);
+        std::vector<void*> fresh;
         int32_t* np = nullptr;
-        HIPCHK(c, dalloc(c->graph_allocs, &np, nc));
         u64 top = 0;
-        HIPCHK(c, hipMemcpy(&top, g.pool_top, sizeof(u64), hipMemcpyDeviceToHost));
-        if (top) HIPCHK(c, hipMemcpy(np, g.pool, top * sizeof(int32_t), hipMemcpyDeviceToDevice));
+        hipError_t e = dalloc(fresh, &np, nc);
+        if (e == hipSuccess) e = hipMemcpy(&top, g.pool_top, sizeof(u64), hipMemcpyDeviceToHost);
+        if (e == hipSuccess && top) e = hipMemcpy(np, g.pool, top * sizeof(int32_t), hipMemcpyDeviceToDevice);
+        if (e != hipSuccess) {  // the old pool stays in use
+            free_list(fresh);
+            const int code = e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation ? MISFIT_E_NOMEM : MISFIT_E_HIP;
+            return fail(c, code, std::string("overlay pool growth: ") + hipGetErrorString(e));
+        }
+        c->graph_allocs.push_back(np);
         auto& v = c->graph_allocs;
         auto it = std::find(v.begin(), v.end(), (void*)g.pool);
         if (it != v.end()) { hipFree(*it); v.erase(it); }
@@ -399,15 +438,30 @@ static int ensure_overlay_capacity(misfit_ctx* c, int64_t extra_ins) {
             nk[s] = keys[i];
             nv[s] = vals[i];
         }
+        // the new table is allocated and filled before the old one goes, so
+        // a failure leaves the context on its old, still valid table
+        std::vector<void*> fresh;
+        u64* nkey = nullptr;
+        int32_t* nval = nullptr;
+        hipError_t e = dalloc(fresh, &nkey, nc);
+        if (e == hipSuccess) e = dalloc(fresh, &nval, nc);
+        if (e == hipSuccess) e = hipMemcpy(nkey, nk.data(), nc * sizeof(u64), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(nval, nv.data(), nc * sizeof(int32_t), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            free_list(fresh);
+            c->S.g = g;
+            const int code = e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation ? MISFIT_E_NOMEM : MISFIT_E_HIP;
+            return fail(c, code, std::string("inserted-pair hash growth: ") + hipGetErrorString(e));
+        }
         auto& v = c->graph_allocs;
         for (void* old : {(void*)g.ph_key, (void*)g.ph_val}) {
             auto it = std::find(v.begin(), v.end(), old);
             if (it != v.end()) { hipFree(*it); v.erase(it); }
         }
-        HIPCHK(c, dalloc(v, &g.ph_key, nc));
-        HIPCHK(c, dalloc(v, &g.ph_val, nc));
-        HIPCHK(c, hipMemcpy(g.ph_key, nk.data(), nc * sizeof(u64), hipMemcpyHostToDevice));
-        HIPCHK(c, hipMemcpy(g.ph_val, nv.data(), nc * sizeof(int32_t), hipMemcpyHostToDevice));
+        v.push_back(nkey);
+        v.push_back(nval);
+        g.ph_key = nkey;
+        g.ph_val = nval;
         c->ph_cap = nc;
         g.ph_mask = (u64)(nc - 1);
     }
@@ -516,7 +570,7 @@ extern "C" int misfit_load_csr(misfit_ctx* c, int32_t n, int64_t m, const void* 
     HIPCHK(c, dalloc(A, &S.d_deg, n));
     HIPCHK(c, dalloc(A, &S.d_tomb_list, n));
     HIPCHK(c, dalloc(A, &S.d_scan_part, ((int64_t)n + 1) / 2048 + 2));
-    S.walk_task_cap = m / 4096 + n / 64 + 1024;
+    S.walk_task_cap = mf_walk_task_need(n, m);
     HIPCHK(c, dalloc(A, (char**)&S.d_walk_tasks, S.walk_task_cap * 16));
     HIPCHK(c, dalloc(A, &S.d_walk_ntasks, 1));
     hipStream_t st = c->stream;
@@ -587,6 +641,7 @@ extern "C" int misfit_set_mis(misfit_ctx* c, const uint8_t* mis) {
     HIPCHK(c, mf_check(c->S, c->d_acc));
     HIPCHK(c, hipMemcpyAsync(c->h_small, c->d_acc, 4 * sizeof(u64), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->h_small[3]) return fail(c, MISFIT_E_CHECK, "edge walk task list overflow");
     c->maximal = c->h_small[1] == 0;
     return MISFIT_OK;
 }
@@ -1111,6 +1166,7 @@ extern "C" int misfit_check(misfit_ctx* c, int64_t* conflicts, int64_t* uncovere
     HIPCHK(c, mf_check(c->S, c->d_acc));
     HIPCHK(c, hipMemcpyAsync(c->h_small, c->d_acc, 4 * sizeof(u64), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->h_small[3]) return fail(c, MISFIT_E_CHECK, "edge walk task list overflow");
     if (conflicts) *conflicts = (int64_t)c->h_small[0];
     if (uncovered) *uncovered = (int64_t)c->h_small[1];
     if (count_errors) *count_errors = (int64_t)c->h_small[2];
@@ -1304,9 +1360,29 @@ extern "C" int misfit_compact(misfit_ctx* c, int64_t* new_m) {
             return fail(c, MISFIT_E_CHECK, "merged sorted row copy does not match the compacted rows");
         }
     }
+    // the edge walk's split tasks follow the new base's size (allocated
+    // before the swap: on failure the old graph stays as it was)
+    void* ntasks = nullptr;
+    const int64_t walk_need = mf_walk_task_need(g.n, total);
+    if (walk_need > c->S.walk_task_cap) {
+        e = grab(&ntasks, (size_t)walk_need * 16);
+        if (e != hipSuccess) {
+            drop_all();
+            return fail(c, MISFIT_E_NOMEM, std::string("edge walk tasks: ") + hipGetErrorString(e));
+        }
+    }
     // the new graph replaces the old one
     keep(nidx);
     keep(nl);
+    if (ntasks) {
+        keep(ntasks);
+        auto& ga = c->graph_allocs;
+        auto it = std::find(ga.begin(), ga.end(), c->S.d_walk_tasks);
+        if (it != ga.end()) { hipFree(*it); ga.erase(it); }
+        ga.push_back(ntasks);
+        c->S.d_walk_tasks = ntasks;
+        c->S.walk_task_cap = walk_need;
+    }
     auto& v = c->graph_allocs;
     for (void* old : {(void*)c->d_nindex, (void*)c->d_nlist}) {
         auto it = std::find(v.begin(), v.end(), old);

@@ -91,6 +91,83 @@ int build_csr(int64_t n, std::vector<uint64_t>& pairs, int64_t* nindex, int32_t*
 
 inline uint64_t dpair(uint32_t a, uint32_t b) { return ((uint64_t)a << 32) | b; }
 
+// CSR of the directed pairs a chunked generator emits, without holding the
+// pair list: gen(ch, emit) calls emit(src, dst) for every pair of chunk ch
+// and must emit the same pairs each time it is called (each chunk draws from
+// its own seeded stream).  Pass 1 counts the row lengths, pass 2 scatters
+// the entries to their rows through per-row cursors; the order inside a row
+// then depends on the threads, but every row is sorted, deduplicated and
+// stripped of self-loops, so the result does not.
+template <class Gen>
+int build_csr_chunked(int64_t n, int64_t nch, const Gen& gen, int64_t* nindex, int32_t* nlist, int64_t* m_out) {
+    std::vector<int64_t> off((size_t)n + 1, 0);
+    int64_t* offp = off.data();
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t ch = 0; ch < nch; ch++)
+        gen(ch, [&](uint32_t a, uint32_t) { __atomic_fetch_add(&offp[(size_t)a + 1], (int64_t)1, __ATOMIC_RELAXED); });
+    for (int64_t i = 0; i < n; i++) off[i + 1] += off[i];
+    std::vector<int32_t> tmp((size_t)off[n]);
+    int32_t* tp = tmp.data();
+    {
+        std::vector<int64_t> cur(off.begin(), off.end() - 1);
+        int64_t* cp = cur.data();
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int64_t ch = 0; ch < nch; ch++)
+            gen(ch, [&](uint32_t a, uint32_t b) {
+                tp[__atomic_fetch_add(&cp[a], (int64_t)1, __ATOMIC_RELAXED)] = (int32_t)b;
+            });
+    }
+    std::vector<int64_t> kept((size_t)n, 0);
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t x = 0; x < n; x++) {
+        int32_t* b = tp + off[x];
+        int32_t* e = tp + off[x + 1];
+        std::sort(b, e);
+        int32_t* w = b;
+        for (int32_t* r = b; r < e; r++) {
+            if (*r == (int32_t)x) continue;
+            if (w > b && *(w - 1) == *r) continue;
+            *w++ = *r;
+        }
+        kept[x] = w - b;
+    }
+    int64_t m = 0;
+    for (int64_t x = 0; x < n; x++) m += kept[x];
+    *m_out = m;
+    if (!nindex) return MISFIT_OK;
+    nindex[0] = 0;
+    for (int64_t x = 0; x < n; x++) nindex[x + 1] = nindex[x] + kept[x];
+    if (nlist) {
+#pragma omp parallel for schedule(dynamic, 1024)
+        for (int64_t x = 0; x < n; x++) memcpy(nlist + nindex[x], tp + off[x], (size_t)kept[x] * 4);
+    }
+    return MISFIT_OK;
+}
+
+// Row of CSR entry p (the largest x with nindex[x] <= p), through a coarse
+// table of the row holding every 4,096th entry, so a lookup searches only
+// the rows between two table entries instead of all n.
+struct EntryRows {
+    const int64_t* nindex;
+    int32_t n;
+    std::vector<int32_t> at;  // at[j] = row of entry j * 4096
+    EntryRows(const int64_t* ni, int32_t nn) : nindex(ni), n(nn) {
+        const int64_t m = ni[nn];
+        at.resize((size_t)(m >> 12) + 2);
+        int32_t x = 0;
+        for (size_t j = 0; j < at.size(); j++) {
+            const int64_t p = std::min<int64_t>((int64_t)j << 12, m);
+            while (x + 1 <= nn && ni[x + 1] <= p) x++;
+            at[j] = x;
+        }
+    }
+    int32_t row(int64_t p) const {
+        const size_t j = (size_t)(p >> 12);
+        const int32_t lo = at[j], hi = std::min<int32_t>(at[j + 1] + 1, n);
+        return (int32_t)(std::upper_bound(nindex + lo, nindex + hi + 1, p) - nindex - 1);
+    }
+};
+
 bool base_adjacent(const int64_t* nindex, const int32_t* nlist, int32_t u, int32_t v) {
     const int32_t* b = nlist + nindex[u];
     const int32_t* e = nlist + nindex[u + 1];
@@ -126,10 +203,10 @@ extern "C" int misfit_gen_rmat(int scale, int edgefactor, double a, double b, do
     const int64_t E = (int64_t)edgefactor * n;
     const int64_t CH = 1 << 16;
     const int64_t nch = (E + CH - 1) / CH;
-    std::vector<uint64_t> pairs((size_t)(2 * E));
     const double ab = a + b, abc = a + b + c;
-#pragma omp parallel for schedule(dynamic, 4)
-    for (int64_t ch = 0; ch < nch; ch++) {
+    // chunk ch draws edges [ch*CH, (ch+1)*CH) from its own stream; each
+    // edge is emitted in both directions
+    auto gen = [&](int64_t ch, auto&& emit) {
         Rng r(stream_seed(seed, (uint64_t)ch));
         int64_t e0 = ch * CH, e1 = std::min(E, e0 + CH);
         for (int64_t e = e0; e < e1; e++) {
@@ -149,11 +226,11 @@ extern "C" int misfit_gen_rmat(int scale, int edgefactor, double a, double b, do
                 u = (u << 1) | bu;
                 v = (v << 1) | bv;
             }
-            pairs[(size_t)(2 * e)] = dpair(u, v);
-            pairs[(size_t)(2 * e + 1)] = dpair(v, u);
+            emit(u, v);
+            emit(v, u);
         }
-    }
-    return build_csr(n, pairs, nindex, nlist, m);
+    };
+    return build_csr_chunked(n, nch, gen, nindex, nlist, m);
 }
 
 extern "C" int misfit_gen_rgg(int32_t n, double avg_degree, uint64_t seed, int64_t* nindex, int32_t* nlist,
@@ -271,14 +348,14 @@ extern "C" int misfit_gen_updates(int kind, int32_t n, const int64_t* nindex, co
     if (kind == 1) {
         int64_t ndel = count / 2, nins = count - ndel;
         if (ndel > m / 2) return MISFIT_E_ARG;
-        std::vector<int32_t> row_of;  // source vertex of entry p via binary search
+        const EntryRows rows(nindex, n);
         std::unordered_set<uint64_t> used;
         used.reserve((size_t)count * 2);
         std::vector<uint64_t> out;
         out.reserve((size_t)count);
         while ((int64_t)out.size() < ndel) {
             int64_t p = (int64_t)r.below((uint64_t)m);
-            int32_t src = (int32_t)(std::upper_bound(nindex, nindex + n + 1, p) - nindex - 1);
+            int32_t src = rows.row(p);
             int32_t dst = nlist[p];
             if (src >= dst) continue;  // each undirected edge once: the entry with src < dst
             if (!used.insert(dpair((uint32_t)src, (uint32_t)dst)).second) continue;
@@ -343,6 +420,7 @@ extern "C" int misfit_gen_updates(int kind, int32_t n, const int64_t* nindex, co
             if (cumw[n] <= 0) return MISFIT_E_ARG;
         }
         if (m <= 0) return MISFIT_E_ARG;
+        const EntryRows rows(nindex, n);
         std::unordered_set<uint64_t> used;
         used.reserve((size_t)count * 2);
         std::vector<uint64_t> out;
@@ -362,12 +440,12 @@ extern "C" int misfit_gen_updates(int kind, int32_t n, const int64_t* nindex, co
                 p = nindex[src] + (int64_t)r.below((uint64_t)d);
             } else if (kind == 6) {
                 int64_t q = (int64_t)r.below((uint64_t)m);  // proportional to degree
-                src = (int32_t)(std::upper_bound(nindex, nindex + n + 1, q) - nindex - 1);
+                src = rows.row(q);
                 int64_t d = nindex[src + 1] - nindex[src];
                 p = nindex[src] + (int64_t)r.below((uint64_t)d);
             } else {
                 p = (int64_t)r.below((uint64_t)m);
-                src = (int32_t)(std::upper_bound(nindex, nindex + n + 1, p) - nindex - 1);
+                src = rows.row(p);
             }
             int32_t dst = nlist[p];
             if (dst == src) continue;

@@ -17,7 +17,12 @@ struct GraphState {
     int64_t walk_task_cap;
 };
 
+// Split tasks of the edge walk (rows cut into pieces of 4,096 entries) a base
+// of n vertices and m entries may need.
+inline int64_t mf_walk_task_need(int64_t n, int64_t m) { return m / 4096 + n / 64 + 1024; }
+
 hipError_t mf_set_mis(GraphState& S, const uint8_t* d_mis);
+// acc[0] conflicts, acc[1] uncovered, acc[2] count errors, acc[3] walk overflow (results invalid)
 hipError_t mf_get_mis(GraphState& S, uint8_t* d_mis);
 hipError_t mf_flags(GraphState& S, const int32_t* d_u, const int32_t* d_v, int64_t count, uint8_t* d_ins);
 hipError_t mf_check(GraphState& S, u64* d_acc);

@@ -279,6 +279,12 @@ __global__ void k_check_finish(const uint32_t* mc, const uint32_t* cnt, int32_t 
     if (bad) atomicAdd(&acc[2], bad);
 }
 
+// acc[3] = 1 when the edge walk's split tasks outgrew their list (some base
+// entries were then not visited and the other three results are invalid).
+__global__ void k_walk_overflow(const u64* ntasks, int64_t cap, u64* acc) {
+    if (threadIdx.x == 0 && (int64_t)*ntasks > cap) acc[3] = 1;
+}
+
 hipError_t mf_check(GraphState& S, u64* d_acc) {
     const int B = S.blocks;
     k_fill<uint32_t><<<B, 256, 0, S.stream>>>(S.d_cnt, 0u, S.g.n);
@@ -286,6 +292,7 @@ hipError_t mf_check(GraphState& S, u64* d_acc) {
     PassCountMis vis{S.g.mc, S.d_cnt, d_acc};
     walk_all(S, vis);
     k_check_finish<<<B, 256, 0, S.stream>>>(S.g.mc, S.d_cnt, S.g.n, d_acc);
+    k_walk_overflow<<<1, 64, 0, S.stream>>>(S.d_walk_ntasks, S.walk_task_cap, d_acc);
     return hipGetLastError();
 }
 

@@ -29,6 +29,7 @@
 // for queue changes only, never around a HIP call.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -79,12 +80,28 @@ struct Graph {
     bool compact_pending = false;
     bool in_call = false;   // a worker is calling the context outside the lock
     int64_t compactions = 0;
+    int64_t overlay = 0;            // overlay pool slots after the last finished batch
+    int64_t compact_at = 0;         // overlay size that triggers the next compaction (0: by the fraction)
+    int compact_rc = MISFIT_OK;     // a failed compaction's error, for the next request to start
+    std::string compact_err;
 };
 
-// Device bytes a graph needs beyond its CSR: per-vertex state, the check
-// scratch, the per-batch lists and a first overlay pool and hash.
+// Device bytes a resident graph may hold: the base (int64 offsets, int32
+// entries), the sorted copy of its rows (present after any compaction, or
+// for unsorted input), about m more for the overlay pool, the inserted-pair
+// hash and the row-chunk tasks, the per-vertex state and compaction scratch
+// (about 256 bytes a vertex), and the first batch buffers.
 int64_t graph_footprint(int32_t n, int64_t m) {
-    return 8 * ((int64_t)n + 1) + 4 * m + 256 * (int64_t)n + (int64_t(64) << 20);
+    return 8 * ((int64_t)n + 1) + 4 * m + 8 * m + m + 256 * (int64_t)n + (int64_t(64) << 20);
+}
+
+// Device bytes a compaction of that graph allocates while it runs: the new
+// offsets and entries and their sorted copy (12 bytes an entry, for a base
+// grown by the compaction fraction plus a tenth), the per-row merge lists and
+// the library sort's scratch.
+int64_t compaction_headroom(int32_t n, int64_t m, double fraction) {
+    const double grown = (double)m * (1.1 + (fraction > 0 ? fraction : 0.0));
+    return 12 * (int64_t)grown + 24 * ((int64_t)n + 1) + (int64_t(16) << 20);
 }
 
 // Device bytes one batch of `count` updates may add (batch buffers doubled
@@ -112,6 +129,8 @@ struct misfit_server {
     bool stop = false;
     std::vector<std::thread> workers;
     std::string err;
+    bool compacting = false;    // one compaction at a time: each needs its headroom only while it runs
+    int64_t headroom_max = 0;   // the largest compaction headroom among the resident graphs
 
     int fail(int rc, const std::string& m) {
         err = m;
@@ -167,21 +186,47 @@ bool misfit_server::service(Graph* g, std::unique_lock<std::mutex>& lk) {
         }
         r->st.batch = bs;
         r->st.device_ms = bs.device_ms;
-        if (opts.compact_fraction > 0 && (double)bs.overlay_entries > opts.compact_fraction * (double)g->base_m)
-            g->compact_pending = true;
+        g->overlay = bs.overlay_entries;
+        const double due = g->compact_at > 0 ? (double)g->compact_at : opts.compact_fraction * (double)g->base_m;
+        if (opts.compact_fraction > 0 && (double)bs.overlay_entries > due) g->compact_pending = true;
         finish(r, MISFIT_OK, "");
     }
-    if (g->compact_pending && g->inflight.empty()) {
+    // compactions run one at a time (each needs its headroom only while it
+    // runs); a graph whose turn has not come keeps its requests waiting
+    if (g->compact_pending && g->inflight.empty() && !compacting) {
+        compacting = true;
         lk.unlock();
         int64_t new_m = 0;
-        int rc = misfit_compact(g->ctx, &new_m);
+        int rc = MISFIT_OK;
+        std::string em;
+        const int64_t need = compaction_headroom(g->n, g->base_m + g->overlay, opts.compact_fraction);
+        const int64_t fr = device_free_bytes(device);
+        if (fr - need < opts.hbm_reserve) {
+            rc = MISFIT_E_NOMEM;
+            em = "compaction not started: needs " + std::to_string(need >> 20) + " MiB, " + std::to_string(fr >> 20) +
+                 " MiB free, " + std::to_string(opts.hbm_reserve >> 20) + " MiB reserved";
+        } else {
+            rc = misfit_compact(g->ctx, &new_m);
+            if (rc != MISFIT_OK) em = std::string("compaction failed: ") + misfit_last_error(g->ctx);
+        }
         lk.lock();
+        compacting = false;
         g->compact_pending = false;
         if (rc == MISFIT_OK) {
             g->base_m = new_m;
+            g->overlay = 0;
+            g->compact_at = 0;
             g->compactions++;
+        } else {
+            // the graph is unchanged; the next request to start reports the
+            // error, and the compaction is tried again only once the overlay
+            // has grown by another compaction fraction of the base
+            g->compact_rc = rc;
+            g->compact_err = em;
+            g->compact_at = g->overlay + (int64_t)(opts.compact_fraction * (double)g->base_m) + 1;
         }
         progress = true;
+        cv_work.notify_all();  // another graph may be waiting for its turn to compact
     }
     // a graph whose batch is still settling its round count gets no
     // further batch until it finishes (the poll above queues its rounds)
@@ -190,6 +235,12 @@ bool misfit_server::service(Graph* g, std::unique_lock<std::mutex>& lk) {
         Request* r = g->pending.front();
         g->pending.pop_front();
         progress = true;
+        if (g->compact_rc != MISFIT_OK) {  // a failed compaction's error goes to the next request
+            const int rc = g->compact_rc;
+            g->compact_rc = MISFIT_OK;
+            finish(r, rc, g->compact_err);
+            continue;
+        }
         lk.unlock();
         int rc = MISFIT_OK;
         std::string em;
@@ -276,12 +327,18 @@ extern "C" const char* misfit_server_last_error(const misfit_server* s) { return
 extern "C" int misfit_server_add_graph(misfit_server* s, int32_t n, int64_t m, const void* nindex, int nindex_is_64,
                                        const int32_t* nlist, const uint8_t* mis, int32_t* graph) {
     if (!s || !graph || !mis) return MISFIT_E_ARG;
+    if (n < 1 || m < 0) return s->fail(MISFIT_E_ARG, "invalid graph size");
     const int64_t need = graph_footprint(n, m);
+    const int64_t headroom = s->opts.compact_fraction > 0 ? compaction_headroom(n, m, s->opts.compact_fraction) : 0;
     {
+        // the graph's resident bytes, and the headroom of the largest
+        // compaction any resident graph (or this one) may run, must fit
         std::lock_guard<std::mutex> lk(s->mu);
-        int64_t fr = device_free_bytes(s->device) - s->loading_bytes;
-        if (fr - need < s->opts.hbm_reserve)
-            return s->fail(MISFIT_E_NOMEM, "graph not admitted: needs " + std::to_string(need >> 20) + " MiB, " +
+        const int64_t fr = device_free_bytes(s->device) - s->loading_bytes;
+        const int64_t room = std::max(headroom, s->headroom_max);
+        if (fr - need - room < s->opts.hbm_reserve)
+            return s->fail(MISFIT_E_NOMEM, "graph not admitted: needs " + std::to_string(need >> 20) + " MiB and " +
+                                               std::to_string(room >> 20) + " MiB of compaction headroom, " +
                                                std::to_string(fr >> 20) + " MiB free, " +
                                                std::to_string(s->opts.hbm_reserve >> 20) + " MiB reserved");
         s->loading_bytes += need;
@@ -307,6 +364,7 @@ extern "C" int misfit_server_add_graph(misfit_server* s, int32_t n, int64_t m, c
     g->worker = g->id % s->opts.workers;
     g->n = n;
     g->base_m = m;
+    s->headroom_max = std::max(s->headroom_max, headroom);
     *graph = g->id;
     s->graphs[g->id] = std::move(g);
     return MISFIT_OK;
@@ -322,6 +380,13 @@ static Graph* graph_idle(misfit_server* s, std::unique_lock<std::mutex>& lk, int
     }
 }
 
+extern "C" int misfit_server_set_reserve(misfit_server* s, int64_t hbm_reserve) {
+    if (!s || hbm_reserve < 0) return MISFIT_E_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->opts.hbm_reserve = hbm_reserve;
+    return MISFIT_OK;
+}
+
 extern "C" int misfit_server_remove_graph(misfit_server* s, int32_t graph) {
     if (!s) return MISFIT_E_ARG;
     std::unique_lock<std::mutex> lk(s->mu);
@@ -331,6 +396,11 @@ extern "C" int misfit_server_remove_graph(misfit_server* s, int32_t graph) {
         return s->fail(MISFIT_E_STATE, "graph has requests in progress");
     misfit_ctx_destroy(it->second->ctx);
     s->graphs.erase(it);
+    s->headroom_max = 0;
+    if (s->opts.compact_fraction > 0)
+        for (auto& kv : s->graphs)
+            s->headroom_max = std::max(s->headroom_max,
+                                       compaction_headroom(kv.second->n, kv.second->base_m, s->opts.compact_fraction));
     return MISFIT_OK;
 }
 

@@ -1,132 +1,112 @@
 #!/usr/bin/env bash
-# run.sh -- build (if needed) and run one of the two MISFIT engines on MI355X.
-# Same flags as the reference's run.sh (run.sh:1-127):
+# run.sh -- build (when needed) and start one of the two drop-in MISFIT
+# binaries on an MI355X.  The command line is the reference's (run.sh:17-36:
+# the same flags, the same defaults, exit status 1 on any usage error):
 #
 #   misfit_amd/run.sh --variant=misfit|misfit_async --graph=<graph.egr> \
 #       --mis=<initial_mis.txt> --batch=<batch_folder> [--num-batches=N] [--gpu=ID]
 #   misfit_amd/run.sh --variant=misfit_async --sample
 #
-# --num-batches defaults to the number of *.mtx files in the batch folder;
-# --gpu defaults to 0.  --sample uses the bundled sample (tests/golden/sample);
-# its graph is copied to a scratch directory first, so the end-of-run
-# compaction rewrites the copy and never the bundled file.
-#
-# Environment: MISFIT_SKIP_COMPACTION=1 (leave the input .egr untouched),
-# MISFIT_RULE=reference (literal rule, no completion), MISFIT_MODE=plain|queued,
-# MISFIT_CHECK=0 (no check after every batch), HIPCC / MISFIT_ARCH (build:
-# default /opt/rocm/bin/hipcc, gfx950).
+# Every flag also takes its value as the next word (--gpu 1).  The sample
+# run copies the bundled graph to a scratch folder, so the compaction at the
+# end of the run rewrites the copy and never the bundled file.
 set -euo pipefail
 
-HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
-ROOT="$(dirname "$HERE")"
+readonly PKG_DIR="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
+readonly REPO_DIR="$(dirname "$PKG_DIR")"
+readonly SAMPLE_DIR="$REPO_DIR/tests/golden/sample"
 
-usage() {
+print_help() {
     cat >&2 <<EOF
-Usage: $0 --variant=misfit|misfit_async --graph=<graph.egr> --mis=<initial_mis.txt> --batch=<batch_folder> [--num-batches=N] [--gpu=ID]
+usage: $0 --variant=misfit|misfit_async --graph=<graph.egr> --mis=<initial_mis.txt> --batch=<batch_folder> [--num-batches=N] [--gpu=ID]
        $0 --variant=misfit|misfit_async --sample
 
-  --variant       misfit (one batch at a time) or misfit_async (batches queued back to back)
-  --graph         base graph, MISFIT .egr binary CSR (int32 or int64-sentinel form)
-  --mis           initial MIS, one 1-indexed vertex id per line
-  --batch         folder of batch files named 1.mtx, 2.mtx, ...
-  --num-batches   how many batch files to replay (default: count of *.mtx files in --batch)
-  --gpu           HIP device id (default: 0)
-  --sample        the small bundled sample dataset instead of --graph/--mis/--batch
+flags
+  --variant=NAME      misfit: one batch at a time; misfit_async: batches queued back to back
+  --graph=FILE        base graph in MISFIT's .egr binary CSR (int32 or int64-sentinel nindex)
+  --mis=FILE          starting MIS, one 1-indexed vertex id per line
+  --batch=DIR         folder of update batches named 1.mtx, 2.mtx, ...
+  --num-batches=N     batches to replay (default: the number of .mtx files in DIR)
+  --gpu=ID            HIP device (default 0)
+  --sample            the bundled sample; fills in --graph, --mis and --batch
 
-Environment overrides:
-  HIPCC                     hipcc binary to build with (default: /opt/rocm/bin/hipcc)
-  MISFIT_ARCH               offload architecture (default: gfx950)
-  MISFIT_SKIP_COMPACTION=1  skip the end-of-run .egr compaction (input graph left untouched)
-  MISFIT_RULE=reference     literal reference rule, without the maximality completion
-  MISFIT_MODE=plain|queued  host-driven or device-driven rounds (default: queued)
-  MISFIT_CHECK=0            skip the independence/maximality check after every batch
+environment
+  HIPCC, MISFIT_ARCH        compiler and offload target of the build (/opt/rocm/bin/hipcc, gfx950)
+  MISFIT_SKIP_COMPACTION=1  leave the input .egr as it is at the end of the run
+  MISFIT_RULE=reference     the literal rule only, without the maximality completion
+  MISFIT_MODE=plain|queued  rounds driven by the host or by the device (default queued)
+  MISFIT_CHECK=0            no independence/maximality check after each batch
 EOF
 }
 
-VARIANT=""
-GRAPH=""
-MIS=""
-BATCH=""
-NUM_BATCHES=""
-GPU="0"
-USE_SAMPLE=0
+# die MESSAGE [help]: the message on stderr, the help text too when asked, status 1
+die() {
+    echo "run.sh: $1" >&2
+    [ "${2:-}" = help ] && print_help
+    exit 1
+}
 
-for arg in "$@"; do
-    case "$arg" in
-        --variant=*)     VARIANT="${arg#--variant=}" ;;
-        --graph=*)       GRAPH="${arg#--graph=}" ;;
-        --mis=*)         MIS="${arg#--mis=}" ;;
-        --batch=*)       BATCH="${arg#--batch=}" ;;
-        --num-batches=*) NUM_BATCHES="${arg#--num-batches=}" ;;
-        --gpu=*)         GPU="${arg#--gpu=}" ;;
-        --sample)        USE_SAMPLE=1 ;;
-        -h|--help)       usage; exit 0 ;;
-        *)
-            echo "ERROR: unrecognized argument: $arg" >&2
-            usage
-            exit 1
+declare -A opt=([gpu]=0)
+sample=no
+
+while [ $# -gt 0 ]; do
+    word="$1"
+    shift
+    case "$word" in
+        -h | --help) print_help; exit 0 ;;
+        --sample) sample=yes; continue ;;
+        --variant | --graph | --mis | --batch | --num-batches | --gpu)
+            [ $# -gt 0 ] || die "$word needs a value" help
+            key="${word#--}"
+            opt[$key]="$1"
+            shift
             ;;
+        --variant=* | --graph=* | --mis=* | --batch=* | --num-batches=* | --gpu=*)
+            key="${word%%=*}"
+            key="${key#--}"
+            opt[$key]="${word#*=}"
+            ;;
+        *) die "unrecognized argument: $word" help ;;
     esac
 done
 
-if [ -z "$VARIANT" ]; then
-    echo "ERROR: --variant=misfit|misfit_async is required" >&2
-    usage
-    exit 1
-fi
-case "$VARIANT" in
-    misfit|misfit_async) BIN="$HERE/bin/$VARIANT" ;;
-    *)
-        echo "ERROR: --variant must be 'misfit' or 'misfit_async', got '$VARIANT'" >&2
-        exit 1
-        ;;
-esac
+variant="${opt[variant]:-}"
+[ -n "$variant" ] || die "no engine chosen: pass --variant=misfit|misfit_async" help
+[ "$variant" = misfit ] || [ "$variant" = misfit_async ] ||
+    die "--variant must be 'misfit' or 'misfit_async' (got '$variant')"
 
-if [ "$USE_SAMPLE" = "1" ]; then
-    SAMPLE_DIR="$ROOT/tests/golden/sample"
-    SCRATCH="$(mktemp -d "${TMPDIR:-/tmp}/misfit_sample.XXXXXX")"
-    cp "$SAMPLE_DIR/sample.egr" "$SCRATCH/sample.egr"
-    GRAPH="${GRAPH:-$SCRATCH/sample.egr}"
-    MIS="${MIS:-$SAMPLE_DIR/sample_mis.txt}"
-    BATCH="${BATCH:-$SAMPLE_DIR/batches}"
+if [ "$sample" = yes ]; then
+    scratch="$(mktemp -d "${TMPDIR:-/tmp}/misfit_sample.XXXXXX")"
+    cp "$SAMPLE_DIR/sample.egr" "$scratch/sample.egr"
+    : "${opt[graph]:=$scratch/sample.egr}"
+    : "${opt[mis]:=$SAMPLE_DIR/sample_mis.txt}"
+    : "${opt[batch]:=$SAMPLE_DIR/batches}"
 fi
 
-if [ -z "$GRAPH" ] || [ -z "$MIS" ] || [ -z "$BATCH" ]; then
-    echo "ERROR: --graph, --mis, and --batch are all required (or pass --sample)" >&2
-    usage
-    exit 1
+for key in graph mis batch; do
+    [ -n "${opt[$key]:-}" ] || die "--graph, --mis and --batch must all be given (or --sample)" help
+done
+[ -f "${opt[graph]}" ] || die "graph not found: ${opt[graph]}"
+[ -f "${opt[mis]}" ] || die "initial MIS not found: ${opt[mis]}"
+[ -d "${opt[batch]}" ] || die "batch folder not found: ${opt[batch]}"
+
+is_count() { [[ "$1" =~ ^[0-9]+$ ]]; }
+
+if [ -z "${opt[num-batches]:-}" ]; then
+    shopt -s nullglob
+    files=("${opt[batch]}"/*.mtx)
+    shopt -u nullglob
+    opt[num-batches]=${#files[@]}
+    echo "replaying all ${opt[num-batches]} batch file(s) of ${opt[batch]} (no --num-batches)"
 fi
-if [ ! -f "$GRAPH" ]; then
-    echo "ERROR: graph not found: $GRAPH" >&2
-    exit 1
-fi
-if [ ! -f "$MIS" ]; then
-    echo "ERROR: initial MIS file not found: $MIS" >&2
-    exit 1
-fi
-if [ ! -d "$BATCH" ]; then
-    echo "ERROR: batch folder not found: $BATCH" >&2
-    exit 1
+is_count "${opt[num-batches]}" || die "--num-batches takes a whole number, not '${opt[num-batches]}'"
+is_count "${opt[gpu]}" || die "--gpu takes a device number, not '${opt[gpu]}'"
+
+binary="$PKG_DIR/bin/$variant"
+if [ ! -x "$binary" ]; then
+    echo "building $binary"
+    make -C "$PKG_DIR/csrc" -j8 HIPCC="${HIPCC:-/opt/rocm/bin/hipcc}" ARCH="${MISFIT_ARCH:-gfx950}" "../bin/$variant"
 fi
 
-if [ -z "$NUM_BATCHES" ]; then
-    NUM_BATCHES="$(find "$BATCH" -maxdepth 1 -name '*.mtx' | wc -l | tr -d ' ')"
-    echo "--num-batches not given; found $NUM_BATCHES batch file(s) in $BATCH"
-fi
-case "$NUM_BATCHES" in
-    ''|*[!0-9]*) echo "ERROR: num-batches must be a positive integer, got '$NUM_BATCHES'" >&2; exit 1 ;;
-esac
-case "$GPU" in
-    ''|*[!0-9]*) echo "ERROR: --gpu must be a non-negative integer, got '$GPU'" >&2; exit 1 ;;
-esac
-
-if [ -x "$BIN" ]; then
-    echo "binary already built: $BIN (delete it to force a rebuild)"
-else
-    echo "building $VARIANT -> $BIN"
-    make -C "$HERE/csrc" -j8 HIPCC="${HIPCC:-/opt/rocm/bin/hipcc}" ARCH="${MISFIT_ARCH:-gfx950}" \
-        "../bin/$VARIANT"
-fi
-
-echo "=== running $VARIANT: graph=$GRAPH mis=$MIS batch=$BATCH num_batches=$NUM_BATCHES gpu=$GPU ==="
-exec "$BIN" "$GRAPH" "$MIS" "$BATCH" "$NUM_BATCHES" "$GPU"
+echo "=== running $variant: graph=${opt[graph]} mis=${opt[mis]} batch=${opt[batch]} num_batches=${opt[num-batches]} gpu=${opt[gpu]} ==="
+exec "$binary" "${opt[graph]}" "${opt[mis]}" "${opt[batch]}" "${opt[num-batches]}" "${opt[gpu]}"

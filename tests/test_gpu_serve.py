@@ -130,6 +130,37 @@ def test_admission_by_free_memory(oracle, engine_mod):
         assert np.array_equal(srv.get_mis(gid, len(ni) - 1), want[0])
 
 
+def test_compaction_without_headroom_fails_the_next_request(oracle, engine_mod):
+    """A compaction whose headroom does not fit in free memory minus the
+    reserve does not run: the graph stays as it was, the next request fails
+    with MISFIT_E_NOMEM without being applied, and once the reserve allows it
+    the compaction runs after the overlay has grown further; the MIS equals
+    the oracle's with compactions at the reported points."""
+    ni, nl, mis0, batches = _workload(engine_mod, oracle, "er", 1 << 14, 61, nb=3)
+    with engine_mod.Server(0, compact_fraction=0.005, max_inflight=1) as srv:
+        gid = srv.add_graph(ni, nl, mis0)
+        with engine_mod.Engine(0) as probe:
+            _, _, free_now = probe.device_info()
+        # room for a 3,000-update batch (3 MB), none for the compaction
+        # (about 20 MiB for this graph)
+        srv.set_reserve(max(free_now - (12 << 20), 0))
+        st1 = srv.wait(srv.submit(gid, engine_mod.make_updates(*batches[0])))
+        assert st1["compactions_before"] == 0
+        with pytest.raises(engine_mod.MisfitError) as ei:
+            srv.wait(srv.submit(gid, engine_mod.make_updates(*batches[1])))
+        assert ei.value.code == -4 and "compaction" in str(ei.value)
+        want, _, _ = H.oracle_replay(oracle, ni, nl, mis0, batches[:1], completion=True, literal=False)
+        assert np.array_equal(srv.get_mis(gid, len(ni) - 1), want[0])
+        srv.set_reserve(0)
+        st2 = srv.wait(srv.submit(gid, engine_mod.make_updates(*batches[1])))
+        st3 = srv.wait(srv.submit(gid, engine_mod.make_updates(*batches[2])))
+        before = [st1["compactions_before"], st2["compactions_before"], st3["compactions_before"]]
+        assert before == [0, 0, 1]
+        want = _oracle_with_compactions(oracle, ni, nl, mis0, batches, before)
+        assert np.array_equal(srv.get_mis(gid, len(ni) - 1), want)
+        assert srv.check(gid) == (0, 0, 0)
+
+
 def test_concurrent_clients_and_graph_churn(oracle, engine_mod):
     """Client threads submit to their own graphs at the same time while a
     graph is removed and another added; every graph ends with the oracle's

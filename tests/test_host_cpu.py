@@ -280,3 +280,7 @@ def test_run_script_arguments():
     r = subprocess.run(["bash", sh, "--variant=misfit", "--graph=/nonexistent.egr", "--mis=/x", "--batch=/y"],
                        capture_output=True, text=True)
     assert r.returncode == 1 and "graph not found" in r.stderr
+    r = subprocess.run(["bash", sh, "--variant", "misfit", "--sample", "--num-batches=x"], capture_output=True, text=True)
+    assert r.returncode == 1 and "--num-batches" in r.stderr
+    r = subprocess.run(["bash", sh, "--variant=misfit", "--sample", "--gpu"], capture_output=True, text=True)
+    assert r.returncode == 1 and "--gpu needs a value" in r.stderr
