@@ -65,6 +65,7 @@ struct BP {
     uint32_t b;        // batch epoch, >= 1
     int completion;
     int64_t window;    // max edges speculated per round
+    int64_t fin_window;// max edges per round of the one-workgroup tail (0: the default)
     int64_t n_ins;     // insertion updates in the batch (bounds evictions per round)
     int may_halt;      // the tail may stop with edges left (nothing of this graph is queued behind)
     int pu_list;       // the MIS was maximal before the batch: the PU scan tests only tagged vertices
@@ -103,8 +104,7 @@ struct BP {
     uint8_t* kind;
     int32_t* res_w;
     int32_t* res_x;
-    uint4* rlog;       // guarded reads of a round, MF_RLOG per wave of the grid
-    int32_t* rlog_cnt; // records per wave
+    uint4* rlog;       // guarded reads of a round, MF_RLOG slots per wave of the grid (slot 0: count)
     int4* cm;          // per wave of the round grid: its first edge's kind, evicted
                        // and admitted vertex and index, left by the speculation for the commit
     uint32_t* relog;   // per edge: = round when its read log overflowed

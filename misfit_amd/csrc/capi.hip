@@ -65,6 +65,7 @@ struct misfit_ctx {
     bool always_halt = false;     // every batch may halt (nothing is ever queued behind one)
     double window_factor = 16.0;  // MISFIT_WINDOW_FACTOR: window = this x expected edges per round
     int round_margin = -1;        // MISFIT_ROUND_MARGIN: rounds queued beyond the planned rate (-1: by the plan)
+    int fin_window = 0;           // MISFIT_TAIL_WINDOW: edges per round of the one-workgroup tail (0: 1024)
     double clock_khz = 0.0;       // device wall clock, for per-batch device times
     hipEvent_t timer0 = nullptr, timer1 = nullptr;  // misfit_timer_begin/end
     int round_per_cu = 0;      // MISFIT_ROUND_WG_PER_CU: round grid (0: by occupancy)
@@ -185,6 +186,8 @@ extern "C" int misfit_ctx_create(int device, misfit_ctx** out) {
     if (rm) c->round_margin = std::max(0, std::min(16, atoi(rm)));
     const char* ps = getenv("MISFIT_PU_SCAN");
     c->force_pu_scan = ps && strcmp(ps, "full") == 0;
+    const char* tw = getenv("MISFIT_TAIL_WINDOW");
+    if (tw) c->fin_window = std::max(0, std::min(1 << 20, atoi(tw)));
     const char* gr = getenv("MISFIT_GRID_ROUNDS");
     if (gr) {
         c->grid_rounds = std::max(0, atoi(gr));
@@ -340,7 +343,6 @@ static int ensure_batch_capacity(misfit_ctx* c, int64_t U) {
     dev(&N.res_x, cap);
     dev(&N.relog, cap);
     dev(&N.rlog, nwaves * MF_RLOG);
-    dev(&N.rlog_cnt, nwaves);
     dev(&N.cm, nwaves);
     dev(&N.fa_z, fa_cap);
     dev(&N.fa_pos, fa_cap);
@@ -595,6 +597,10 @@ extern "C" int misfit_load_csr(misfit_ctx* c, int32_t n, int64_t m, const void* 
     mf_fill_u32(S.d_bmark, 0u, n, st);
     mf_fill_u32(c->d_round, 0u, 1, st);
     mf_fill_u64(c->d_ctr, 0ull, C_NCOUNTERS * MF_CS, st);
+    // the counters' state at a batch start (each batch's last kernel
+    // restores it): zero, no conflict
+    mf_fill_u64(c->d_ctr + MF_CS * C_CONF0, ~0ull, 1, st);
+    mf_fill_u64(c->d_ctr + MF_CS * C_CONF1, ~0ull, 1, st);
     c->epoch = 0;
     c->bytes_tag = 0;
     S.g = g;
@@ -810,6 +816,11 @@ static RoundPlan plan_rounds(misfit_ctx* c, int64_t count, int64_t full_window) 
             int64_t w = (int64_t)std::ceil(c->window_factor * per_round);
             p.window = std::min(std::max<int64_t>(256, (w + 31) / 32 * 32), full_window);
         }
+        if (c->grid_rounds_fixed) {  // a fixed count; the tail finishes whatever is left
+            p.k = c->grid_rounds;
+            p.sure = true;
+            return p;
+        }
         double k = std::ceil((double)count * std::max(recent, 0.0)) + 1.0;
         k = std::max(k, std::ceil((double)count / (double)std::max<int64_t>(p.window, 1)) + 1.0);
         p.k = (int)std::min(k, 65536.0);
@@ -1001,14 +1012,15 @@ static int submit(misfit_ctx* c, const misfit_update* ups, const misfit_staged* 
     c->epoch++;
     BP P = c->P;
     P.g = c->S.g;
-    P.bu = s.d_u;
-    P.bv = s.d_v;
-    P.bins = s.d_ins;
-    // the first kernel reads the batch: the slot's pinned staging (not
-    // written again before this batch's end event) or the device copy
-    P.hu = !count ? nullptr : st ? st->u : s.g_u;
-    P.hv = !count ? nullptr : st ? st->v : s.g_v;
-    P.hins = !count ? nullptr : st ? st->ins : s.g_ins;
+    // a batch staged in device memory is read where it lies; one in the
+    // slot's pinned staging (not written again before this batch's end
+    // event) is read by the first kernel into the slot's device arrays
+    P.bu = st ? st->u : s.d_u;
+    P.bv = st ? st->v : s.d_v;
+    P.bins = st ? st->ins : s.d_ins;
+    P.hu = (!count || st) ? nullptr : s.g_u;
+    P.hv = (!count || st) ? nullptr : s.g_v;
+    P.hins = (!count || st) ? nullptr : s.g_ins;
     P.U = count;
     P.b = c->epoch;
     P.completion = c->completion;
@@ -1023,6 +1035,7 @@ static int submit(misfit_ctx* c, const misfit_update* ups, const misfit_staged* 
     L.round_grid = (int)std::max<int64_t>(plan.window / wpg, 1);
     P.window = (int64_t)L.round_grid * wpg;
     P.n_ins = nins;
+    P.fin_window = c->fin_window;
     P.ktask = c->d_ktask;
     P.ktask_cap = c->ktask_cap;
     P.h_out = s.h_ctr;
@@ -1179,7 +1192,12 @@ extern "C" int misfit_batch_bytes(misfit_ctx* c, int index_bytes, int64_t* bytes
     HIPCHK(c, hipSetDevice(c->device));
     if (int rc_ = settle(c)) return rc_;
     c->bytes_tag++;
-    HIPCHK(c, mf_batch_bytes(c->S, c->P.seeds, c->P.ctr + MF_CS * C_SEEDS, c->bytes_tag, c->d_acc));
+    // the last batch's seed count, from its published counters (the device
+    // counters are reset when a batch ends)
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->h_small[0] = c->next_ticket > 1 ? c->slots[(c->next_ticket - 1) % MF_NSLOT].h_ctr[C_SEEDS] : 0ull;
+    HIPCHK(c, hipMemcpyAsync(c->d_acc + 4, c->h_small, sizeof(u64), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, mf_batch_bytes(c->S, c->P.seeds, c->d_acc + 4, c->bytes_tag, c->d_acc));
     HIPCHK(c, hipMemcpyAsync(c->h_small, c->d_acc, 4 * sizeof(u64), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     int64_t n1 = (int64_t)c->h_small[0], n2 = (int64_t)c->h_small[1], deg = (int64_t)c->h_small[2];

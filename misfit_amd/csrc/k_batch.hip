@@ -102,9 +102,12 @@ __device__ __forceinline__ void vread(const BP& P, int32_t x, uint32_t val, uint
 }
 
 // Per-wave log of the guarded reads of a speculation round: a fixed segment
-// of MF_RLOG records (vertex, value read, threshold, edge) per wave, so
-// logging needs no global atomics; the cursor lives in LDS.  A read that does
-// not fit marks its edge for a full re-walk instead.
+// of MF_RLOG 16-byte slots per wave, so logging needs no global atomics; the
+// cursor lives in LDS.  Slot 0 is the header (record count in .x), written
+// when the wave's speculation ends; slots 1 .. MF_RLOG-1 hold the records
+// (vertex, value read, threshold, edge), so the header and the first seven
+// records share one 128-byte line.  A read that does not fit marks its edge
+// for a full re-walk instead.
 struct ReadLog {
     uint4* rec;        // this wave's segment
     int* cur;          // this wave's cursor (LDS)
@@ -119,7 +122,7 @@ __device__ __forceinline__ void vlog(const ReadLog& lg, int32_t x, uint32_t val,
     if (lane == leader) base = atomicAdd(lg.cur, __popcll(bal));
     base = __shfl(base, leader, MF_WAVE);
     int pos = base + __popcll(bal & ((1ull << lane) - 1ull));
-    if (pos < MF_RLOG) lg.rec[pos] = make_uint4((uint32_t)x, val, T, (uint32_t)e);
+    if (pos < MF_RLOG - 1) lg.rec[1 + pos] = make_uint4((uint32_t)x, val, T, (uint32_t)e);
     else lg.relog[e] = R;
 }
 
@@ -1195,50 +1198,46 @@ __device__ void ph_spec_w(const BP& P, const GC& c, const RoundView& v, int* cur
         if (k) spec_edge(P, (int32_t)e, k, v.R, (u64)P.n_ins, c.lane, rc, cmw);
     }
     if (c.lane == 0) {
-        int n = cur[wiw];
-        P.rlog_cnt[c.wave] = n < MF_RLOG ? n : MF_RLOG;
+        const int n = cur[wiw];
+        P.rlog[c.wave * MF_RLOG] = make_uint4((uint32_t)(n < MF_RLOG - 1 ? n : MF_RLOG - 1), 0u, 0u, 0u);
     }
 }
 
 // Validation: wave w checks what wave w of the speculation (same grid) did:
 // the reads logged in its segment rlog[w * MF_RLOG ...] and its edge's
 // endpoints (the classification read them), with a re-walk when the log
-// overflowed.  The segment's first records are loaded before the count says
-// which are filled, together with the edge, so a check is two or three
-// dependent loads deep instead of a chain through the count, the record and
-// the edge.
-static_assert(MF_RLOG == 2 * MF_WAVE, "two log records per lane");
+// overflowed.  A wave with no edge in the window reads nothing.  One load
+// of the segment's first line gives the count and up to seven records
+// (most segments hold only a few), issued together with the edge's own
+// loads; the rest of a longer segment is loaded when the count asks for it.
+static_assert(MF_RLOG == 2 * MF_WAVE, "two log slots per lane");
 __device__ __forceinline__ bool endpoint_conflict(const BP& P, int64_t e, uint8_t k, int32_t a, int32_t bb,
                                                   uint32_t R) {
     if (k && P.relog[e] == R) return validate_edge(P, (int32_t)e, k, R);
     return mk_before(P.g.mmark, a, R, (int32_t)e) || mk_before(P.g.mmark, bb, R, (int32_t)e);
 }
 
+#define MF_RLOG_LINE 8  // 16-byte slots per 128-byte line
 __device__ void ph_validate_w(const BP& P, const GC& c, const RoundView& v, int par) {
     const uint4* seg = P.rlog + c.wave * MF_RLOG;
     const int64_t e0 = v.start + c.wave;
-    const bool has = e0 < v.end;
-    const int n = P.rlog_cnt[c.wave];
-    // the first record of each lane before the count is known (most
-    // segments hold at most 64), the second only when the count says it is
-    // filled
-    const uint4 r0 = seg[c.lane];
-    uint4 r1 = make_uint4(0u, 0u, 0u, 0u);
-    if (c.lane + MF_WAVE < n) r1 = seg[c.lane + MF_WAVE];
-    uint8_t k0 = 0;
-    int32_t a0 = 0, b0 = 0;
-    if (has) {
-        k0 = P.kind[e0];
-        a0 = P.bu[e0];
-        b0 = P.bv[e0];
-    }
+    if (e0 >= v.end) return;  // wave-uniform: no edge, no records
+    const uint4 zero = make_uint4(0u, 0u, 0u, 0u);
+    uint4 r0 = c.lane < MF_RLOG_LINE ? seg[c.lane] : zero;
+    const uint8_t k0 = P.kind[e0];
+    const int32_t a0 = P.bu[e0], b0 = P.bv[e0];
+    const int n = __shfl((int)r0.x, 0, MF_WAVE);  // the header
+    // slot l (1 <= l <= n) is checked by lane l, slot 64 + l by lane l
+    if (c.lane >= MF_RLOG_LINE && c.lane <= n) r0 = seg[c.lane];
+    uint4 r1 = zero;
+    if (c.lane + MF_WAVE <= n) r1 = seg[c.lane + MF_WAVE];
     bool c0 = false, c1 = false;
-    if (c.lane < n) vread(P, (int32_t)r0.x, r0.y, r0.z, v.R, (int32_t)r0.w, c0);
-    if (c.lane + MF_WAVE < n) vread(P, (int32_t)r1.x, r1.y, r1.z, v.R, (int32_t)r1.w, c1);
+    if (c.lane >= 1 && c.lane <= n) vread(P, (int32_t)r0.x, r0.y, r0.z, v.R, (int32_t)r0.w, c0);
+    if (c.lane + MF_WAVE <= n) vread(P, (int32_t)r1.x, r1.y, r1.z, v.R, (int32_t)r1.w, c1);
     u64 mine = ~0ull;
     if (c0) mine = (u64)r0.w;
     if (c1 && (u64)r1.w < mine) mine = (u64)r1.w;
-    if (has && endpoint_conflict(P, e0, k0, a0, b0, v.R) && (u64)e0 < mine) mine = (u64)e0;  // wave-uniform call
+    if (endpoint_conflict(P, e0, k0, a0, b0, v.R) && (u64)e0 < mine) mine = (u64)e0;  // wave-uniform call
     // a window wider than the grid gives a wave further edges
     for (int64_t e = e0 + c.nwaves; e < v.end; e += c.nwaves)
         if (endpoint_conflict(P, e, P.kind[e], P.bu[e], P.bv[e], v.R) && (u64)e < mine) mine = (u64)e;
@@ -1414,28 +1413,28 @@ __device__ void ph_stats(const BP& P, const GC& c) {
 }
 
 // ---------------------------------------------------------------- kernels
-// Counters, the batch start on the device clock, and the batch itself from
-// the pinned staging buffers (9 bytes per update over PCIe, read here rather
+// A batch handed over in pinned host memory (P.hu set): its updates are
+// read here into device arrays (9 bytes per update over PCIe, read rather
 // than copied on another stream: a cross-stream wait costs more than these
-// reads).
+// reads), and the batch start is stamped on the device clock.  A batch
+// staged in device memory needs no such kernel: the claim kernel stamps the
+// start.  The counters need no reset here: every batch's last kernel leaves
+// them reset for the next (publish).
 __device__ void ph_init(const BP& P, const GC& c) {
-    for (int64_t i = c.tid; i < C_NCOUNTERS; i += c.nthreads)
-        if (i != C_T0) P.ctr[MF_CS * (i)] = 0;
-    if (c.tid == 0) {
-        P.ctr[MF_CS * (C_T0)] = wall_clock64();
-        P.ctr[MF_CS * (C_CONF0)] = ~0ull;
-        P.ctr[MF_CS * (C_CONF1)] = ~0ull;
+    if (c.tid == 0) P.ctr[MF_CS * (C_T0)] = wall_clock64();
+    int32_t* du = const_cast<int32_t*>(P.bu);
+    int32_t* dv = const_cast<int32_t*>(P.bv);
+    uint8_t* di = const_cast<uint8_t*>(P.bins);
+    for (int64_t i = c.tid; i < P.U; i += c.nthreads) {
+        du[i] = P.hu[i];
+        dv[i] = P.hv[i];
+        di[i] = P.hins[i];
     }
-    if (P.hu) {
-        int32_t* du = const_cast<int32_t*>(P.bu);
-        int32_t* dv = const_cast<int32_t*>(P.bv);
-        uint8_t* di = const_cast<uint8_t*>(P.bins);
-        for (int64_t i = c.tid; i < P.U; i += c.nthreads) {
-            du[i] = P.hu[i];
-            dv[i] = P.hv[i];
-            di[i] = P.hins[i];
-        }
-    }
+}
+
+// The counters' state at a batch start: every count zero, no conflict.
+__device__ __forceinline__ u64 counter_reset_value(int64_t i) {
+    return (i == C_CONF0 || i == C_CONF1) ? ~0ull : 0ull;
 }
 
 __device__ __forceinline__ GC block_coords() {
@@ -1455,7 +1454,7 @@ __device__ __forceinline__ GC block_coords() {
     }
 
 MF_KERNEL(k_init, ph_init(P, c))
-MF_KERNEL(k_claim, ph_claim(P, c))
+MF_KERNEL(k_claim, if (!P.hu && c.tid == 0) P.ctr[MF_CS * (C_T0)] = wall_clock64(); ph_claim(P, c))
 MF_KERNEL(k_resolve, ph_resolve(P, c))
 MF_KERNEL(k_append, ph_append(P, c))
 MF_KERNEL(k_partner, ph_partner(P, c))
@@ -1511,11 +1510,16 @@ __device__ __forceinline__ void phase_barrier() {
 
 // Stamps the end of the batch on the device clock, then copies the counters
 // and the pool top to the pinned host block of the batch (one workgroup), so
-// the host reads the batch's results without a copy on the stream.
-__device__ void publish(const BP& P, const GC& c) {
+// the host reads the batch's results without a copy on the stream.  At the
+// batch's end (`last`) each counter is reset for the next batch right after
+// it is copied; a halted tail keeps them for the rounds queued after it.
+__device__ void publish(const BP& P, const GC& c, bool last) {
     if (c.tid == 0) P.ctr[MF_CS * (C_T1)] = wall_clock64();
     phase_barrier();
-    for (int64_t i = c.tid; i < C_NCOUNTERS; i += c.nthreads) P.h_out[i] = P.ctr[MF_CS * (i)];
+    for (int64_t i = c.tid; i < C_NCOUNTERS; i += c.nthreads) {
+        P.h_out[i] = P.ctr[MF_CS * (i)];
+        if (last) P.ctr[MF_CS * (i)] = counter_reset_value(i);
+    }
     if (c.tid == 0) P.h_out[C_NCOUNTERS] = *P.g.pool_top;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
 }
@@ -1528,7 +1532,8 @@ __device__ bool finish_rounds(const BP& P, const GC& c, int par, bool& ran) {
     ran = false;
     while (true) {
         if (ran) phase_barrier();
-        const int64_t win = P.window < MF_FIN_WINDOW ? P.window : MF_FIN_WINDOW;
+        const int64_t fw = P.fin_window > 0 ? P.fin_window : MF_FIN_WINDOW;
+        const int64_t win = P.window < fw ? P.window : fw;
         RoundView v = round_view(P, par, win);
         if (!v.live) return true;
         ran = true;
@@ -1583,7 +1588,7 @@ __global__ __launch_bounds__(1024) void k_finish(BP P, int par) {
         const bool halt = P.U - start > MF_HALT_EDGES;
         if (c.tid == 0) P.ctr[MF_CS * (C_HALT)] = halt ? 1ull : 0ull;
         if (halt) {
-            publish(P, c);
+            publish(P, c, false);
             return;
         }
     }
@@ -1596,12 +1601,12 @@ __global__ __launch_bounds__(1024) void k_finish(BP P, int par) {
             finish_completion(P, c);
         }
     }
-    publish(P, c);
+    publish(P, c, true);
 }
 
 __global__ __launch_bounds__(256) void k_publish(BP P) {
     GC c = block_coords();
-    publish(P, c);
+    publish(P, c, true);
 }
 
 // ---------------------------------------------------------------- launchers
@@ -1677,7 +1682,7 @@ int mf_round_grid(const LaunchCfg& L) {
 #define LAUNCH_ROUND(k, ...) hipLaunchKernelGGL(k, dim3(mf_round_grid(L)), dim3(256), 0, L.stream, __VA_ARGS__)
 
 void mf_launch_batch_prologue(const BP& P, const LaunchCfg& L) {
-    LAUNCH_C(k_init, G_UPD, P);
+    if (P.hu) LAUNCH_C(k_init, G_UPD, P);
     LAUNCH_C(k_claim, G_CLAIM, P);
     LAUNCH_C(k_resolve, G_UPD, P);
     LAUNCH_C(k_append, G_WAVE_UPD, P);
