@@ -64,6 +64,7 @@ typedef struct {
     int64_t overlay_entries;  /* pool slots in use after the batch */
     double device_ms;         /* batch time on the device (device wall clock, first kernel to end of tail) */
     double prologue_ms;       /* ... of which the overlay update, may-uncover set and admissible index */
+    int64_t window;           /* edges speculated per round (the rounds' window) */
 } misfit_batch_stats;
 
 /* Execution modes of the repair (results are identical). */
@@ -220,6 +221,11 @@ typedef struct {
                                  batch is queued with the rounds it is expected to need and gets more
                                  from the host if it needs them, so no rounds run empty (many graphs) */
     int workers;              /* dispatcher threads, each owning a share of the graphs (0: 8) */
+    int persistent;           /* with max_inflight 1: 0 every batch on its graph's stream; 1 small batches,
+                                 contended graphs' batches and first batches run whole, each in one
+                                 workgroup of one long-running serving kernel shared by all graphs (its
+                                 workgroups take batches from a queue as they free up), the rest on their
+                                 streams; 2 every batch in the serving kernel */
 } misfit_server_opts;
 
 typedef struct {

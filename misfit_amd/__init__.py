@@ -49,7 +49,8 @@ class BatchStats(ctypes.Structure):
     _fields_ = [(k, ctypes.c_int64) for k in (
         "updates", "seeds", "active_ins", "evictions", "admit_ins", "active_del", "admit_del",
         "completion_adds", "rounds", "evaluations", "maybe_uncovered", "admissible", "delta_ops",
-        "overlay_entries")] + [("device_ms", ctypes.c_double), ("prologue_ms", ctypes.c_double)]
+        "overlay_entries")] + [("device_ms", ctypes.c_double), ("prologue_ms", ctypes.c_double),
+                               ("window", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -57,7 +58,7 @@ class BatchStats(ctypes.Structure):
 
 class ServerOpts(ctypes.Structure):
     _fields_ = [("hbm_reserve", ctypes.c_int64), ("compact_fraction", ctypes.c_double),
-                ("max_inflight", ctypes.c_int), ("workers", ctypes.c_int)]
+                ("max_inflight", ctypes.c_int), ("workers", ctypes.c_int), ("persistent", ctypes.c_int)]
 
 
 class RequestStats(ctypes.Structure):
@@ -499,9 +500,9 @@ class Server:
     """Serving layer (misfit_server_*): many graphs on one GPU, update batches
     submitted as they arrive, continuous batching across graphs."""
 
-    def __init__(self, device=0, hbm_reserve=0, compact_fraction=0.05, max_inflight=2, workers=0):
+    def __init__(self, device=0, hbm_reserve=0, compact_fraction=0.05, max_inflight=2, workers=0, persistent=0):
         self.srv = ctypes.c_void_p()
-        opts = ServerOpts(hbm_reserve, compact_fraction, max_inflight, workers)
+        opts = ServerOpts(hbm_reserve, compact_fraction, max_inflight, workers, persistent)
         _ck(lib().misfit_server_create(device, ctypes.byref(opts), ctypes.byref(self.srv)))
 
     def _ck(self, rc):

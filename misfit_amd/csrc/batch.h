@@ -117,6 +117,30 @@ struct BP {
                        // by the batch's last kernel (C_NCOUNTERS + 1 words)
 };
 
+// The serving kernel's work queue, in pinned host memory (coherent): the
+// host writes an item, then raises `tail`; a workgroup of the kernel claims
+// the next item with a compare-and-swap on a head counter in device memory,
+// runs the batch's speculative rounds, rule counters and completion (and
+// its prologue too for a `whole` item; otherwise the prologue's kernels ran
+// on the graph's own stream before the item was published), publishes its
+// counters to P.h_out and then writes the item's index plus one to
+// P.h_out[MF_DONE_WORD].
+#define MF_SERVE_CAP 4096
+struct ServeItem {
+    BP P;
+    u64 whole;      // 1: the workgroup runs the prologue too; 0: it ran on the graph's stream
+    u64 pad[4];
+};
+struct ServeQueue {
+    u64 tail;       // items published
+    u64 stop;       // nonzero: every workgroup leaves at its next claim
+    u64 pad[14];
+    ServeItem items[MF_SERVE_CAP];
+};
+// Word of P.h_out that reports a finished batch of the serving kernel.
+#define MF_DONE_WORD (C_NCOUNTERS + 1)
+#define MF_HOUT_WORDS (C_NCOUNTERS + 2)
+
 // host-side launchers (k_batch.hip)
 struct LaunchCfg {
     int blocks;        // grid cap for grid-stride phases (8 workgroups per CU)
@@ -138,4 +162,9 @@ void mf_launch_finish(const BP& P, const LaunchCfg& L, int par);
 void mf_launch_completion_collect(const BP& P, const LaunchCfg& L);
 void mf_launch_stats(const BP& P, const LaunchCfg& L);
 void mf_launch_completion_round(const BP& P, const LaunchCfg& L, int par);
+// The serving kernel: `grid` workgroups of `threads` threads (256, 512 or
+// 1024), each taking whole batches from the queue (q and head as the device
+// addresses them) until the host sets stop or no item comes for idle_ticks
+// of the device clock.
+void mf_launch_serve(ServeQueue* q, u64* head, int grid, int threads, u64 idle_ticks, hipStream_t st);
 void mf_launch_publish(const BP& P, const LaunchCfg& L);

@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -45,7 +46,12 @@ struct Slot {
     LaunchCfg L{0, nullptr};
     int par_next = 0;      // parity of the batch's next round and tail
     bool may_halt = false; // queued without a reliable round estimate; not yet settled
+    bool external = false; // handed to the serving kernel: done when h_ctr[MF_DONE_WORD] is set
 };
+
+// A batch handed to the serving kernel has finished once the kernel wrote
+// its item number (plus one) after the counters.
+static bool external_done(const Slot& s) { return __atomic_load_n(&s.h_ctr[MF_DONE_WORD], __ATOMIC_ACQUIRE) != 0; }
 
 struct misfit_ctx {
     int device = 0;
@@ -66,6 +72,8 @@ struct misfit_ctx {
     double window_factor = 16.0;  // MISFIT_WINDOW_FACTOR: window = this x expected edges per round
     int round_margin = -1;        // MISFIT_ROUND_MARGIN: rounds queued beyond the planned rate (-1: by the plan)
     int fin_window = 0;           // MISFIT_TAIL_WINDOW: edges per round of the one-workgroup tail (0: 1024)
+    int serve_window = 0;         // MISFIT_SERVE_WINDOW: edges per round of batches in the serving kernel (0: adaptive)
+    double last_epr = 0.0;        // edges committed per round by the last batch (0: none yet)
     double clock_khz = 0.0;       // device wall clock, for per-batch device times
     hipEvent_t timer0 = nullptr, timer1 = nullptr;  // misfit_timer_begin/end
     int round_per_cu = 0;      // MISFIT_ROUND_WG_PER_CU: round grid (0: by occupancy)
@@ -122,6 +130,56 @@ struct misfit_staged {
     int32_t n = 0;  // the vertex count the endpoints were checked against
 };
 
+// ---------------------------------------------------------------- frees
+namespace {
+std::mutex g_free_mu;
+int g_serving = 0;                 // serving kernels that may be running
+std::vector<void*> g_dev_deferred;  // frees waiting for them to end
+std::vector<void*> g_host_deferred;
+}  // namespace
+
+void mf_free_device(void* p) {
+    if (!p) return;
+    {
+        std::lock_guard<std::mutex> lk(g_free_mu);
+        if (g_serving > 0) {
+            g_dev_deferred.push_back(p);
+            return;
+        }
+    }
+    hipFree(p);
+}
+
+void mf_free_host(void* p) {
+    if (!p) return;
+    {
+        std::lock_guard<std::mutex> lk(g_free_mu);
+        if (g_serving > 0) {
+            g_host_deferred.push_back(p);
+            return;
+        }
+    }
+    hipHostFree(p);
+}
+
+void mf_serving_begin() {
+    std::lock_guard<std::mutex> lk(g_free_mu);
+    g_serving++;
+}
+
+void mf_serving_end() {
+    std::vector<void*> dev, host;
+    {
+        std::lock_guard<std::mutex> lk(g_free_mu);
+        if (--g_serving > 0) return;
+        g_serving = 0;
+        dev.swap(g_dev_deferred);
+        host.swap(g_host_deferred);
+    }
+    for (void* p : dev) hipFree(p);
+    for (void* p : host) hipHostFree(p);
+}
+
 // ---------------------------------------------------------------- errors
 static int fail(misfit_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
@@ -134,6 +192,17 @@ static int fail(misfit_ctx* c, int code, const std::string& msg) {
         if (_e != hipSuccess)                                                                   \
             return fail((c), MISFIT_E_HIP, std::string(#expr ": ") + hipGetErrorString(_e));    \
     } while (0)
+
+// A copy on the context's stream, then a wait for that stream: every copy is
+// ordered with the context's kernels on both sides.  (The null stream of a
+// plain hipMemcpy does not order against the context's non-blocking stream,
+// and a device-to-device or pageable host-to-device hipMemcpy may return
+// before its copy has landed.)
+static hipError_t copy_sync(misfit_ctx* c, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return e;
+}
 
 static int build_row_index(misfit_ctx* c);
 static void free_compact_scratch(misfit_ctx* c);
@@ -148,7 +217,7 @@ static hipError_t dalloc(std::vector<void*>& owner, T** p, int64_t count) {
 }
 
 static void free_list(std::vector<void*>& v) {
-    for (void* p : v) hipFree(p);
+    for (void* p : v) mf_free_device(p);
     v.clear();
 }
 
@@ -186,6 +255,8 @@ extern "C" int misfit_ctx_create(int device, misfit_ctx** out) {
     if (rm) c->round_margin = std::max(0, std::min(16, atoi(rm)));
     const char* ps = getenv("MISFIT_PU_SCAN");
     c->force_pu_scan = ps && strcmp(ps, "full") == 0;
+    const char* sw = getenv("MISFIT_SERVE_WINDOW");
+    if (sw) c->serve_window = std::max(0, std::min(1 << 20, atoi(sw)));
     const char* tw = getenv("MISFIT_TAIL_WINDOW");
     if (tw) c->fin_window = std::max(0, std::min(1 << 20, atoi(tw)));
     const char* gr = getenv("MISFIT_GRID_ROUNDS");
@@ -209,7 +280,7 @@ extern "C" int misfit_ctx_create(int device, misfit_ctx** out) {
     for (auto& s : c->slots) {
         hipEventCreateWithFlags(&s.done, hipEventDisableTiming);
         // written by the batch's last kernel, read after its event
-        hipHostMalloc((void**)&s.h_ctr, (C_NCOUNTERS + 1) * sizeof(u64), hipHostMallocCoherent);
+        hipHostMalloc((void**)&s.h_ctr, MF_HOUT_WORDS * sizeof(u64), hipHostMallocCoherent);
     }
     *out = c;
     return MISFIT_OK;
@@ -220,9 +291,9 @@ extern "C" int misfit_ctx_create(int device, misfit_ctx** out) {
 // (the regrowth synchronises the stream first).
 static void free_slots(misfit_ctx* c) {
     for (auto& s : c->slots) {
-        if (s.h_u) hipHostFree(s.h_u);
-        if (s.h_v) hipHostFree(s.h_v);
-        if (s.h_ins) hipHostFree(s.h_ins);
+        if (s.h_u) mf_free_host(s.h_u);
+        if (s.h_v) mf_free_host(s.h_v);
+        if (s.h_ins) mf_free_host(s.h_ins);
         s.h_u = nullptr; s.h_v = nullptr; s.h_ins = nullptr;
         s.g_u = nullptr; s.g_v = nullptr; s.g_ins = nullptr;
         s.d_u = nullptr; s.d_v = nullptr; s.d_ins = nullptr;
@@ -235,7 +306,7 @@ extern "C" int misfit_ctx_destroy(misfit_ctx* c) {
     hipStreamSynchronize(c->stream);
     free_slots(c);
     for (auto& s : c->slots) {
-        if (s.h_ctr) hipHostFree(s.h_ctr);
+        if (s.h_ctr) mf_free_host(s.h_ctr);
         hipEventDestroy(s.done);
     }
     hipEventDestroy(c->timer0);
@@ -244,13 +315,13 @@ extern "C" int misfit_ctx_destroy(misfit_ctx* c) {
     free_list(c->graph_allocs);
     free_compact_scratch(c);
     for (misfit_staged* b : c->staged) {
-        hipFree(b->u);
+        mf_free_device(b->u);
         delete b;
     }
-    if (c->d_ktask) hipFree(c->d_ktask);
-    if (c->d_sval) hipFree(c->d_sval);
-    if (c->d_spos) hipFree(c->d_spos);
-    if (c->h_small) hipHostFree(c->h_small);
+    if (c->d_ktask) mf_free_device(c->d_ktask);
+    if (c->d_sval) mf_free_device(c->d_sval);
+    if (c->d_spos) mf_free_device(c->d_spos);
+    if (c->h_small) mf_free_host(c->h_small);
     hipStreamDestroy(c->stream);
     delete c;
     return MISFIT_OK;
@@ -371,7 +442,7 @@ static int ensure_batch_capacity(misfit_ctx* c, int64_t U) {
     }
     if (e != hipSuccess) {
         free_list(A);
-        for (void* p : pinned) hipHostFree(p);
+        for (void* p : pinned) mf_free_host(p);
         const int code = e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation ? MISFIT_E_NOMEM : MISFIT_E_HIP;
         return fail(c, code, std::string("batch buffers for ") + std::to_string(cap) + " updates: " + hipGetErrorString(e));
     }
@@ -405,8 +476,8 @@ static int ensure_overlay_capacity(misfit_ctx* c, int64_t extra_ins) {
         int32_t* np = nullptr;
         u64 top = 0;
         hipError_t e = dalloc(fresh, &np, nc);
-        if (e == hipSuccess) e = hipMemcpy(&top, g.pool_top, sizeof(u64), hipMemcpyDeviceToHost);
-        if (e == hipSuccess && top) e = hipMemcpy(np, g.pool, top * sizeof(int32_t), hipMemcpyDeviceToDevice);
+        if (e == hipSuccess) e = copy_sync(c, &top, g.pool_top, sizeof(u64), hipMemcpyDeviceToHost);
+        if (e == hipSuccess && top) e = copy_sync(c, np, g.pool, top * sizeof(int32_t), hipMemcpyDeviceToDevice);
         if (e != hipSuccess) {  // the old pool stays in use
             free_list(fresh);
             const int code = e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation ? MISFIT_E_NOMEM : MISFIT_E_HIP;
@@ -415,7 +486,7 @@ static int ensure_overlay_capacity(misfit_ctx* c, int64_t extra_ins) {
         c->graph_allocs.push_back(np);
         auto& v = c->graph_allocs;
         auto it = std::find(v.begin(), v.end(), (void*)g.pool);
-        if (it != v.end()) { hipFree(*it); v.erase(it); }
+        if (it != v.end()) { mf_free_device(*it); v.erase(it); }
         g.pool = np;
         g.pool_cap = nc;
     }
@@ -424,8 +495,8 @@ static int ensure_overlay_capacity(misfit_ctx* c, int64_t extra_ins) {
         int64_t nc = (int64_t)pow2_at_least((u64)std::max<int64_t>(need_ph * 2, c->ph_cap * 2));
         std::vector<u64> keys((size_t)c->ph_cap);
         std::vector<int32_t> vals((size_t)c->ph_cap);
-        HIPCHK(c, hipMemcpy(keys.data(), g.ph_key, c->ph_cap * sizeof(u64), hipMemcpyDeviceToHost));
-        HIPCHK(c, hipMemcpy(vals.data(), g.ph_val, c->ph_cap * sizeof(int32_t), hipMemcpyDeviceToHost));
+        HIPCHK(c, copy_sync(c, keys.data(), g.ph_key, c->ph_cap * sizeof(u64), hipMemcpyDeviceToHost));
+        HIPCHK(c, copy_sync(c, vals.data(), g.ph_val, c->ph_cap * sizeof(int32_t), hipMemcpyDeviceToHost));
         std::vector<u64> nk((size_t)nc, MF_EMPTY_KEY);
         std::vector<int32_t> nv((size_t)nc, 0);
         auto mixh = [](u64 z) {
@@ -447,8 +518,8 @@ static int ensure_overlay_capacity(misfit_ctx* c, int64_t extra_ins) {
         int32_t* nval = nullptr;
         hipError_t e = dalloc(fresh, &nkey, nc);
         if (e == hipSuccess) e = dalloc(fresh, &nval, nc);
-        if (e == hipSuccess) e = hipMemcpy(nkey, nk.data(), nc * sizeof(u64), hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(nval, nv.data(), nc * sizeof(int32_t), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = copy_sync(c, nkey, nk.data(), nc * sizeof(u64), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = copy_sync(c, nval, nv.data(), nc * sizeof(int32_t), hipMemcpyHostToDevice);
         if (e != hipSuccess) {
             free_list(fresh);
             c->S.g = g;
@@ -458,7 +529,7 @@ static int ensure_overlay_capacity(misfit_ctx* c, int64_t extra_ins) {
         auto& v = c->graph_allocs;
         for (void* old : {(void*)g.ph_key, (void*)g.ph_val}) {
             auto it = std::find(v.begin(), v.end(), old);
-            if (it != v.end()) { hipFree(*it); v.erase(it); }
+            if (it != v.end()) { mf_free_device(*it); v.erase(it); }
         }
         v.push_back(nkey);
         v.push_back(nval);
@@ -510,17 +581,17 @@ extern "C" int misfit_load_csr(misfit_ctx* c, int32_t n, int64_t m, const void* 
     HIPCHK(c, dalloc(A, &c->d_nindex, (int64_t)n + 1));
     HIPCHK(c, dalloc(A, &c->d_nlist, m));
     if (nindex_is_64) {
-        HIPCHK(c, hipMemcpy(c->d_nindex, nindex, ((size_t)n + 1) * 8, hipMemcpyHostToDevice));
+        HIPCHK(c, copy_sync(c, c->d_nindex, nindex, ((size_t)n + 1) * 8, hipMemcpyHostToDevice));
     } else {
         int32_t* tmp = nullptr;
         HIPCHK(c, hipMalloc((void**)&tmp, ((size_t)n + 1) * 4));
-        hipError_t e = hipMemcpy(tmp, nindex, ((size_t)n + 1) * 4, hipMemcpyHostToDevice);
+        hipError_t e = copy_sync(c, tmp, nindex, ((size_t)n + 1) * 4, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = mf_widen(tmp, c->d_nindex, (int64_t)n + 1, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-        hipFree(tmp);
+        mf_free_device(tmp);
         HIPCHK(c, e);
     }
-    if (m) HIPCHK(c, hipMemcpy(c->d_nlist, nlist, (size_t)m * 4, hipMemcpyHostToDevice));
+    if (m) HIPCHK(c, copy_sync(c, c->d_nlist, nlist, (size_t)m * 4, hipMemcpyHostToDevice));
     g.nindex = c->d_nindex;
     g.nlist = c->d_nlist;
     HIPCHK(c, dalloc(A, &g.ov_start, n));
@@ -634,10 +705,10 @@ extern "C" int misfit_set_mis(misfit_ctx* c, const uint8_t* mis) {
     int32_t n = c->P.g.n;
     uint8_t* d = nullptr;
     HIPCHK(c, hipMalloc((void**)&d, n));
-    hipError_t e = hipMemcpy(d, mis, n, hipMemcpyHostToDevice);
+    hipError_t e = copy_sync(c, d, mis, n, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = mf_set_mis(c->S, d);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    hipFree(d);
+    mf_free_device(d);
     HIPCHK(c, e);
     // a maximal start stays maximal while the completion runs (a vertex can
     // only lose its last MIS neighbour next to an eviction or at a deleted
@@ -662,8 +733,8 @@ extern "C" int misfit_get_mis(misfit_ctx* c, uint8_t* mis) {
     HIPCHK(c, hipMalloc((void**)&d, n));
     hipError_t e = mf_get_mis(c->S, d);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e == hipSuccess) e = hipMemcpy(mis, d, n, hipMemcpyDeviceToHost);
-    hipFree(d);
+    if (e == hipSuccess) e = copy_sync(c, mis, d, n, hipMemcpyDeviceToHost);
+    mf_free_device(d);
     HIPCHK(c, e);
     return MISFIT_OK;
 }
@@ -682,15 +753,15 @@ extern "C" int misfit_flags(misfit_ctx* c, const int32_t* u, const int32_t* v, i
     hipError_t e = hipMalloc((void**)&du, count * 4);
     if (e == hipSuccess) e = hipMalloc((void**)&dv, count * 4);
     if (e == hipSuccess) e = hipMalloc((void**)&di, count);
-    if (e == hipSuccess) e = hipMemcpy(du, u, count * 4, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(dv, v, count * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = copy_sync(c, du, u, count * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = copy_sync(c, dv, v, count * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess) e = mf_flags(c->S, du, dv, count, di);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e == hipSuccess) e = hipMemcpy(ins, di, count, hipMemcpyDeviceToHost);
-    hipFree(du);
-    hipFree(dv);
-    hipFree(di);
+    if (e == hipSuccess) e = copy_sync(c, ins, di, count, hipMemcpyDeviceToHost);
+    mf_free_device(du);
+    mf_free_device(dv);
+    mf_free_device(di);
     HIPCHK(c, e);
     return MISFIT_OK;
 }
@@ -713,6 +784,7 @@ static void fill_stats(const misfit_ctx* c, const Slot& s, misfit_batch_stats* s
     st->admissible = (int64_t)k[C_ADM];
     st->delta_ops = (int64_t)k[C_DELTA_OPS];
     st->overlay_entries = (int64_t)k[C_NCOUNTERS];
+    st->window = s.P.window;
     // device wall clock (kHz ticks per ms): first kernel start to the end of
     // the tail, and to the start of the first round
     if (c->clock_khz > 0 && k[C_T1] >= k[C_T0]) {
@@ -902,6 +974,9 @@ static int continue_batch(misfit_ctx* c, Slot& s) {
 // finished (queueing further rounds where its tail halted).  Called before
 // anything that queues more work on the graph or reads its state.
 static int settle(misfit_ctx* c) {
+    for (auto& s : c->slots)
+        if (s.busy && s.external && !external_done(s))
+            return fail(c, MISFIT_E_STATE, "a batch of this graph is still running in the serving kernel");
     for (auto& s : c->slots) {
         if (!s.busy || !s.may_halt) continue;
         while (true) {
@@ -931,8 +1006,8 @@ void mf_ctx_set_halting(misfit_ctx* c, bool on) { c->always_halt = on; }
 // adjacency tests binary-search them (any older copy is dropped first).
 static int build_row_index(misfit_ctx* c) {
     DevGraph& g = c->P.g;
-    if (c->d_sval) hipFree(c->d_sval);
-    if (c->d_spos) hipFree(c->d_spos);
+    if (c->d_sval) mf_free_device(c->d_sval);
+    if (c->d_spos) mf_free_device(c->d_spos);
     c->d_sval = c->d_spos = nullptr;
     g.sval = g.spos = nullptr;
     c->S.g = g;
@@ -958,7 +1033,7 @@ static int ensure_ktask_capacity(misfit_ctx* c, int64_t U) {
     int64_t need = U + (c->P.g.m + c->P.g.pool_cap) / MF_KCH + 2;
     if (need <= c->ktask_cap) return MISFIT_OK;
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (c->d_ktask) hipFree(c->d_ktask);
+    if (c->d_ktask) mf_free_device(c->d_ktask);
     c->d_ktask = nullptr;
     c->ktask_cap = 0;
     int64_t cap = need + need / 2;
@@ -983,8 +1058,21 @@ static int check_updates(misfit_ctx* c, const misfit_update* ups, int64_t count,
 
 // Queues one batch: from the caller's host array (ups, copied into the
 // slot's pinned staging) or from a batch already in device memory (st).
+// The window (edges per round) of a batch run whole by one workgroup of the
+// serving kernel: four times the edges the graph's last batch committed per
+// round, rounded up to 32, within [64, 1024]; 128 without history.
+static int64_t serve_window(const misfit_ctx* c) {
+    if (c->serve_window > 0) return c->serve_window;
+    if (c->last_epr <= 0.0) return 128;
+    const int64_t w = ((int64_t)std::ceil(4.0 * c->last_epr) + 31) / 32 * 32;
+    return std::min<int64_t>(1024, std::max<int64_t>(64, w));
+}
+
+// Queues one batch (ext null), or prepares it for the serving kernel (ext:
+// its parameters are returned and nothing is launched; the batch is done
+// when the kernel has written its done word, see misfit_poll).
 static int submit(misfit_ctx* c, const misfit_update* ups, const misfit_staged* st, int64_t count,
-                  int64_t nins, uint64_t* ticket) {
+                  int64_t nins, uint64_t* ticket, BP* ext = nullptr, bool ext_whole = false) {
     HIPCHK(c, hipSetDevice(c->device));
     int rc = settle(c);
     if (rc) return rc;
@@ -996,6 +1084,7 @@ static int submit(misfit_ctx* c, const misfit_update* ups, const misfit_staged* 
     if (rc) return rc;
     uint64_t t = c->next_ticket++;
     Slot& s = c->slots[t % MF_NSLOT];
+    if (s.busy && s.external) return fail(c, MISFIT_E_STATE, "staging slot still held by a batch in the serving kernel");
     if (s.busy) HIPCHK(c, hipEventSynchronize(s.done));  // staging still owned by an older batch
     if (!st) {
         for (int64_t i = 0; i < count; i++) {
@@ -1043,11 +1132,35 @@ static int submit(misfit_ctx* c, const misfit_update* ups, const misfit_staged* 
     const bool sure = !queued || plan.sure;
     const int k = queued ? plan.k : 0;
     P.may_halt = sure ? 0 : 1;
+    c->last_U = count;
+    if (ext) {  // the batch's rounds (and with ext_whole its prologue) in the serving kernel
+        P.window = P.fin_window = serve_window(c);
+        P.may_halt = 0;
+        s.P = P;
+        s.may_halt = false;
+        s.external = true;
+        s.h_ctr[MF_DONE_WORD] = 0;
+        if (ext_whole) {
+            // the kernel runs on another stream: whatever this graph's
+            // stream still has queued (buffer fills after a growth or a
+            // compaction) must be done before it starts
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        } else {
+            // the prologue's grid-wide kernels on the graph's stream; the
+            // item is published once they have ended (s.done)
+            mf_launch_batch_prologue(P, L);
+            HIPCHK(c, hipGetLastError());
+        }
+        HIPCHK(c, hipEventRecord(s.done, c->stream));
+        *ext = P;
+        if (ticket) *ticket = t;
+        return MISFIT_OK;
+    }
+    s.external = false;
     s.P = P;
     s.L = L;
     s.par_next = k & 1;
     s.may_halt = !sure;
-    c->last_U = count;
     rc = queued ? drive_queued(c, P, L, k) : drive_plain(c, P, L);
     if (rc) return rc;
     // the counters, the pool top and the device clock stamps reached s.h_ctr
@@ -1063,6 +1176,32 @@ extern "C" int misfit_submit_batch(misfit_ctx* c, const misfit_update* ups, int6
     int64_t nins = 0;
     if (int rc = check_updates(c, ups, count, &nins)) return rc;
     return submit(c, ups, nullptr, count, nins, ticket);
+}
+
+// For the serving layer: checks and prepares one batch for the serving
+// kernel (capacity, staging, parameters); *P receives its parameters.  With
+// `whole` nothing is launched: the kernel runs the whole batch.  Otherwise
+// the prologue's kernels are queued on the graph's stream, and the item may
+// be published once mf_ctx_external_ready reports them ended.  misfit_poll
+// reports the batch done once the kernel has written its done word;
+// misfit_wait then collects it.
+int mf_ctx_prepare_external(misfit_ctx* c, const misfit_update* ups, int64_t count, bool whole, uint64_t* ticket,
+                            BP* P) {
+    if (!c || count < 0 || (count && !ups) || !P) return MISFIT_E_ARG;
+    int64_t nins = 0;
+    if (int rc = check_updates(c, ups, count, &nins)) return rc;
+    return submit(c, ups, nullptr, count, nins, ticket, P, whole);
+}
+
+// 1 once the prologue of a batch prepared for the serving kernel has ended
+// (the item may be published), 0 before, or an error code.
+int mf_ctx_external_ready(misfit_ctx* c, uint64_t ticket) {
+    Slot& s = c->slots[ticket % MF_NSLOT];
+    if (!s.busy || s.ticket != ticket || !s.external) return fail(c, MISFIT_E_ARG, "unknown ticket");
+    const hipError_t e = hipEventQuery(s.done);
+    if (e == hipErrorNotReady) return 0;
+    HIPCHK(c, e);
+    return 1;
 }
 
 extern "C" int misfit_stage_batch(misfit_ctx* c, const misfit_update* ups, int64_t count, misfit_staged** out) {
@@ -1083,9 +1222,9 @@ extern "C" int misfit_stage_batch(misfit_ctx* c, const misfit_update* ups, int64
     }
     void* d = nullptr;
     HIPCHK(c, hipMalloc(&d, h.size()));
-    hipError_t e = hipMemcpy(d, h.data(), h.size(), hipMemcpyHostToDevice);
+    hipError_t e = copy_sync(c, d, h.data(), h.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-        hipFree(d);
+        mf_free_device(d);
         return fail(c, MISFIT_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
     }
     misfit_staged* b = new misfit_staged;
@@ -1119,7 +1258,7 @@ extern "C" int misfit_release_staged(misfit_ctx* c, misfit_staged* b) {
     // batches that read it may still be queued
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->staged.erase(std::find(c->staged.begin(), c->staged.end(), b));
-    hipFree(b->u);
+    mf_free_device(b->u);
     delete b;
     return MISFIT_OK;
 }
@@ -1128,6 +1267,7 @@ extern "C" int misfit_poll(misfit_ctx* c, uint64_t ticket) {
     if (!c) return MISFIT_E_ARG;
     Slot& s = c->slots[ticket % MF_NSLOT];
     if (!s.busy || s.ticket != ticket) return fail(c, MISFIT_E_ARG, "unknown or expired ticket");
+    if (s.external) return external_done(s) ? 1 : 0;
     hipError_t e = hipEventQuery(s.done);
     if (e == hipErrorNotReady) return 0;
     HIPCHK(c, e);
@@ -1144,15 +1284,24 @@ extern "C" int misfit_wait(misfit_ctx* c, uint64_t ticket, misfit_batch_stats* s
     Slot& s = c->slots[ticket % MF_NSLOT];
     if (!s.busy || s.ticket != ticket) return fail(c, MISFIT_E_ARG, "unknown or expired ticket");
     HIPCHK(c, hipSetDevice(c->device));
-    while (true) {
-        HIPCHK(c, hipEventSynchronize(s.done));
-        if (!(s.may_halt && s.h_ctr[C_HALT])) break;
-        int rc = continue_batch(c, s);
-        if (rc) return rc;
+    if (s.external) {
+        // the serving kernel's caller waits with misfit_poll (it may have to
+        // relaunch the kernel); here the batch must be done already
+        if (!external_done(s)) return fail(c, MISFIT_E_STATE, "batch still running in the serving kernel");
+    } else {
+        while (true) {
+            HIPCHK(c, hipEventSynchronize(s.done));
+            if (!(s.may_halt && s.h_ctr[C_HALT])) break;
+            int rc = continue_batch(c, s);
+            if (rc) return rc;
+        }
     }
     s.busy = false;
     s.may_halt = false;
-    if (s.count >= 512 && s.h_ctr[C_ROUNDS] >= 1) {
+    const bool was_external = s.external;
+    s.external = false;
+    if (s.h_ctr[C_ROUNDS] >= 1) c->last_epr = (double)s.count / (double)s.h_ctr[C_ROUNDS];
+    if (!was_external && s.count >= 512 && s.h_ctr[C_ROUNDS] >= 1) {
         c->hist_rpu[c->hist_pos & 15] = (double)(s.h_ctr[C_ROUNDS] - 1) / (double)s.count;
         c->hist_count[c->hist_pos & 15] = s.count;
         c->hist_pos++;
@@ -1210,9 +1359,9 @@ extern "C" int misfit_batch_bytes(misfit_ctx* c, int index_bytes, int64_t* bytes
 // ---------------------------------------------------------------- compaction
 // Compaction scratch: freed with the graph (reload, destroy).
 static void free_compact_scratch(misfit_ctx* c) {
-    for (void* p : c->cs_allocs) hipFree(p);
+    for (void* p : c->cs_allocs) mf_free_device(p);
     c->cs_allocs.clear();
-    if (c->cs_tmp) hipFree(c->cs_tmp);
+    if (c->cs_tmp) mf_free_device(c->cs_tmp);
     c->cs_tmp = nullptr;
     c->cs_tmp_bytes = 0;
     c->cs_n = c->cs_d = c->cs_t = 0;
@@ -1235,7 +1384,7 @@ static void cs_release(misfit_ctx* c, void* p) {
     auto it = std::find(c->cs_allocs.begin(), c->cs_allocs.end(), p);
     if (it == c->cs_allocs.end()) return;
     c->cs_allocs.erase(it);
-    hipFree(p);
+    mf_free_device(p);
 }
 
 // Per-vertex scratch (once per graph) and the deletion and tail lists
@@ -1301,7 +1450,7 @@ extern "C" int misfit_compact(misfit_ctx* c, int64_t* new_m) {
         return e;
     };
     auto drop_all = [&]() {
-        for (void* p : mine) hipFree(p);
+        for (void* p : mine) mf_free_device(p);
         mine.clear();
     };
     auto keep = [&](void* p) { mine.erase(std::find(mine.begin(), mine.end(), p)); };
@@ -1396,7 +1545,7 @@ extern "C" int misfit_compact(misfit_ctx* c, int64_t* new_m) {
         keep(ntasks);
         auto& ga = c->graph_allocs;
         auto it = std::find(ga.begin(), ga.end(), c->S.d_walk_tasks);
-        if (it != ga.end()) { hipFree(*it); ga.erase(it); }
+        if (it != ga.end()) { mf_free_device(*it); ga.erase(it); }
         ga.push_back(ntasks);
         c->S.d_walk_tasks = ntasks;
         c->S.walk_task_cap = walk_need;
@@ -1404,14 +1553,14 @@ extern "C" int misfit_compact(misfit_ctx* c, int64_t* new_m) {
     auto& v = c->graph_allocs;
     for (void* old : {(void*)c->d_nindex, (void*)c->d_nlist}) {
         auto it = std::find(v.begin(), v.end(), old);
-        if (it != v.end()) { hipFree(*it); v.erase(it); }
+        if (it != v.end()) { mf_free_device(*it); v.erase(it); }
     }
     v.push_back(nidx);
     v.push_back(nl);
     c->d_nindex = nidx;
     c->d_nlist = nl;
-    if (c->d_sval) hipFree(c->d_sval);
-    if (c->d_spos) hipFree(c->d_spos);
+    if (c->d_sval) mf_free_device(c->d_sval);
+    if (c->d_spos) mf_free_device(c->d_spos);
     c->d_sval = c->d_spos = nullptr;
     if (merged) {
         keep(nsval);
@@ -1451,7 +1600,7 @@ extern "C" int misfit_graph_size(misfit_ctx* c, int32_t* n, int64_t* base_m, int
         int64_t total = 0;
         if (e == hipSuccess) e = hipMemcpyAsync(&total, nidx + g.n, 8, hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-        hipFree(nidx);
+        mf_free_device(nidx);
         HIPCHK(c, e);
         *live_m = total;
     }
@@ -1465,10 +1614,10 @@ extern "C" int misfit_export_csr(misfit_ctx* c, int64_t* nindex, int32_t* nlist)
     if (int rc_ = settle(c)) return rc_;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     DevGraph& g = c->P.g;
-    HIPCHK(c, hipMemcpy(nindex, g.nindex, ((size_t)g.n + 1) * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_sync(c, nindex, g.nindex, ((size_t)g.n + 1) * 8, hipMemcpyDeviceToHost));
     if (g.m) {
         if (!nlist) return MISFIT_E_ARG;
-        HIPCHK(c, hipMemcpy(nlist, g.nlist, (size_t)g.m * 4, hipMemcpyDeviceToHost));
+        HIPCHK(c, copy_sync(c, nlist, g.nlist, (size_t)g.m * 4, hipMemcpyDeviceToHost));
         // deleted base entries are stored complemented; export live values
         for (int64_t i = 0; i < g.m; i++)
             if (nlist[i] < 0) return fail(c, MISFIT_E_STATE, "base holds deletions: compact before export");
@@ -1487,12 +1636,12 @@ extern "C" int misfit_neighbors(misfit_ctx* c, int32_t v, int32_t* out, int64_t 
     int64_t be[2];
     int64_t ovs = 0;
     int32_t ovl = 0;
-    HIPCHK(c, hipMemcpy(be, g.nindex + v, 16, hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(&ovl, g.ov_len + v, 4, hipMemcpyDeviceToHost));
-    if (ovl) HIPCHK(c, hipMemcpy(&ovs, g.ov_start + v, 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_sync(c, be, g.nindex + v, 16, hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_sync(c, &ovl, g.ov_len + v, 4, hipMemcpyDeviceToHost));
+    if (ovl) HIPCHK(c, copy_sync(c, &ovs, g.ov_start + v, 8, hipMemcpyDeviceToHost));
     std::vector<int32_t> row((size_t)(be[1] - be[0]) + ovl);
-    if (be[1] > be[0]) HIPCHK(c, hipMemcpy(row.data(), g.nlist + be[0], (be[1] - be[0]) * 4, hipMemcpyDeviceToHost));
-    if (ovl) HIPCHK(c, hipMemcpy(row.data() + (be[1] - be[0]), g.pool + ovs, (size_t)ovl * 4, hipMemcpyDeviceToHost));
+    if (be[1] > be[0]) HIPCHK(c, copy_sync(c, row.data(), g.nlist + be[0], (be[1] - be[0]) * 4, hipMemcpyDeviceToHost));
+    if (ovl) HIPCHK(c, copy_sync(c, row.data() + (be[1] - be[0]), g.pool + ovs, (size_t)ovl * 4, hipMemcpyDeviceToHost));
     int64_t k = 0;
     for (int32_t y : row) {
         if (y < 0) continue;

@@ -17,6 +17,16 @@ struct GraphState {
     int64_t walk_task_cap;
 };
 
+// Frees device or pinned host memory.  hipFree and hipHostFree wait for
+// every kernel on the device to end, on any stream, the long-running serving
+// kernel included; while a serving kernel may run, frees are therefore
+// deferred until it has ended (the memory is only released later, never
+// used again).
+void mf_free_device(void* p);
+void mf_free_host(void* p);
+void mf_serving_begin();  // before a serving kernel's launch
+void mf_serving_end();    // after its end was seen: deferred frees run once none is left
+
 // Split tasks of the edge walk (rows cut into pieces of 4,096 entries) a base
 // of n vertices and m entries may need.
 inline int64_t mf_walk_task_need(int64_t n, int64_t m) { return m / 4096 + n / 64 + 1024; }

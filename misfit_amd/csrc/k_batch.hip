@@ -1313,7 +1313,7 @@ __device__ void ph_commit_l(const BP& P, const GC& c, int par) {
 }
 
 // ---------------------------------------------------------------- completion
-__device__ void ph_comp_collect(const BP& P, const GC& c) {
+__device__ __forceinline__ void ph_comp_collect(const BP& P, const GC& c) {
     const DevGraph& g = P.g;
     int64_t np = (int64_t)P.ctr[MF_CS * (C_PU)];
     if (np > P.pu_cap) np = P.pu_cap;
@@ -1513,7 +1513,7 @@ __device__ __forceinline__ void phase_barrier() {
 // the host reads the batch's results without a copy on the stream.  At the
 // batch's end (`last`) each counter is reset for the next batch right after
 // it is copied; a halted tail keeps them for the rounds queued after it.
-__device__ void publish(const BP& P, const GC& c, bool last) {
+__device__ __forceinline__ void publish(const BP& P, const GC& c, bool last) {
     if (c.tid == 0) P.ctr[MF_CS * (C_T1)] = wall_clock64();
     phase_barrier();
     for (int64_t i = c.tid; i < C_NCOUNTERS; i += c.nthreads) {
@@ -1527,7 +1527,7 @@ __device__ void publish(const BP& P, const GC& c, bool last) {
 // Remaining speculative rounds in one workgroup; returns false when a loop
 // bound was hit.  `ran`: some round ran (nothing before the first needs a
 // barrier: the kernel's start sees what earlier kernels wrote).
-__device__ bool finish_rounds(const BP& P, const GC& c, int par, bool& ran) {
+__device__ __forceinline__ bool finish_rounds(const BP& P, const GC& c, int par, bool& ran) {
     uint32_t guard = 0;
     ran = false;
     while (true) {
@@ -1554,7 +1554,7 @@ __device__ bool finish_rounds(const BP& P, const GC& c, int par, bool& ran) {
 }
 
 // Completion iterations over the collected candidates in one workgroup.
-__device__ void finish_completion(const BP& P, const GC& c) {
+__device__ __forceinline__ void finish_completion(const BP& P, const GC& c) {
     if (!P.completion) return;
     for (uint32_t k = 0;; k++) {
         phase_barrier();
@@ -1607,6 +1607,118 @@ __global__ __launch_bounds__(1024) void k_finish(BP P, int par) {
 __global__ __launch_bounds__(256) void k_publish(BP P) {
     GC c = block_coords();
     publish(P, c, true);
+}
+
+// ---------------------------------------------------------------- serving kernel
+// A batch in one workgroup: (whole) the prologue's phases, then the
+// speculative rounds (window P.fin_window), the rule counters, the
+// completion and the publication, separated by workgroup barriers instead
+// of kernel boundaries.  Every phase is the grid-stride body the grid-wide
+// kernels run, here over the workgroup's threads.  Without `whole` the
+// prologue ran as grid-wide kernels before the item was published.
+__device__ __forceinline__ void batch_in_workgroup(const BP& P, const GC& c, bool whole) {
+    if (!whole) {
+        if (c.tid == 0) P.ctr[MF_CS * (C_TP)] = wall_clock64();
+        bool ran = false;
+        if (finish_rounds(P, c, 0, ran)) {
+            if (ran) phase_barrier();
+            ph_stats(P, c);
+            if (P.completion) {
+                ph_comp_collect(P, c);
+                finish_completion(P, c);
+            }
+        }
+        publish(P, c, true);
+        return;
+    }
+    if (P.hu) ph_init(P, c);
+    else if (c.tid == 0) P.ctr[MF_CS * (C_T0)] = wall_clock64();
+    phase_barrier();
+    ph_claim(P, c);
+    phase_barrier();
+    ph_resolve(P, c);
+    phase_barrier();
+    ph_append(P, c);
+    phase_barrier();
+    ph_partner(P, c);
+    phase_barrier();
+    ph_kcount(P, c);
+    phase_barrier();
+    ph_pu_scan(P, c);
+    phase_barrier();
+    ph_fa_count(P, c);
+    phase_barrier();
+    ph_fa_alloc(P, c);
+    phase_barrier();
+    ph_fa_fill(P, c);
+    phase_barrier();
+    if (c.tid == 0) P.ctr[MF_CS * (C_TP)] = wall_clock64();
+    bool ran = false;
+    if (finish_rounds(P, c, 0, ran)) {
+        if (ran) phase_barrier();
+        ph_stats(P, c);
+        if (P.completion) {
+            ph_comp_collect(P, c);
+            finish_completion(P, c);
+        }
+    }
+    publish(P, c, true);
+}
+
+__device__ __forceinline__ u64 host_word(const u64* p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// The claim of the next item by the workgroup's first thread: the item's
+// index, or -1 when the host asked the kernel to stop or no item came for
+// idle_ticks of the device clock.  Waits sleep about 3 us between looks.
+__device__ int64_t serve_claim(ServeQueue* q, u64* head, u64 idle_ticks) {
+    const u64 t0 = wall_clock64();
+    while (true) {
+        if (host_word(&q->stop)) return -1;
+        const u64 tail = host_word(&q->tail);
+        const u64 h = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (h < tail) {
+            if (atomicCAS(head, h, h + 1) == h) return (int64_t)h;
+            continue;  // another workgroup took it
+        }
+        if (wall_clock64() - t0 > idle_ticks) return -1;
+        __builtin_amdgcn_s_sleep(127);
+    }
+}
+
+// Workgroups take whole batches from the queue, any graph's, one at a time.
+// A batch may run on a different die than its graph's last one, or after
+// kernels of its graph's own stream: each item starts with an agent-scope
+// acquire (this die's L2 drops lines others may have written) and its
+// publication ends with a system-scope release (the batch's writes leave
+// the L2), so every batch sees its graph's state as the last one left it.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_serve(ServeQueue* q, u64* head, u64 idle_ticks) {
+    __shared__ BP sP;
+    __shared__ int64_t s_item;
+    __shared__ int s_whole;
+    GC c = block_coords();
+    while (true) {
+        if (c.tid == 0) s_item = serve_claim(q, head, idle_ticks);
+        __syncthreads();
+        const int64_t item = s_item;
+        if (item < 0) return;  // every thread read the same value
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(&q->items[item % MF_SERVE_CAP].P);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(&sP);
+        for (int i = (int)c.tid; i < (int)(sizeof(BP) / 4); i += (int)c.nthreads)
+            dst[i] = __hip_atomic_load(src + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        if (c.tid == 0)
+            s_whole = __hip_atomic_load(&q->items[item % MF_SERVE_CAP].whole, __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_SYSTEM) != 0;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        __syncthreads();
+        batch_in_workgroup(sP, c, s_whole != 0);
+        __syncthreads();
+        if (c.tid == 0)
+            __hip_atomic_store(&sP.h_out[MF_DONE_WORD], (u64)item + 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        __syncthreads();  // sP is rewritten by the next item only after this
+    }
 }
 
 // ---------------------------------------------------------------- launchers
@@ -1711,6 +1823,12 @@ void mf_launch_stats(const BP& P, const LaunchCfg& L) { LAUNCH(k_stats, P); }
 
 void mf_launch_publish(const BP& P, const LaunchCfg& L) {
     hipLaunchKernelGGL(k_publish, dim3(1), dim3(256), 0, L.stream, P);
+}
+
+void mf_launch_serve(ServeQueue* q, u64* head, int grid, int threads, u64 idle_ticks, hipStream_t st) {
+    if (threads >= 1024) hipLaunchKernelGGL(k_serve<1024>, dim3(grid), dim3(1024), 0, st, q, head, idle_ticks);
+    else if (threads >= 512) hipLaunchKernelGGL(k_serve<512>, dim3(grid), dim3(512), 0, st, q, head, idle_ticks);
+    else hipLaunchKernelGGL(k_serve<256>, dim3(grid), dim3(256), 0, st, q, head, idle_ticks);
 }
 
 void mf_launch_completion_round(const BP& P, const LaunchCfg& L, int par) {

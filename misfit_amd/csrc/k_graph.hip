@@ -849,7 +849,8 @@ hipError_t mf_build_row_index(GraphState& S, int32_t* sval, int32_t* spos, int* 
     std::vector<int32_t> cuts{0};
     if (g.m > chunk) {  // row ranges of at most 2^30 entries
         ni.resize((size_t)g.n + 1);
-        hipError_t e = hipMemcpy(ni.data(), g.nindex, ni.size() * 8, hipMemcpyDeviceToHost);
+        hipError_t e = hipMemcpyAsync(ni.data(), g.nindex, ni.size() * 8, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) return e;
         int64_t start = 0;
         for (int32_t r = 0; r < g.n; r++) {
@@ -891,7 +892,7 @@ hipError_t mf_build_row_index(GraphState& S, int32_t* sval, int32_t* spos, int* 
                                                     r1 - r0, beg, end, st);
         if (e != hipSuccess) break;
         if (need > tmp_bytes) {
-            if (tmp) hipFree(tmp);
+            if (tmp) mf_free_device(tmp);
             tmp = nullptr;
             tmp_bytes = 0;
             e = hipMalloc(&tmp, need);
@@ -902,8 +903,8 @@ hipError_t mf_build_row_index(GraphState& S, int32_t* sval, int32_t* spos, int* 
                                                     r1 - r0, beg, end, st);
     }
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (tmp) hipFree(tmp);
-    hipFree(pos_in);
+    if (tmp) mf_free_device(tmp);
+    mf_free_device(pos_in);
     if (e != hipSuccess) return e;
     k_check_row_index<<<S.blocks, 256, 0, st>>>(g.nindex, g.nlist, sval, spos, g.n, d_flags);
     return hipGetLastError();
@@ -944,7 +945,7 @@ hipError_t sort_lists(GraphState& S, const int64_t* off, const int32_t* rows, co
                                                          beg, end, st);
     else e = hipcub::DeviceSegmentedSort::SortKeys(nullptr, need, kin, kout, (int)total, (int)nbig, beg, end, st);
     if (e == hipSuccess && need > *L.tmp_bytes) {
-        if (*L.tmp) hipFree(*L.tmp);
+        if (*L.tmp) mf_free_device(*L.tmp);
         *L.tmp = nullptr;
         *L.tmp_bytes = 0;
         e = hipMalloc(L.tmp, need);

@@ -42,10 +42,17 @@
 #include <vector>
 
 #include "../../include/misfit.h"
+#include "graph.h"
 
 // capi.hip: 1 while a batch of the context may still need rounds queued
 // from the host (queued without a reliable round estimate).
 int mf_ctx_settling(const misfit_ctx* c);
+// capi.hip: checks and prepares a batch for the serving kernel: with
+// `whole` nothing is launched, else its prologue is queued on the graph's
+// stream and mf_ctx_external_ready reports when it has ended.
+int mf_ctx_prepare_external(misfit_ctx* c, const misfit_update* ups, int64_t count, bool whole, uint64_t* ticket,
+                            BP* P);
+int mf_ctx_external_ready(misfit_ctx* c, uint64_t ticket);
 // capi.hip: every batch of the context may halt (one batch per graph).
 void mf_ctx_set_halting(misfit_ctx* c, bool on);
 
@@ -64,6 +71,9 @@ struct Request {
     std::vector<misfit_update> ups;
     clk::time_point t_submit, t_enqueue;
     uint64_t ticket = 0;
+    bool external = false;  // runs in the serving kernel, not on the graph's stream
+    bool published = false; // ... and its item is in the kernel's queue (after its prologue)
+    BP P{};                 // ... its parameters, until published
     ReqState state = PENDING;
     int rc = MISFIT_OK;
     std::string err;
@@ -84,6 +94,131 @@ struct Graph {
     int64_t compact_at = 0;         // overlay size that triggers the next compaction (0: by the fraction)
     int compact_rc = MISFIT_OK;     // a failed compaction's error, for the next request to start
     std::string compact_err;
+    double last_epr = 0.0;          // edges the graph's last batch committed per round (0: no batch yet)
+    int64_t last_window = 0;        // ... the edges it speculated per round
+    bool last_kernel = false;       // ... and whether it ran in the serving kernel
+    int external = 0;               // its batches running in the serving kernel
+};
+
+// The serving kernel's host side: the pinned queue (coherent, so the
+// kernel sees each item and the raised tail without a copy), the device
+// head counter the workgroups claim items with, the kernel's stream and
+// the end event of its latest launch.  A launch ends on its own once no
+// item comes for the idle time; a caller whose item has not finished calls
+// keep_running, which launches again when the last launch has ended (a
+// claimed item always finishes before its workgroup leaves, so an
+// unfinished item after the end was never claimed).
+struct Runner {
+    ServeQueue* q = nullptr;
+    ServeQueue* q_dev = nullptr;
+    u64* d_head = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ended = nullptr;
+    bool launched = false;
+    u64 pushed = 0, finished = 0;
+    int cus = 0;
+    int threads = 1024;  // per workgroup (MISFIT_SERVE_THREADS: 256, 512 or 1024)
+    u64 idle_ticks = 0;
+    int err = MISFIT_OK;
+    std::string err_msg;
+    std::mutex mu;
+
+    int init(int device, double idle_ms) {
+        hipSetDevice(device);
+        int khz = 0;
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) != hipSuccess || khz <= 0)
+            khz = 100000;
+        idle_ticks = (u64)(idle_ms * (double)khz);
+        const char* th = getenv("MISFIT_SERVE_THREADS");
+        if (th) threads = atoi(th) >= 1024 ? 1024 : atoi(th) >= 512 ? 512 : 256;
+        void* dp = nullptr;
+        if (hipHostMalloc((void**)&q, sizeof(ServeQueue), hipHostMallocCoherent) != hipSuccess) return MISFIT_E_NOMEM;
+        memset((void*)q, 0, sizeof(ServeQueue));
+        if (hipHostGetDevicePointer(&dp, q, 0) != hipSuccess) return MISFIT_E_HIP;
+        q_dev = (ServeQueue*)dp;
+        if (hipMalloc((void**)&d_head, sizeof(u64)) != hipSuccess) return MISFIT_E_NOMEM;
+        // A stream with a CU mask gets a hardware queue of its own: the
+        // long-running kernel must not hold up the graphs' streams, which
+        // share the other hardware queues (a queue runs its kernels in
+        // order).  The mask covers every CU.
+        std::vector<uint32_t> mask((size_t)(cus + 31) / 32, 0u);
+        for (int cu = 0; cu < cus; cu++) mask[(size_t)cu / 32] |= 1u << (cu % 32);
+        if (hipExtStreamCreateWithCUMask(&stream, (uint32_t)mask.size(), mask.data()) != hipSuccess)
+            return MISFIT_E_HIP;
+        // on the kernel's own stream: ordered before its first launch
+        if (hipMemsetAsync(d_head, 0, sizeof(u64), stream) != hipSuccess) return MISFIT_E_HIP;
+        if (hipStreamSynchronize(stream) != hipSuccess) return MISFIT_E_HIP;
+        if (hipEventCreateWithFlags(&ended, hipEventDisableTiming) != hipSuccess) return MISFIT_E_HIP;
+        return MISFIT_OK;
+    }
+    // Relaunches the kernel (grid: one workgroup per graph that may have a
+    // batch in it, at most one per CU) when its last launch has ended.
+    int keep_running(int device, int graphs) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (err) return err;
+        if (launched) {
+            const hipError_t e = hipEventQuery(ended);
+            if (e == hipErrorNotReady) return MISFIT_OK;
+            if (e != hipSuccess) return fail_locked(MISFIT_E_HIP, std::string("serving kernel: ") + hipGetErrorString(e));
+            launched = false;
+            mf_serving_end();
+        }
+        if (finished >= pushed) return MISFIT_OK;  // nothing left for a kernel to do
+        hipSetDevice(device);
+        // workgroups of 512 or 1024 threads take a CU each, of 256 half of one
+        const int grid = std::max(1, std::min(threads <= 256 ? 2 * cus : cus, graphs));
+        mf_serving_begin();  // frees wait from now on until the launch has ended
+        mf_launch_serve(q_dev, d_head, grid, threads, idle_ticks, stream);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipEventRecord(ended, stream);
+        if (e != hipSuccess) {
+            mf_serving_end();
+            return fail_locked(MISFIT_E_HIP, std::string("serving kernel launch: ") + hipGetErrorString(e));
+        }
+        launched = true;
+        return MISFIT_OK;
+    }
+    // Publishes one batch; the queue holds at most MF_SERVE_CAP unfinished.
+    int push(const BP& P, bool whole, int device, int graphs) {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            if (err) return err;
+            if (pushed - finished >= (u64)MF_SERVE_CAP) return MISFIT_E_STATE;
+            memcpy((void*)&q->items[pushed % MF_SERVE_CAP].P, &P, sizeof(BP));
+            q->items[pushed % MF_SERVE_CAP].whole = whole ? 1 : 0;
+            pushed++;
+            __atomic_store_n(&q->tail, pushed, __ATOMIC_RELEASE);
+        }
+        return keep_running(device, graphs);
+    }
+    std::string error() {
+        std::lock_guard<std::mutex> lk(mu);
+        return err_msg;
+    }
+    void done_one() {
+        std::lock_guard<std::mutex> lk(mu);
+        finished++;
+    }
+    int fail_locked(int rc, const std::string& m) {
+        err = rc;
+        err_msg = m;
+        return rc;
+    }
+    void shutdown() {
+        if (!q) return;
+        __atomic_store_n(&q->stop, 1ull, __ATOMIC_RELEASE);
+        if (launched) {
+            hipEventSynchronize(ended);
+            mf_serving_end();
+        }
+        launched = false;
+        if (ended) hipEventDestroy(ended);
+        if (stream) hipStreamDestroy(stream);
+        if (d_head) hipFree(d_head);
+        hipHostFree(q);
+        q = nullptr;
+    }
 };
 
 // Device bytes a resident graph may hold: the base (int64 offsets, int32
@@ -131,6 +266,26 @@ struct misfit_server {
     std::string err;
     bool compacting = false;    // one compaction at a time: each needs its headroom only while it runs
     int64_t headroom_max = 0;   // the largest compaction headroom among the resident graphs
+    Runner runner;              // the serving kernel (opts.persistent)
+    bool check_each = false;    // MISFIT_SERVE_CHECK=1: the full check after every batch (a debugging aid)
+
+    // Whether request r of graph g runs its rounds in the serving kernel:
+    // every batch (persistent 2), or (persistent 1) a small one, or one of a
+    // contended graph, whose last batch committed few edges per round: fewer
+    // than 256 on the graph's stream (its grid-wide rounds leave most of the
+    // GPU idle and cost three launches each), or under three quarters of
+    // the window in the kernel (a wider grid-wide window would not commit
+    // more).  The rest, and a graph's first large batch, run on its stream.
+    bool use_kernel(const Graph* g, const Request* r) const {
+        if (opts.persistent <= 0 || opts.max_inflight != 1) return false;
+        if (opts.persistent >= 2 || r->ups.size() <= 4096) return true;
+        if (g->last_epr <= 0.0) return false;
+        if (g->last_kernel) return g->last_epr < 0.75 * (double)g->last_window;
+        return g->last_epr < 256.0;
+    }
+    // ... and whether the kernel runs its prologue too: tiny batches, whose
+    // prologue kernels would each cost more to launch than to run.
+    static bool whole_in_kernel(const Request* r) { return r->ups.size() <= 1024; }
 
     int fail(int rc, const std::string& m) {
         err = m;
@@ -169,20 +324,63 @@ bool misfit_server::service(Graph* g, std::unique_lock<std::mutex>& lk) {
     g->in_call = true;
     while (!g->inflight.empty()) {
         Request* r = g->inflight.front();
+        const int ngraphs = (int)graphs.size();
         lk.unlock();
-        int p = misfit_poll(g->ctx, r->ticket);
+        int p = 0;
+        std::string em;
+        if (r->external && !r->published) {  // its prologue on the stream first, then the kernel
+            p = mf_ctx_external_ready(g->ctx, r->ticket);
+            if (p < 0) {
+                em = misfit_last_error(g->ctx);
+            } else if (p == 1) {
+                // a batch whose prologue ran but whose rounds cannot (the
+                // kernel failed to launch) leaves its graph unrepaired
+                p = runner.push(r->P, false, device, ngraphs);
+                if (p == MISFIT_OK) r->published = true;
+                else em = "serving kernel unavailable: " + runner.error();
+            }
+        } else {
+            p = misfit_poll(g->ctx, r->ticket);
+            if (p < 0) em = misfit_last_error(g->ctx);
+        }
         misfit_batch_stats bs{};
         int rc = p < 0 ? p : MISFIT_OK;
-        std::string em;
-        if (p == 1) rc = misfit_wait(g->ctx, r->ticket, &bs);
-        if (rc != MISFIT_OK) em = misfit_last_error(g->ctx);
+        if (p == 0 && r->external && r->published) {  // not finished: the kernel must still be running
+            rc = runner.keep_running(device, ngraphs);
+            if (rc != MISFIT_OK) {
+                p = rc;
+                em = "serving kernel unavailable: " + runner.error();
+            }
+        }
+        if (p == 1) {
+            rc = misfit_wait(g->ctx, r->ticket, &bs);
+            if (rc != MISFIT_OK) em = misfit_last_error(g->ctx);
+            if (r->external) runner.done_one();
+            if (rc == MISFIT_OK && check_each) {  // MISFIT_SERVE_CHECK=1: the full check after every batch
+                int64_t cf = 0, un = 0, ce = 0;
+                rc = misfit_check(g->ctx, &cf, &un, &ce);
+                if (rc == MISFIT_OK && (cf || un || ce)) {
+                    rc = MISFIT_E_CHECK;
+                    em = "check after the batch failed (" + std::string(r->external ? "serving kernel" : "stream") +
+                         ", " + std::to_string(r->ups.size()) + " updates, " + std::to_string(g->compactions) +
+                         " compactions so far): conflicts " + std::to_string(cf) + ", uncovered " +
+                         std::to_string(un) + ", count errors " + std::to_string(ce);
+                }
+            }
+        }
         lk.lock();
         if (p == 0) break;
         g->inflight.pop_front();
+        if (r->external) g->external--;
         progress = true;
         if (rc != MISFIT_OK) {
             finish(r, rc, em);
             continue;
+        }
+        if (bs.rounds > 0) {
+            g->last_epr = (double)bs.updates / (double)bs.rounds;
+            g->last_window = bs.window;
+            g->last_kernel = r->external;
         }
         r->st.batch = bs;
         r->st.device_ms = bs.device_ms;
@@ -241,12 +439,24 @@ bool misfit_server::service(Graph* g, std::unique_lock<std::mutex>& lk) {
             finish(r, rc, g->compact_err);
             continue;
         }
+        const bool ext = use_kernel(g, r);
+        const int ngraphs = (int)graphs.size();
         lk.unlock();
         int rc = MISFIT_OK;
         std::string em;
         if (device_free_bytes(device) - batch_footprint((int64_t)r->ups.size()) < opts.hbm_reserve) {
             rc = MISFIT_E_NOMEM;
             em = "batch not admitted: not enough free device memory";
+        } else if (ext) {
+            const bool whole = whole_in_kernel(r);
+            rc = mf_ctx_prepare_external(g->ctx, r->ups.data(), (int64_t)r->ups.size(), whole, &r->ticket, &r->P);
+            if (rc != MISFIT_OK) {
+                em = misfit_last_error(g->ctx);
+            } else if (whole) {
+                rc = runner.push(r->P, true, device, ngraphs);
+                if (rc == MISFIT_OK) r->published = true;
+                else em = "serving kernel unavailable: " + runner.error();
+            }
         } else {
             rc = misfit_submit_batch(g->ctx, r->ups.data(), (int64_t)r->ups.size(), &r->ticket);
             if (rc != MISFIT_OK) em = misfit_last_error(g->ctx);
@@ -256,6 +466,8 @@ bool misfit_server::service(Graph* g, std::unique_lock<std::mutex>& lk) {
             finish(r, rc, em);
             continue;
         }
+        r->external = ext;
+        if (ext) g->external++;
         r->t_enqueue = clk::now();
         r->st.compactions_before = g->compactions;
         r->st.queue_ms = ms_between(r->t_submit, r->t_enqueue);
@@ -304,6 +516,18 @@ extern "C" int misfit_server_create(int device, const misfit_server_opts* opts, 
     int nw = opts ? opts->workers : 0;
     if (nw <= 0) nw = 8;
     s->opts.workers = nw > 64 ? 64 : nw;
+    s->opts.persistent = opts ? opts->persistent : 0;
+    const char* ck = getenv("MISFIT_SERVE_CHECK");
+    s->check_each = ck && ck[0] == '1';
+    if (s->opts.persistent > 0) {
+        const char* idle = getenv("MISFIT_SERVE_IDLE_MS");
+        const int rc = s->runner.init(device, idle && atof(idle) > 0 ? atof(idle) : 20.0);
+        if (rc != MISFIT_OK) {
+            s->runner.shutdown();
+            delete s;
+            return rc;
+        }
+    }
     for (int w = 0; w < s->opts.workers; w++) s->workers.emplace_back([s, w] { s->run(w); });
     *out = s;
     return MISFIT_OK;
@@ -317,6 +541,7 @@ extern "C" int misfit_server_destroy(misfit_server* s) {
     }
     s->cv_work.notify_all();
     for (auto& t : s->workers) t.join();
+    s->runner.shutdown();  // every request has finished: the kernel leaves at its next claim
     for (auto& kv : s->graphs) misfit_ctx_destroy(kv.second->ctx);
     delete s;
     return MISFIT_OK;
@@ -371,11 +596,12 @@ extern "C" int misfit_server_add_graph(misfit_server* s, int32_t n, int64_t m, c
 }
 
 // Waits (holding the lock on return) until no worker is inside graph id's
-// context; null when the graph is unknown.
+// context and none of its batches is in the serving kernel (whose writes
+// the graph's own stream does not order); null when the graph is unknown.
 static Graph* graph_idle(misfit_server* s, std::unique_lock<std::mutex>& lk, int32_t id) {
     while (true) {
         Graph* g = s->graph(id);
-        if (!g || !g->in_call) return g;
+        if (!g || (!g->in_call && g->external == 0)) return g;
         s->cv_done.wait(lk);
     }
 }

@@ -234,3 +234,86 @@ def test_tuner_on_the_device(engine_mod, tmp_path, monkeypatch):
     p = str(tmp_path / "tune.json")
     T.save(res, p)
     assert T.apply(p) == res["env"]
+
+
+def _mixed_graphs(engine_mod, oracle, count, nb, seed0):
+    """`count` small graphs of the three types, with low-degree update
+    batches of uneven sizes (dense contention on the smallest graphs)."""
+    rng = np.random.default_rng(seed0)
+    out = []
+    for i in range(count):
+        kind = ("er", "rmat", "rgg")[i % 3]
+        size = int(rng.integers(12, 15))
+        seed = seed0 * 100 + i
+        if kind == "er":
+            ni, nl = engine_mod.gen_er(1 << size, 8 << size, seed)
+        elif kind == "rmat":
+            ni, nl = engine_mod.gen_rmat(size, 16, seed)
+        else:
+            ni, nl = engine_mod.gen_rgg(1 << size, 32.0, seed)
+        mis0 = engine_mod.greedy_mis(ni, nl)
+        sizes = [int(x) for x in rng.integers(200, 4000, nb)]
+        u, v = engine_mod.gen_updates(engine_mod.LOW_DEGREE, ni, nl, sum(sizes), seed)
+        ins = oracle.Graph.from_csr(ni, nl).flags(u, v)
+        offs = np.concatenate([[0], np.cumsum(sizes)])
+        batches = [(u[a:b], v[a:b], ins[a:b]) for a, b in zip(offs[:-1], offs[1:])]
+        out.append((ni, nl, mis0, batches))
+    return out
+
+
+def _serve_all(engine_mod, graphs, **opts):
+    """Every request submitted at once, interleaved across graphs; per graph
+    the MIS after all its batches, the request stats and the final check."""
+    with engine_mod.Server(0, **opts) as srv:
+        gids = [srv.add_graph(ni, nl, mis0) for (ni, nl, mis0, _) in graphs]
+        nb = max(len(g[3]) for g in graphs)
+        reqs = [(gi, srv.submit(gids[gi], engine_mod.make_updates(*graphs[gi][3][b])))
+                for b in range(nb) for gi in range(len(graphs)) if b < len(graphs[gi][3])]
+        stats = {gi: [] for gi in range(len(graphs))}
+        for gi, r in reqs:
+            stats[gi].append(srv.wait(r))
+        mis = [srv.get_mis(gids[gi], len(g[0]) - 1) for gi, g in enumerate(graphs)]
+        checks = [srv.check(g) for g in gids]
+    return mis, stats, checks
+
+
+def test_serving_kernel_graphs_alone_and_in_the_mix(oracle, engine_mod):
+    """Sixteen graphs whose batches all run in the serving kernel (each
+    batch whole in one workgroup, workgroups taking batches of any graph
+    from one queue): each graph's MIS is the same served alone and in the
+    mix of all sixteen, and the oracle's."""
+    graphs = _mixed_graphs(engine_mod, oracle, 16, 3, 7)
+    want = [H.oracle_replay(oracle, ni, nl, mis0, batches, completion=True, literal=False)[0][-1]
+            for (ni, nl, mis0, batches) in graphs]
+    mix, stats, checks = _serve_all(engine_mod, graphs, compact_fraction=0.0, max_inflight=1, workers=4,
+                                    persistent=2)
+    for gi in range(len(graphs)):
+        assert np.array_equal(mix[gi], want[gi]), "graph %d in the mix" % gi
+        assert checks[gi] == (0, 0, 0)
+        assert all(s["device_ms"] > 0 and s["batch"]["rounds"] >= 0 for s in stats[gi])
+    for gi in (0, 5, 10, 15):  # alone: a server holding only this graph
+        alone, _, checks1 = _serve_all(engine_mod, graphs[gi:gi + 1], compact_fraction=0.0, max_inflight=1,
+                                       persistent=2)
+        assert np.array_equal(alone[0], mix[gi]), "graph %d alone" % gi
+        assert checks1[0] == (0, 0, 0)
+
+
+def test_serving_kernel_beside_streams_with_compactions(oracle, engine_mod):
+    """Batches split between the serving kernel (small or contended ones)
+    and the graphs' own streams (the rest), with compactions between them:
+    every graph ends with the MIS of the oracle compacting at the reported
+    points."""
+    graphs = _mixed_graphs(engine_mod, oracle, 9, 4, 11)
+    # two larger, uncontended graphs whose big batches take the streams
+    for kind, size, seed in (("er", 1 << 15, 91), ("rmat", 15, 92)):
+        graphs.append(_workload(engine_mod, oracle, kind, size, seed, nb=4, bs=6000))
+    mis, stats, checks = _serve_all(engine_mod, graphs, compact_fraction=0.01, max_inflight=1, workers=4,
+                                    persistent=1)
+    compacted = 0
+    for gi, (ni, nl, mis0, batches) in enumerate(graphs):
+        before = [s["compactions_before"] for s in stats[gi]]
+        compacted += before[-1]
+        want = _oracle_with_compactions(oracle, ni, nl, mis0, batches, before)
+        assert np.array_equal(mis[gi], want), "graph %d" % gi
+        assert checks[gi] == (0, 0, 0)
+    assert compacted > 0
