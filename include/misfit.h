@@ -65,6 +65,8 @@ typedef struct {
     double device_ms;         /* batch time on the device (device wall clock, first kernel to end of tail) */
     double prologue_ms;       /* ... of which the overlay update, may-uncover set and admissible index */
     int64_t window;           /* edges speculated per round (the rounds' window) */
+    double device_start_ms;   /* the device wall clock at the batch's start and end, in ms (comparable */
+    double device_end_ms;     /* across the batches of every context on the device) */
 } misfit_batch_stats;
 
 /* Execution modes of the repair (results are identical). */

@@ -50,7 +50,8 @@ class BatchStats(ctypes.Structure):
         "updates", "seeds", "active_ins", "evictions", "admit_ins", "active_del", "admit_del",
         "completion_adds", "rounds", "evaluations", "maybe_uncovered", "admissible", "delta_ops",
         "overlay_entries")] + [("device_ms", ctypes.c_double), ("prologue_ms", ctypes.c_double),
-                               ("window", ctypes.c_int64)]
+                               ("window", ctypes.c_int64), ("device_start_ms", ctypes.c_double),
+                               ("device_end_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

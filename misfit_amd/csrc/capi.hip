@@ -73,6 +73,7 @@ struct misfit_ctx {
     int round_margin = -1;        // MISFIT_ROUND_MARGIN: rounds queued beyond the planned rate (-1: by the plan)
     int fin_window = 0;           // MISFIT_TAIL_WINDOW: edges per round of the one-workgroup tail (0: 1024)
     int serve_window = 0;         // MISFIT_SERVE_WINDOW: edges per round of batches in the serving kernel (0: adaptive)
+    double serve_window_factor = 2.0;  // MISFIT_SERVE_WINDOW_FACTOR: ... adaptive: this x the last edges per round
     double last_epr = 0.0;        // edges committed per round by the last batch (0: none yet)
     double clock_khz = 0.0;       // device wall clock, for per-batch device times
     hipEvent_t timer0 = nullptr, timer1 = nullptr;  // misfit_timer_begin/end
@@ -255,6 +256,8 @@ extern "C" int misfit_ctx_create(int device, misfit_ctx** out) {
     if (rm) c->round_margin = std::max(0, std::min(16, atoi(rm)));
     const char* ps = getenv("MISFIT_PU_SCAN");
     c->force_pu_scan = ps && strcmp(ps, "full") == 0;
+    const char* swf = getenv("MISFIT_SERVE_WINDOW_FACTOR");
+    if (swf && atof(swf) > 0.0) c->serve_window_factor = atof(swf);
     const char* sw = getenv("MISFIT_SERVE_WINDOW");
     if (sw) c->serve_window = std::max(0, std::min(1 << 20, atoi(sw)));
     const char* tw = getenv("MISFIT_TAIL_WINDOW");
@@ -788,6 +791,8 @@ static void fill_stats(const misfit_ctx* c, const Slot& s, misfit_batch_stats* s
     // device wall clock (kHz ticks per ms): first kernel start to the end of
     // the tail, and to the start of the first round
     if (c->clock_khz > 0 && k[C_T1] >= k[C_T0]) {
+        st->device_start_ms = (double)k[C_T0] / c->clock_khz;
+        st->device_end_ms = (double)k[C_T1] / c->clock_khz;
         st->device_ms = (double)(k[C_T1] - k[C_T0]) / c->clock_khz;
         st->prologue_ms = k[C_TP] >= k[C_T0] ? (double)(k[C_TP] - k[C_T0]) / c->clock_khz : st->device_ms;
     }
@@ -1058,13 +1063,14 @@ static int check_updates(misfit_ctx* c, const misfit_update* ups, int64_t count,
 
 // Queues one batch: from the caller's host array (ups, copied into the
 // slot's pinned staging) or from a batch already in device memory (st).
-// The window (edges per round) of a batch run whole by one workgroup of the
-// serving kernel: four times the edges the graph's last batch committed per
-// round, rounded up to 32, within [64, 1024]; 128 without history.
+// The window (edges per round) of a batch whose rounds run in one workgroup
+// of the serving kernel: MISFIT_SERVE_WINDOW_FACTOR (default 2) times the
+// edges the graph's last batch committed per round, rounded up to 32,
+// within [64, 1024]; 128 without history.
 static int64_t serve_window(const misfit_ctx* c) {
     if (c->serve_window > 0) return c->serve_window;
     if (c->last_epr <= 0.0) return 128;
-    const int64_t w = ((int64_t)std::ceil(4.0 * c->last_epr) + 31) / 32 * 32;
+    const int64_t w = ((int64_t)std::ceil(c->serve_window_factor * c->last_epr) + 31) / 32 * 32;
     return std::min<int64_t>(1024, std::max<int64_t>(64, w));
 }
 

