@@ -103,8 +103,40 @@ KERNEL_NOTE = ("queued batch (prologue, grid rounds, finisher): device time of t
                "events on the engine's stream, divided by the batches in it")
 
 
+def physical_cores():
+    """Physical cores among the CPUs this process may run on: distinct
+    (package, core) pairs in /proc/cpuinfo (the logical CPU count where that
+    file does not say)."""
+    allowed = os.sched_getaffinity(0) if hasattr(os, "sched_getaffinity") else None
+    cores, proc, phys = set(), None, None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                key, _, val = line.partition(":")
+                key, val = key.strip(), val.strip()
+                if key == "processor":
+                    proc, phys = int(val), None
+                elif key == "physical id":
+                    phys = val
+                elif key == "core id" and proc is not None and (allowed is None or proc in allowed):
+                    cores.add((phys, val))
+    except (OSError, ValueError):
+        pass
+    return len(cores) or (len(allowed) if allowed else os.cpu_count() or 1)
+
+
+def cpu_threads():
+    """Threads of the CPU baseline: one per physical core (at most
+    OMP_NUM_THREADS when that is set)."""
+    n = physical_cores()
+    try:
+        return max(1, min(n, int(os.environ["OMP_NUM_THREADS"])))
+    except (KeyError, ValueError):
+        return n
+
+
 def host_info():
-    """The host the CPU baseline ran on: CPU model and logical CPUs."""
+    """The host the CPU baseline ran on: CPU model, logical CPUs, physical cores."""
     model = None
     try:
         with open("/proc/cpuinfo") as f:
@@ -114,7 +146,7 @@ def host_info():
                     break
     except OSError:
         pass
-    return {"cpu_model": model, "nproc": os.cpu_count()}
+    return {"cpu_model": model, "nproc": os.cpu_count(), "physical_cores": physical_cores()}
 
 
 def pct(v, q):
@@ -144,7 +176,7 @@ def run_plain(M, eng, ni, nl, mis0, ups, W, K, batch, measure_bytes=True, index_
     eng.load_csr(ni, nl)
     eng.set_mis(mis0)
     eng.set_mode(M.MODE_PLAIN)
-    lat, dev, nbytes, stats = [], [], [], []
+    lat, dev, nbytes, stats, all_bytes = [], [], [], [], []
     hs = stage_all(eng, ups, W + K, batch)
     eng.sync()
     t_all = 0.0
@@ -158,16 +190,18 @@ def run_plain(M, eng, ni, nl, mis0, ups, W, K, batch, measure_bytes=True, index_
         print("plain batch %d: %.2f ms, rounds %d, evictions %d, PU %d, ADM %d" % (
             b, dt * 1e3, st["rounds"], st["evictions"], st["maybe_uncovered"], st["admissible"]),
             file=sys.stderr, flush=True)
+        nb = eng.batch_bytes(index_bytes)[0] if measure_bytes else None
+        all_bytes.append(nb)
         if b >= W:
             t_all += dt
             lat.append(dt * 1e3)
             dev.append(st["device_ms"])
             stats.append(st)
             if measure_bytes:
-                nbytes.append(eng.batch_bytes(index_bytes)[0])
+                nbytes.append(nb)
     release_all(eng, hs)
-    return {"lat_ms": lat, "dev_ms": dev, "bytes": nbytes, "stats": stats, "wall_s": t_all,
-            "mis": eng.get_mis()}
+    return {"lat_ms": lat, "dev_ms": dev, "bytes": nbytes, "bytes_all": all_bytes, "stats": stats,
+            "wall_s": t_all, "mis": eng.get_mis()}
 
 
 def run_served(M, eng, ni, nl, mis0, ups, W, K, batch, depth=2, staged=True):
@@ -210,7 +244,7 @@ def cpu_baseline(M, eng, ni, nl, mis0, ups, budget_s):
     first N updates of the stream as one batch, N doubled until the time
     budget is used; the GPU replays the same sample and must agree."""
     from oracle import pyoracle as O
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = cpu_threads()
     g0 = O.Graph.from_csr(ni, nl)
     best = None
     n_up = 250
@@ -311,12 +345,13 @@ def workload_c3(M, W, K):
     return ni, nl, mis0, u, v, time.time() - t
 
 
-def cpu_stream(M, eng, ni, nl, mis0, ups, batch, nb, budget_s):
+def cpu_stream(M, eng, ni, nl, mis0, ups, batch, nb, budget_s, batch_bytes=None):
     """The oracle's literal engine replaying whole batches of the stream in
     order from the initial MIS until the time budget is used (at least one
-    batch); the GPU replays the same batches and must agree bit for bit."""
+    batch); the GPU replays the same batches and must agree bit for bit.
+    batch_bytes: the algorithmic bytes of each batch (for the bytes rate)."""
     from oracle import pyoracle as O
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = cpu_threads()
     g = O.Graph.from_csr(ni, nl)
     m = mis0.copy()
     lat = []
@@ -335,11 +370,16 @@ def cpu_stream(M, eng, ni, nl, mis0, ups, batch, nb, budget_s):
         eng.process_batch(ups[b * batch:(b + 1) * batch])
     same = bool(np.array_equal(eng.get_mis(), m))
     del g
-    return {"value": done * batch / sum(lat), "unit": "edge updates/s", "cores": threads, "kind": "port", **host_info(),
-            "sample": "first %d of %d batches of the stream, in order, literal 2-hop rows (OpenMP, %d threads) "
-                      "+ sequential rule replay + completion; %.2f s" % (done, nb, threads, sum(lat)),
-            "p50_ms": round(pct([x * 1e3 for x in lat], 50), 3), "p99_ms": round(pct([x * 1e3 for x in lat], 99), 3),
-            "bit_exact_vs_gpu": same}
+    out = {"value": done * batch / sum(lat), "unit": "edge updates/s", "cores": threads, "kind": "port",
+           **host_info(),
+           "sample": "first %d of %d batches of the stream, whole and in order, literal 2-hop rows (OpenMP, %d "
+                     "threads) + sequential rule replay + completion; %.2f s" % (done, nb, threads, sum(lat)),
+           "batches": done, "per_batch_ms": [round(x * 1e3, 2) for x in lat],
+           "p50_ms": round(pct([x * 1e3 for x in lat], 50), 3), "p99_ms": round(pct([x * 1e3 for x in lat], 99), 3),
+           "bit_exact_vs_gpu": same}
+    if batch_bytes is not None and len(batch_bytes) >= done:
+        out["algorithmic_bytes_per_s"] = round(float(np.sum(batch_bytes[:done])) / sum(lat), 1)
+    return out
 
 
 def single_graph_line(M, args, name, ni, nl, mis0, u, v, W, K, B, index_bytes, copy, workload, cpu_mode):
@@ -367,7 +407,7 @@ def single_graph_line(M, args, name, ni, nl, mis0, u, v, W, K, B, index_bytes, c
         cpu = None
         if not args.no_cpu_baseline:
             if cpu_mode == "stream":
-                cpu = cpu_stream(M, eng, ni, nl, mis0, ups, B, W + K, args.cpu_budget)
+                cpu = cpu_stream(M, eng, ni, nl, mis0, ups, B, W + K, args.cpu_budget, plain["bytes_all"])
             else:
                 cpu = cpu_baseline(M, eng, ni, nl, mis0, ups, args.cpu_budget)
             print("cpu baseline: %.1f updates/s" % cpu["value"], file=sys.stderr, flush=True)
@@ -714,7 +754,7 @@ def cpu_many(M, graphs, budget_s):
     used; an item whose predicted time would overrun the budget is skipped.
     The GPU replays the same batches (plain) and must agree bit for bit."""
     from oracle import pyoracle as O
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = cpu_threads()
     items = sorted(range(len(graphs)), key=lambda gi: len(graphs[gi]["batches"][0]) * max(1, len(graphs[gi]["nl"])))
     spent, done, ups_done, lat, cost = 0.0, [], 0, [], None
     for gi in items:
@@ -850,7 +890,10 @@ def main():
             loads = latency_at_load(M, eng, ni, nl, mis0, ups, B, W + K, K * B / plain["wall_s"])
         cpu = None
         if not args.no_cpu_baseline:
-            cpu = cpu_baseline(M, eng, ni, nl, mis0, ups, args.cpu_budget)
+            # whole batches of the stream in order from the initial state, as
+            # the reference's host path would run them
+            cpu = cpu_stream(M, eng, ni, nl, mis0, ups, B, W + K, args.cpu_budget,
+                             plain["bytes_all"] if plain is not None else None)
             print("cpu baseline: %.1f updates/s" % cpu["value"], file=sys.stderr, flush=True)
             if not cpu["bit_exact_vs_gpu"]:
                 raise SystemExit("GPU differs from the oracle on the CPU-baseline sample")
