@@ -75,6 +75,7 @@ struct misfit_ctx {
     int serve_window = 0;         // MISFIT_SERVE_WINDOW: edges per round of batches in the serving kernel (0: adaptive)
     double serve_window_factor = 2.0;  // MISFIT_SERVE_WINDOW_FACTOR: ... adaptive: this x the last edges per round
     double last_epr = 0.0;        // edges committed per round by the last batch (0: none yet)
+    bool fail_growth = false;     // MISFIT_TEST_FAIL_GROWTH=1: every capacity growth fails (fault injection)
     double clock_khz = 0.0;       // device wall clock, for per-batch device times
     hipEvent_t timer0 = nullptr, timer1 = nullptr;  // misfit_timer_begin/end
     int round_per_cu = 0;      // MISFIT_ROUND_WG_PER_CU: round grid (0: by occupancy)
@@ -256,6 +257,8 @@ extern "C" int misfit_ctx_create(int device, misfit_ctx** out) {
     if (rm) c->round_margin = std::max(0, std::min(16, atoi(rm)));
     const char* ps = getenv("MISFIT_PU_SCAN");
     c->force_pu_scan = ps && strcmp(ps, "full") == 0;
+    const char* fg = getenv("MISFIT_TEST_FAIL_GROWTH");
+    c->fail_growth = fg && fg[0] == '1';
     const char* swf = getenv("MISFIT_SERVE_WINDOW_FACTOR");
     if (swf && atof(swf) > 0.0) c->serve_window_factor = atof(swf);
     const char* sw = getenv("MISFIT_SERVE_WINDOW");
@@ -386,7 +389,8 @@ static int ensure_batch_capacity(misfit_ctx* c, int64_t U) {
     // read logs: one segment per wave of the round grid (256-thread workgroups)
     const int64_t nwaves = (int64_t)c->S.blocks * (256 / MF_WAVE);
     const int64_t fa_cap = std::max<int64_t>(1 << 22, 16 * cap);
-    hipError_t e = hipSuccess;
+    // a growth past the first capacity fails outright under fault injection
+    hipError_t e = c->fail_growth && c->U_cap > 0 ? hipErrorOutOfMemory : hipSuccess;
     auto dev = [&](auto** p, int64_t count) {
         if (e == hipSuccess) e = dalloc(A, p, count);
     };
@@ -478,7 +482,7 @@ static int ensure_overlay_capacity(misfit_ctx* c, int64_t extra_ins) {
         std::vector<void*> fresh;
         int32_t* np = nullptr;
         u64 top = 0;
-        hipError_t e = dalloc(fresh, &np, nc);
+        hipError_t e = c->fail_growth ? hipErrorOutOfMemory : dalloc(fresh, &np, nc);
         if (e == hipSuccess) e = copy_sync(c, &top, g.pool_top, sizeof(u64), hipMemcpyDeviceToHost);
         if (e == hipSuccess && top) e = copy_sync(c, np, g.pool, top * sizeof(int32_t), hipMemcpyDeviceToDevice);
         if (e != hipSuccess) {  // the old pool stays in use
@@ -519,7 +523,7 @@ static int ensure_overlay_capacity(misfit_ctx* c, int64_t extra_ins) {
         std::vector<void*> fresh;
         u64* nkey = nullptr;
         int32_t* nval = nullptr;
-        hipError_t e = dalloc(fresh, &nkey, nc);
+        hipError_t e = c->fail_growth ? hipErrorOutOfMemory : dalloc(fresh, &nkey, nc);
         if (e == hipSuccess) e = dalloc(fresh, &nval, nc);
         if (e == hipSuccess) e = copy_sync(c, nkey, nk.data(), nc * sizeof(u64), hipMemcpyHostToDevice);
         if (e == hipSuccess) e = copy_sync(c, nval, nv.data(), nc * sizeof(int32_t), hipMemcpyHostToDevice);
@@ -1038,11 +1042,15 @@ static int ensure_ktask_capacity(misfit_ctx* c, int64_t U) {
     int64_t need = U + (c->P.g.m + c->P.g.pool_cap) / MF_KCH + 2;
     if (need <= c->ktask_cap) return MISFIT_OK;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    // the new list first: on failure the old one stays in use
+    const int64_t cap = need + need / 2;
+    u64* nt = nullptr;
+    const hipError_t e = c->fail_growth && c->ktask_cap > 0 ? hipErrorOutOfMemory
+                                                             : hipMalloc((void**)&nt, cap * sizeof(u64));
+    if (e != hipSuccess)
+        return fail(c, MISFIT_E_NOMEM, std::string("row-chunk task list growth: ") + hipGetErrorString(e));
     if (c->d_ktask) mf_free_device(c->d_ktask);
-    c->d_ktask = nullptr;
-    c->ktask_cap = 0;
-    int64_t cap = need + need / 2;
-    HIPCHK(c, hipMalloc((void**)&c->d_ktask, cap * sizeof(u64)));
+    c->d_ktask = nt;
     c->ktask_cap = cap;
     return MISFIT_OK;
 }

@@ -175,3 +175,19 @@ def test_report_lines_match_reference(tmp_path, binary):
     if async_:
         i = _match_run(lines, i, _SUMMARY)
     assert lines[i] == "" and lines[i + 1] == "Engine summary:"
+
+
+def test_engine_class_shim_on_the_sample(tmp_path):
+    """The reference-shaped engine class (processBatchWithVertexClustering
+    batch by batch, then launchBatch / syncAndCollect two in flight) ends the
+    bundled sample with its known final MIS."""
+    from tests.test_host_cpu import _build_shim_sample
+    golden = json.load(open(os.path.join(GOLDEN, "sample_known_answer.json")))["completion"]
+    exe = _build_shim_sample(tmp_path)
+    print("running the engine class shim on the sample", flush=True)
+    r = subprocess.run([exe, os.path.join(SAMPLE, "sample.egr"), os.path.join(SAMPLE, "sample_mis.txt"),
+                        os.path.join(SAMPLE, "batches"), "0"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    runs = [[int(x) for x in line.split()] for line in r.stdout.strip().splitlines()]
+    assert len(runs) == 2
+    assert runs[0] == golden["final_ids"] and runs[1] == golden["final_ids"]

@@ -472,3 +472,35 @@ def test_invalid_updates_refused_without_effect(oracle, engine_mod, eng, mode):
     eng.wait(eng.submit(engine_mod.make_updates(u, v, ins)))
     assert np.array_equal(eng.get_mis(), want[1])
     assert eng.check() == (0, 0, 0)
+
+
+def test_failed_capacity_growth_leaves_the_context_usable(oracle, engine_mod, monkeypatch):
+    """Fault injection (MISFIT_TEST_FAIL_GROWTH=1: every capacity growth of
+    the context fails): a batch larger than the batch buffers fails with
+    MISFIT_E_NOMEM and changes nothing; the next, smaller batches, the flags
+    and a compaction work on the old buffers, and the MIS equals the oracle's
+    over the batches that ran."""
+    monkeypatch.setenv("MISFIT_TEST_FAIL_GROWTH", "1")
+    ni, nl = engine_mod.gen_er(1 << 13, 8 << 13, 71)
+    mis0 = engine_mod.greedy_mis(ni, nl)
+    u, v = engine_mod.gen_updates(engine_mod.MIXED_EXISTING, ni, nl, 24000, 71)
+    ins = oracle.Graph.from_csr(ni, nl).flags(u, v)
+    big = (u[:20000], v[:20000], ins[:20000])          # past the first 16,384-update buffers
+    small = [(u[a:a + 2000], v[a:a + 2000], ins[a:a + 2000]) for a in (20000, 22000)]
+    with engine_mod.Engine(0) as eng:
+        eng.load_csr(ni, nl)
+        eng.set_mis(mis0)
+        with pytest.raises(engine_mod.MisfitError) as ei:
+            eng.process_batch(engine_mod.make_updates(*big))
+        assert ei.value.code == -4
+        assert np.array_equal(eng.get_mis(), mis0)
+        assert np.array_equal(eng.flags(u[20000:], v[20000:]), ins[20000:])
+        for b in small:
+            eng.process_batch(engine_mod.make_updates(*b))
+        assert eng.check() == (0, 0, 0)
+        want, _, og = H.oracle_replay(oracle, ni, nl, mis0, small, completion=True, literal=False)
+        assert np.array_equal(eng.get_mis(), want[-1])
+        eng.compact()
+        cni, cnl = og.compact()
+        gni, gnl = eng.export_csr()
+        assert np.array_equal(gni, cni) and np.array_equal(gnl, cnl)

@@ -284,3 +284,23 @@ def test_run_script_arguments():
     assert r.returncode == 1 and "--num-batches" in r.stderr
     r = subprocess.run(["bash", sh, "--variant=misfit", "--sample", "--gpu"], capture_output=True, text=True)
     assert r.returncode == 1 and "--gpu needs a value" in r.stderr
+
+
+def _build_shim_sample(out_dir):
+    """Compiles tests/shim/shim_sample.cpp (the reference-shaped engine class
+    over the C ABI) against the built library."""
+    lib_dir = os.path.join(ROOT, "misfit_amd")
+    exe = os.path.join(str(out_dir), "shim_sample")
+    r = subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-Werror", "-o", exe,
+                        os.path.join(ROOT, "tests", "shim", "shim_sample.cpp"), "-L" + lib_dir, "-lmisfit_amd",
+                        "-Wl,-rpath," + lib_dir], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    return exe
+
+
+def test_engine_class_shim_compiles(engine_mod, tmp_path):
+    """misfit_amd/shim/gpu_vertex_clustering_graph.h (the reference's
+    GPUVertexClusteringGraph interface, src/misfit/misfit.cu:360-693, over the
+    C ABI) compiles and links as C++17 with a caller's own edge and timing
+    records; the GPU test runs it on the sample."""
+    assert os.path.exists(_build_shim_sample(tmp_path))
