@@ -149,6 +149,21 @@ def host_info():
     return {"cpu_model": model, "nproc": os.cpu_count(), "physical_cores": physical_cores()}
 
 
+def phase_means(stats):
+    """Mean device-clock split of served batches: the prologue (overlay
+    update, may-uncover set, admissible index), the speculative rounds, the
+    rule counters with the completion's candidates, and the completion with
+    the publication."""
+    if not stats:
+        return None
+    pro = float(np.mean([s["prologue_ms"] for s in stats]))
+    rnd = float(np.mean([s["rounds_ms"] for s in stats]))
+    cnt = float(np.mean([s["counters_ms"] for s in stats]))
+    dev = float(np.mean([s["device_ms"] for s in stats]))
+    return {"prologue": round(pro, 4), "rounds": round(rnd, 4), "counters": round(cnt, 4),
+            "completion_and_publication": round(dev - pro - rnd - cnt, 4)}
+
+
 def pct(v, q):
     return float(np.percentile(np.asarray(v, np.float64), q)) if len(v) else 0.0
 
@@ -430,6 +445,7 @@ def single_graph_line(M, args, name, ni, nl, mis0, u, v, W, K, B, index_bytes, c
            "served_updates_per_s": round(served_ups, 1),
            "served_p50_ms": round(pct(served["lat_ms"], 50), 3), "served_p99_ms": round(pct(served["lat_ms"], 99), 3),
            "served_kernel_ms_mean": round(float(np.mean(served["dev_ms"])), 3),
+           "served_phase_ms_mean": phase_means(served["stats"]),
            "served_host_handover_updates_per_s": round(K * B / served_h["wall_s"], 1),
            "plain_updates_per_s": round(plain_ups, 1),
            "plain_p50_ms": round(pct(plain["lat_ms"], 50), 3), "plain_p99_ms": round(pct(plain["lat_ms"], 99), 3),
@@ -923,6 +939,7 @@ def main():
         "served_p50_ms": round(pct(served["lat_ms"], 50), 3),
         "served_p99_ms": round(pct(served["lat_ms"], 99), 3),
         "served_kernel_ms_mean": round(float(np.mean(served["dev_ms"])), 3),
+        "served_phase_ms_mean": phase_means(served["stats"]),
         "served_host_handover_updates_per_s": round(K * B / served_h["wall_s"], 1) if served_h else None,
         "mis_plain_equals_served": same,
         "check_after_every_plain_batch": "independent, maximal, counts consistent" if plain else None,

@@ -67,6 +67,9 @@ typedef struct {
     int64_t window;           /* edges speculated per round (the rounds' window) */
     double device_start_ms;   /* the device wall clock at the batch's start and end, in ms (comparable */
     double device_end_ms;     /* across the batches of every context on the device) */
+    double rounds_ms;         /* of device_ms: the speculative rounds, grid-wide and in the tail */
+    double counters_ms;       /* ... the rule counters and the completion's candidates (the rest after the
+                                 rounds is the completion and the publication) */
 } misfit_batch_stats;
 
 /* Execution modes of the repair (results are identical). */

@@ -51,7 +51,8 @@ class BatchStats(ctypes.Structure):
         "completion_adds", "rounds", "evaluations", "maybe_uncovered", "admissible", "delta_ops",
         "overlay_entries")] + [("device_ms", ctypes.c_double), ("prologue_ms", ctypes.c_double),
                                ("window", ctypes.c_int64), ("device_start_ms", ctypes.c_double),
-                               ("device_end_ms", ctypes.c_double)]
+                               ("device_end_ms", ctypes.c_double), ("rounds_ms", ctypes.c_double),
+                               ("counters_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

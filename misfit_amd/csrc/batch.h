@@ -44,6 +44,7 @@ enum BatchCounter {
     C_GUARD,         // a loop bound was hit or a list overflowed (engine bug): results invalid
     C_HALT,          // the tail stopped with many edges left: the host queues more rounds
     C_T0, C_TP, C_T1,  // device wall clock: batch start, first round, tail end
+    C_TR, C_TS,        // ... the tail's last round done, its rule counters done
     C_NCOUNTERS
 };
 

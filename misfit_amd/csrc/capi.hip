@@ -799,6 +799,10 @@ static void fill_stats(const misfit_ctx* c, const Slot& s, misfit_batch_stats* s
         st->device_end_ms = (double)k[C_T1] / c->clock_khz;
         st->device_ms = (double)(k[C_T1] - k[C_T0]) / c->clock_khz;
         st->prologue_ms = k[C_TP] >= k[C_T0] ? (double)(k[C_TP] - k[C_T0]) / c->clock_khz : st->device_ms;
+        if (k[C_TR] >= k[C_TP] && k[C_TR] <= k[C_T1]) {
+            st->rounds_ms = (double)(k[C_TR] - k[C_TP]) / c->clock_khz;
+            st->counters_ms = k[C_TS] >= k[C_TR] ? (double)(k[C_TS] - k[C_TR]) / c->clock_khz : 0.0;
+        }
     }
 }
 

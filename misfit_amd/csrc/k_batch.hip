@@ -1571,6 +1571,26 @@ __device__ __forceinline__ void finish_completion(const BP& P, const GC& c) {
     }
 }
 
+// The rounds a tail still finds (parity par first), the rule counters, the
+// completion and the publication, in one workgroup; the end of the rounds
+// and of the counters are stamped on the device clock.
+__device__ __forceinline__ void tail_phases(const BP& P, const GC& c, int par) {
+    bool ran = false;
+    if (finish_rounds(P, c, par, ran)) {
+        if (ran) phase_barrier();
+        if (c.tid == 0) P.ctr[MF_CS * (C_TR)] = wall_clock64();
+        ph_stats(P, c);
+        if (P.completion) {
+            ph_comp_collect(P, c);
+            if (c.tid == 0) P.ctr[MF_CS * (C_TS)] = wall_clock64();
+            finish_completion(P, c);
+        } else if (c.tid == 0) {
+            P.ctr[MF_CS * (C_TS)] = wall_clock64();
+        }
+    }
+    publish(P, c, true);
+}
+
 // The tail of a batch in one workgroup, after the queued grid-wide rounds:
 // whatever rounds they left, the rule counters, the completion candidates
 // and iterations, then the counters are published to the host.  Its phases
@@ -1592,16 +1612,7 @@ __global__ __launch_bounds__(1024) void k_finish(BP P, int par) {
             return;
         }
     }
-    bool ran = false;
-    if (finish_rounds(P, c, par, ran)) {
-        if (ran) phase_barrier();
-        ph_stats(P, c);
-        if (P.completion) {
-            ph_comp_collect(P, c);
-            finish_completion(P, c);
-        }
-    }
-    publish(P, c, true);
+    tail_phases(P, c, par);
 }
 
 __global__ __launch_bounds__(256) void k_publish(BP P) {
@@ -1619,16 +1630,7 @@ __global__ __launch_bounds__(256) void k_publish(BP P) {
 __device__ __forceinline__ void batch_in_workgroup(const BP& P, const GC& c, bool whole) {
     if (!whole) {
         if (c.tid == 0) P.ctr[MF_CS * (C_TP)] = wall_clock64();
-        bool ran = false;
-        if (finish_rounds(P, c, 0, ran)) {
-            if (ran) phase_barrier();
-            ph_stats(P, c);
-            if (P.completion) {
-                ph_comp_collect(P, c);
-                finish_completion(P, c);
-            }
-        }
-        publish(P, c, true);
+        tail_phases(P, c, 0);
         return;
     }
     if (P.hu) ph_init(P, c);
@@ -1653,16 +1655,7 @@ __device__ __forceinline__ void batch_in_workgroup(const BP& P, const GC& c, boo
     ph_fa_fill(P, c);
     phase_barrier();
     if (c.tid == 0) P.ctr[MF_CS * (C_TP)] = wall_clock64();
-    bool ran = false;
-    if (finish_rounds(P, c, 0, ran)) {
-        if (ran) phase_barrier();
-        ph_stats(P, c);
-        if (P.completion) {
-            ph_comp_collect(P, c);
-            finish_completion(P, c);
-        }
-    }
-    publish(P, c, true);
+    tail_phases(P, c, 0);
 }
 
 __device__ __forceinline__ u64 host_word(const u64* p) {
