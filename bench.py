@@ -230,7 +230,7 @@ def run_served(M, eng, ni, nl, mis0, ups, W, K, batch, depth=2, staged=True):
         eng.process_batch(ups[b * batch:(b + 1) * batch])
     hs = [None] * W + [eng.stage(ups[b * batch:(b + 1) * batch]) for b in range(W, W + K)] if staged else None
     eng.sync()
-    starts, lat, dev, tickets, stats = {}, [], [], {}, []
+    starts, lat, dev, tickets, stats, sub = {}, [], [], {}, [], []
     eng.timer_begin()  # HIP event on the engine's stream: the timed region's device time
     t0 = time.perf_counter()
     nxt = W
@@ -239,6 +239,7 @@ def run_served(M, eng, ni, nl, mis0, ups, W, K, batch, depth=2, staged=True):
             starts[nxt] = time.perf_counter()
             tickets[nxt] = (eng.submit_staged(hs[nxt]) if staged
                             else eng.submit(ups[nxt * batch:(nxt + 1) * batch]))
+            sub.append((time.perf_counter() - starts[nxt]) * 1e3)
             nxt += 1
         st = eng.wait(tickets[b])
         lat.append((time.perf_counter() - starts[b]) * 1e3)
@@ -251,7 +252,7 @@ def run_served(M, eng, ni, nl, mis0, ups, W, K, batch, depth=2, staged=True):
         release_all(eng, hs[W:])
     conf, unc, bad = eng.check()
     return {"lat_ms": lat, "dev_ms": dev, "stats": stats, "wall_s": wall, "mis": eng.get_mis(),
-            "check": (conf, unc, bad), "region_ms": region_ms}
+            "check": (conf, unc, bad), "region_ms": region_ms, "submit_ms": sub}
 
 
 def cpu_baseline(M, eng, ni, nl, mis0, ups, budget_s):
@@ -446,6 +447,7 @@ def single_graph_line(M, args, name, ni, nl, mis0, u, v, W, K, B, index_bytes, c
            "served_p50_ms": round(pct(served["lat_ms"], 50), 3), "served_p99_ms": round(pct(served["lat_ms"], 99), 3),
            "served_kernel_ms_mean": round(float(np.mean(served["dev_ms"])), 3),
            "served_phase_ms_mean": phase_means(served["stats"]),
+           "served_submit_ms_mean": round(float(np.mean(served["submit_ms"])), 4),
            "served_host_handover_updates_per_s": round(K * B / served_h["wall_s"], 1),
            "plain_updates_per_s": round(plain_ups, 1),
            "plain_p50_ms": round(pct(plain["lat_ms"], 50), 3), "plain_p99_ms": round(pct(plain["lat_ms"], 99), 3),
@@ -940,6 +942,7 @@ def main():
         "served_p99_ms": round(pct(served["lat_ms"], 99), 3),
         "served_kernel_ms_mean": round(float(np.mean(served["dev_ms"])), 3),
         "served_phase_ms_mean": phase_means(served["stats"]),
+        "served_submit_ms_mean": round(float(np.mean(served["submit_ms"])), 4),
         "served_host_handover_updates_per_s": round(K * B / served_h["wall_s"], 1) if served_h else None,
         "mis_plain_equals_served": same,
         "check_after_every_plain_batch": "independent, maximal, counts consistent" if plain else None,

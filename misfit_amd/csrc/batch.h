@@ -105,9 +105,14 @@ struct BP {
     uint8_t* kind;
     int32_t* res_w;
     int32_t* res_x;
-    uint4* rlog;       // guarded reads of a round, MF_RLOG slots per wave of the grid (slot 0: count)
+    uint4* rlog;       // guarded reads of a round, MF_RLOG slots per wave of the grid (slot 0: header)
     int4* cm;          // per wave of the round grid: its first edge's kind, evicted
                        // and admitted vertex and index, left by the speculation for the commit
+    int4* stage;       // per wave of the round grid: the endpoints, batch epoch and
+                       // insertion flag (epoch << 1 | ins) and index of the edge it speculates
+                       // next round, left by the commit that fixes where that round starts
+                       // (index -1: none)
+    int64_t nstage;    // entries of stage (waves of the largest round grid)
     uint32_t* relog;   // per edge: = round when its read log overflowed
     // completion
     int32_t* cand;

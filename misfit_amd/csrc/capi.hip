@@ -422,6 +422,7 @@ static int ensure_batch_capacity(misfit_ctx* c, int64_t U) {
     dev(&N.relog, cap);
     dev(&N.rlog, nwaves * MF_RLOG);
     dev(&N.cm, nwaves);
+    dev(&N.stage, nwaves);
     dev(&N.fa_z, fa_cap);
     dev(&N.fa_pos, fa_cap);
     for (auto& s : ns) {
@@ -442,6 +443,8 @@ static int ensure_batch_capacity(misfit_ctx* c, int64_t U) {
     if (e == hipSuccess) {
         N.bt_mask = bt_size - 1;
         N.fa_cap = fa_cap;
+        N.nstage = nwaves;
+        mf_fill_i32((int32_t*)N.stage, -1, 4 * nwaves, c->stream);
         mf_fill_u64(N.bt_key, MF_EMPTY_KEY, (int64_t)bt_size, c->stream);
         mf_fill_u32(N.relog, 0u, cap, c->stream);
         mf_fill_i32(N.bt_head, -1, (int64_t)bt_size, c->stream);

@@ -106,14 +106,19 @@ struct Row {
     __device__ __forceinline__ int32_t at(int64_t j) const { return j < blen ? base[j] : add[j - blen]; }
 };
 
+// The four words of a row's bounds are loaded together: the run's start
+// (every vertex has one, 0 until it gets a run) is read whatever the run's
+// length, so no load waits for another; an empty run's pointer is never
+// dereferenced.
 __device__ __forceinline__ Row get_row(const DevGraph& g, int32_t x) {
     Row r;
-    int64_t s = g.nindex[x];
+    const int64_t s = g.nindex[x], e = g.nindex[x + 1];
+    const int32_t al = g.ov_len[x];
+    const int64_t os = g.ov_start[x];
     r.base = g.nlist + s;
-    r.blen = g.nindex[x + 1] - s;
-    int32_t al = g.ov_len[x];
+    r.blen = e - s;
     r.alen = al;
-    r.add = al ? g.pool + g.ov_start[x] : nullptr;
+    r.add = g.pool + os;
     return r;
 }
 

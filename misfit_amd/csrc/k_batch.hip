@@ -777,14 +777,14 @@ struct SpecMarks {
 
 // Insertion with both ends in the MIS, w evicted: first admissible vertex of
 // row(w) in BFS order, or -1.  Wave-cooperative; all lanes return the same.
-// With `em`, w's neighbours are marked as part 2 passes them.
-__device__ int32_t eval_insertion(const BP& P, int32_t w, Recorder& rc, SpecMarks* em = nullptr) {
+// With `em`, w's neighbours are marked as part 2 passes them.  pw, mw and rw
+// are w's PU word, count word and row, loaded by the caller (the
+// speculation issues them with the edge's classification).
+__device__ int32_t eval_insertion_from(const BP& P, int32_t w, uint32_t pw, uint32_t mw, const Row& rw, Recorder& rc,
+                                       SpecMarks* em) {
     const DevGraph& g = P.g;
     const int lane = lane_id();
     Recorder none{REC_NONE};
-    // loads that do not depend on each other first
-    const uint32_t pw = g.pub[w], mw = g.mc[w];
-    const Row rw = get_row(g, w);
     const int64_t L = rw.len();
     const bool fa_ok = P.ctr[MF_CS * (C_FA_OVF)] == 0;
     // position 0: w itself.  Outside PU, w keeps an MIS neighbour whenever it
@@ -832,7 +832,13 @@ __device__ int32_t eval_insertion(const BP& P, int32_t w, Recorder& rc, SpecMark
             }
         }
     }
-    // part 2: N1(w) in adjacency order
+    // part 2: N1(w) in adjacency order.  A row of at most one wave's length
+    // (the usual case) also loads each neighbour's index words here, with its
+    // other words, for part 3's pass over the same neighbours.
+    const bool short_row = L <= MF_WAVE;
+    int32_t sx1 = -1, sfill = 0;
+    uint32_t stag = 0;
+    int64_t sstart = -1;
     for (int64_t j0 = 0; j0 < L; j0 += MF_WAVE) {
         int64_t j = j0 + lane;
         int32_t x = -1;
@@ -842,6 +848,12 @@ __device__ int32_t eval_insertion(const BP& P, int32_t w, Recorder& rc, SpecMark
             x = rw.at(j);
             if (x >= 0 && x != w) {
                 const uint32_t px = g.pub[x], mx = g.mc[x], sx = g.seedb[x];
+                if (short_row) {
+                    sx1 = x;
+                    stag = g.fa_tag[x];
+                    sstart = g.fa_start[x];
+                    sfill = g.fa_fill[x];
+                }
                 if (em && (u64)(mx & MF_CMASK) < 2ull + em->bound) {
                     atomicMax(&g.cmark[x], em->mv);
                     round_count_add(&g.cdec[x], em->R);
@@ -869,17 +881,17 @@ __device__ int32_t eval_insertion(const BP& P, int32_t w, Recorder& rc, SpecMark
         const int32_t* pa = nullptr;
         int64_t blen = 0, slen = 0;
         if (j < L) {
-            int32_t x1 = rw.at(j);
+            int32_t x1 = short_row ? sx1 : rw.at(j);
             if (x1 >= 0 && x1 != w) {
                 // every row with an admissible neighbour has index entries
                 // this batch; only an overflowed index falls back to the row
-                bool indexed = g.fa_tag[x1] == P.b;
-                int64_t st = indexed && fa_ok ? g.fa_start[x1] : -1;
+                bool indexed = (short_row ? stag : g.fa_tag[x1]) == P.b;
+                int64_t st = indexed && fa_ok ? (short_row ? sstart : g.fa_start[x1]) : -1;
                 if (st >= 0) {
                     kindl = 2;
                     pb = P.fa_z + st;
                     pa = P.fa_pos + st;
-                    slen = g.fa_fill[x1];
+                    slen = short_row ? sfill : g.fa_fill[x1];
                 } else if (indexed || !fa_ok) {
                     Row r1 = get_row(g, x1);
                     kindl = 1; pb = r1.base; pa = r1.add; blen = r1.blen; slen = r1.len();
@@ -959,14 +971,18 @@ __device__ int32_t eval_insertion(const BP& P, int32_t w, Recorder& rc, SpecMark
     return -1;
 }
 
+__device__ __forceinline__ int32_t eval_insertion(const BP& P, int32_t w, Recorder& rc, SpecMarks* em = nullptr) {
+    return eval_insertion_from(P, w, P.g.pub[w], P.g.mc[w], get_row(P.g, w), rc, em);
+}
+
 // Deletion, exactly one end `other` in the MIS: may tgt join?  row(tgt) =
 // {tgt} ∪ N1 ∪ N2 must hold no MIS member other than `other`.
 // With `em` (its `done` unused), the admission marks of tgt's neighbours
 // (mark_vertex) are issued by the final pass that records the reads.
-__device__ bool eval_deletion(const BP& P, int32_t tgt, int32_t other, Recorder& rc, const SpecMarks* em = nullptr) {
+__device__ bool eval_deletion_from(const BP& P, int32_t tgt, int32_t other, const Row& r, Recorder& rc,
+                                   const SpecMarks* em) {
     const DevGraph& g = P.g;
     const int lane = lane_id();
-    Row r = get_row(g, tgt);
     int64_t L = r.len();
     for (int64_t j0 = 0; j0 < L; j0 += MF_WAVE) {
         int64_t j = j0 + lane;
@@ -1010,6 +1026,11 @@ __device__ bool eval_deletion(const BP& P, int32_t tgt, int32_t other, Recorder&
     return true;
 }
 
+__device__ __forceinline__ bool eval_deletion(const BP& P, int32_t tgt, int32_t other, Recorder& rc,
+                                              const SpecMarks* em = nullptr) {
+    return eval_deletion_from(P, tgt, other, get_row(P.g, tgt), rc, em);
+}
+
 // ---------------------------------------------------------------- rounds
 // The round state lives on the device: ctr[C_START0 + par] is the first edge
 // not yet committed, read by the rounds of parity par, and *P.round the id of
@@ -1041,10 +1062,37 @@ __device__ __forceinline__ RoundView round_view(const BP& P, int par, int64_t wi
     return v;
 }
 
+// Kind of an edge from its endpoints' count words: 1 an insertion with both
+// ends in the MIS, 2 a deletion with exactly one end in it, 0 inactive.
+__device__ __forceinline__ uint8_t edge_kind(bool ins, uint32_t ma, uint32_t mb) {
+    const bool ia = (ma & MF_MISBIT) != 0, ib = (mb & MF_MISBIT) != 0;
+    if (ins) return (ia && ib) ? 1 : 0;
+    return (ia != ib) ? 2 : 0;
+}
+
 __device__ __forceinline__ uint8_t classify_edge(const BP& P, bool ins, int32_t a, int32_t bb) {
-    bool ma = (P.g.mc[a] & MF_MISBIT) != 0, mb = (P.g.mc[bb] & MF_MISBIT) != 0;
-    if (ins) return (ma && mb) ? 1 : 0;
-    return (ma != mb) ? 2 : 0;
+    return edge_kind(ins, P.g.mc[a], P.g.mc[bb]);
+}
+
+// What the evaluation of an edge needs first that depends on its endpoints
+// alone, loaded together with the classification's count words: an
+// insertion evaluates row(max(a, b)), a deletion the row of the end outside
+// the MIS, so both rows' bounds come with the PU word of the larger end.
+struct EdgeLoads {
+    int32_t a, b;
+    uint32_t ma, mb, pw;
+    Row ra, rb;
+};
+__device__ __forceinline__ EdgeLoads edge_loads(const BP& P, int32_t a, int32_t b) {
+    EdgeLoads l;
+    l.a = a;
+    l.b = b;
+    l.ma = P.g.mc[a];
+    l.mb = P.g.mc[b];
+    l.pw = P.g.pub[a > b ? a : b];
+    l.ra = get_row(P.g, a);
+    l.rb = get_row(P.g, b);
+    return l;
 }
 
 // Marks the writes of a speculated membership flip of z: z's bit and the
@@ -1054,10 +1102,11 @@ __device__ __forceinline__ uint8_t classify_edge(const BP& P, bool ins, int32_t 
 // change its outcome this round (every count threshold is at most 2).
 // Entries before `from` were marked already (by the evaluation's first pass
 // over an evicted vertex's row, SpecMarks).
+// `pre`: z's row, when the caller has it.
 __device__ __forceinline__ void mark_vertex(const BP& P, int32_t z, u64 mv, bool evict, uint32_t R, u64 bound,
-                                            int lane, int64_t from = 0) {
+                                            int lane, int64_t from = 0, const Row* pre = nullptr) {
     if (lane == 0) atomicMax(&P.g.mmark[z], mv);
-    Row r = get_row(P.g, z);
+    const Row r = pre ? *pre : get_row(P.g, z);
     int64_t L = r.len();
     for (int64_t j = from + lane; j < L; j += MF_WAVE) {
         int32_t y = r.at(j);
@@ -1069,30 +1118,33 @@ __device__ __forceinline__ void mark_vertex(const BP& P, int32_t z, u64 mv, bool
     }
 }
 
-// Speculative evaluation of active edge e (kind k) by one wave; the reads
-// its outcome rests on go to `rc` (nowhere, or the wave's read log).
-__device__ __forceinline__ void spec_edge(const BP& P, int32_t e, uint8_t k, uint32_t R, u64 bound, int lane,
-                                          Recorder& rc, int4* cmw) {
-    int32_t a = P.bu[e], bb = P.bv[e];
+// Speculative evaluation of active edge e (kind k) by one wave, from its
+// endpoints' loads; the reads its outcome rests on go to `rc` (nowhere, or
+// the wave's read log).
+__device__ __forceinline__ void spec_edge_from(const BP& P, int32_t e, uint8_t k, uint32_t R, u64 bound, int lane,
+                                               Recorder& rc, int4* cmw, const EdgeLoads& el) {
+    const int32_t a = el.a, bb = el.b;
     rc.R = R;
     rc.e = e;
     u64 mv = ((u64)R << 32) | (u64)(0xFFFFFFFFu - (uint32_t)e);
     if (k == 1) {
-        int32_t w = a > bb ? a : bb;
+        const bool wa = a > bb;
+        const int32_t w = wa ? a : bb;
+        const Row rw = wa ? el.ra : el.rb;
         SpecMarks em{mv, R, bound, 0};
-        int32_t x = eval_insertion(P, w, rc, &em);
+        int32_t x = eval_insertion_from(P, w, el.pw, wa ? el.ma : el.mb, rw, rc, &em);
         if (lane == 0) {
             P.res_w[e] = w;
             P.res_x[e] = x;
             if (cmw) *cmw = make_int4(1, w, x, e);
         }
-        mark_vertex(P, w, mv, true, R, bound, lane, em.done);
+        mark_vertex(P, w, mv, true, R, bound, lane, em.done, &rw);
         if (x >= 0) mark_vertex(P, x, mv, false, R, bound, lane);
     } else {
-        bool ma = (P.g.mc[a] & MF_MISBIT) != 0;
+        const bool ma = (el.ma & MF_MISBIT) != 0;
         int32_t tgt = ma ? bb : a, other = ma ? a : bb;
         const SpecMarks em{mv, R, bound, 0};
-        bool ok = eval_deletion(P, tgt, other, rc, &em);
+        bool ok = eval_deletion_from(P, tgt, other, ma ? el.rb : el.ra, rc, &em);
         if (lane == 0) {
             P.res_w[e] = -1;
             P.res_x[e] = ok ? tgt : -1;
@@ -1100,6 +1152,11 @@ __device__ __forceinline__ void spec_edge(const BP& P, int32_t e, uint8_t k, uin
             if (ok) atomicMax(&P.g.mmark[tgt], mv);  // its neighbours were marked by the evaluation
         }
     }
+}
+
+__device__ __forceinline__ void spec_edge(const BP& P, int32_t e, uint8_t k, uint32_t R, u64 bound, int lane,
+                                          Recorder& rc, int4* cmw) {
+    spec_edge_from(P, e, k, R, bound, lane, rc, cmw, edge_loads(P, P.bu[e], P.bv[e]));
 }
 
 // Re-walk of active edge e: true when a read its outcome rests on may have
@@ -1179,27 +1236,48 @@ __device__ __forceinline__ void advance_round(const BP& P, int par, int64_t conf
 // Grid-wide rounds: one wave per edge of the window.  Speculation classifies
 // the edge first (the kind depends only on the committed state) and logs its
 // guarded reads, so a round is three kernels: speculate, validate, commit.
-// `cur` is the LDS cursor array of the workgroup's waves.
-__device__ void ph_spec_w(const BP& P, const GC& c, const RoundView& v, int* cur) {
+// `cur` is the LDS cursor array of the workgroup's waves.  `st` is the
+// wave's staged edge (stage[wave], loaded with the round's start): when it
+// is the wave's first edge, its endpoints need no load after the start's.
+// The segment's header (slot 0) gets the record count and the kind (bits 16
+// and up), the index and the endpoints of the wave's first edge (index
+// 0xFFFFFFFF: none), so the validation needs no load of its own for them.
+#define MF_NO_EDGE 0xFFFFFFFFu
+__device__ void ph_spec_w(const BP& P, const GC& c, const RoundView& v, int* cur, int4 st) {
     const int wiw = (int)(threadIdx.x / MF_WAVE);
     if (c.lane == 0) cur[wiw] = 0;
     __syncthreads();
     Recorder rc{REC_LOG, ReadLog{P.rlog + c.wave * MF_RLOG, cur + wiw, P.relog}};
+    uint4 hdr = make_uint4(0u, MF_NO_EDGE, 0u, 0u);
     for (int64_t e = v.start + c.wave; e < v.end; e += c.nwaves) {
-        int32_t a = P.bu[e], bb = P.bv[e];
-        uint8_t k = classify_edge(P, P.bins[e] != 0, a, bb);
+        const bool first = e == v.start + c.wave;
+        int32_t a, bb;
+        bool ins;
+        if (first && st.w == (int32_t)e && ((uint32_t)st.z & ~1u) == (P.b << 1)) {  // this batch's edge e
+            a = st.x;
+            bb = st.y;
+            ins = (st.z & 1) != 0;
+        } else {
+            a = P.bu[e];
+            bb = P.bv[e];
+            ins = P.bins[e] != 0;
+        }
+        const EdgeLoads el = edge_loads(P, a, bb);
+        const uint8_t k = edge_kind(ins, el.ma, el.mb);
         // the commit's record of the wave's first edge (wave w of the commit
         // commits edge start + w)
-        int4* cmw = e == v.start + c.wave ? P.cm + c.wave : nullptr;
+        int4* cmw = first ? P.cm + c.wave : nullptr;
         if (c.lane == 0) {
             P.kind[e] = k;
             if (cmw && !k) *cmw = make_int4(0, -1, -1, (int32_t)e);
         }
-        if (k) spec_edge(P, (int32_t)e, k, v.R, (u64)P.n_ins, c.lane, rc, cmw);
+        if (first) hdr = make_uint4((uint32_t)k << 16, (uint32_t)e, (uint32_t)a, (uint32_t)bb);
+        if (k) spec_edge_from(P, (int32_t)e, k, v.R, (u64)P.n_ins, c.lane, rc, cmw, el);
     }
     if (c.lane == 0) {
         const int n = cur[wiw];
-        P.rlog[c.wave * MF_RLOG] = make_uint4((uint32_t)(n < MF_RLOG - 1 ? n : MF_RLOG - 1), 0u, 0u, 0u);
+        hdr.x |= (uint32_t)(n < MF_RLOG - 1 ? n : MF_RLOG - 1);
+        P.rlog[c.wave * MF_RLOG] = hdr;
     }
 }
 
@@ -1218,17 +1296,33 @@ __device__ __forceinline__ bool endpoint_conflict(const BP& P, int64_t e, uint8_
 }
 
 #define MF_RLOG_LINE 8  // 16-byte slots per 128-byte line
-__device__ void ph_validate_w(const BP& P, const GC& c, const RoundView& v, int par) {
+// `r0`: slot `lane` of the wave's segment (lanes below MF_RLOG_LINE), loaded
+// with the round's start.
+__device__ void ph_validate_w(const BP& P, const GC& c, const RoundView& v, int par, uint4 r0) {
     const uint4* seg = P.rlog + c.wave * MF_RLOG;
     const int64_t e0 = v.start + c.wave;
     if (e0 >= v.end) return;  // wave-uniform: no edge, no records
     const uint4 zero = make_uint4(0u, 0u, 0u, 0u);
-    uint4 r0 = c.lane < MF_RLOG_LINE ? seg[c.lane] : zero;
-    const uint8_t k0 = P.kind[e0];
-    const int32_t a0 = P.bu[e0], b0 = P.bv[e0];
-    const int n = __shfl((int)r0.x, 0, MF_WAVE);  // the header
+    const uint32_t hx = (uint32_t)__shfl((int)r0.x, 0, MF_WAVE);
+    const uint32_t he = (uint32_t)__shfl((int)r0.y, 0, MF_WAVE);
+    uint8_t k0;
+    int32_t a0, b0;
+    if (he == (uint32_t)e0) {  // the header names the wave's edge (always, with the speculation's grid)
+        k0 = (uint8_t)(hx >> 16);
+        a0 = __shfl((int)r0.z, 0, MF_WAVE);
+        b0 = __shfl((int)r0.w, 0, MF_WAVE);
+    } else {
+        k0 = P.kind[e0];
+        a0 = P.bu[e0];
+        b0 = P.bv[e0];
+    }
+    const int n = (int)(hx & 0xFFFFu);
     // slot l (1 <= l <= n) is checked by lane l, slot 64 + l by lane l
-    if (c.lane >= MF_RLOG_LINE && c.lane <= n) r0 = seg[c.lane];
+    if (c.lane < MF_RLOG_LINE) {
+        if (c.lane > n) r0 = zero;
+    } else {
+        r0 = c.lane <= n ? seg[c.lane] : zero;
+    }
     uint4 r1 = zero;
     if (c.lane + MF_WAVE <= n) r1 = seg[c.lane + MF_WAVE];
     bool c0 = false, c1 = false;
@@ -1247,12 +1341,21 @@ __device__ void ph_validate_w(const BP& P, const GC& c, const RoundView& v, int 
 
 // Wave w commits edge start + w from the record wave w of the speculation
 // left (loaded beside the conflict index, not after it), further edges of a
-// wider window from the arrays.
+// wider window from the arrays.  The grid also stages the next round's
+// edges: stage[i] = edge next + i, where next (the first conflict, or the
+// window's end) is where the next round starts, tagged with the batch epoch
+// (z = epoch << 1 | insertion flag) and the edge's index.
 __device__ void ph_commit_w(const BP& P, const GC& c, const RoundView& v, int par) {
     const int64_t e0 = v.start + c.wave;
     const int4 m = e0 < v.end ? P.cm[c.wave] : make_int4(0, -1, -1, -1);
     const int64_t conflict = round_conflict(P, par);
     const int64_t lim = conflict < v.end ? conflict : v.end;
+    const int64_t ns = P.window < P.nstage ? P.window : P.nstage;
+    for (int64_t i = c.tid; i < ns; i += c.nthreads) {
+        const int64_t e = lim + i;
+        P.stage[i] = e < P.U ? make_int4(P.bu[e], P.bv[e], (int)((P.b << 1) | (uint32_t)(P.bins[e] != 0)), (int32_t)e)
+                             : make_int4(-1, -1, 0, -1);
+    }
     if (e0 < lim) {
         if (m.w == (int32_t)e0) {
             if (m.x) commit_flips(P, (uint8_t)m.x, m.y, m.z, c.lane);
@@ -1467,21 +1570,26 @@ MF_KERNEL(k_comp_collect, ph_comp_collect(P, c))
 MF_KERNEL(k_comp_decide, ph_comp_decide(P, c))
 MF_KERNEL(k_stats, ph_stats(P, c))
 
-// par bit 1: the batch's first round, which stamps the end of the prologue.
+// par bit 1: the batch's first round, which stamps the end of the prologue
+// (no commit staged its edges).
 __global__ __launch_bounds__(256) void k_spec(BP P, int par) {
     __shared__ int cur[256 / MF_WAVE];
     GC c = grid_coords();
-    if (par & 2) {
+    const bool first = (par & 2) != 0;
+    const int4 st = !first && c.wave < P.nstage ? P.stage[c.wave] : make_int4(-1, -1, 0, -1);
+    if (first) {
         if (c.tid == 0) P.ctr[MF_CS * (C_TP)] = wall_clock64();
         par &= 1;
     }
     RoundView v = round_view(P, par, P.window);
-    if (v.live) ph_spec_w(P, c, v, cur);
+    if (v.live) ph_spec_w(P, c, v, cur, st);
 }
 __global__ __launch_bounds__(256) void k_validate(BP P, int par) {
     GC c = grid_coords();
+    // the segment's first line (every wave of the round grid has one)
+    const uint4 r0 = c.lane < MF_RLOG_LINE ? P.rlog[c.wave * MF_RLOG + c.lane] : make_uint4(0u, 0u, 0u, 0u);
     RoundView v = round_view(P, par, P.window);
-    if (v.live) ph_validate_w(P, c, v, par);
+    if (v.live) ph_validate_w(P, c, v, par, r0);
 }
 __global__ __launch_bounds__(256) void k_commit(BP P, int par) {
     GC c = grid_coords();
