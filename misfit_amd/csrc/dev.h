@@ -364,10 +364,14 @@ struct GC {
     int lane;
 };
 
+// Every grid-stride kernel runs workgroups of MF_BLOCK threads.  The size is
+// a constant here, not blockDim.x: reading the workgroup size is a load from
+// the dispatch's hidden arguments that every wave would wait for first.
+#define MF_BLOCK 256
 __device__ __forceinline__ GC grid_coords() {
     GC c;
-    c.tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    c.nthreads = (int64_t)gridDim.x * blockDim.x;
+    c.tid = (int64_t)blockIdx.x * MF_BLOCK + threadIdx.x;
+    c.nthreads = (int64_t)gridDim.x * MF_BLOCK;
     c.wave = c.tid / MF_WAVE;
     c.nwaves = c.nthreads / MF_WAVE;
     c.lane = lane_id();

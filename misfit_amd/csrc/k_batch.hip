@@ -144,6 +144,12 @@ __device__ __forceinline__ Rd rd_none() { return rd(-1, 0u, 0u); }
 // Where the reads an outcome rests on go: nowhere (plain speculation),
 // checked against this round's marks (validation by re-walk), or appended to
 // the wave's read log (speculation before a flat validation).
+//
+// Log records are held in registers (up to four per lane) until
+// flush_records: a vector memory counter counts stores with loads, so a
+// store issued before a load holds up the load's use until the store is
+// done, and an evaluation's records would otherwise delay its own later
+// loads.
 #define REC_NONE 0
 #define REC_CHECK 1
 #define REC_LOG 2
@@ -153,12 +159,34 @@ struct Recorder {
     uint32_t R;
     int32_t e;
     bool conf;  // REC_CHECK: a recorded read conflicts (per lane)
+    int np = 0; // REC_LOG: records held back
+    Rd p0, p1, p2, p3;
 };
 
 __device__ __forceinline__ void record(const BP& P, Recorder& rc, const Rd& r) {
     if (r.x < 0) return;
-    if (rc.mode == REC_CHECK) vread(P, r.x, r.val, r.T, rc.R, rc.e, rc.conf);
-    else if (rc.mode == REC_LOG) vlog(rc.lg, r.x, r.val, r.T, rc.e, rc.R);
+    if (rc.mode == REC_CHECK) {
+        vread(P, r.x, r.val, r.T, rc.R, rc.e, rc.conf);
+    } else if (rc.mode == REC_LOG) {
+        // named slots, not an array: a lane-dependent index would live in scratch memory
+        if (rc.np == 0) rc.p0 = r;
+        else if (rc.np == 1) rc.p1 = r;
+        else if (rc.np == 2) rc.p2 = r;
+        else if (rc.np == 3) rc.p3 = r;
+        else vlog(rc.lg, r.x, r.val, r.T, rc.e, rc.R);
+        if (rc.np < 4) rc.np++;
+    }
+}
+
+// Appends the records held back to the wave's log (any subset of lanes may
+// hold some).
+__device__ __forceinline__ void flush_records(Recorder& rc) {
+    if (rc.mode != REC_LOG) return;
+    if (rc.np > 0) vlog(rc.lg, rc.p0.x, rc.p0.val, rc.p0.T, rc.e, rc.R);
+    if (rc.np > 1) vlog(rc.lg, rc.p1.x, rc.p1.val, rc.p1.T, rc.e, rc.R);
+    if (rc.np > 2) vlog(rc.lg, rc.p2.x, rc.p2.val, rc.p2.T, rc.e, rc.R);
+    if (rc.np > 3) vlog(rc.lg, rc.p3.x, rc.p3.val, rc.p3.T, rc.e, rc.R);
+    rc.np = 0;
 }
 
 // Round-tagged counter word R<<32 | count.  Rounds only grow, so raising
@@ -759,20 +787,19 @@ __device__ bool cand_ok_v(const BP& P, int32_t x, int32_t w, bool xadj, uint32_t
     return true;
 }
 
-template <bool ALL>
-__device__ __forceinline__ bool cand_ok(const BP& P, int32_t x, int32_t w, bool xadj, Rd& wit, Recorder& rc) {
-    return cand_ok_v<ALL>(P, x, w, xadj, P.g.mc[x], P.g.seedb[x], wit, rc);
-}
-
 // The marks of a speculated flip's neighbours (mark_vertex), issued by an
 // evaluation's pass over the flipped vertex's row from the counts it loads
 // anyway: an evicted w's in part 2 of eval_insertion (`done`: the prefix of
 // the row covered), an admitted deletion endpoint's in eval_deletion.
+// A short row's marks are issued by the caller after the evaluation's last
+// load (`dm`: this lane's neighbour `dx` is to be marked).
 struct SpecMarks {
     u64 mv;
     uint32_t R;
     u64 bound;
     int64_t done;
+    bool dm = false;
+    int32_t dx = -1;
 };
 
 // Insertion with both ends in the MIS, w evicted: first admissible vertex of
@@ -844,10 +871,13 @@ __device__ int32_t eval_insertion_from(const BP& P, int32_t w, uint32_t pw, uint
         int32_t x = -1;
         bool cnd = false, ok = false;
         Rd wt = rd_none();
+        uint32_t mx = 0, sx = 0;
         if (j < L) {
             x = rw.at(j);
             if (x >= 0 && x != w) {
-                const uint32_t px = g.pub[x], mx = g.mc[x], sx = g.seedb[x];
+                const uint32_t px = g.pub[x];
+                mx = g.mc[x];
+                sx = g.seedb[x];
                 if (short_row) {
                     sx1 = x;
                     stag = g.fa_tag[x];
@@ -855,8 +885,13 @@ __device__ int32_t eval_insertion_from(const BP& P, int32_t w, uint32_t pw, uint
                     sfill = g.fa_fill[x];
                 }
                 if (em && (u64)(mx & MF_CMASK) < 2ull + em->bound) {
-                    atomicMax(&g.cmark[x], em->mv);
-                    round_count_add(&g.cdec[x], em->R);
+                    if (short_row) {
+                        em->dm = true;
+                        em->dx = x;
+                    } else {
+                        atomicMax(&g.cmark[x], em->mv);
+                        round_count_add(&g.cdec[x], em->R);
+                    }
                 }
                 if (px == 4u * P.b + 3u) {  // admissible
                     cnd = true;
@@ -869,7 +904,7 @@ __device__ int32_t eval_insertion_from(const BP& P, int32_t w, uint32_t pw, uint
         const int f = bal ? first_lane(bal) : MF_WAVE;
         if (cnd && !ok && lane < f) record(P, rc, wt);  // earlier candidates stay blocked
         if (bal) {
-            if (lane == f) (void)cand_ok<true>(P, x, w, true, wt, rc);
+            if (lane == f) (void)cand_ok_v<true>(P, x, w, true, mx, sx, wt, rc);  // the words it read above
             return __shfl(x, f, MF_WAVE);
         }
     }
@@ -903,6 +938,7 @@ __device__ int32_t eval_insertion_from(const BP& P, int32_t w, uint32_t pw, uint
         int64_t total = __shfl(incl, MF_WAVE - 1, MF_WAVE);
         u64 best = ~0ull;
         int32_t best_x = -1;
+        uint32_t best_m = 0, best_s = 0;  // the best candidate's count and seed words
         for (int64_t k0 = 0; k0 < total; k0 += MF_WAVE) {
             int64_t k = k0 + lane;
             int64_t kk = k < total ? k : total - 1;
@@ -923,6 +959,7 @@ __device__ int32_t eval_insertion_from(const BP& P, int32_t w, uint32_t pw, uint
             int64_t oblen = __shfl(blen, owner, MF_WAVE);
             u64 key = ~0ull;
             int32_t x = -1;
+            uint32_t px = 0, mx = 0, sx = 0;
             if (k < total) {
                 int64_t off = k - oexcl;
                 int64_t pos;
@@ -933,7 +970,6 @@ __device__ int32_t eval_insertion_from(const BP& P, int32_t w, uint32_t pw, uint
                     x = off < oblen ? obase[off] : oadd[off - oblen];
                     pos = off;
                 }
-                uint32_t px = 0, mx = 0, sx = 0;
                 if (x >= 0 && x != w) {
                     px = g.pub[x];
                     mx = g.mc[x];
@@ -950,8 +986,11 @@ __device__ int32_t eval_insertion_from(const BP& P, int32_t w, uint32_t pw, uint
             u64 mn = wave_min_u64(key);
             if (mn < best) {
                 u64 bal = __ballot(key == mn);
+                const int fl = first_lane(bal);
                 best = mn;
-                best_x = __shfl(x, first_lane(bal), MF_WAVE);
+                best_x = __shfl(x, fl, MF_WAVE);
+                best_m = (uint32_t)__shfl((int)mx, fl, MF_WAVE);
+                best_s = (uint32_t)__shfl((int)sx, fl, MF_WAVE);
             }
             if (best != ~0ull) {
                 // stop once every element of the best owner's segment is seen
@@ -963,7 +1002,7 @@ __device__ int32_t eval_insertion_from(const BP& P, int32_t w, uint32_t pw, uint
         if (best != ~0ull) {
             if (lane == 0) {
                 Rd wt;
-                (void)cand_ok<true>(P, best_x, w, false, wt, rc);
+                (void)cand_ok_v<true>(P, best_x, w, false, best_m, best_s, wt, rc);
             }
             return best_x;
         }
@@ -1120,7 +1159,8 @@ __device__ __forceinline__ void mark_vertex(const BP& P, int32_t z, u64 mv, bool
 
 // Speculative evaluation of active edge e (kind k) by one wave, from its
 // endpoints' loads; the reads its outcome rests on go to `rc` (nowhere, or
-// the wave's read log).
+// the wave's read log).  Loads come first, then the marks, the results and
+// the log records (stores and atomics ahead of a load delay its use).
 __device__ __forceinline__ void spec_edge_from(const BP& P, int32_t e, uint8_t k, uint32_t R, u64 bound, int lane,
                                                Recorder& rc, int4* cmw, const EdgeLoads& el) {
     const int32_t a = el.a, bb = el.b;
@@ -1133,25 +1173,30 @@ __device__ __forceinline__ void spec_edge_from(const BP& P, int32_t e, uint8_t k
         const Row rw = wa ? el.ra : el.rb;
         SpecMarks em{mv, R, bound, 0};
         int32_t x = eval_insertion_from(P, w, el.pw, wa ? el.ma : el.mb, rw, rc, &em);
+        if (x >= 0) mark_vertex(P, x, mv, false, R, bound, lane);
+        mark_vertex(P, w, mv, true, R, bound, lane, em.done, &rw);
+        if (em.dm) {
+            atomicMax(&P.g.cmark[em.dx], mv);
+            round_count_add(&P.g.cdec[em.dx], R);
+        }
         if (lane == 0) {
             P.res_w[e] = w;
             P.res_x[e] = x;
             if (cmw) *cmw = make_int4(1, w, x, e);
         }
-        mark_vertex(P, w, mv, true, R, bound, lane, em.done, &rw);
-        if (x >= 0) mark_vertex(P, x, mv, false, R, bound, lane);
     } else {
         const bool ma = (el.ma & MF_MISBIT) != 0;
         int32_t tgt = ma ? bb : a, other = ma ? a : bb;
         const SpecMarks em{mv, R, bound, 0};
         bool ok = eval_deletion_from(P, tgt, other, ma ? el.rb : el.ra, rc, &em);
         if (lane == 0) {
+            if (ok) atomicMax(&P.g.mmark[tgt], mv);  // its neighbours were marked by the evaluation
             P.res_w[e] = -1;
             P.res_x[e] = ok ? tgt : -1;
             if (cmw) *cmw = make_int4(2, -1, ok ? tgt : -1, e);
-            if (ok) atomicMax(&P.g.mmark[tgt], mv);  // its neighbours were marked by the evaluation
         }
     }
+    flush_records(rc);
 }
 
 __device__ __forceinline__ void spec_edge(const BP& P, int32_t e, uint8_t k, uint32_t R, u64 bound, int lane,
@@ -1197,12 +1242,41 @@ __device__ __forceinline__ void apply_flip(const BP& P, int32_t z, bool join, in
     }
 }
 
+// An edge's flips: the evicted w leaves (k == 1), the admitted x joins (x >=
+// 0).  Both rows' bounds and first wave of entries are loaded before any
+// atomic is issued; longer rows continue as apply_flip does.
 __device__ __forceinline__ void commit_flips(const BP& P, uint8_t k, int32_t w, int32_t x, int lane) {
-    if (k == 1) {
-        apply_flip(P, w, false, lane);
-        if (lane == 0) P.g.evb[w] = P.b;
+    const DevGraph& g = P.g;
+    const bool ev = k == 1, ad = x >= 0;
+    Row rw{}, rx{};
+    if (ev) rw = get_row(g, w);
+    if (ad) rx = get_row(g, x);
+    const int64_t Lw = ev ? rw.len() : 0, Lx = ad ? rx.len() : 0;
+    const int32_t yw = lane < Lw ? rw.at(lane) : -1;
+    const int32_t yx = lane < Lx ? rx.at(lane) : -1;
+    if (lane == 0) {
+        if (ev) {
+            atomicAnd(&g.mc[w], MF_CMASK);
+            g.evb[w] = P.b;
+        }
+        if (ad) atomicOr(&g.mc[x], MF_MISBIT);
     }
-    if (x >= 0) apply_flip(P, x, true, lane);
+    if (yw >= 0 && yw != w) {
+        atomicSub(&g.mc[yw], 1u);
+        g.nevb[yw] = P.b;
+    }
+    if (yx >= 0 && yx != x) atomicAdd(&g.mc[yx], 1u);
+    for (int64_t j = MF_WAVE + lane; j < Lw; j += MF_WAVE) {
+        const int32_t y = rw.at(j);
+        if (y >= 0 && y != w) {
+            atomicSub(&g.mc[y], 1u);
+            g.nevb[y] = P.b;
+        }
+    }
+    for (int64_t j = MF_WAVE + lane; j < Lx; j += MF_WAVE) {
+        const int32_t y = rx.at(j);
+        if (y >= 0 && y != x) atomicAdd(&g.mc[y], 1u);
+    }
 }
 
 __device__ __forceinline__ void commit_edge(const BP& P, int32_t e, uint8_t k, int lane) {
@@ -1267,12 +1341,12 @@ __device__ void ph_spec_w(const BP& P, const GC& c, const RoundView& v, int* cur
         // the commit's record of the wave's first edge (wave w of the commit
         // commits edge start + w)
         int4* cmw = first ? P.cm + c.wave : nullptr;
-        if (c.lane == 0) {
+        if (first) hdr = make_uint4((uint32_t)k << 16, (uint32_t)e, (uint32_t)a, (uint32_t)bb);
+        if (k) spec_edge_from(P, (int32_t)e, k, v.R, (u64)P.n_ins, c.lane, rc, cmw, el);
+        if (c.lane == 0) {  // after the evaluation's loads
             P.kind[e] = k;
             if (cmw && !k) *cmw = make_int4(0, -1, -1, (int32_t)e);
         }
-        if (first) hdr = make_uint4((uint32_t)k << 16, (uint32_t)e, (uint32_t)a, (uint32_t)bb);
-        if (k) spec_edge_from(P, (int32_t)e, k, v.R, (u64)P.n_ins, c.lane, rc, cmw, el);
     }
     if (c.lane == 0) {
         const int n = cur[wiw];
@@ -1350,12 +1424,6 @@ __device__ void ph_commit_w(const BP& P, const GC& c, const RoundView& v, int pa
     const int4 m = e0 < v.end ? P.cm[c.wave] : make_int4(0, -1, -1, -1);
     const int64_t conflict = round_conflict(P, par);
     const int64_t lim = conflict < v.end ? conflict : v.end;
-    const int64_t ns = P.window < P.nstage ? P.window : P.nstage;
-    for (int64_t i = c.tid; i < ns; i += c.nthreads) {
-        const int64_t e = lim + i;
-        P.stage[i] = e < P.U ? make_int4(P.bu[e], P.bv[e], (int)((P.b << 1) | (uint32_t)(P.bins[e] != 0)), (int32_t)e)
-                             : make_int4(-1, -1, 0, -1);
-    }
     if (e0 < lim) {
         if (m.w == (int32_t)e0) {
             if (m.x) commit_flips(P, (uint8_t)m.x, m.y, m.z, c.lane);
@@ -1367,6 +1435,13 @@ __device__ void ph_commit_w(const BP& P, const GC& c, const RoundView& v, int pa
     for (int64_t e = e0 + c.nwaves; e < lim; e += c.nwaves) {
         uint8_t k = P.kind[e];
         if (k) commit_edge(P, (int32_t)e, k, c.lane);
+    }
+    // after the commit's own loads
+    const int64_t ns = P.window < P.nstage ? P.window : P.nstage;
+    for (int64_t i = c.tid; i < ns; i += c.nthreads) {
+        const int64_t e = lim + i;
+        P.stage[i] = e < P.U ? make_int4(P.bu[e], P.bv[e], (int)((P.b << 1) | (uint32_t)(P.bins[e] != 0)), (int32_t)e)
+                             : make_int4(-1, -1, 0, -1);
     }
     if (c.tid == 0) advance_round(P, par, conflict, P.window);
 }
@@ -1540,10 +1615,12 @@ __device__ __forceinline__ u64 counter_reset_value(int64_t i) {
     return (i == C_CONF0 || i == C_CONF1) ? ~0ull : 0ull;
 }
 
+// One workgroup of NT threads (a constant, as in grid_coords).
+template <int NT>
 __device__ __forceinline__ GC block_coords() {
     GC c;
     c.tid = threadIdx.x;
-    c.nthreads = blockDim.x;
+    c.nthreads = NT;
     c.wave = c.tid / MF_WAVE;
     c.nwaves = c.nthreads / MF_WAVE;
     c.lane = lane_id();
@@ -1709,7 +1786,7 @@ __device__ __forceinline__ void tail_phases(const BP& P, const GC& c, int par) {
 // when more than MF_HALT_EDGES edges are left, publishing C_HALT = 1, and
 // the host queues more grid-wide rounds and another tail.
 __global__ __launch_bounds__(1024) void k_finish(BP P, int par) {
-    GC c = block_coords();
+    GC c = block_coords<1024>();
     if (P.may_halt) {
         phase_barrier();
         const int64_t start = (int64_t)P.ctr[MF_CS * (C_START0 + par)];
@@ -1724,7 +1801,7 @@ __global__ __launch_bounds__(1024) void k_finish(BP P, int par) {
 }
 
 __global__ __launch_bounds__(256) void k_publish(BP P) {
-    GC c = block_coords();
+    GC c = block_coords<256>();
     publish(P, c, true);
 }
 
@@ -1799,7 +1876,7 @@ __global__ __launch_bounds__(NT) void k_serve(ServeQueue* q, u64* head, u64 idle
     __shared__ BP sP;
     __shared__ int64_t s_item;
     __shared__ int s_whole;
-    GC c = block_coords();
+    GC c = block_coords<NT>();
     while (true) {
         if (c.tid == 0) s_item = serve_claim(q, head, idle_ticks);
         __syncthreads();
