@@ -114,6 +114,18 @@ int misfit_set_mode(misfit_ctx* ctx, int mode);
  * (core/cli_setup.h:77, src/misfit/misfit.cu:822). */
 int misfit_flags(misfit_ctx* ctx, const int32_t* u, const int32_t* v, int64_t count, uint8_t* ins);
 
+/* misfit_flags without any allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (serving loops that parse
+ * batches as they arrive, harnesses that forbid hipMalloc).  The pairs are
+ * staged in a static device array of MISFIT_NOALLOC_MAX_PAIRS pairs (9 MiB,
+ * held by every GPU the library runs on until the process ends); larger
+ * counts return MISFIT_E_ARG.  Calls on one GPU share that array: the
+ * library serialises them within a process (the array keeps the last
+ * call's contents), and two processes each have their own.  Results equal
+ * misfit_flags'. */
+#define MISFIT_NOALLOC_MAX_PAIRS (1 << 20)
+int misfit_flags_noalloc(misfit_ctx* ctx, const int32_t* u, const int32_t* v, int64_t count, uint8_t* ins);
+
 /* Queues one batch: overlay update, exact repair, completion.  Returns at
  * once in queued mode (the host buffer is copied before the call returns).
  * The speculative rounds are queued from the recent rounds per update of

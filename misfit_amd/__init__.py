@@ -25,6 +25,7 @@ BIN_DIR = os.path.join(_HERE, "bin")
 
 MODE_PLAIN = 0
 MODE_QUEUED = 1
+NOALLOC_MAX_PAIRS = 1 << 20  # MISFIT_NOALLOC_MAX_PAIRS
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _i64p = ctypes.POINTER(ctypes.c_int64)
@@ -90,6 +91,7 @@ class EgrReport(ctypes.Structure):
 EXPORTS = (
     "misfit_ctx_create", "misfit_ctx_destroy", "misfit_last_error", "misfit_load_csr",
     "misfit_set_mis", "misfit_get_mis", "misfit_set_rule", "misfit_set_mode", "misfit_flags",
+    "misfit_flags_noalloc",
     "misfit_submit_batch", "misfit_wait", "misfit_process_batch", "misfit_sync", "misfit_check",
     "misfit_compact", "misfit_graph_size", "misfit_export_csr", "misfit_neighbors",
     "misfit_batch_bytes", "misfit_device_info", "misfit_egr_open", "misfit_egr_close", "misfit_write_egr",
@@ -123,6 +125,7 @@ def lib():
         L.misfit_set_rule.argtypes = [vp, ctypes.c_int]
         L.misfit_set_mode.argtypes = [vp, ctypes.c_int]
         L.misfit_flags.argtypes = [vp, _i32p, _i32p, ctypes.c_int64, _u8p]
+        L.misfit_flags_noalloc.argtypes = [vp, _i32p, _i32p, ctypes.c_int64, _u8p]
         L.misfit_submit_batch.argtypes = [vp, vp, ctypes.c_int64, _u64p]
         L.misfit_stage_batch.argtypes = [vp, vp, ctypes.c_int64, ctypes.POINTER(vp)]
         L.misfit_submit_staged.argtypes = [vp, vp, _u64p]
@@ -395,6 +398,16 @@ class Engine:
         ins = np.zeros(max(len(u), 1), np.uint8)
         if len(u):
             self._ck(lib().misfit_flags(self.ctx, _p(u, _i32p), _p(v, _i32p), len(u), _p(ins, _u8p)))
+        return ins[: len(u)]
+
+    def flags_noalloc(self, u, v):
+        """As flags(), staged in the library's static device array: no
+        allocation, at most NOALLOC_MAX_PAIRS pairs."""
+        u = np.ascontiguousarray(u, np.int32)
+        v = np.ascontiguousarray(v, np.int32)
+        ins = np.zeros(max(len(u), 1), np.uint8)
+        if len(u):
+            self._ck(lib().misfit_flags_noalloc(self.ctx, _p(u, _i32p), _p(v, _i32p), len(u), _p(ins, _u8p)))
         return ins[: len(u)]
 
     def submit(self, updates):

@@ -209,6 +209,7 @@ static hipError_t copy_sync(misfit_ctx* c, void* dst, const void* src, size_t by
 static int build_row_index(misfit_ctx* c);
 static void free_compact_scratch(misfit_ctx* c);
 static int settle(misfit_ctx* c);
+static hipError_t resolve_scratch(int device);
 
 template <typename T>
 static hipError_t dalloc(std::vector<void*>& owner, T** p, int64_t count) {
@@ -244,6 +245,11 @@ extern "C" int misfit_ctx_create(int device, misfit_ctx** out) {
     misfit_ctx* c = new misfit_ctx();
     c->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return MISFIT_E_HIP;
+    }
+    if (resolve_scratch(device) != hipSuccess) {  // misfit_flags_noalloc's staging on this device
+        hipStreamDestroy(c->stream);
         delete c;
         return MISFIT_E_HIP;
     }
@@ -773,6 +779,63 @@ extern "C" int misfit_flags(misfit_ctx* c, const int32_t* u, const int32_t* v, i
     mf_free_device(dv);
     mf_free_device(di);
     HIPCHK(c, e);
+    return MISFIT_OK;
+}
+
+// ---------------------------------------------------------------- flags without allocation
+// Staging of misfit_flags_noalloc: the u and v arrays (4 bytes a pair each)
+// and the flag bytes of at most MISFIT_NOALLOC_MAX_PAIRS pairs, 9 MiB that
+// every GPU the library runs on holds until the process ends.  On the host
+// the array's name stands for a host-side stand-in, not device memory, so
+// host code never hands it to a kernel or a copy: it uses the address
+// hipGetSymbolAddress returned on each device, kept in scratch_ptr.
+static constexpr int kScratchDevices = 64;
+static constexpr size_t kScratchPairs = (size_t)MISFIT_NOALLOC_MAX_PAIRS;
+static __device__ __align__(256) unsigned char scratch[kScratchPairs * 9];
+static std::mutex scratch_resolve_mu;
+static unsigned char* scratch_ptr[kScratchDevices];
+static std::mutex scratch_call_mu[kScratchDevices];  // one call at a time per GPU
+
+// With `device` current: this device's address of the staging array, looked
+// up once per device.
+static hipError_t resolve_scratch(int device) {
+    if (device < 0 || device >= kScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(scratch_resolve_mu);
+    if (scratch_ptr[device]) return hipSuccess;
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    scratch_ptr[device] = (unsigned char*)p;
+    return hipSuccess;
+}
+
+extern "C" int misfit_flags_noalloc(misfit_ctx* c, const int32_t* u, const int32_t* v, int64_t count,
+                                    uint8_t* ins) {
+    if (!c || count < 0 || (count && (!u || !v || !ins))) return MISFIT_E_ARG;
+    if (count > (int64_t)kScratchPairs)
+        return fail(c, MISFIT_E_ARG, "misfit_flags_noalloc: " + std::to_string(count) +
+                                         " pairs exceed its static staging of MISFIT_NOALLOC_MAX_PAIRS = " +
+                                         std::to_string(kScratchPairs) + " (use misfit_flags)");
+    if (!c->has_graph) return fail(c, MISFIT_E_STATE, "no graph loaded");
+    if (!count) return MISFIT_OK;
+    int32_t n = c->P.g.n;
+    for (int64_t i = 0; i < count; i++)
+        if (u[i] < 0 || u[i] >= n || v[i] < 0 || v[i] >= n) return fail(c, MISFIT_E_ARG, "vertex out of range");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (int rc_ = settle(c)) return rc_;
+    unsigned char* base = c->device < kScratchDevices ? scratch_ptr[c->device] : nullptr;
+    if (!base) return fail(c, MISFIT_E_STATE, "misfit_flags_noalloc: staging array not resolved on this device");
+    std::lock_guard<std::mutex> lk(scratch_call_mu[c->device]);
+    int32_t* du = (int32_t*)base;
+    int32_t* dv = (int32_t*)(base + kScratchPairs * 4);
+    uint8_t* di = base + kScratchPairs * 8;
+    // every flag byte of the call is written by the kernel, so nothing left
+    // by an earlier call needs clearing
+    HIPCHK(c, copy_sync(c, du, u, (size_t)count * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, copy_sync(c, dv, v, (size_t)count * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, mf_flags(c->S, du, dv, count, di));
+    HIPCHK(c, copy_sync(c, ins, di, (size_t)count, hipMemcpyDeviceToHost));
     return MISFIT_OK;
 }
 
