@@ -93,14 +93,14 @@ struct misfit_ctx {
     int32_t* d_spos = nullptr;
     // compaction scratch, kept between compactions (grown as needed, freed
     // with the graph): per-vertex counts, offsets and row lists, and the
-    // deletion and tail lists of the merge
+    // tail lists of the merge
     std::vector<void*> cs_allocs;
-    int64_t cs_n = 0, cs_d = 0, cs_t = 0;
+    int64_t cs_n = 0, cs_t = 0;
     int64_t *cs_dcnt = nullptr, *cs_tcnt = nullptr, *cs_doff = nullptr, *cs_toff = nullptr;
     int32_t *cs_trows = nullptr, *cs_big = nullptr;
     u64* cs_cnt = nullptr;  // [0] rows with a tail, [1] long lists
-    int32_t *cs_dval = nullptr, *cs_dpos = nullptr, *cs_dsv = nullptr;
     int32_t *cs_tval = nullptr, *cs_tidx = nullptr, *cs_tsv = nullptr, *cs_tsi = nullptr;
+    bool compact_selfcheck = false;  // MISFIT_COMPACT_SELFCHECK=1: full checks of every compaction's output
     void* cs_tmp = nullptr;
     size_t cs_tmp_bytes = 0;
     int64_t ktask_cap = 0;
@@ -263,6 +263,8 @@ extern "C" int misfit_ctx_create(int device, misfit_ctx** out) {
     if (rm) c->round_margin = std::max(0, std::min(16, atoi(rm)));
     const char* ps = getenv("MISFIT_PU_SCAN");
     c->force_pu_scan = ps && strcmp(ps, "full") == 0;
+    const char* sc = getenv("MISFIT_COMPACT_SELFCHECK");
+    c->compact_selfcheck = sc && sc[0] == '1';
     const char* fg = getenv("MISFIT_TEST_FAIL_GROWTH");
     c->fail_growth = fg && fg[0] == '1';
     const char* swf = getenv("MISFIT_SERVE_WINDOW_FACTOR");
@@ -657,7 +659,6 @@ extern "C" int misfit_load_csr(misfit_ctx* c, int32_t n, int64_t m, const void* 
     HIPCHK(c, dalloc(A, &S.d_cnt, n));
     HIPCHK(c, dalloc(A, &S.d_bmark, n));
     HIPCHK(c, dalloc(A, &S.d_deg, n));
-    HIPCHK(c, dalloc(A, &S.d_tomb_list, n));
     HIPCHK(c, dalloc(A, &S.d_scan_part, ((int64_t)n + 1) / 2048 + 2));
     S.walk_task_cap = mf_walk_task_need(n, m);
     HIPCHK(c, dalloc(A, (char**)&S.d_walk_tasks, S.walk_task_cap * 16));
@@ -1448,11 +1449,10 @@ static void free_compact_scratch(misfit_ctx* c) {
     if (c->cs_tmp) mf_free_device(c->cs_tmp);
     c->cs_tmp = nullptr;
     c->cs_tmp_bytes = 0;
-    c->cs_n = c->cs_d = c->cs_t = 0;
+    c->cs_n = c->cs_t = 0;
     c->cs_dcnt = c->cs_tcnt = c->cs_doff = c->cs_toff = nullptr;
     c->cs_trows = c->cs_big = nullptr;
     c->cs_cnt = nullptr;
-    c->cs_dval = c->cs_dpos = c->cs_dsv = nullptr;
     c->cs_tval = c->cs_tidx = c->cs_tsv = c->cs_tsi = nullptr;
 }
 
@@ -1471,9 +1471,9 @@ static void cs_release(misfit_ctx* c, void* p) {
     mf_free_device(p);
 }
 
-// Per-vertex scratch (once per graph) and the deletion and tail lists
-// (grown by half again when short).
-static hipError_t ensure_compact_scratch(misfit_ctx* c, int64_t ndel, int64_t ntail) {
+// Per-vertex scratch (once per graph) and the tail lists (grown by half
+// again when short).
+static hipError_t ensure_compact_scratch(misfit_ctx* c, int64_t ntail) {
     const int64_t n = c->P.g.n;
     hipError_t e = hipSuccess;
     if (c->cs_n != n) {
@@ -1490,18 +1490,6 @@ static hipError_t ensure_compact_scratch(misfit_ctx* c, int64_t ndel, int64_t nt
             return e;
         }
         c->cs_n = n;
-    }
-    if (ndel > c->cs_d) {
-        for (int32_t** p : {&c->cs_dval, &c->cs_dpos, &c->cs_dsv}) {
-            cs_release(c, *p);
-            *p = nullptr;
-        }
-        c->cs_d = 0;
-        const int64_t want = ndel + ndel / 2;
-        for (int32_t** p : {&c->cs_dval, &c->cs_dpos, &c->cs_dsv})
-            if (e == hipSuccess) e = cs_alloc(c, (void**)p, (size_t)want * 4);
-        if (e != hipSuccess) return e;
-        c->cs_d = want;
     }
     if (ntail > c->cs_t) {
         for (int32_t** p : {&c->cs_tval, &c->cs_tidx, &c->cs_tsv, &c->cs_tsi}) {
@@ -1540,28 +1528,32 @@ extern "C" int misfit_compact(misfit_ctx* c, int64_t* new_m) {
     auto keep = [&](void* p) { mine.erase(std::find(mine.begin(), mine.end(), p)); };
     int64_t* nidx = nullptr;
     int32_t* nl = nullptr;
-    int64_t total = 0, ndel = 0, ntail = 0;
+    int64_t total = 0, ntail = 0;
     MergeLists L;
     hipError_t e = grab((void**)&nidx, ((size_t)g.n + 1) * 8);
     if (e == hipSuccess) e = mf_compact_plan(c->S, nidx);
-    // per-row deletion and tail lists, for merging the sorted row copy
-    if (e == hipSuccess) e = ensure_compact_scratch(c, 0, 0);
+    // per-row deletion counts and tail lists; the deletion marks of the old
+    // rows (position order, and sorted order when a sorted copy exists) and
+    // the counting chunks' sums are transient
+    if (e == hipSuccess) e = ensure_compact_scratch(c, 0);
     if (e == hipSuccess)
         e = mf_merge_counts(c->S, c->cs_dcnt, c->cs_tcnt, c->cs_doff, c->cs_toff, c->cs_trows, c->cs_cnt);
     if (e == hipSuccess) e = hipMemcpyAsync(&total, nidx + g.n, 8, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(&ndel, c->cs_doff + g.n, 8, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipMemcpyAsync(&ntail, c->cs_toff + g.n, 8, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e == hipSuccess) e = ensure_compact_scratch(c, ndel, ntail);
-    L.dval = c->cs_dval;
-    L.dpos = c->cs_dpos;
+    if (e == hipSuccess) e = ensure_compact_scratch(c, ntail);
+    const int64_t nwords = mf_tomb_words(g.m), nchunks = mf_tomb_chunks(g.m);
+    if (e == hipSuccess) e = grab((void**)&L.tw, (size_t)nwords * sizeof(TombWord));
+    if (e == hipSuccess && g.sval) e = grab((void**)&L.tws, (size_t)nwords * sizeof(TombWord));
+    if (e == hipSuccess) e = grab((void**)&L.ccnt, (size_t)nchunks * 8);
+    if (e == hipSuccess) e = grab((void**)&L.cpre, ((size_t)nchunks + 1) * 8);
+    if (e == hipSuccess) e = grab((void**)&L.cpart, ((size_t)nchunks / 2048 + 2) * 8);
     L.doff = c->cs_doff;
     L.tval = c->cs_tval;
     L.tidx = c->cs_tidx;
     L.toff = c->cs_toff;
     L.trows = c->cs_trows;
     L.ntrows = c->cs_cnt;
-    L.dsv = c->cs_dsv;
     L.tsv = c->cs_tsv;
     L.tsi = c->cs_tsi;
     L.big = c->cs_big;
@@ -1569,15 +1561,10 @@ extern "C" int misfit_compact(misfit_ctx* c, int64_t* new_m) {
     L.tmp = &c->cs_tmp;
     L.tmp_bytes = &c->cs_tmp_bytes;
     if (e == hipSuccess) e = grab((void**)&nl, (size_t)total * 4);
-    if (e == hipSuccess) e = mf_compact_copy(c->S, nidx, nl, L);
-    // rows are base order then insertion order, so they may be unsorted now
-    GraphState NS = c->S;
-    NS.g.nindex = nidx;
-    NS.g.nlist = nl;
-    NS.g.m = total;
+    if (e == hipSuccess) e = mf_compact_copy(c->S, nidx, nl, L, c->d_flags);
+    // the new rows are ascending when the old ones were and every tail
+    // continues its row in ascending order (k_copy_runs tests that)
     int flags = 0;
-    if (e == hipSuccess) e = mf_validate_csr(NS, c->d_flags);
-    if (e == hipSuccess) e = mf_validate_entries(NS, c->d_flags);
     if (e == hipSuccess) e = hipMemcpyAsync(&flags, c->d_flags, sizeof(int), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
@@ -1585,6 +1572,25 @@ extern "C" int misfit_compact(misfit_ctx* c, int64_t* new_m) {
         HIPCHK(c, e);
     }
     const bool sorted = (flags & 1) == 0;
+    GraphState NS = c->S;
+    NS.g.nindex = nidx;
+    NS.g.nlist = nl;
+    NS.g.m = total;
+    if (c->compact_selfcheck) {  // the new rows' offsets, entries and order, against the flag above
+        int vflags = 0;
+        e = mf_validate_csr(NS, c->d_flags);
+        if (e == hipSuccess) e = mf_validate_entries(NS, c->d_flags);
+        if (e == hipSuccess) e = hipMemcpyAsync(&vflags, c->d_flags, sizeof(int), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) {
+            drop_all();
+            HIPCHK(c, e);
+        }
+        if ((vflags & ~1) || (sorted && (vflags & 1))) {
+            drop_all();
+            return fail(c, MISFIT_E_CHECK, "compacted rows failed the self-check (flags " + std::to_string(vflags) + ")");
+        }
+    }
     int32_t *nsval = nullptr, *nspos = nullptr;
     bool merged = false;
     if (!sorted && total > 0) {
@@ -1592,7 +1598,7 @@ extern "C" int misfit_compact(misfit_ctx* c, int64_t* new_m) {
         if (e == hipSuccess) e = grab((void**)&nspos, (size_t)total * 4);
         if (e == hipSuccess) {
             mf_fill_i32((int32_t*)c->d_flags, 0, 1, st);
-            e = mf_merge_row_index(c->S, L, ndel, ntail, nidx, nl, nsval, nspos, c->d_flags);
+            e = mf_merge_row_index(c->S, L, ntail, nidx, nl, nsval, nspos, c->d_flags, c->compact_selfcheck);
             if (e == hipSuccess) e = hipMemcpyAsync(&flags, c->d_flags, sizeof(int), hipMemcpyDeviceToHost, st);
             if (e == hipSuccess) e = hipStreamSynchronize(st);
             if (e == hipSuccess) merged = true;

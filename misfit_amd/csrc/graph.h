@@ -10,7 +10,6 @@ struct GraphState {
     uint32_t* d_cnt;         // n: recount scratch
     uint32_t* d_bmark;       // n: byte-measurement marks
     int64_t* d_deg;          // n: compaction degrees
-    int32_t* d_tomb_list;    // n: rows with deletions
     int64_t* d_scan_part;    // scan block sums
     void* d_walk_tasks;      // edge-walk split tasks
     u64* d_walk_ntasks;
@@ -37,22 +36,37 @@ hipError_t mf_get_mis(GraphState& S, uint8_t* d_mis);
 hipError_t mf_flags(GraphState& S, const int32_t* d_u, const int32_t* d_v, int64_t count, uint8_t* d_ins);
 hipError_t mf_check(GraphState& S, u64* d_acc);
 hipError_t mf_compact_plan(GraphState& S, int64_t* d_newidx);
-// Per-row lists a compaction hands to the merge of the sorted row copy
-// (all null: not collected).  doff/toff: n+1 offsets (exclusive scans of
-// the deleted and live-overlay counts per row).
+// Deletion marks of 64 consecutive entries of an array of rows (old entries
+// in row order, or the old sorted copy): bit i set = entry 64w+i is a
+// deleted base edge; pre = deleted entries before entry 64w in the whole
+// array.  pre - (deleted entries in rows before x) = deleted entries of row
+// x before the word, so one 16-byte load answers "is entry p deleted" and
+// "how many deletions precede it in its row".
+struct __align__(16) TombWord {
+    u64 mask;
+    u64 pre;
+};
+// words covering m entries (one past the end, so the end of the last row has a word)
+__host__ __device__ inline int64_t mf_tomb_words(int64_t m) { return m / 64 + 1; }
+// entries per counting chunk (a wave's task; a multiple of 64)
+#define MF_TCHUNK 8192
+__host__ __device__ inline int64_t mf_tomb_chunks(int64_t m) { return (mf_tomb_words(m) * 64 + MF_TCHUNK - 1) / MF_TCHUNK; }
+
+// What a compaction hands from the copy to the merge of the sorted row copy.
 struct MergeLists {
-    int32_t* dval = nullptr;   // deleted entries' values, row order
-    int32_t* dpos = nullptr;   // ... their positions in the old row (ascending per row)
-    const int64_t* doff = nullptr;
-    int32_t* tval = nullptr;   // live overlay entries, insertion order
-    int32_t* tidx = nullptr;   // ... their index in that order
-    const int64_t* toff = nullptr;
-    const int32_t* trows = nullptr;  // rows with a tail (rows with deletions: the tomb list)
+    const int64_t* doff = nullptr;   // n+1: deleted base entries in rows before x (exclusive scan)
+    int32_t* tval = nullptr;         // live overlay entries, insertion order
+    int32_t* tidx = nullptr;         // ... their index in that order
+    const int64_t* toff = nullptr;   // n+1: offsets of the rows' tails
+    const int32_t* trows = nullptr;  // rows with a tail
     const u64* ntrows = nullptr;
-    // merge scratch: sorted deletion values, sorted tail values and their
-    // insertion indices, rows whose list is long (n), a counter, and the
-    // library sort's temporary storage (grown as needed)
-    int32_t* dsv = nullptr;
+    TombWord* tw = nullptr;          // deletion marks of the old rows (position order)
+    TombWord* tws = nullptr;         // ... of the old sorted copy (null: rows were ascending, use tw)
+    int64_t* ccnt = nullptr;         // per counting chunk: deletions in it
+    int64_t* cpre = nullptr;         // ... exclusive scan (chunks + 1)
+    int64_t* cpart = nullptr;        // scan scratch
+    // tail sort: sorted tail values and their insertion indices, rows whose
+    // tail is long, a counter, and the library sort's temporary storage
     int32_t* tsv = nullptr;
     int32_t* tsi = nullptr;
     int32_t* big = nullptr;
@@ -60,7 +74,12 @@ struct MergeLists {
     void** tmp = nullptr;
     size_t* tmp_bytes = nullptr;
 };
-hipError_t mf_compact_copy(GraphState& S, const int64_t* d_newidx, int32_t* d_out, const MergeLists& L);
+// Deletion marks of the old rows (L.tw), then the kept base entries of every
+// row into d_out at their compacted positions, then the live overlay
+// entries after them (collected into L.tval/L.tidx).  flags |= 1 when a
+// compacted row may not be ascending (the old rows were not, or a row's
+// tail does not continue it in ascending order).
+hipError_t mf_compact_copy(GraphState& S, const int64_t* d_newidx, int32_t* d_out, const MergeLists& L, int* d_flags);
 hipError_t mf_batch_bytes(GraphState& S, const int32_t* d_seeds, const u64* d_nseeds, uint32_t t, u64* d_acc);
 hipError_t mf_widen(const int32_t* d_in, int64_t* d_out, int64_t count, hipStream_t st);
 hipError_t mf_validate_csr(GraphState& S, int* d_flags);
@@ -72,9 +91,10 @@ hipError_t mf_build_row_index(GraphState& S, int32_t* sval, int32_t* spos, int* 
 hipError_t mf_merge_counts(GraphState& S, int64_t* dcnt, int64_t* tcnt, int64_t* doff, int64_t* toff, int32_t* trows,
                            u64* ntrows);
 // sorted copy of the compacted rows nidx/nnl by merging the old graph's (S) sorted order with the
-// lists compaction collected; same flags as mf_build_row_index
-hipError_t mf_merge_row_index(GraphState& S, const MergeLists& L, int64_t ndel, int64_t ntail, const int64_t* nidx,
-                              const int32_t* nnl, int32_t* nsval, int32_t* nspos, int* d_flags);
+// tails compaction collected; with `check`, the result is checked entry by entry (same flags as
+// mf_build_row_index)
+hipError_t mf_merge_row_index(GraphState& S, const MergeLists& L, int64_t ntail, const int64_t* nidx,
+                              const int32_t* nnl, int32_t* nsval, int32_t* nspos, int* d_flags, bool check);
 void mf_scan_excl(const int64_t* in, int64_t count, int64_t* out, int64_t* part, hipStream_t st);
 void mf_fill_u32(uint32_t* p, uint32_t v, int64_t count, hipStream_t st);
 void mf_fill_i32(int32_t* p, int32_t v, int64_t count, hipStream_t st);

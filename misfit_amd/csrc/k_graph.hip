@@ -381,112 +381,199 @@ void mf_scan_excl(const int64_t* in, int64_t count, int64_t* out, int64_t* part,
 }
 
 // ---------------------------------------------------------------- compaction
-// New row lengths, and the rows with deletions listed.  Appends to one
-// counter serialise across the chip even one per wave, so each workgroup
-// counts its rows in LDS, reserves its share with one atomic and fills it
-// from an LDS cursor (list order is free: each row is copied on its own).
-__global__ __launch_bounds__(256) void k_newdeg(DevGraph g, int64_t* deg, int32_t* tomb_list, u64* ntomb) {
-    __shared__ unsigned int cnt, cur;
-    __shared__ u64 base;
-    if (threadIdx.x == 0) {
-        cnt = 0;
-        cur = 0;
-    }
-    __syncthreads();
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    unsigned int mine = 0;
-    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < g.n; x += stride) {
-        int32_t t = g.ov_tomb[x];
-        deg[x] = (g.nindex[x + 1] - g.nindex[x]) - t + g.ov_alive[x];
-        mine += t ? 1u : 0u;
-    }
-    if (mine) atomicAdd(&cnt, mine);
-    __syncthreads();
-    if (threadIdx.x == 0 && cnt) base = atomicAdd(ntomb, (u64)cnt);
-    __syncthreads();
-    if (!cnt) return;
-    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < g.n; x += stride)
-        if (g.ov_tomb[x]) tomb_list[base + atomicAdd(&cur, 1u)] = (int32_t)x;
+// New row lengths: old length - deleted base entries + live overlay entries.
+__global__ __launch_bounds__(256) void k_newdeg(DevGraph g, int64_t* deg) {
+    for (int64_t x = (int64_t)blockIdx.x * MF_BLOCK + threadIdx.x; x < g.n; x += (int64_t)gridDim.x * MF_BLOCK)
+        deg[x] = (g.nindex[x + 1] - g.nindex[x]) - g.ov_tomb[x] + g.ov_alive[x];
 }
 
-// rows without deletions: straight copy, balanced by entries (a row with
-// deletions gets -1 and is left to k_copy_tomb)
-__global__ __launch_bounds__(256) void k_copy_clean(DevGraph g, const int64_t* nidx, int32_t* out) {
-    entry_walk(
-        g.nindex, 0, g.n, [&](int64_t x) { return g.ov_tomb[x] == 0 ? nidx[x] : (int64_t)-1; },
-        [&](int64_t p, int64_t, int64_t start, int64_t dst) {
-            if (dst >= 0) out[dst + (p - start)] = g.nlist[p];
-        });
+__device__ __forceinline__ u64 below_mask(int64_t p) { return (1ull << (p & 63)) - 1ull; }
+
+// deleted entries of the array before entry p
+__device__ __forceinline__ int64_t tomb_rank(const TombWord* tw, int64_t p) {
+    const TombWord t = tw[p >> 6];
+    return (int64_t)t.pre + __popcll(t.mask & below_mask(p));
 }
 
-// rows with deletions: order-preserving filter, one wave per row; with
-// L.dval, the deleted entries (value and position in the old row, in row
-// order) also go to the row's slice of the deletion lists
-__global__ __launch_bounds__(256) void k_copy_tomb(DevGraph g, const int64_t* nidx, int32_t* out,
-                                                   const int32_t* tomb_list, const u64* ntomb, MergeLists L) {
+// Deletion marks of the old rows in position order (a deleted base edge is
+// stored as ~v): one wave per counting chunk, a ballot per word, `pre`
+// counted from the chunk's start (k_words_fix adds the chunks before), and
+// the chunk's count.  Words are loaded 16 at a time so each wave keeps
+// 16 loads in flight.
+#define MF_TW_UNROLL 16
+__global__ __launch_bounds__(256) void k_tomb_words(const int32_t* nlist, int64_t m, TombWord* tw, int64_t nchunks,
+                                                    int64_t* ccnt) {
     GC c = grid_coords();
-    int64_t nt = (int64_t)*ntomb;
-    for (int64_t i = c.wave; i < nt; i += c.nwaves) {
-        int32_t x = tomb_list[i];
-        int64_t pb = g.nindex[x], pe = g.nindex[x + 1];
-        int64_t dst = nidx[x];
-        int64_t ddst = L.dval ? L.doff[x] : 0;
-        for (int64_t p0 = pb; p0 < pe; p0 += MF_WAVE) {
-            int64_t p = p0 + c.lane;
-            int32_t y = p < pe ? g.nlist[p] : 0;
-            bool keep = p < pe && y >= 0;
-            bool del = p < pe && y < 0;
-            u64 bal = __ballot(keep);
-            u64 bald = __ballot(del);
-            u64 below = (1ull << c.lane) - 1ull;
-            if (keep) out[dst + __popcll(bal & below)] = y;
-            if (del && L.dval) {
-                int64_t k = ddst + __popcll(bald & below);
-                L.dval[k] = ~y;
-                L.dpos[k] = (int32_t)(p - pb);
+    const int64_t nw = mf_tomb_words(m);
+    for (int64_t t = c.wave; t < nchunks; t += c.nwaves) {
+        const int64_t w0 = t * (MF_TCHUNK / 64);
+        const int64_t w1 = w0 + MF_TCHUNK / 64 < nw ? w0 + MF_TCHUNK / 64 : nw;
+        u64 run = 0;
+        for (int64_t w = w0; w < w1; w += MF_TW_UNROLL) {
+            int32_t y[MF_TW_UNROLL];
+#pragma unroll
+            for (int j = 0; j < MF_TW_UNROLL; j++) {
+                const int64_t p = (w + j) * 64 + c.lane;
+                y[j] = (w + j < w1 && p < m) ? nlist[p] : 0;
             }
-            dst += __popcll(bal);
-            ddst += __popcll(bald);
+#pragma unroll
+            for (int j = 0; j < MF_TW_UNROLL; j++) {
+                const u64 mask = __ballot(y[j] < 0);
+                if (c.lane == 0 && w + j < w1) tw[w + j] = TombWord{mask, run};
+                run += __popcll(mask);
+            }
         }
+        if (c.lane == 0) ccnt[t] = (int64_t)run;
     }
+}
+
+// Deletion marks of the old sorted copy (entry p of row x is deleted when
+// its row position spos[p] is): the chunk's entries are walked in order
+// with their rows found as in entry_walk, each iteration's ballot is merged
+// into the aligned word it covers (an iteration of at most 64 contiguous
+// entries straddles at most one word boundary), and every word of the
+// chunk is written, the one past the last entry included.
+__global__ __launch_bounds__(256) void k_tomb_words_sorted(const int64_t* nindex, const int32_t* spos, int32_t n,
+                                                           const TombWord* tw, TombWord* tws, int64_t nchunks,
+                                                           int64_t* ccnt) {
+    GC c = grid_coords();
+    const int64_t m = nindex[n];
+    const int64_t nw = mf_tomb_words(m);
+    for (int64_t t = c.wave; t < nchunks; t += c.nwaves) {
+        const int64_t tb = t * MF_TCHUNK;
+        const int64_t w1 = (t + 1) * (MF_TCHUNK / 64) < nw ? (t + 1) * (MF_TCHUNK / 64) : nw;
+        const int64_t tend = tb + MF_TCHUNK < m ? tb + MF_TCHUNK : m;
+        int64_t wcur = tb >> 6;
+        u64 mcur = 0, run = 0;
+        if (tb < tend) {
+            int64_t lo = 0, hi = n - 1;  // largest row x with nindex[x] <= tb
+            while (lo < hi) {
+                int64_t mid = (lo + hi + 1) >> 1;
+                if (nindex[mid] <= tb) lo = mid; else hi = mid - 1;
+            }
+            int64_t x = lo, p = tb;
+            while (p < tend && x < n) {
+                const int64_t myv = x + c.lane;
+                const int64_t off = myv < n ? nindex[myv] : INT64_MAX;
+                const int64_t xw = x + MF_WAVE < n ? x + MF_WAVE : n;
+                const int64_t wend = nindex[xw];
+                const int64_t lim = wend < tend ? wend : tend;
+                for (int64_t p0 = p; p0 < lim; p0 += MF_WAVE) {
+                    const int64_t pp = p0 + c.lane;
+                    const int64_t q0 = pp < lim ? pp : lim - 1;
+                    int l = 0, h = MF_WAVE - 1;
+#pragma unroll
+                    for (int s = 0; s < 6; s++) {
+                        int mid = (l + h + 1) >> 1;
+                        int64_t om = __shfl(off, mid, MF_WAVE);
+                        if (om <= q0) l = mid; else h = mid - 1;
+                    }
+                    const int64_t start = __shfl(off, l, MF_WAVE);
+                    bool del = false;
+                    if (pp < lim) {
+                        const int64_t a = start + spos[pp];
+                        del = (tw[a >> 6].mask >> (a & 63)) & 1ull;
+                    }
+                    const u64 bal = __ballot(del);
+                    const int cnt = lim - p0 < MF_WAVE ? (int)(lim - p0) : MF_WAVE;
+                    const int sh = (int)(p0 & 63);
+                    mcur |= bal << sh;
+                    if (sh + cnt >= 64) {  // the word is complete
+                        if (c.lane == 0) tws[wcur] = TombWord{mcur, run};
+                        run += __popcll(mcur);
+                        wcur++;
+                        mcur = sh ? (bal >> (64 - sh)) : 0ull;
+                    }
+                }
+                if (lim > p) p = lim;
+                x = xw;
+            }
+        }
+        for (; wcur < w1; wcur++) {  // the last, partial word and any word past the end
+            if (c.lane == 0) tws[wcur] = TombWord{mcur, run};
+            run += __popcll(mcur);
+            mcur = 0;
+        }
+        if (c.lane == 0) ccnt[t] = (int64_t)run;
+    }
+}
+
+__global__ void k_words_fix(TombWord* tw, int64_t nw, const int64_t* cpre) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += (int64_t)gridDim.x * blockDim.x)
+        tw[i].pre += (u64)cpre[i / (MF_TCHUNK / 64)];
+}
+
+// Marks of an array of m entries, with `pre` global (tw: position order
+// from nlist; spos given: the sorted copy's order, from the position marks).
+static void tomb_marks(GraphState& S, const int32_t* spos, const TombWord* tw_pos, TombWord* out, const MergeLists& L) {
+    const int64_t m = S.g.m;
+    const int64_t nch = mf_tomb_chunks(m);
+    const int64_t nw = mf_tomb_words(m);
+    const int B = (int)std::min<int64_t>(S.blocks, (nch + 3) / 4);
+    if (spos) k_tomb_words_sorted<<<B, 256, 0, S.stream>>>(S.g.nindex, spos, S.g.n, tw_pos, out, nch, L.ccnt);
+    else k_tomb_words<<<B, 256, 0, S.stream>>>(S.g.nlist, m, out, nch, L.ccnt);
+    mf_scan_excl(L.ccnt, nch, L.cpre, L.cpart, S.stream);
+    k_words_fix<<<S.blocks, 256, 0, S.stream>>>(out, nw, L.cpre);
+}
+
+// Every kept base entry to its compacted position: row x starts at nidx[x],
+// entry p keeps its place minus the deletions before it in the row, i.e.
+// dst = nidx[x] + doff[x] + (p - start) - (deletions before p in the array).
+// Balanced by entries over all rows, hub rows spread over many waves.
+__global__ __launch_bounds__(256) void k_copy_base(DevGraph g, const int64_t* nidx, const int64_t* doff,
+                                                   const TombWord* tw, int32_t* out) {
+    entry_walk(
+        g.nindex, 0, g.n, [&](int64_t x) { return nidx[x] + doff[x]; },
+        [&](int64_t p, int64_t, int64_t start, int64_t r) {
+            const int32_t y = g.nlist[p];
+            if (y >= 0) out[r + (p - start) - tomb_rank(tw, p)] = y;
+        });
 }
 
 // live overlay entries follow the base part, in insertion order; with
 // L.tval, they also go to the row's slice of the tail lists with their
-// insertion index
-__global__ void k_copy_runs(DevGraph g, const int64_t* nidx, int32_t* out, MergeLists L) {
+// insertion index.  With check_order, a row whose entries are not strictly
+// ascending after its tail was appended sets flags |= 1 (its kept part is
+// ascending when the old rows were, so only the tail and its first entry's
+// step need testing).
+__global__ void k_copy_runs(DevGraph g, const int64_t* nidx, int32_t* out, MergeLists L, int check_order, int* flags) {
+    bool unsorted = false;
     for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < g.n; x += (int64_t)gridDim.x * blockDim.x) {
         int32_t len = g.ov_len[x];
         if (!len) continue;
-        int64_t dst = nidx[x] + (g.nindex[x + 1] - g.nindex[x]) - g.ov_tomb[x];
+        const int64_t row0 = nidx[x];
+        int64_t dst = row0 + (g.nindex[x + 1] - g.nindex[x]) - g.ov_tomb[x];
         int64_t t0 = L.tval ? L.toff[x] : 0;
+        int32_t prev = (check_order && dst > row0) ? out[dst - 1] : -1;
         int32_t j = 0;
         const int32_t* r = g.pool + g.ov_start[x];
-        for (int32_t i = 0; i < len; i++)
-            if (r[i] >= 0) {
-                out[dst++] = r[i];
-                if (L.tval) {
-                    L.tval[t0 + j] = r[i];
-                    L.tidx[t0 + j] = j;
-                }
-                j++;
+        for (int32_t i = 0; i < len; i++) {
+            const int32_t y = r[i];
+            if (y < 0) continue;
+            out[dst++] = y;
+            if (y <= prev) unsorted = true;
+            prev = y;
+            if (L.tval) {
+                L.tval[t0 + j] = y;
+                L.tidx[t0 + j] = j;
             }
+            j++;
+        }
     }
+    if (check_order && unsorted) atomicOr(flags, 1);
 }
 
 hipError_t mf_compact_plan(GraphState& S, int64_t* d_newidx) {
-    const int B = S.blocks;
-    k_fill<u64><<<1, 64, 0, S.stream>>>(S.d_walk_ntasks, 0ull, 1);
-    k_newdeg<<<B, 256, 0, S.stream>>>(S.g, S.d_deg, S.d_tomb_list, S.d_walk_ntasks);
+    k_newdeg<<<S.blocks, 256, 0, S.stream>>>(S.g, S.d_deg);
     mf_scan_excl(S.d_deg, S.g.n, d_newidx, S.d_scan_part, S.stream);
     return hipGetLastError();
 }
 
-hipError_t mf_compact_copy(GraphState& S, const int64_t* d_newidx, int32_t* d_out, const MergeLists& L) {
-    const int B = S.blocks;
-    k_copy_clean<<<B, 256, 0, S.stream>>>(S.g, d_newidx, d_out);
-    k_copy_tomb<<<B, 256, 0, S.stream>>>(S.g, d_newidx, d_out, S.d_tomb_list, S.d_walk_ntasks, L);
-    k_copy_runs<<<B, 256, 0, S.stream>>>(S.g, d_newidx, d_out, L);
+hipError_t mf_compact_copy(GraphState& S, const int64_t* d_newidx, int32_t* d_out, const MergeLists& L, int* d_flags) {
+    tomb_marks(S, nullptr, nullptr, L.tw, L);
+    k_copy_base<<<S.blocks, 256, 0, S.stream>>>(S.g, d_newidx, L.doff, L.tw, d_out);
+    k_fill<int><<<1, 64, 0, S.stream>>>(d_flags, S.g.sorted ? 0 : 1, 1);
+    k_copy_runs<<<S.blocks, 256, 0, S.stream>>>(S.g, d_newidx, d_out, L, S.g.sorted ? 1 : 0, d_flags);
     return hipGetLastError();
 }
 
@@ -628,6 +715,9 @@ __global__ __launch_bounds__(256) void k_check_row_index(const int64_t* nindex, 
 //     #{old values < w} - #{deleted values < w} + j, new position |K| + i.
 // Tail values never occur in the old row (re-inserting a deleted base edge
 // restores it in place), so all values of the new row are distinct ranks.
+// The deletion counts come from the deletion marks: in position order for
+// #{deleted positions < q}, in sorted order for #{deleted values < v} (the
+// same marks when rows were ascending), each one load and a popcount.
 struct OldRows {
     const int64_t* nindex;
     const int32_t* nlist;  // with deletions as ~v
@@ -649,13 +739,14 @@ __device__ __forceinline__ int64_t lower32(const int32_t* a, int64_t lo, int64_t
     return lo;
 }
 
-// Entries of the old rows in balanced chunks (as entry_walk), with each
-// row's list offsets and destination fetched once per 64-row window by the
+// Entries of the old sorted order in balanced chunks (as entry_walk), with
+// each row's offsets and destination fetched once per 64-row window by the
 // lane that owns the row and shuffled to its entries.
-__global__ __launch_bounds__(256) void k_merge_base(OldRows o, int32_t n, MergeLists L, const int32_t* dsv,
-                                                    const int32_t* tsv, const int64_t* nidx, int32_t* nsval,
-                                                    int32_t* nspos) {
+__global__ __launch_bounds__(256) void k_merge_base(OldRows o, int32_t n, MergeLists L, const int32_t* tsv,
+                                                    const int64_t* nidx, int32_t* nsval, int32_t* nspos) {
     GC c = grid_coords();
+    const TombWord* tw = L.tw;
+    const TombWord* tws = o.sval ? L.tws : L.tw;
     const int64_t pb = o.nindex[0], pe = o.nindex[n];
     const int64_t ntask = (pe - pb + MF_ECHUNK - 1) / MF_ECHUNK;
     for (int64_t t = c.wave; t < ntask; t += c.nwaves) {
@@ -670,11 +761,10 @@ __global__ __launch_bounds__(256) void k_merge_base(OldRows o, int32_t n, MergeL
         while (p < tend && x < n) {
             const int64_t myv = x + c.lane;
             int64_t off = INT64_MAX, d0 = 0, t0 = 0, dst = 0;
-            int32_t dn = 0, tn = 0;
+            int32_t tn = 0;
             if (myv < n) {
                 off = o.nindex[myv];
                 d0 = L.doff[myv];
-                dn = (int32_t)(L.doff[myv + 1] - d0);
                 t0 = L.toff[myv];
                 tn = (int32_t)(L.toff[myv + 1] - t0);
                 dst = nidx[myv];
@@ -695,29 +785,30 @@ __global__ __launch_bounds__(256) void k_merge_base(OldRows o, int32_t n, MergeL
                 const int64_t start = __shfl(off, l, MF_WAVE);
                 const int64_t rd0 = __shfl(d0, l, MF_WAVE), rt0 = __shfl(t0, l, MF_WAVE);
                 const int64_t rdst = __shfl(dst, l, MF_WAVE);
-                const int32_t rdn = __shfl(dn, l, MF_WAVE), rtn = __shfl(tn, l, MF_WAVE);
+                const int32_t rtn = __shfl(tn, l, MF_WAVE);
                 if (pp >= lim) continue;
                 const int64_t k = pp - start;
-                int64_t q;
+                int64_t q, dpos, dval;
                 int32_t v;
                 if (o.sval) {
                     q = o.spos[pp];
-                    if (o.nlist[start + q] < 0) continue;  // deleted
+                    const int64_t a = start + q;
+                    const TombWord w = tw[a >> 6];
+                    if ((w.mask >> (a & 63)) & 1ull) continue;  // deleted
                     v = o.sval[pp];
+                    dpos = (int64_t)w.pre + __popcll(w.mask & below_mask(a)) - rd0;
+                    dval = tomb_rank(tws, pp) - rd0;
                 } else {
                     const int32_t e = o.nlist[pp];
                     if (e < 0) continue;
                     v = e;
                     q = k;
+                    dpos = dval = tomb_rank(tw, pp) - rd0;
                 }
-                int64_t at = k, np = q;
-                if (rdn) {
-                    at -= lower32(dsv, rd0, rd0 + rdn, v) - rd0;
-                    np -= lower32(L.dpos, rd0, rd0 + rdn, (int32_t)q) - rd0;
-                }
+                int64_t at = k - dval;
                 if (rtn) at += lower32(tsv, rt0, rt0 + rtn, v) - rt0;
                 nsval[rdst + at] = v;
-                nspos[rdst + at] = (int32_t)np;
+                nspos[rdst + at] = (int32_t)(q - dpos);
             }
             if (lim > p) p = lim;
             x = xw;
@@ -725,9 +816,10 @@ __global__ __launch_bounds__(256) void k_merge_base(OldRows o, int32_t n, MergeL
     }
 }
 
-__global__ __launch_bounds__(256) void k_merge_tail(OldRows o, int32_t n, MergeLists L, const int32_t* dsv,
-                                                    const int32_t* tsv, const int32_t* tsi, const int64_t* nidx,
-                                                    int32_t* nsval, int32_t* nspos) {
+__global__ __launch_bounds__(256) void k_merge_tail(OldRows o, int32_t n, MergeLists L, const int32_t* tsv,
+                                                    const int32_t* tsi, const int64_t* nidx, int32_t* nsval,
+                                                    int32_t* nspos) {
+    const TombWord* tws = o.sval ? L.tws : L.tw;
     entry_walk(
         L.toff, 0, n, [](int64_t) { return (int64_t)0; },
         [&](int64_t p, int64_t x, int64_t start, int64_t) {
@@ -740,7 +832,7 @@ __global__ __launch_bounds__(256) void k_merge_tail(OldRows o, int32_t n, MergeL
                 else hi = mid;
             }
             const int64_t d0 = L.doff[x], d1 = L.doff[x + 1];
-            const int64_t at = (lo - b) - (lower32(dsv, d0, d1, w) - d0) + (p - start);
+            const int64_t at = (lo - b) - (tomb_rank(tws, lo) - d0) + (p - start);
             const int64_t kept = (e - b) - (d1 - d0);
             nsval[nidx[x] + at] = w;
             nspos[nidx[x] + at] = (int32_t)(kept + tsi[p]);
@@ -961,19 +1053,18 @@ hipError_t sort_lists(GraphState& S, const int64_t* off, const int32_t* rows, co
 }
 }  // namespace
 
-hipError_t mf_merge_row_index(GraphState& S, const MergeLists& L, int64_t ndel, int64_t ntail, const int64_t* nidx,
-                              const int32_t* nnl, int32_t* nsval, int32_t* nspos, int* d_flags) {
+hipError_t mf_merge_row_index(GraphState& S, const MergeLists& L, int64_t ntail, const int64_t* nidx,
+                              const int32_t* nnl, int32_t* nsval, int32_t* nspos, int* d_flags, bool check) {
     const DevGraph& g = S.g;
     hipStream_t st = S.stream;
-    if (ndel > (int64_t)INT32_MAX || ntail > (int64_t)INT32_MAX) return hipErrorInvalidValue;
-    // rows with deletions are the compaction's tomb list
-    hipError_t e = sort_lists(S, L.doff, S.d_tomb_list, S.d_walk_ntasks, ndel, L.dval, nullptr, L.dsv, nullptr, L);
-    if (e == hipSuccess) e = sort_lists(S, L.toff, L.trows, L.ntrows, ntail, L.tval, L.tidx, L.tsv, L.tsi, L);
+    if (ntail > (int64_t)INT32_MAX) return hipErrorInvalidValue;
+    if (g.sval) tomb_marks(S, g.spos, L.tw, L.tws, L);  // deletion marks in sorted order
+    hipError_t e = sort_lists(S, L.toff, L.trows, L.ntrows, ntail, L.tval, L.tidx, L.tsv, L.tsi, L);
     if (e == hipSuccess) {
         OldRows o{g.nindex, g.nlist, g.sval, g.spos};
-        k_merge_base<<<S.blocks, 256, 0, st>>>(o, g.n, L, L.dsv, L.tsv, nidx, nsval, nspos);
-        if (ntail) k_merge_tail<<<S.blocks, 256, 0, st>>>(o, g.n, L, L.dsv, L.tsv, L.tsi, nidx, nsval, nspos);
-        k_check_row_index<<<S.blocks, 256, 0, st>>>(nidx, nnl, nsval, nspos, g.n, d_flags);
+        k_merge_base<<<S.blocks, 256, 0, st>>>(o, g.n, L, L.tsv, nidx, nsval, nspos);
+        if (ntail) k_merge_tail<<<S.blocks, 256, 0, st>>>(o, g.n, L, L.tsv, L.tsi, nidx, nsval, nspos);
+        if (check) k_check_row_index<<<S.blocks, 256, 0, st>>>(nidx, nnl, nsval, nspos, g.n, d_flags);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(st);

@@ -232,11 +232,12 @@ int64_t graph_footprint(int32_t n, int64_t m) {
 
 // Device bytes a compaction of that graph allocates while it runs: the new
 // offsets and entries and their sorted copy (12 bytes an entry, for a base
-// grown by the compaction fraction plus a tenth), the per-row merge lists and
-// the library sort's scratch.
+// grown by the compaction fraction plus a tenth), the old rows' deletion
+// marks in position and in sorted order (16 bytes per 64 entries each), the
+// per-row lists and the library sort's scratch.
 int64_t compaction_headroom(int32_t n, int64_t m, double fraction) {
     const double grown = (double)m * (1.1 + (fraction > 0 ? fraction : 0.0));
-    return 12 * (int64_t)grown + 24 * ((int64_t)n + 1) + (int64_t(16) << 20);
+    return 12 * (int64_t)grown + m / 2 + 24 * ((int64_t)n + 1) + (int64_t(16) << 20);
 }
 
 // Device bytes one batch of `count` updates may add (batch buffers doubled

@@ -7,6 +7,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
+# every compaction the tests run also checks its output in full (offsets,
+# entries, row order, and the merged sorted row copy entry by entry)
+os.environ.setdefault("MISFIT_COMPACT_SELFCHECK", "1")
+
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 SAMPLE = os.path.join(GOLDEN, "sample")
 
