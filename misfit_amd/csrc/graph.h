@@ -48,9 +48,17 @@ struct __align__(16) TombWord {
 };
 // words covering m entries (one past the end, so the end of the last row has a word)
 __host__ __device__ inline int64_t mf_tomb_words(int64_t m) { return m / 64 + 1; }
-// entries per counting chunk (a wave's task; a multiple of 64)
+// entries per counting chunk of the position-order marks (a wave's task; a multiple of 64)
 #define MF_TCHUNK 8192
 __host__ __device__ inline int64_t mf_tomb_chunks(int64_t m) { return (mf_tomb_words(m) * 64 + MF_TCHUNK - 1) / MF_TCHUNK; }
+// Merge-path chunks of an array of rows: chunk t starts where rows passed
+// plus entries passed reach t * MF_MCHUNK (rounded down to a multiple of 64
+// entries), so a chunk holds at most about MF_MCHUNK entries and crosses at
+// most about MF_MCHUNK rows, however sparse the rows are.
+#define MF_MCHUNK 8192
+__host__ __device__ inline int64_t mf_path_chunks(int64_t rows, int64_t entries) {
+    return (rows + entries + MF_MCHUNK - 1) / MF_MCHUNK;
+}
 
 // What a compaction hands from the copy to the merge of the sorted row copy.
 struct MergeLists {
@@ -65,6 +73,10 @@ struct MergeLists {
     int64_t* ccnt = nullptr;         // per counting chunk: deletions in it
     int64_t* cpre = nullptr;         // ... exclusive scan (chunks + 1)
     int64_t* cpart = nullptr;        // scan scratch
+    int64_t* cst = nullptr;          // merge-path chunk starts of the old rows (ncst + 1 entries)
+    int64_t ncst = 0;
+    int64_t* tcst = nullptr;         // ... of the tail lists (ntcst + 1 entries)
+    int64_t ntcst = 0;
     // tail sort: sorted tail values and their insertion indices, rows whose
     // tail is long, a counter, and the library sort's temporary storage
     int32_t* tsv = nullptr;

@@ -68,6 +68,192 @@ __device__ __forceinline__ void entry_walk(const int64_t* nindex, int32_t r0, in
     }
 }
 
+// Largest row x in [r0, r1) with nindex[x] <= p (nindex[r0] <= p), by a
+// 64-ary search in which every lane probes one position per step: 4
+// dependent loads for 2^24 rows instead of 24.  Wave-uniform result; all
+// lanes of the wave must call it together.
+__device__ __forceinline__ int64_t row_of(const int64_t* nindex, int64_t r0, int64_t r1, int64_t p, int lane) {
+    int64_t lo = r0, hi = r1 - 1;  // the answer lies in [lo, hi]
+    while (lo < hi) {
+        const int64_t span = hi - lo;
+        const int64_t probe = lo + ((int64_t)(lane + 1) * span + 63) / 64;  // ascending, lane 63 at hi
+        const u64 le = __ballot(nindex[probe] <= p);
+        const int k = __popcll(le);  // probes 0 .. k-1 are <= p
+        const int64_t plo = __shfl(probe, k > 0 ? k - 1 : 0, MF_WAVE);
+        const int64_t phi = __shfl(probe, k < 64 ? k : 63, MF_WAVE);
+        if (k > 0) lo = plo;
+        if (k < 64) hi = phi - 1;
+    }
+    return lo;
+}
+
+// Value of v on lane l (l wave-uniform): scalar reads, no shuffle.
+__device__ __forceinline__ int64_t read_lane(int64_t v, int l) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((u64)v >> 32), l);
+    return (int64_t)(((u64)hi << 32) | lo);
+}
+__device__ __forceinline__ int32_t read_lane(int32_t v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
+// One 64-entry group of an entry walk, as a lane sees it.
+template <class Pay>
+struct WalkGroup {
+    int64_t pp;     // the lane's entry
+    int64_t q;      // pp clamped into the window: always an entry index, for unconditional loads
+    int64_t row, start;
+    Pay pay;        // the row's values (Pay::shfl of the window's)
+    bool valid;     // the lane holds an entry (pp == q)
+};
+
+// Visits every entry of rows [r0, r1) once, balanced by entries like
+// entry_walk (chunks of MF_ECHUNK, 64-row windows, rows found by a 6-step
+// shuffle search), for kernels bound by memory: each step takes
+// MF_EW_GROUPS groups of 64 entries, and steps are software-pipelined in
+// three phases,
+//     load1(next step)  ->  store(this step)  ->  load2(next step),
+// where load2 may depend on load1's values and store on both.  On this
+// GPU one counter tracks loads and stores together and in order, so a
+// wave waiting for a load also waits for every older store: issuing the
+// next step's loads before this step's stores keeps the store latency
+// out of every wait.  The next window's row data is also loaded while the
+// current window is walked.  Phases run on every lane (so `store` may use
+// ballots); groups past the window's end have no valid lane.  rowpay(x)
+// loads the per-row values a lane hands to its row's entries.
+#define MF_EW_GROUPS 4
+template <class Pay>
+struct WalkStep {
+    WalkGroup<Pay> g[MF_EW_GROUPS];
+    int64_t chunk;  // the merge-path chunk of these entries
+};
+template <class Pay, class RowPay, class Load1, class Load2, class Store>
+__device__ __forceinline__ void entry_walk4(const int64_t* nindex, int32_t r0, int32_t r1, const int64_t* cst,
+                                            int64_t nch, RowPay rowpay, Load1 load1, Load2 load2, Store store) {
+    GC c = grid_coords();
+    using D = decltype(load1(WalkStep<Pay>{}));
+    WalkStep<Pay> cur;
+    D dcur{};
+    bool have = false;  // wave-uniform: cur holds a step whose stores are due
+    for (int64_t t = c.wave; t < nch; t += c.nwaves) {
+        int64_t p = cst[t];
+        const int64_t tend = cst[t + 1];
+        if (p >= tend) continue;  // a chunk of empty rows only
+        int64_t x = row_of(nindex, r0, r1, p, c.lane);
+        // window loads at clamped indices, selected after
+        auto win = [&](int64_t x0, int64_t& off, Pay& pay, int64_t& wend) {
+            const int64_t myv = x0 + c.lane;
+            const int64_t lo_ = nindex[myv < r1 ? myv : r1];
+            const Pay py = rowpay(myv < r1 ? myv : r1 - 1);
+            const int64_t xw = x0 + MF_WAVE < r1 ? x0 + MF_WAVE : r1;
+            wend = nindex[xw];
+            off = myv < r1 ? lo_ : INT64_MAX;
+            pay = myv < r1 ? py : Pay{};
+        };
+        int64_t off, wend;
+        Pay pay;
+        win(x, off, pay, wend);
+        while (p < tend && x < r1) {
+            const int64_t xw = x + MF_WAVE < r1 ? x + MF_WAVE : r1;
+            int64_t noff = INT64_MAX, nwend = 0;
+            Pay npay{};
+            const int64_t lim = wend < tend ? wend : tend;
+            if (lim < tend && xw < r1) win(xw, noff, npay, nwend);  // wave-uniform: the chunk goes on
+            int lf = 0;  // window row of the step's first entry (wave-uniform; only grows)
+            for (int64_t p0 = p; p0 < lim; p0 += MF_WAVE * MF_EW_GROUPS) {
+                WalkStep<Pay> nx;
+                nx.chunk = t;
+                while (lf < MF_WAVE - 1 && read_lane(off, lf + 1) <= p0) lf++;
+                const int64_t rend = lf < MF_WAVE - 1 ? read_lane(off, lf + 1) : wend;
+                if (p0 + MF_WAVE * MF_EW_GROUPS <= rend) {
+                    // the whole step lies in row lf: no search, the row's
+                    // values read from its lane by scalar reads
+                    const int64_t st = read_lane(off, lf);
+                    const Pay py = pay.read(lf);
+#pragma unroll
+                    for (int j = 0; j < MF_EW_GROUPS; j++) {
+                        WalkGroup<Pay>& g = nx.g[j];
+                        g.pp = p0 + MF_WAVE * j + c.lane;
+                        g.valid = g.pp < lim;
+                        g.q = g.valid ? g.pp : lim - 1;
+                        g.start = st;
+                        g.pay = py;
+                        g.row = x + lf;
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < MF_EW_GROUPS; j++) {
+                        WalkGroup<Pay>& g = nx.g[j];
+                        g.pp = p0 + MF_WAVE * j + c.lane;
+                        g.valid = g.pp < lim;
+                        g.q = g.valid ? g.pp : lim - 1;
+                        int l = lf, h = MF_WAVE - 1;  // rows before lf end before p0
+#pragma unroll
+                        for (int s = 0; s < 6; s++) {
+                            int mid = (l + h + 1) >> 1;
+                            int64_t om = __shfl(off, mid, MF_WAVE);
+                            if (om <= g.q) l = mid; else h = mid - 1;
+                        }
+                        g.start = __shfl(off, l, MF_WAVE);
+                        g.pay = pay.shfl(l);
+                        g.row = x + l;
+                    }
+                }
+                D dn = load1(nx);
+                if (have) store(cur, dcur);
+                load2(nx, dn);
+                cur = nx;
+                dcur = dn;
+                have = true;
+            }
+            if (lim > p) p = lim;
+            x = xw;
+            off = noff;
+            pay = npay;
+            wend = nwend;
+        }
+    }
+    if (have) store(cur, dcur);
+}
+
+// Merge-path chunk starts of rows [r0, r1) (see MF_MCHUNK): cst[t] for
+// t = 0 .. nch, cst[nch] = the rows' end.  Chunk t starts on the largest
+// row x whose (x - r0) + (nindex[x] - nindex[r0]) <= t * MF_MCHUNK, at the
+// entry that brings the sum to t * MF_MCHUNK, rounded down to a multiple
+// of 64 entries.
+__global__ void k_merge_path(const int64_t* nindex, int32_t r0, int32_t r1, int64_t nch, int64_t* cst) {
+    const int64_t pb = nindex[r0], pe = nindex[r1];
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t <= nch; t += (int64_t)gridDim.x * blockDim.x) {
+        if (t == nch) {
+            cst[t] = pe;
+            continue;
+        }
+        const int64_t d = t * MF_MCHUNK;
+        int64_t lo = r0, hi = r1 - 1;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi + 1) >> 1;
+            if ((mid - r0) + (nindex[mid] - pb) <= d) lo = mid;
+            else hi = mid - 1;
+        }
+        int64_t j = d - (lo - r0);
+        if (j > pe - pb) j = pe - pb;
+        cst[t] = pb + (j & ~(int64_t)63);
+    }
+}
+
+static void merge_path(const int64_t* nindex, int32_t r0, int32_t r1, int64_t nch, int64_t* cst, hipStream_t st) {
+    const int64_t nb = (nch + 1 + 255) / 256;
+    k_merge_path<<<(unsigned)std::min<int64_t>(nb, 4096), 256, 0, st>>>(nindex, r0, r1, nch, cst);
+}
+
+struct PayNone {
+    __device__ PayNone shfl(int) const { return PayNone{}; }
+    __device__ PayNone read(int) const { return PayNone{}; }
+};
+struct PayI64 {
+    int64_t v;
+    __device__ PayI64 shfl(int l) const { return PayI64{__shfl(v, l, MF_WAVE)}; }
+    __device__ PayI64 read(int l) const { return PayI64{read_lane(v, l)}; }
+};
+
 // ---------------------------------------------------------------- basics
 __global__ void k_widen32(const int32_t* in, int64_t* out, int64_t count) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
@@ -405,20 +591,30 @@ __global__ __launch_bounds__(256) void k_tomb_words(const int32_t* nlist, int64_
                                                     int64_t* ccnt) {
     GC c = grid_coords();
     const int64_t nw = mf_tomb_words(m);
+    if (m == 0) {  // one word, no entries
+        if (c.tid == 0) {
+            tw[0] = TombWord{0ull, 0ull};
+            ccnt[0] = 0;
+        }
+        return;
+    }
     for (int64_t t = c.wave; t < nchunks; t += c.nwaves) {
         const int64_t w0 = t * (MF_TCHUNK / 64);
         const int64_t w1 = w0 + MF_TCHUNK / 64 < nw ? w0 + MF_TCHUNK / 64 : nw;
         u64 run = 0;
         for (int64_t w = w0; w < w1; w += MF_TW_UNROLL) {
+            // every load in bounds (clamped) and unconditional, so all of
+            // them are in flight before the first ballot
             int32_t y[MF_TW_UNROLL];
 #pragma unroll
             for (int j = 0; j < MF_TW_UNROLL; j++) {
                 const int64_t p = (w + j) * 64 + c.lane;
-                y[j] = (w + j < w1 && p < m) ? nlist[p] : 0;
+                y[j] = nlist[p < m ? p : m - 1];
             }
 #pragma unroll
             for (int j = 0; j < MF_TW_UNROLL; j++) {
-                const u64 mask = __ballot(y[j] < 0);
+                const int64_t p = (w + j) * 64 + c.lane;
+                const u64 mask = __ballot(y[j] < 0 && p < m && w + j < w1);
                 if (c.lane == 0 && w + j < w1) tw[w + j] = TombWord{mask, run};
                 run += __popcll(mask);
             }
@@ -428,74 +624,81 @@ __global__ __launch_bounds__(256) void k_tomb_words(const int32_t* nlist, int64_
 }
 
 // Deletion marks of the old sorted copy (entry p of row x is deleted when
-// its row position spos[p] is): the chunk's entries are walked in order
-// with their rows found as in entry_walk, each iteration's ballot is merged
-// into the aligned word it covers (an iteration of at most 64 contiguous
-// entries straddles at most one word boundary), and every word of the
-// chunk is written, the one past the last entry included.
+// its row position spos[p] is): each merge-path chunk's entries are walked
+// in order (entry_walk4), each group's ballot is merged into the aligned
+// word it covers (a group of at most 64 contiguous entries straddles at
+// most one word boundary), and every word of the chunk is written, `pre`
+// counted from the chunk's start, with the chunk's count in ccnt (zeroed
+// before: chunks of empty rows are not visited).  Chunk starts are
+// multiples of 64 entries, so every word belongs to one chunk; the last
+// chunk also owns the word past the end.
 __global__ __launch_bounds__(256) void k_tomb_words_sorted(const int64_t* nindex, const int32_t* spos, int32_t n,
-                                                           const TombWord* tw, TombWord* tws, int64_t nchunks,
-                                                           int64_t* ccnt) {
+                                                           const TombWord* tw, TombWord* tws, const int64_t* cst,
+                                                           int64_t nch, int64_t* ccnt) {
     GC c = grid_coords();
     const int64_t m = nindex[n];
     const int64_t nw = mf_tomb_words(m);
-    for (int64_t t = c.wave; t < nchunks; t += c.nwaves) {
-        const int64_t tb = t * MF_TCHUNK;
-        const int64_t w1 = (t + 1) * (MF_TCHUNK / 64) < nw ? (t + 1) * (MF_TCHUNK / 64) : nw;
-        const int64_t tend = tb + MF_TCHUNK < m ? tb + MF_TCHUNK : m;
-        int64_t wcur = tb >> 6;
-        u64 mcur = 0, run = 0;
-        if (tb < tend) {
-            int64_t lo = 0, hi = n - 1;  // largest row x with nindex[x] <= tb
-            while (lo < hi) {
-                int64_t mid = (lo + hi + 1) >> 1;
-                if (nindex[mid] <= tb) lo = mid; else hi = mid - 1;
-            }
-            int64_t x = lo, p = tb;
-            while (p < tend && x < n) {
-                const int64_t myv = x + c.lane;
-                const int64_t off = myv < n ? nindex[myv] : INT64_MAX;
-                const int64_t xw = x + MF_WAVE < n ? x + MF_WAVE : n;
-                const int64_t wend = nindex[xw];
-                const int64_t lim = wend < tend ? wend : tend;
-                for (int64_t p0 = p; p0 < lim; p0 += MF_WAVE) {
-                    const int64_t pp = p0 + c.lane;
-                    const int64_t q0 = pp < lim ? pp : lim - 1;
-                    int l = 0, h = MF_WAVE - 1;
-#pragma unroll
-                    for (int s = 0; s < 6; s++) {
-                        int mid = (l + h + 1) >> 1;
-                        int64_t om = __shfl(off, mid, MF_WAVE);
-                        if (om <= q0) l = mid; else h = mid - 1;
-                    }
-                    const int64_t start = __shfl(off, l, MF_WAVE);
-                    bool del = false;
-                    if (pp < lim) {
-                        const int64_t a = start + spos[pp];
-                        del = (tw[a >> 6].mask >> (a & 63)) & 1ull;
-                    }
-                    const u64 bal = __ballot(del);
-                    const int cnt = lim - p0 < MF_WAVE ? (int)(lim - p0) : MF_WAVE;
-                    const int sh = (int)(p0 & 63);
-                    mcur |= bal << sh;
-                    if (sh + cnt >= 64) {  // the word is complete
-                        if (c.lane == 0) tws[wcur] = TombWord{mcur, run};
-                        run += __popcll(mcur);
-                        wcur++;
-                        mcur = sh ? (bal >> (64 - sh)) : 0ull;
-                    }
-                }
-                if (lim > p) p = lim;
-                x = xw;
-            }
-        }
-        for (; wcur < w1; wcur++) {  // the last, partial word and any word past the end
+    auto chunk_end_word = [&](int64_t t) { return t + 1 < nch ? cst[t + 1] >> 6 : nw; };
+    // state of the wave's current chunk (chunks come in order, and a
+    // chunk's groups in order)
+    int64_t chunk = -1, wcur = 0, wend = 0;
+    u64 mcur = 0, run = 0;
+    auto close_chunk = [&]() {
+        if (chunk < 0) return;
+        for (; wcur < wend; wcur++) {  // the last, partial word and any word past the end
             if (c.lane == 0) tws[wcur] = TombWord{mcur, run};
             run += __popcll(mcur);
             mcur = 0;
         }
-        if (c.lane == 0) ccnt[t] = (int64_t)run;
-    }
+        if (c.lane == 0) ccnt[chunk] = (int64_t)run;
+    };
+    struct D {
+        int64_t a[MF_EW_GROUPS];
+        u64 word[MF_EW_GROUPS];
+    };
+    entry_walk4<PayNone>(
+        nindex, 0, n, cst, nch, [](int64_t) { return PayNone{}; },
+        [&](const WalkStep<PayNone>& st) {
+            D d;
+#pragma unroll
+            for (int j = 0; j < MF_EW_GROUPS; j++) d.a[j] = st.g[j].start + (int64_t)spos[st.g[j].q];
+            return d;
+        },
+        [&](const WalkStep<PayNone>&, D& d) {
+#pragma unroll
+            for (int j = 0; j < MF_EW_GROUPS; j++) d.word[j] = tw[d.a[j] >> 6].mask;
+        },
+        [&](const WalkStep<PayNone>& st, const D& d) {
+            if (st.chunk != chunk) {  // wave-uniform
+                close_chunk();
+                chunk = st.chunk;
+                wcur = cst[chunk] >> 6;
+                wend = chunk_end_word(chunk);
+                mcur = 0;
+                run = 0;
+            }
+#pragma unroll
+            for (int j = 0; j < MF_EW_GROUPS; j++) {
+                const WalkGroup<PayNone>& g = st.g[j];
+                const int cnt = __popcll(__ballot(g.valid));
+                if (cnt == 0) break;  // this and later groups are past the window's end (wave-uniform)
+                const int64_t p0 = g.pp - c.lane;  // the group's first entry
+                const u64 bal = __ballot(g.valid && ((d.word[j] >> (d.a[j] & 63)) & 1ull));
+                const int sh = (int)(p0 & 63);
+                mcur |= bal << sh;
+                if (sh + cnt >= 64) {  // the word is complete
+                    if (c.lane == 0) tws[wcur] = TombWord{mcur, run};
+                    run += __popcll(mcur);
+                    wcur++;
+                    mcur = sh ? (bal >> (64 - sh)) : 0ull;
+                }
+            }
+        });
+    close_chunk();
+    // the last chunk holds no entry (m a multiple of 64, its only word is
+    // the one past the end): written here
+    if (c.tid == 0 && cst[nch - 1] == cst[nch])
+        for (int64_t w = cst[nch - 1] >> 6; w < nw; w++) tws[w] = TombWord{0ull, 0ull};
 }
 
 __global__ void k_words_fix(TombWord* tw, int64_t nw, const int64_t* cpre) {
@@ -503,17 +706,34 @@ __global__ void k_words_fix(TombWord* tw, int64_t nw, const int64_t* cpre) {
         tw[i].pre += (u64)cpre[i / (MF_TCHUNK / 64)];
 }
 
-// Marks of an array of m entries, with `pre` global (tw: position order
-// from nlist; spos given: the sorted copy's order, from the position marks).
+// As k_words_fix for marks counted per merge-path chunk: a wave per chunk
+// adds the deletions of the chunks before it to its words.
+__global__ __launch_bounds__(256) void k_words_fix_path(TombWord* tw, int64_t nw, const int64_t* cst, int64_t nch,
+                                                        const int64_t* cpre) {
+    GC c = grid_coords();
+    for (int64_t t = c.wave; t < nch; t += c.nwaves) {
+        const int64_t w0 = cst[t] >> 6, w1 = t + 1 < nch ? cst[t + 1] >> 6 : nw;
+        const u64 add = (u64)cpre[t];
+        for (int64_t w = w0 + c.lane; w < w1; w += MF_WAVE) tw[w].pre += add;
+    }
+}
+
 static void tomb_marks(GraphState& S, const int32_t* spos, const TombWord* tw_pos, TombWord* out, const MergeLists& L) {
     const int64_t m = S.g.m;
-    const int64_t nch = mf_tomb_chunks(m);
     const int64_t nw = mf_tomb_words(m);
-    const int B = (int)std::min<int64_t>(S.blocks, (nch + 3) / 4);
-    if (spos) k_tomb_words_sorted<<<B, 256, 0, S.stream>>>(S.g.nindex, spos, S.g.n, tw_pos, out, nch, L.ccnt);
-    else k_tomb_words<<<B, 256, 0, S.stream>>>(S.g.nlist, m, out, nch, L.ccnt);
-    mf_scan_excl(L.ccnt, nch, L.cpre, L.cpart, S.stream);
-    k_words_fix<<<S.blocks, 256, 0, S.stream>>>(out, nw, L.cpre);
+    if (spos) {  // sorted order: counted per merge-path chunk of the old rows
+        k_fill<int64_t><<<1024, 256, 0, S.stream>>>(L.ccnt, 0, L.ncst);
+        k_tomb_words_sorted<<<S.blocks, 256, 0, S.stream>>>(S.g.nindex, spos, S.g.n, tw_pos, out, L.cst, L.ncst,
+                                                            L.ccnt);
+        mf_scan_excl(L.ccnt, L.ncst, L.cpre, L.cpart, S.stream);
+        k_words_fix_path<<<S.blocks, 256, 0, S.stream>>>(out, nw, L.cst, L.ncst, L.cpre);
+    } else {  // position order: counted per MF_TCHUNK entries
+        const int64_t nch = mf_tomb_chunks(m);
+        const int B = (int)std::min<int64_t>(S.blocks, (nch + 3) / 4);
+        k_tomb_words<<<B, 256, 0, S.stream>>>(S.g.nlist, m, out, nch, L.ccnt);
+        mf_scan_excl(L.ccnt, nch, L.cpre, L.cpart, S.stream);
+        k_words_fix<<<S.blocks, 256, 0, S.stream>>>(out, nw, L.cpre);
+    }
 }
 
 // Every kept base entry to its compacted position: row x starts at nidx[x],
@@ -521,12 +741,39 @@ static void tomb_marks(GraphState& S, const int32_t* spos, const TombWord* tw_po
 // dst = nidx[x] + doff[x] + (p - start) - (deletions before p in the array).
 // Balanced by entries over all rows, hub rows spread over many waves.
 __global__ __launch_bounds__(256) void k_copy_base(DevGraph g, const int64_t* nidx, const int64_t* doff,
-                                                   const TombWord* tw, int32_t* out) {
-    entry_walk(
-        g.nindex, 0, g.n, [&](int64_t x) { return nidx[x] + doff[x]; },
-        [&](int64_t p, int64_t, int64_t start, int64_t r) {
-            const int32_t y = g.nlist[p];
-            if (y >= 0) out[r + (p - start) - tomb_rank(tw, p)] = y;
+                                                   const TombWord* tw, const int64_t* cst, int64_t nch,
+                                                   int32_t* out) {
+    // Entries come in order within a chunk, so the deletions before an
+    // entry are counted as the walk goes (a ballot per group), from the
+    // chunk's start (a multiple of 64 entries: its word's `pre`).
+    const int lane = lane_id();
+    int64_t chunk = -1;
+    u64 grun = 0;
+    struct D {
+        int32_t y[MF_EW_GROUPS];
+    };
+    entry_walk4<PayI64>(
+        g.nindex, 0, g.n, cst, nch, [&](int64_t x) { return PayI64{nidx[x] + doff[x]}; },
+        [&](const WalkStep<PayI64>& st) {
+            D d;
+#pragma unroll
+            for (int j = 0; j < MF_EW_GROUPS; j++) d.y[j] = g.nlist[st.g[j].q];
+            return d;
+        },
+        [](const WalkStep<PayI64>&, D&) {},
+        [&](const WalkStep<PayI64>& st, const D& d) {
+            if (st.chunk != chunk) {  // wave-uniform
+                chunk = st.chunk;
+                grun = tw[cst[chunk] >> 6].pre;
+            }
+#pragma unroll
+            for (int j = 0; j < MF_EW_GROUPS; j++) {
+                const WalkGroup<PayI64>& w = st.g[j];
+                const u64 bal = __ballot(w.valid && d.y[j] < 0);
+                if (w.valid && d.y[j] >= 0)
+                    out[w.pay.v + (w.pp - w.start) - (int64_t)(grun + __popcll(bal & below_mask(lane)))] = d.y[j];
+                grun += __popcll(bal);
+            }
         });
 }
 
@@ -570,8 +817,9 @@ hipError_t mf_compact_plan(GraphState& S, int64_t* d_newidx) {
 }
 
 hipError_t mf_compact_copy(GraphState& S, const int64_t* d_newidx, int32_t* d_out, const MergeLists& L, int* d_flags) {
+    merge_path(S.g.nindex, 0, S.g.n, L.ncst, L.cst, S.stream);
     tomb_marks(S, nullptr, nullptr, L.tw, L);
-    k_copy_base<<<S.blocks, 256, 0, S.stream>>>(S.g, d_newidx, L.doff, L.tw, d_out);
+    k_copy_base<<<S.blocks, 256, 0, S.stream>>>(S.g, d_newidx, L.doff, L.tw, L.cst, L.ncst, d_out);
     k_fill<int><<<1, 64, 0, S.stream>>>(d_flags, S.g.sorted ? 0 : 1, 1);
     k_copy_runs<<<S.blocks, 256, 0, S.stream>>>(S.g, d_newidx, d_out, L, S.g.sorted ? 1 : 0, d_flags);
     return hipGetLastError();
@@ -739,103 +987,174 @@ __device__ __forceinline__ int64_t lower32(const int32_t* a, int64_t lo, int64_t
     return lo;
 }
 
-// Entries of the old sorted order in balanced chunks (as entry_walk), with
-// each row's offsets and destination fetched once per 64-row window by the
-// lane that owns the row and shuffled to its entries.
-__global__ __launch_bounds__(256) void k_merge_base(OldRows o, int32_t n, MergeLists L, const int32_t* tsv,
-                                                    const int64_t* nidx, int32_t* nsval, int32_t* nspos) {
-    GC c = grid_coords();
-    const TombWord* tw = L.tw;
-    const TombWord* tws = o.sval ? L.tws : L.tw;
-    const int64_t pb = o.nindex[0], pe = o.nindex[n];
-    const int64_t ntask = (pe - pb + MF_ECHUNK - 1) / MF_ECHUNK;
-    for (int64_t t = c.wave; t < ntask; t += c.nwaves) {
-        int64_t p = pb + t * MF_ECHUNK;
-        const int64_t tend = p + MF_ECHUNK < pe ? p + MF_ECHUNK : pe;
-        int64_t lo = 0, hi = n - 1;
-        while (lo < hi) {
-            int64_t mid = (lo + hi + 1) >> 1;
-            if (o.nindex[mid] <= p) lo = mid; else hi = mid - 1;
-        }
-        int64_t x = lo;
-        while (p < tend && x < n) {
-            const int64_t myv = x + c.lane;
-            int64_t off = INT64_MAX, d0 = 0, t0 = 0, dst = 0;
-            int32_t tn = 0;
-            if (myv < n) {
-                off = o.nindex[myv];
-                d0 = L.doff[myv];
-                t0 = L.toff[myv];
-                tn = (int32_t)(L.toff[myv + 1] - t0);
-                dst = nidx[myv];
-            }
-            const int64_t xw = x + MF_WAVE < n ? x + MF_WAVE : n;
-            const int64_t wend = o.nindex[xw];
-            const int64_t lim = wend < tend ? wend : tend;
-            for (int64_t p0 = p; p0 < lim; p0 += MF_WAVE) {
-                const int64_t pp = p0 + c.lane;
-                const int64_t q0 = pp < lim ? pp : lim - 1;
-                int l = 0, h = MF_WAVE - 1;
-#pragma unroll
-                for (int s = 0; s < 6; s++) {
-                    int mid = (l + h + 1) >> 1;
-                    int64_t om = __shfl(off, mid, MF_WAVE);
-                    if (om <= q0) l = mid; else h = mid - 1;
-                }
-                const int64_t start = __shfl(off, l, MF_WAVE);
-                const int64_t rd0 = __shfl(d0, l, MF_WAVE), rt0 = __shfl(t0, l, MF_WAVE);
-                const int64_t rdst = __shfl(dst, l, MF_WAVE);
-                const int32_t rtn = __shfl(tn, l, MF_WAVE);
-                if (pp >= lim) continue;
-                const int64_t k = pp - start;
-                int64_t q, dpos, dval;
-                int32_t v;
-                if (o.sval) {
-                    q = o.spos[pp];
-                    const int64_t a = start + q;
-                    const TombWord w = tw[a >> 6];
-                    if ((w.mask >> (a & 63)) & 1ull) continue;  // deleted
-                    v = o.sval[pp];
-                    dpos = (int64_t)w.pre + __popcll(w.mask & below_mask(a)) - rd0;
-                    dval = tomb_rank(tws, pp) - rd0;
-                } else {
-                    const int32_t e = o.nlist[pp];
-                    if (e < 0) continue;
-                    v = e;
-                    q = k;
-                    dpos = dval = tomb_rank(tw, pp) - rd0;
-                }
-                int64_t at = k - dval;
-                if (rtn) at += lower32(tsv, rt0, rt0 + rtn, v) - rt0;
-                nsval[rdst + at] = v;
-                nspos[rdst + at] = (int32_t)(q - dpos);
-            }
-            if (lim > p) p = lim;
-            x = xw;
-        }
+// Entries of the old sorted order (entry_walk4), with each row's deletion
+// and tail offsets and destination loaded once per 64-row window by the
+// lane that owns the row and shuffled to its entries.  HasSorted: the old
+// rows had a sorted copy (else they were ascending and are their own
+// sorted order).
+struct PayMerge {
+    int64_t d0, t0, dst;
+    int32_t tn;
+    int32_t tv0, tv1;  // the row's first two sorted tail values (INT32_MAX past its tail)
+    __device__ PayMerge shfl(int l) const {
+        return PayMerge{__shfl(d0, l, MF_WAVE), __shfl(t0, l, MF_WAVE), __shfl(dst, l, MF_WAVE), __shfl(tn, l, MF_WAVE),
+                        __shfl(tv0, l, MF_WAVE), __shfl(tv1, l, MF_WAVE)};
     }
+    __device__ PayMerge read(int l) const {
+        return PayMerge{read_lane(d0, l), read_lane(t0, l), read_lane(dst, l), read_lane(tn, l), read_lane(tv0, l),
+                        read_lane(tv1, l)};
+    }
+};
+
+template <bool HasSorted>
+__global__ __launch_bounds__(256) void k_merge_base(OldRows o, int32_t n, MergeLists L, const int32_t* tsv,
+                                                    int64_t ntail, const int64_t* nidx, int32_t* nsval,
+                                                    int32_t* nspos) {
+    const TombWord* tw = L.tw;
+    const TombWord* tws = HasSorted ? L.tws : L.tw;
+    struct D {
+        int32_t v[MF_EW_GROUPS], q[MF_EW_GROUPS];
+        TombWord ws[MF_EW_GROUPS], wp[MF_EW_GROUPS];
+    };
+    entry_walk4<PayMerge>(
+        o.nindex, 0, n, L.cst, L.ncst,
+        [&](int64_t x) {
+            const int64_t t0 = L.toff[x];
+            const int32_t tn = (int32_t)(L.toff[x + 1] - t0);
+            // a row's tail of at most two entries rides along with the row
+            // (clamped loads: tsv holds at least one entry even when no row has a tail)
+            const int32_t a = tsv[t0 < ntail ? t0 : 0], b = tsv[t0 + 1 < ntail ? t0 + 1 : 0];
+            return PayMerge{L.doff[x], t0, nidx[x], tn, tn >= 1 ? a : INT32_MAX, tn >= 2 ? b : INT32_MAX};
+        },
+        [&](const WalkStep<PayMerge>& st) {
+            D d;
+#pragma unroll
+            for (int j = 0; j < MF_EW_GROUPS; j++) {
+                const WalkGroup<PayMerge>& w = st.g[j];
+                if (HasSorted) {
+                    d.q[j] = o.spos[w.q];
+                    d.v[j] = o.sval[w.q];
+                } else {
+                    d.v[j] = o.nlist[w.q];
+                    d.q[j] = (int32_t)(w.q - w.start);
+                }
+                d.ws[j] = tws[w.q >> 6];
+            }
+            return d;
+        },
+        [&](const WalkStep<PayMerge>& st, D& d) {
+            if (HasSorted) {
+#pragma unroll
+                for (int j = 0; j < MF_EW_GROUPS; j++) d.wp[j] = tw[(st.g[j].start + d.q[j]) >> 6];
+            }
+        },
+        [&](const WalkStep<PayMerge>& st, const D& d) {
+#pragma unroll
+            for (int j = 0; j < MF_EW_GROUPS; j++) {
+                const WalkGroup<PayMerge>& w = st.g[j];
+                if (!w.valid) continue;
+                const PayMerge& r = w.pay;
+                const int64_t dsort = (int64_t)d.ws[j].pre + __popcll(d.ws[j].mask & below_mask(w.pp)) - r.d0;
+                int64_t dpos;
+                if (HasSorted) {
+                    const int64_t a = w.start + d.q[j];
+                    if ((d.wp[j].mask >> (a & 63)) & 1ull) continue;  // deleted
+                    dpos = (int64_t)d.wp[j].pre + __popcll(d.wp[j].mask & below_mask(a)) - r.d0;
+                } else {
+                    if (d.v[j] < 0) continue;  // deleted
+                    dpos = dsort;
+                }
+                int64_t at = (w.pp - w.start) - dsort;
+                if (r.tn <= 2) at += (r.tv0 < d.v[j] ? 1 : 0) + (r.tv1 < d.v[j] ? 1 : 0);
+                else at += lower32(tsv, r.t0, r.t0 + r.tn, d.v[j]) - r.t0;
+                nsval[r.dst + at] = d.v[j];
+                nspos[r.dst + at] = d.q[j] - (int32_t)dpos;
+            }
+        });
 }
+
+// Tail entries (entry_walk4 over the tail lists): each one's rank among
+// the old row's values by binary search, the searches of all groups in
+// lock step so their loads are in flight together.
+struct PayTail {
+    int64_t b, e, d0, d1, dst;
+    __device__ PayTail shfl(int l) const {
+        return PayTail{__shfl(b, l, MF_WAVE), __shfl(e, l, MF_WAVE), __shfl(d0, l, MF_WAVE), __shfl(d1, l, MF_WAVE),
+                       __shfl(dst, l, MF_WAVE)};
+    }
+    __device__ PayTail read(int l) const {
+        return PayTail{read_lane(b, l), read_lane(e, l), read_lane(d0, l), read_lane(d1, l), read_lane(dst, l)};
+    }
+};
 
 __global__ __launch_bounds__(256) void k_merge_tail(OldRows o, int32_t n, MergeLists L, const int32_t* tsv,
                                                     const int32_t* tsi, const int64_t* nidx, int32_t* nsval,
                                                     int32_t* nspos) {
     const TombWord* tws = o.sval ? L.tws : L.tw;
-    entry_walk(
-        L.toff, 0, n, [](int64_t) { return (int64_t)0; },
-        [&](int64_t p, int64_t x, int64_t start, int64_t) {
-            const int32_t w = tsv[p];
-            const int64_t b = o.nindex[x], e = o.nindex[x + 1];
-            int64_t lo = b, hi = e;  // old values below w
-            while (lo < hi) {
-                int64_t mid = (lo + hi) >> 1;
-                if (old_sorted_val(o, mid) < w) lo = mid + 1;
-                else hi = mid;
+    const int64_t mlast = o.nindex[n] - 1;  // rows with a tail may all be empty in the old graph (then no search runs)
+    struct D {
+        int32_t wv[MF_EW_GROUPS], wi[MF_EW_GROUPS];
+        int64_t lo[MF_EW_GROUPS];
+        TombWord t[MF_EW_GROUPS];
+    };
+    entry_walk4<PayTail>(
+        L.toff, 0, n, L.tcst, L.ntcst,
+        [&](int64_t x) { return PayTail{o.nindex[x], o.nindex[x + 1], L.doff[x], L.doff[x + 1], nidx[x]}; },
+        [&](const WalkStep<PayTail>& st) {
+            D d;
+#pragma unroll
+            for (int j = 0; j < MF_EW_GROUPS; j++) {
+                d.wv[j] = tsv[st.g[j].q];
+                d.wi[j] = tsi[st.g[j].q];
             }
-            const int64_t d0 = L.doff[x], d1 = L.doff[x + 1];
-            const int64_t at = (lo - b) - (tomb_rank(tws, lo) - d0) + (p - start);
-            const int64_t kept = (e - b) - (d1 - d0);
-            nsval[nidx[x] + at] = w;
-            nspos[nidx[x] + at] = (int32_t)(kept + tsi[p]);
+            return d;
+        },
+        [&](const WalkStep<PayTail>& st, D& d) {
+            // old values below w, all searches a step at a time
+            int64_t lo[MF_EW_GROUPS], hi[MF_EW_GROUPS];
+#pragma unroll
+            for (int j = 0; j < MF_EW_GROUPS; j++) {
+                lo[j] = st.g[j].pay.b;
+                hi[j] = st.g[j].valid ? st.g[j].pay.e : st.g[j].pay.b;
+            }
+            while (true) {
+                bool any = false;
+#pragma unroll
+                for (int j = 0; j < MF_EW_GROUPS; j++) any |= lo[j] < hi[j];
+                if (!__ballot(any)) break;
+                int32_t mv[MF_EW_GROUPS];
+                int64_t mid[MF_EW_GROUPS];
+#pragma unroll
+                for (int j = 0; j < MF_EW_GROUPS; j++) {
+                    mid[j] = (lo[j] + hi[j]) >> 1;
+                    const int64_t cl = mid[j] < mlast ? mid[j] : mlast;
+                    mv[j] = o.sval ? o.sval[cl] : decode(o.nlist[cl]);
+                }
+#pragma unroll
+                for (int j = 0; j < MF_EW_GROUPS; j++)
+                    if (lo[j] < hi[j]) {
+                        if (mv[j] < d.wv[j]) lo[j] = mid[j] + 1;
+                        else hi[j] = mid[j];
+                    }
+            }
+#pragma unroll
+            for (int j = 0; j < MF_EW_GROUPS; j++) {
+                d.lo[j] = lo[j];
+                d.t[j] = tws[lo[j] >> 6];
+            }
+        },
+        [&](const WalkStep<PayTail>& st, const D& d) {
+#pragma unroll
+            for (int j = 0; j < MF_EW_GROUPS; j++) {
+                const WalkGroup<PayTail>& g = st.g[j];
+                if (!g.valid) continue;
+                const PayTail& r = g.pay;
+                const int64_t dl = (int64_t)d.t[j].pre + __popcll(d.t[j].mask & below_mask(d.lo[j])) - r.d0;
+                const int64_t at = (d.lo[j] - r.b) - dl + (g.pp - g.start);
+                const int64_t kept = (r.e - r.b) - (r.d1 - r.d0);
+                nsval[r.dst + at] = d.wv[j];
+                nspos[r.dst + at] = (int32_t)(kept + d.wi[j]);
+            }
         });
 }
 
@@ -1062,8 +1381,12 @@ hipError_t mf_merge_row_index(GraphState& S, const MergeLists& L, int64_t ntail,
     hipError_t e = sort_lists(S, L.toff, L.trows, L.ntrows, ntail, L.tval, L.tidx, L.tsv, L.tsi, L);
     if (e == hipSuccess) {
         OldRows o{g.nindex, g.nlist, g.sval, g.spos};
-        k_merge_base<<<S.blocks, 256, 0, st>>>(o, g.n, L, L.tsv, nidx, nsval, nspos);
-        if (ntail) k_merge_tail<<<S.blocks, 256, 0, st>>>(o, g.n, L, L.tsv, L.tsi, nidx, nsval, nspos);
+        if (g.sval) k_merge_base<true><<<S.blocks, 256, 0, st>>>(o, g.n, L, L.tsv, ntail, nidx, nsval, nspos);
+        else k_merge_base<false><<<S.blocks, 256, 0, st>>>(o, g.n, L, L.tsv, ntail, nidx, nsval, nspos);
+        if (ntail) {
+            merge_path(L.toff, 0, g.n, L.ntcst, L.tcst, st);
+            k_merge_tail<<<S.blocks, 256, 0, st>>>(o, g.n, L, L.tsv, L.tsi, nidx, nsval, nspos);
+        }
         if (check) k_check_row_index<<<S.blocks, 256, 0, st>>>(nidx, nnl, nsval, nspos, g.n, d_flags);
         e = hipGetLastError();
     }
