@@ -61,6 +61,32 @@ import time
 
 import numpy as np
 
+
+class _Ranges:
+    """Named ranges for rocprofv3 --marker-trace around the timed regions
+    (roctx from the ROCm install; nothing happens when it is absent)."""
+
+    def __init__(self):
+        self.lib = None
+        try:
+            import ctypes
+            root = os.environ.get("ROCM_PATH", "/opt/rocm")
+            self.lib = ctypes.CDLL(os.path.join(root, "lib", "librocprofiler-sdk-roctx.so.1"))
+            self.lib.roctxRangePushA.argtypes = [ctypes.c_char_p]
+        except OSError:
+            self.lib = None
+
+    def push(self, name):
+        if self.lib:
+            self.lib.roctxRangePushA(name.encode())
+
+    def pop(self):
+        if self.lib:
+            self.lib.roctxRangePop()
+
+
+RANGES = _Ranges()
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -232,6 +258,7 @@ def run_served(M, eng, ni, nl, mis0, ups, W, K, batch, depth=2, staged=True):
     eng.sync()
     starts, lat, dev, tickets, stats, sub = {}, [], [], {}, [], []
     eng.timer_begin()  # HIP event on the engine's stream: the timed region's device time
+    RANGES.push("bench:served")
     t0 = time.perf_counter()
     nxt = W
     for b in range(W, W + K):
@@ -247,6 +274,7 @@ def run_served(M, eng, ni, nl, mis0, ups, W, K, batch, depth=2, staged=True):
         stats.append(st)
     eng.sync()
     wall = time.perf_counter() - t0
+    RANGES.pop()
     region_ms = eng.timer_end()
     if staged:
         release_all(eng, hs[W:])
@@ -632,10 +660,12 @@ def many_graph_line(M, args, copy):
                       persistent=persistent) as srv:
             gids = [srv.add_graph(g["ni"], g["nl"], g["mis0"]) for g in graphs]
             order = [(gi, b) for b in range(args.steps) for gi in range(len(graphs))]
+            RANGES.push("bench:served_c5")
             t = time.perf_counter()
             reqs = [(gi, srv.submit(gids[gi], graphs[gi]["batches"][b])) for gi, b in order]
             st = [(gi, srv.wait(r)) for gi, r in reqs]
             w = time.perf_counter() - t
+            RANGES.pop()
             for gi, g in enumerate(graphs):
                 if not np.array_equal(srv.get_mis(gids[gi], g["n"]), g["mis_plain"]):
                     raise SystemExit("served (persistent %d) and plain MIS differ on graph %d" % (persistent, gi))

@@ -160,7 +160,10 @@ struct LaunchCfg {
     int64_t U = -1;
     int64_t n = 0;
     int round_grid = 0;
+    int round_threads = 256;  // workgroup size of the speculation, validation and commit kernels: 128, 256 or 512
 };
+// Waves per workgroup of the round kernels (a round's window is round_grid times this).
+inline int mf_round_wpg(const LaunchCfg& L) { return L.round_threads / MF_WAVE; }
 void mf_launch_batch_prologue(const BP& P, const LaunchCfg& L);
 int mf_round_grid(const LaunchCfg& L);  // workgroups of the speculation/validation kernels
 void mf_launch_round(const BP& P, const LaunchCfg& L, int par, bool first = false);  // first: stamps C_TP

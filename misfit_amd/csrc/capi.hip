@@ -22,8 +22,23 @@
 #include <string>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "graph.h"
 #include "../../include/misfit.h"
+
+// A named range for rocprofv3 --marker-trace (no cost beyond a call when no
+// tool is attached).  Host-side: in plain mode, where the host waits for
+// each round, a range spans the device work it launched; in queued mode it
+// spans the enqueue.
+namespace {
+struct Range {
+    explicit Range(const char* name) { roctxRangePushA(name); }
+    ~Range() { roctxRangePop(); }
+    Range(const Range&) = delete;
+    Range& operator=(const Range&) = delete;
+};
+}  // namespace
 
 #define MF_NSLOT 4
 
@@ -62,6 +77,11 @@ struct misfit_ctx {
     int completion = 1;
     bool maximal = false;  // the MIS has no uncovered non-member (checked at set_mis, kept by the completion)
     bool force_pu_scan = false;  // MISFIT_PU_SCAN=full: always scan every vertex for PU
+    // MISFIT_PU_DENSE_FRACTION=f: the may-uncover scan switches from the
+    // tagged vertices only (sparse) to every vertex (dense) when the batch
+    // may tag more than f*n vertices (its updates times the mean degree);
+    // below 0, never
+    double pu_dense_fraction = -1.0;
     int mode = MISFIT_MODE_QUEUED;
     int grid_rounds = 0;       // MISFIT_GRID_ROUNDS: fixed number of queued grid rounds
     bool grid_rounds_fixed = false;
@@ -79,6 +99,7 @@ struct misfit_ctx {
     double clock_khz = 0.0;       // device wall clock, for per-batch device times
     hipEvent_t timer0 = nullptr, timer1 = nullptr;  // misfit_timer_begin/end
     int round_per_cu = 0;      // MISFIT_ROUND_WG_PER_CU: round grid (0: by occupancy)
+    int round_threads = 256;   // MISFIT_ROUND_THREADS: workgroup size of the round kernels (128, 256 or 512)
     bool debug_rounds = false;
     uint32_t epoch = 0;
     uint32_t bytes_tag = 0;
@@ -257,12 +278,16 @@ extern "C" int misfit_ctx_create(int device, misfit_ctx** out) {
     c->debug_rounds = dbg && dbg[0] == '1';
     const char* rw = getenv("MISFIT_ROUND_WG_PER_CU");
     if (rw) c->round_per_cu = std::max(0, std::min(8, atoi(rw)));
+    const char* rt = getenv("MISFIT_ROUND_THREADS");
+    if (rt && (atoi(rt) == 128 || atoi(rt) == 256 || atoi(rt) == 512)) c->round_threads = atoi(rt);
     const char* wf = getenv("MISFIT_WINDOW_FACTOR");
     if (wf && atof(wf) >= 1.0) c->window_factor = atof(wf);
     const char* rm = getenv("MISFIT_ROUND_MARGIN");
     if (rm) c->round_margin = std::max(0, std::min(16, atoi(rm)));
     const char* ps = getenv("MISFIT_PU_SCAN");
     c->force_pu_scan = ps && strcmp(ps, "full") == 0;
+    const char* pd = getenv("MISFIT_PU_DENSE_FRACTION");
+    if (pd) c->pu_dense_fraction = atof(pd);
     const char* sc = getenv("MISFIT_COMPACT_SELFCHECK");
     c->compact_selfcheck = sc && sc[0] == '1';
     const char* fg = getenv("MISFIT_TEST_FAIL_GROWTH");
@@ -876,7 +901,12 @@ static void fill_stats(const misfit_ctx* c, const Slot& s, misfit_batch_stats* s
 // Host-driven rounds: one round = 4 kernels, then the host reads where the
 // committed prefix ends; the completion iterations likewise.
 static int drive_plain(misfit_ctx* c, BP& P, const LaunchCfg& L) {
-    mf_launch_batch_prologue(P, L);
+    Range batch("misfit:batch");
+    {
+        Range pro("misfit:prologue");
+        mf_launch_batch_prologue(P, L);
+    }
+    Range rounds("misfit:rounds");
     for (uint32_t r = 0;; r++) {
         if ((int64_t)r > P.U + 1) return fail(c, MISFIT_E_CHECK, "speculative rounds made no progress");
         int par = (int)(r & 1u);
@@ -893,6 +923,7 @@ static int drive_plain(misfit_ctx* c, BP& P, const LaunchCfg& L) {
         }
         if (start >= P.U) break;
     }
+    Range tail("misfit:tail");
     mf_launch_stats(P, L);
     if (P.completion) {
         mf_launch_completion_collect(P, L);
@@ -914,8 +945,16 @@ static int drive_plain(misfit_ctx* c, BP& P, const LaunchCfg& L) {
 // The queued sequence of one batch: prologue, grid-wide rounds, then the
 // one-workgroup tail (leftover rounds, counters, completion, publication).
 static void launch_queued(const BP& P, const LaunchCfg& L, int k) {
-    mf_launch_batch_prologue(P, L);
-    for (int r = 0; r < k; r++) mf_launch_round(P, L, r & 1, r == 0);
+    Range batch("misfit:batch_enqueue");
+    {
+        Range pro("misfit:prologue");
+        mf_launch_batch_prologue(P, L);
+    }
+    {
+        Range rounds("misfit:rounds");
+        for (int r = 0; r < k; r++) mf_launch_round(P, L, r & 1, r == 0);
+    }
+    Range tail("misfit:tail");
     mf_launch_finish(P, L, k & 1);
 }
 
@@ -1024,11 +1063,12 @@ static int drive_queued(misfit_ctx* c, BP& P, const LaunchCfg& L, int k) {
 // the batch's end event is recorded again.  Nothing else of this graph
 // was queued behind it.
 static int continue_batch(misfit_ctx* c, Slot& s) {
+    Range range("misfit:continue_rounds");
     const u64* h = s.h_ctr;
     int64_t start = (int64_t)h[C_START0 + s.par_next];
     int64_t left = std::max<int64_t>(s.P.U - start, 0);
     if (h[C_ROUNDS] > 0) {
-        const int64_t wpg = 256 / MF_WAVE;
+        const int64_t wpg = mf_round_wpg(s.L);
         LaunchCfg full = s.L;
         full.round_grid = 0;
         const int64_t full_window = std::max<int64_t>((int64_t)mf_round_grid(full) * wpg, wpg);
@@ -1198,13 +1238,19 @@ static int submit(misfit_ctx* c, const misfit_update* ups, const misfit_staged* 
     P.U = count;
     P.b = c->epoch;
     P.completion = c->completion;
-    P.pu_list = c->maximal && !c->force_pu_scan ? 1 : 0;
+    // sparse scan (tagged vertices only) unless forced dense, the MIS is not
+    // known maximal, or the batch may tag more than the dense fraction of n
+    const double may_tag = (double)count * (double)P.g.m / (double)std::max<int32_t>(P.g.n, 1);
+    const bool dense = c->force_pu_scan || !c->maximal ||
+                       (c->pu_dense_fraction >= 0.0 && may_tag > c->pu_dense_fraction * (double)P.g.n);
+    P.pu_list = dense ? 0 : 1;
     if (!c->completion) c->maximal = false;
     // grids sized to the batch; one edge per wave of the round grid and round
     LaunchCfg L{c->S.blocks, c->stream, c->round_per_cu};
+    L.round_threads = c->round_threads;
     L.U = count;
     L.n = c->P.g.n;
-    const int64_t wpg = 256 / MF_WAVE;
+    const int64_t wpg = mf_round_wpg(L);
     RoundPlan plan = plan_rounds(c, count, std::max<int64_t>((int64_t)mf_round_grid(L) * wpg, wpg));
     L.round_grid = (int)std::max<int64_t>(plan.window / wpg, 1);
     P.window = (int64_t)L.round_grid * wpg;
@@ -1406,6 +1452,7 @@ extern "C" int misfit_process_batch(misfit_ctx* c, const misfit_update* ups, int
 
 // ---------------------------------------------------------------- checks
 extern "C" int misfit_check(misfit_ctx* c, int64_t* conflicts, int64_t* uncovered, int64_t* count_errors) {
+    Range range("misfit:check");
     if (!c) return MISFIT_E_ARG;
     if (!c->has_graph) return fail(c, MISFIT_E_STATE, "no graph loaded");
     HIPCHK(c, hipSetDevice(c->device));
@@ -1508,6 +1555,7 @@ static hipError_t ensure_compact_scratch(misfit_ctx* c, int64_t ntail) {
 
 extern "C" int misfit_compact(misfit_ctx* c, int64_t* new_m) {
     if (!c) return MISFIT_E_ARG;
+    Range range("misfit:compact");
     if (!c->has_graph) return fail(c, MISFIT_E_STATE, "no graph loaded");
     HIPCHK(c, hipSetDevice(c->device));
     if (int rc_ = settle(c)) return rc_;

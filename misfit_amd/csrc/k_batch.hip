@@ -1647,11 +1647,25 @@ MF_KERNEL(k_comp_collect, ph_comp_collect(P, c))
 MF_KERNEL(k_comp_decide, ph_comp_decide(P, c))
 MF_KERNEL(k_stats, ph_stats(P, c))
 
+// The round kernels come in workgroups of NT = 128, 256 or 512 threads
+// (a tuning knob); they work per wave, so only the grid's shape changes.
+template <int NT>
+__device__ __forceinline__ GC grid_coords_nt() {
+    GC c;
+    c.tid = (int64_t)blockIdx.x * NT + threadIdx.x;
+    c.nthreads = (int64_t)gridDim.x * NT;
+    c.wave = c.tid / MF_WAVE;
+    c.nwaves = c.nthreads / MF_WAVE;
+    c.lane = lane_id();
+    return c;
+}
+
 // par bit 1: the batch's first round, which stamps the end of the prologue
 // (no commit staged its edges).
-__global__ __launch_bounds__(256) void k_spec(BP P, int par) {
-    __shared__ int cur[256 / MF_WAVE];
-    GC c = grid_coords();
+template <int NT>
+__global__ __launch_bounds__(NT) void k_spec(BP P, int par) {
+    __shared__ int cur[NT / MF_WAVE];
+    GC c = grid_coords_nt<NT>();
     const bool first = (par & 2) != 0;
     const int4 st = !first && c.wave < P.nstage ? P.stage[c.wave] : make_int4(-1, -1, 0, -1);
     if (first) {
@@ -1661,15 +1675,17 @@ __global__ __launch_bounds__(256) void k_spec(BP P, int par) {
     RoundView v = round_view(P, par, P.window);
     if (v.live) ph_spec_w(P, c, v, cur, st);
 }
-__global__ __launch_bounds__(256) void k_validate(BP P, int par) {
-    GC c = grid_coords();
+template <int NT>
+__global__ __launch_bounds__(NT) void k_validate(BP P, int par) {
+    GC c = grid_coords_nt<NT>();
     // the segment's first line (every wave of the round grid has one)
     const uint4 r0 = c.lane < MF_RLOG_LINE ? P.rlog[c.wave * MF_RLOG + c.lane] : make_uint4(0u, 0u, 0u, 0u);
     RoundView v = round_view(P, par, P.window);
     if (v.live) ph_validate_w(P, c, v, par, r0);
 }
-__global__ __launch_bounds__(256) void k_commit(BP P, int par) {
-    GC c = grid_coords();
+template <int NT>
+__global__ __launch_bounds__(NT) void k_commit(BP P, int par) {
+    GC c = grid_coords_nt<NT>();
     RoundView v = round_view(P, par, P.window);
     if (v.live) ph_commit_w(P, c, v, par);
     else if (c.tid == 0) advance_round(P, par, INT64_MAX, P.window);
@@ -1927,11 +1943,25 @@ int occ_per_cu(const void* fn, int threads) {
     return nb;
 }
 
-// L.blocks is the cap (8 workgroups per CU); the grid is min(cap, resident).
-int grid_for(const void* fn, const LaunchCfg& L) {
-    int per_cu = occ_per_cu(fn, 256);
+// L.blocks is the cap (8 workgroups of 256 threads per CU); the grid is
+// min(cap, resident), both counted in workgroups of `threads`.
+int grid_for(const void* fn, const LaunchCfg& L, int threads = 256) {
+    int per_cu = occ_per_cu(fn, threads);
     int resident = per_cu * (L.blocks / 8);
-    return resident < L.blocks ? resident : L.blocks;
+    const int cap = (int)((int64_t)L.blocks * 256 / threads);
+    return resident < cap ? resident : cap;
+}
+
+// The round kernels of one workgroup size.
+struct RoundFns {
+    const void* spec;
+    const void* validate;
+    const void* commit;
+};
+RoundFns round_fns(int threads) {
+    if (threads == 128) return {(const void*)k_spec<128>, (const void*)k_validate<128>, (const void*)k_commit<128>};
+    if (threads == 512) return {(const void*)k_spec<512>, (const void*)k_validate<512>, (const void*)k_commit<512>};
+    return {(const void*)k_spec<256>, (const void*)k_validate<256>, (const void*)k_commit<256>};
 }
 }  // namespace
 
@@ -1955,13 +1985,17 @@ int grid_cls(const void* fn, const LaunchCfg& L, GridClass cls) {
 // writes read-log segment w, which wave w's slots of the validation check.
 // L.round_grid > 0 sets it for the batch; otherwise L.round_per_cu > 0 sets
 // it (workgroups per CU) instead of the occupancy.
+// The grid never holds more waves than 4 per workgroup of L.blocks: the
+// read log has one segment per such wave.
 int mf_round_grid(const LaunchCfg& L) {
     if (L.round_grid > 0) return L.round_grid;
+    const int cap = (int)((int64_t)L.blocks * 256 / L.round_threads);
     if (L.round_per_cu > 0) {
         int g = L.round_per_cu * (L.blocks / 8);
-        return g < L.blocks ? g : L.blocks;
+        return g < cap ? g : cap;
     }
-    int a = grid_for((const void*)k_spec, L), b = grid_for((const void*)k_validate, L);
+    const RoundFns f = round_fns(L.round_threads);
+    int a = grid_for(f.spec, L, L.round_threads), b = grid_for(f.validate, L, L.round_threads);
     return a < b ? a : b;
 }
 
@@ -1969,7 +2003,8 @@ int mf_round_grid(const LaunchCfg& L) {
     hipLaunchKernelGGL(k, dim3(grid_for((const void*)k, L)), dim3(256), 0, L.stream, __VA_ARGS__)
 #define LAUNCH_C(k, cls, ...) \
     hipLaunchKernelGGL(k, dim3(grid_cls((const void*)k, L, cls)), dim3(256), 0, L.stream, __VA_ARGS__)
-#define LAUNCH_ROUND(k, ...) hipLaunchKernelGGL(k, dim3(mf_round_grid(L)), dim3(256), 0, L.stream, __VA_ARGS__)
+#define LAUNCH_ROUND(k, NT, ...) \
+    hipLaunchKernelGGL(k<NT>, dim3(mf_round_grid(L)), dim3(NT), 0, L.stream, __VA_ARGS__)
 
 void mf_launch_batch_prologue(const BP& P, const LaunchCfg& L) {
     if (P.hu) LAUNCH_C(k_init, G_UPD, P);
@@ -1984,11 +2019,20 @@ void mf_launch_batch_prologue(const BP& P, const LaunchCfg& L) {
     LAUNCH_C(k_fa_fill, G_WAVE_UPD, P);
 }
 
+template <int NT>
+static void launch_round_nt(const BP& P, const LaunchCfg& L, int par, bool first) {
+    LAUNCH_ROUND(k_spec, NT, P, par | (first ? 2 : 0));
+    LAUNCH_ROUND(k_validate, NT, P, par);
+    if (L.round_grid > 0) LAUNCH_ROUND(k_commit, NT, P, par);
+    else
+        hipLaunchKernelGGL(k_commit<NT>, dim3(grid_for((const void*)k_commit<NT>, L, NT)), dim3(NT), 0, L.stream, P,
+                           par);
+}
+
 void mf_launch_round(const BP& P, const LaunchCfg& L, int par, bool first) {
-    LAUNCH_ROUND(k_spec, P, par | (first ? 2 : 0));
-    LAUNCH_ROUND(k_validate, P, par);
-    if (L.round_grid > 0) LAUNCH_ROUND(k_commit, P, par);
-    else LAUNCH(k_commit, P, par);
+    if (L.round_threads == 128) launch_round_nt<128>(P, L, par, first);
+    else if (L.round_threads == 512) launch_round_nt<512>(P, L, par, first);
+    else launch_round_nt<256>(P, L, par, first);
 }
 
 void mf_launch_finish(const BP& P, const LaunchCfg& L, int par) {

@@ -1,10 +1,14 @@
 """Autotuner for the engine's speed knobs on a caller's own graph and stream.
 
-The engine reads three knobs when a context is created (MISFIT_ROUND_MARGIN,
-MISFIT_WINDOW_FACTOR, MISFIT_ROUND_WG_PER_CU).  They change how many
-speculative rounds are queued per batch, how many edges a round speculates
-on, and how many workgroups run a round; they never change a result.  Their
-defaults come from measurements on the R-MAT scale 22 stream.  A graph or
+The engine reads five knobs when a context is created:
+  MISFIT_ROUND_MARGIN       speculative rounds queued beyond the plan,
+  MISFIT_WINDOW_FACTOR      edges a round speculates on (per expected commit),
+  MISFIT_ROUND_WG_PER_CU    workgroups per CU that run a round,
+  MISFIT_ROUND_THREADS      the round kernels' workgroup size (128/256/512),
+  MISFIT_PU_DENSE_FRACTION  where the may-uncover scan switches from the
+                            tagged vertices (sparse) to all vertices (dense).
+They never change a result.  Their defaults come from measurements on the
+R-MAT scale 22 stream.  A graph or
 batch size far from that may prefer other values; this module finds them by
 timing the queued driver on a short stream of the caller's own shape:
 
@@ -37,6 +41,8 @@ SPACE = {
     "MISFIT_ROUND_MARGIN": ["", "1", "2", "3", "4"],
     "MISFIT_WINDOW_FACTOR": ["", "8", "12", "24", "32"],
     "MISFIT_ROUND_WG_PER_CU": ["", "2", "4"],
+    "MISFIT_ROUND_THREADS": ["", "128", "512"],
+    "MISFIT_PU_DENSE_FRACTION": ["", "0.01", "0.1", "1"],
 }
 
 

@@ -373,6 +373,36 @@ def test_tagged_scan_equals_full_scan(oracle, engine_mod, mode, monkeypatch):
 
 
 @pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("knobs", [{"MISFIT_ROUND_THREADS": "128"}, {"MISFIT_ROUND_THREADS": "512"},
+                                   {"MISFIT_PU_DENSE_FRACTION": "0"},
+                                   {"MISFIT_ROUND_THREADS": "512", "MISFIT_ROUND_WG_PER_CU": "1"}])
+def test_tuning_knobs_keep_results(oracle, engine_mod, mode, knobs, monkeypatch):
+    """The tuner's knobs change how the work is laid out, never the result:
+    the round kernels in workgroups of 128 or 512 threads (their windows
+    follow the grid's waves), and the may-uncover scan switched to every
+    vertex for every batch, each give the oracle's MIS after every batch,
+    with the device check passing."""
+    for k in ("MISFIT_ROUND_THREADS", "MISFIT_PU_DENSE_FRACTION", "MISFIT_ROUND_WG_PER_CU", "MISFIT_PU_SCAN"):
+        monkeypatch.delenv(k, raising=False)
+    for k, v in knobs.items():
+        monkeypatch.setenv(k, v)
+    ni, nl = engine_mod.gen_rmat(15, 16, 13)
+    mis0 = engine_mod.greedy_mis(ni, nl)
+    u, v = engine_mod.gen_updates(engine_mod.MIXED_EXISTING, ni, nl, 12000, 13)
+    g = oracle.Graph.from_csr(ni, nl)
+    ins = g.flags(u, v)
+    sizes = [500, 3000, 6000, 2500]
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    batches = [(u[a:b], v[a:b], ins[a:b]) for a, b in zip(offs[:-1], offs[1:])]
+    with engine_mod.Engine(0) as e:
+        got, stats, checks = H.engine_replay(engine_mod, e, ni, nl, mis0, batches, mode=mode)
+    assert all(c == (0, 0, 0) for c in checks)
+    want, _, _ = H.oracle_replay(oracle, ni, nl, mis0, batches, completion=True, literal=False)
+    for a, b in zip(want, got):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("mode", MODES)
 def test_device_times_and_timer(engine_mod, eng, mode):
     """Per-batch device times come from the device clock (the prologue is part
     of the batch), and the event timer brackets queued work."""

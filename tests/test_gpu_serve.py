@@ -220,7 +220,8 @@ def test_tuner_on_the_device(engine_mod, tmp_path, monkeypatch):
             eng.process_batch(ups[b * B:(b + 1) * B])
         want = eng.get_mis()
     space = {"MISFIT_ROUND_MARGIN": ["", "1", "4"], "MISFIT_WINDOW_FACTOR": ["", "8"],
-             "MISFIT_ROUND_WG_PER_CU": ["", "2"]}
+             "MISFIT_ROUND_WG_PER_CU": ["", "2"], "MISFIT_ROUND_THREADS": ["", "128", "512"],
+             "MISFIT_PU_DENSE_FRACTION": ["", "0"]}
     seen = []
 
     def fn(env):
@@ -228,7 +229,7 @@ def test_tuner_on_the_device(engine_mod, tmp_path, monkeypatch):
         seen.append(mis)
         return r, mis
     res = T.tune(fn, space=space, repeats=1)
-    assert len(res["trials"]) == 1 + 2 + 1 + 1
+    assert len(res["trials"]) == 1 + 2 + 1 + 1 + 2 + 1
     assert all(np.array_equal(m, want) for m in seen)
     assert res["updates_per_s"] >= res["default_updates_per_s"] > 0
     p = str(tmp_path / "tune.json")
