@@ -99,7 +99,7 @@ struct misfit_ctx {
     double clock_khz = 0.0;       // device wall clock, for per-batch device times
     hipEvent_t timer0 = nullptr, timer1 = nullptr;  // misfit_timer_begin/end
     int round_per_cu = 0;      // MISFIT_ROUND_WG_PER_CU: round grid (0: by occupancy)
-    int round_threads = 256;   // MISFIT_ROUND_THREADS: workgroup size of the round kernels (128, 256 or 512)
+    int round_threads = 512;   // MISFIT_ROUND_THREADS: workgroup size of the round kernels (128, 256 or 512)
     bool debug_rounds = false;
     uint32_t epoch = 0;
     uint32_t bytes_tag = 0;

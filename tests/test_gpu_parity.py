@@ -373,12 +373,12 @@ def test_tagged_scan_equals_full_scan(oracle, engine_mod, mode, monkeypatch):
 
 
 @pytest.mark.parametrize("mode", MODES)
-@pytest.mark.parametrize("knobs", [{"MISFIT_ROUND_THREADS": "128"}, {"MISFIT_ROUND_THREADS": "512"},
+@pytest.mark.parametrize("knobs", [{"MISFIT_ROUND_THREADS": "128"}, {"MISFIT_ROUND_THREADS": "256"},
                                    {"MISFIT_PU_DENSE_FRACTION": "0"},
-                                   {"MISFIT_ROUND_THREADS": "512", "MISFIT_ROUND_WG_PER_CU": "1"}])
+                                   {"MISFIT_ROUND_THREADS": "256", "MISFIT_ROUND_WG_PER_CU": "1"}])
 def test_tuning_knobs_keep_results(oracle, engine_mod, mode, knobs, monkeypatch):
     """The tuner's knobs change how the work is laid out, never the result:
-    the round kernels in workgroups of 128 or 512 threads (their windows
+    the round kernels in workgroups of 128 or 256 threads (the default is 512; windows
     follow the grid's waves), and the may-uncover scan switched to every
     vertex for every batch, each give the oracle's MIS after every batch,
     with the device check passing."""

@@ -220,7 +220,7 @@ def test_tuner_on_the_device(engine_mod, tmp_path, monkeypatch):
             eng.process_batch(ups[b * B:(b + 1) * B])
         want = eng.get_mis()
     space = {"MISFIT_ROUND_MARGIN": ["", "1", "4"], "MISFIT_WINDOW_FACTOR": ["", "8"],
-             "MISFIT_ROUND_WG_PER_CU": ["", "2"], "MISFIT_ROUND_THREADS": ["", "128", "512"],
+             "MISFIT_ROUND_WG_PER_CU": ["", "2"], "MISFIT_ROUND_THREADS": ["", "128", "256"],
              "MISFIT_PU_DENSE_FRACTION": ["", "0"]}
     seen = []
 
