@@ -99,7 +99,13 @@ struct misfit_ctx {
     double clock_khz = 0.0;       // device wall clock, for per-batch device times
     hipEvent_t timer0 = nullptr, timer1 = nullptr;  // misfit_timer_begin/end
     int round_per_cu = 0;      // MISFIT_ROUND_WG_PER_CU: round grid (0: by occupancy)
-    int round_threads = 512;   // MISFIT_ROUND_THREADS: workgroup size of the round kernels (128, 256 or 512)
+    // MISFIT_ROUND_THREADS: workgroup size of the round kernels (128, 256 or
+    // 512); 0 (the default) chooses per batch: 512 when a round is expected
+    // to commit few enough edges for the 512-thread grid (fewer, larger
+    // workgroups were 2.7% faster on R-MAT scale 22's 10k-update batches),
+    // else 256, whose grid holds more waves (50k-update batches on the RGG,
+    // about 2,900 edges a round, ran 25 rounds instead of 17 at 512)
+    int round_threads = 0;
     bool debug_rounds = false;
     uint32_t epoch = 0;
     uint32_t bytes_tag = 0;
@@ -1247,11 +1253,25 @@ static int submit(misfit_ctx* c, const misfit_update* ups, const misfit_staged* 
     if (!c->completion) c->maximal = false;
     // grids sized to the batch; one edge per wave of the round grid and round
     LaunchCfg L{c->S.blocks, c->stream, c->round_per_cu};
-    L.round_threads = c->round_threads;
+    L.round_threads = c->round_threads > 0 ? c->round_threads : 256;
     L.U = count;
     L.n = c->P.g.n;
+    RoundPlan plan = plan_rounds(c, count, std::max<int64_t>((int64_t)mf_round_grid(L) * mf_round_wpg(L),
+                                                             mf_round_wpg(L)));
+    if (c->round_threads == 0) {
+        // per batch: 512 threads when the edges a round is expected to
+        // commit (the batch over its planned rounds), with half again as
+        // margin, fit in the 512-thread grid's waves
+        LaunchCfg L5 = L;
+        L5.round_threads = 512;
+        const double per_round = (double)count / (double)std::max(plan.k, 1);
+        const int64_t waves5 = (int64_t)mf_round_grid(L5) * mf_round_wpg(L5);
+        if (1.5 * per_round <= (double)waves5) {
+            L.round_threads = 512;
+            plan.window = std::min(plan.window, waves5);  // no more workgroups than the GPU holds
+        }
+    }
     const int64_t wpg = mf_round_wpg(L);
-    RoundPlan plan = plan_rounds(c, count, std::max<int64_t>((int64_t)mf_round_grid(L) * wpg, wpg));
     L.round_grid = (int)std::max<int64_t>(plan.window / wpg, 1);
     P.window = (int64_t)L.round_grid * wpg;
     P.n_ins = nins;

@@ -41,7 +41,7 @@ SPACE = {
     "MISFIT_ROUND_MARGIN": ["", "1", "2", "3", "4"],
     "MISFIT_WINDOW_FACTOR": ["", "8", "12", "24", "32"],
     "MISFIT_ROUND_WG_PER_CU": ["", "2", "4"],
-    "MISFIT_ROUND_THREADS": ["", "128", "256"],
+    "MISFIT_ROUND_THREADS": ["", "128", "256", "512"],
     "MISFIT_PU_DENSE_FRACTION": ["", "0.01", "0.1", "1"],
 }
 
