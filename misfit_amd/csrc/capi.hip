@@ -1074,6 +1074,14 @@ static int continue_batch(misfit_ctx* c, Slot& s) {
     int64_t start = (int64_t)h[C_START0 + s.par_next];
     int64_t left = std::max<int64_t>(s.P.U - start, 0);
     if (h[C_ROUNDS] > 0) {
+        if (c->round_threads == 0) {  // the round kernels' size from the batch's own edges per round so far
+            LaunchCfg L5 = s.L;
+            L5.round_threads = 512;
+            L5.round_grid = 0;
+            const double per_round = (double)start / (double)h[C_ROUNDS];
+            s.L.round_threads =
+                1.5 * per_round <= (double)((int64_t)mf_round_grid(L5) * mf_round_wpg(L5)) ? 512 : 256;
+        }
         const int64_t wpg = mf_round_wpg(s.L);
         LaunchCfg full = s.L;
         full.round_grid = 0;
@@ -1258,10 +1266,11 @@ static int submit(misfit_ctx* c, const misfit_update* ups, const misfit_staged* 
     L.n = c->P.g.n;
     RoundPlan plan = plan_rounds(c, count, std::max<int64_t>((int64_t)mf_round_grid(L) * mf_round_wpg(L),
                                                              mf_round_wpg(L)));
-    if (c->round_threads == 0) {
+    if (c->round_threads == 0 && plan.sure) {
         // per batch: 512 threads when the edges a round is expected to
         // commit (the batch over its planned rounds), with half again as
-        // margin, fit in the 512-thread grid's waves
+        // margin, fit in the 512-thread grid's waves; without a reliable
+        // plan, 256 (a halting batch's continuation decides again)
         LaunchCfg L5 = L;
         L5.round_threads = 512;
         const double per_round = (double)count / (double)std::max(plan.k, 1);
