@@ -731,7 +731,7 @@ def many_graph_line(M, args, copy):
             cur_e = max(cur_e, b)
     if cur_e is not None:
         busy += cur_e - cur_s
-    dev_span = iv[-1][1] - iv[0][0] if iv else 0.0
+    dev_span = max(b for _, b in iv) - iv[0][0] if iv else 0.0  # first start to the latest end
     busy_frac = busy / dev_span if dev_span > 0 else 0.0
     concurrency = sum(b - a for a, b in iv) / dev_span if dev_span > 0 else 0.0
     _, wall_streams = serve_closed(0)
