@@ -1039,12 +1039,12 @@ static RoundPlan plan_rounds(misfit_ctx* c, int64_t count, int64_t full_window) 
     }
     // with enough history the second largest rate (about the 90th
     // percentile of 16), else the largest with 15% more, then a margin that
-    // grows with the planned rounds (30% of them, 1 to 3): a round
+    // grows with the planned rounds (30% of them, 1 to 2): a round
     // queued past the end costs one empty round of three kernels, a round
     // missing costs a one-workgroup round in the tail, far slower on large
     // batches
     const double base = eligible >= 6 ? std::ceil((double)count * rpu2) : std::ceil((double)count * rpu * 1.15);
-    const double mg = c->round_margin >= 0 ? (double)c->round_margin : std::min(3.0, std::max(1.0, std::ceil(0.3 * base)));
+    const double mg = c->round_margin >= 0 ? (double)c->round_margin : std::min(2.0, std::max(1.0, std::ceil(0.3 * base)));
     double k = base + mg;
     double per_window = std::ceil((double)count / (double)std::max<int64_t>(p.window, 1)) + 2.0;
     k = std::max(k, per_window);
