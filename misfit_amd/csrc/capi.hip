@@ -130,6 +130,24 @@ struct misfit_ctx {
     bool compact_selfcheck = false;  // MISFIT_COMPACT_SELFCHECK=1: full checks of every compaction's output
     void* cs_tmp = nullptr;
     size_t cs_tmp_bytes = 0;
+    // MISFIT_COMPACT_KEEP (default 1): the buffers a compaction releases (the
+    // old base rows and offsets, the old sorted copy, the deletion marks) are
+    // kept as the next compaction's outputs instead of being freed, since
+    // freeing and allocating them cost about 0.3 ms per GB; fresh compaction
+    // buffers get 1/32 more room, so the next, slightly larger output still
+    // fits.  The graph then holds about twice its base arrays.  Only the
+    // latest compaction's buffers are kept, and all of them go at reload,
+    // destroy or a failed allocation.  The serving layer turns it off: its
+    // graphs share one compaction headroom.
+    bool keep_compact = true;
+    struct Spare {
+        void* p;
+        size_t cap;
+        uint64_t gen;  // the compaction that released it
+    };
+    std::vector<Spare> spares;
+    uint64_t compact_gen = 0;
+    size_t cap_nindex = 0, cap_nlist = 0, cap_sval = 0, cap_spos = 0;  // bytes held by the graph's arrays
     int64_t ktask_cap = 0;
     int64_t ins_bound = 0;    // inserted pairs since the overlay was last cleared (upper bound)
     int64_t pool_bound = 0;   // pool slots possibly used (upper bound)
@@ -296,6 +314,8 @@ extern "C" int misfit_ctx_create(int device, misfit_ctx** out) {
     if (pd) c->pu_dense_fraction = atof(pd);
     const char* sc = getenv("MISFIT_COMPACT_SELFCHECK");
     c->compact_selfcheck = sc && sc[0] == '1';
+    const char* ck = getenv("MISFIT_COMPACT_KEEP");
+    if (ck) c->keep_compact = ck[0] != '0';
     const char* fg = getenv("MISFIT_TEST_FAIL_GROWTH");
     c->fail_growth = fg && fg[0] == '1';
     const char* swf = getenv("MISFIT_SERVE_WINDOW_FACTOR");
@@ -629,6 +649,8 @@ extern "C" int misfit_load_csr(misfit_ctx* c, int32_t n, int64_t m, const void* 
     g.m = m;
     HIPCHK(c, dalloc(A, &c->d_nindex, (int64_t)n + 1));
     HIPCHK(c, dalloc(A, &c->d_nlist, m));
+    c->cap_nindex = ((size_t)n + 1) * 8;
+    c->cap_nlist = (size_t)std::max<int64_t>(m, 1) * 4;
     if (nindex_is_64) {
         HIPCHK(c, copy_sync(c, c->d_nindex, nindex, ((size_t)n + 1) * 8, hipMemcpyHostToDevice));
     } else {
@@ -1148,6 +1170,7 @@ static int build_row_index(misfit_ctx* c) {
     if (g.sorted || g.m == 0) return MISFIT_OK;
     HIPCHK(c, hipMalloc((void**)&c->d_sval, (size_t)g.m * 4));
     HIPCHK(c, hipMalloc((void**)&c->d_spos, (size_t)g.m * 4));
+    c->cap_sval = c->cap_spos = (size_t)g.m * 4;
     mf_fill_i32((int32_t*)c->d_flags, 0, 1, c->stream);
     HIPCHK(c, mf_build_row_index(c->S, c->d_sval, c->d_spos, c->d_flags));
     int flags = 0;
@@ -1518,8 +1541,43 @@ extern "C" int misfit_batch_bytes(misfit_ctx* c, int index_bytes, int64_t* bytes
 }
 
 // ---------------------------------------------------------------- compaction
-// Compaction scratch: freed with the graph (reload, destroy).
+static void free_spares(misfit_ctx* c) {
+    for (const misfit_ctx::Spare& s : c->spares) mf_free_device(s.p);
+    c->spares.clear();
+}
+
+// A kept buffer of at least `bytes` and not much more (best fit), taken out
+// of the spares, or null; its capacity in *cap.
+static void* take_spare(misfit_ctx* c, size_t bytes, size_t* cap) {
+    int best = -1;
+    for (size_t i = 0; i < c->spares.size(); i++) {
+        const size_t k = c->spares[i].cap;
+        if (k >= bytes && k - bytes <= bytes / 4 + 4096 && (best < 0 || k < c->spares[best].cap)) best = (int)i;
+    }
+    if (best < 0) return nullptr;
+    void* p = c->spares[best].p;
+    *cap = c->spares[best].cap;
+    c->spares.erase(c->spares.begin() + best);
+    return p;
+}
+
+// A buffer the compaction no longer needs: kept for the next one, or freed.
+static void release_buf(misfit_ctx* c, void* p, size_t cap) {
+    if (!p) return;
+    if (c->keep_compact) c->spares.push_back(misfit_ctx::Spare{p, cap, c->compact_gen});
+    else mf_free_device(p);
+}
+
+// The serving layer's graphs share one compaction headroom: they keep no
+// buffers between compactions.
+void mf_ctx_set_keep_compact(misfit_ctx* c, bool on) {
+    c->keep_compact = on;
+    if (!on) free_spares(c);
+}
+
+// Compaction scratch and kept buffers: freed with the graph (reload, destroy).
 static void free_compact_scratch(misfit_ctx* c) {
+    free_spares(c);
     for (void* p : c->cs_allocs) mf_free_device(p);
     c->cs_allocs.clear();
     if (c->cs_tmp) mf_free_device(c->cs_tmp);
@@ -1590,19 +1648,37 @@ extern "C" int misfit_compact(misfit_ctx* c, int64_t* new_m) {
     if (int rc_ = settle(c)) return rc_;
     DevGraph& g = c->P.g;
     hipStream_t st = c->stream;
-    // every device buffer made here is freed unless it becomes the new graph;
-    // until then the old graph is untouched
-    std::vector<void*> mine;
+    // every device buffer taken here is released (kept or freed) unless it
+    // becomes the new graph; until then the old graph is untouched
+    c->compact_gen++;
+    std::vector<misfit_ctx::Spare> mine;
     auto grab = [&](void** p, size_t bytes) {
-        hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 4));
-        if (e == hipSuccess) mine.push_back(*p);
+        bytes = std::max<size_t>(bytes, 4);
+        size_t cap = 0;
+        if (c->keep_compact && (*p = take_spare(c, bytes, &cap))) {
+            mine.push_back(misfit_ctx::Spare{*p, cap, 0});
+            return hipSuccess;
+        }
+        cap = c->keep_compact ? bytes + bytes / 32 : bytes;
+        hipError_t e = hipMalloc(p, cap);
+        if (e != hipSuccess && !c->spares.empty()) {  // out of memory: give the kept buffers back first
+            (void)hipGetLastError();
+            free_spares(c);
+            e = hipMalloc(p, cap);
+        }
+        if (e == hipSuccess) mine.push_back(misfit_ctx::Spare{*p, cap, 0});
         return e;
     };
     auto drop_all = [&]() {
-        for (void* p : mine) mf_free_device(p);
+        for (const misfit_ctx::Spare& b : mine) release_buf(c, b.p, b.cap);
         mine.clear();
     };
-    auto keep = [&](void* p) { mine.erase(std::find(mine.begin(), mine.end(), p)); };
+    auto keep = [&](void* p) {  // the buffer's capacity
+        auto it = std::find_if(mine.begin(), mine.end(), [&](const misfit_ctx::Spare& b) { return b.p == p; });
+        const size_t cap = it->cap;
+        mine.erase(it);
+        return cap;
+    };
     int64_t* nidx = nullptr;
     int32_t* nl = nullptr;
     int64_t total = 0, ntail = 0;
@@ -1712,8 +1788,9 @@ extern "C" int misfit_compact(misfit_ctx* c, int64_t* new_m) {
         }
     }
     // the new graph replaces the old one
-    keep(nidx);
-    keep(nl);
+    Range release("misfit:compact_release");
+    const size_t cap_nidx = keep(nidx);
+    const size_t cap_nl = keep(nl);
     if (ntasks) {
         keep(ntasks);
         auto& ga = c->graph_allocs;
@@ -1724,24 +1801,38 @@ extern "C" int misfit_compact(misfit_ctx* c, int64_t* new_m) {
         c->S.walk_task_cap = walk_need;
     }
     auto& v = c->graph_allocs;
-    for (void* old : {(void*)c->d_nindex, (void*)c->d_nlist}) {
-        auto it = std::find(v.begin(), v.end(), old);
-        if (it != v.end()) { mf_free_device(*it); v.erase(it); }
+    for (auto old : {std::make_pair((void*)c->d_nindex, c->cap_nindex), std::make_pair((void*)c->d_nlist, c->cap_nlist)}) {
+        auto it = std::find(v.begin(), v.end(), old.first);
+        if (it != v.end()) {
+            v.erase(it);
+            release_buf(c, old.first, old.second);
+        }
     }
     v.push_back(nidx);
     v.push_back(nl);
     c->d_nindex = nidx;
     c->d_nlist = nl;
-    if (c->d_sval) mf_free_device(c->d_sval);
-    if (c->d_spos) mf_free_device(c->d_spos);
+    c->cap_nindex = cap_nidx;
+    c->cap_nlist = cap_nl;
+    release_buf(c, c->d_sval, c->cap_sval);
+    release_buf(c, c->d_spos, c->cap_spos);
     c->d_sval = c->d_spos = nullptr;
     if (merged) {
-        keep(nsval);
-        keep(nspos);
+        c->cap_sval = keep(nsval);
+        c->cap_spos = keep(nspos);
         c->d_sval = nsval;
         c->d_spos = nspos;
     }
     drop_all();
+    // buffers kept by an earlier compaction and not used by this one
+    for (size_t i = 0; i < c->spares.size();) {
+        if (c->spares[i].gen < c->compact_gen) {
+            mf_free_device(c->spares[i].p);
+            c->spares.erase(c->spares.begin() + i);
+        } else {
+            i++;
+        }
+    }
     g.nindex = nidx;
     g.nlist = nl;
     g.m = total;
@@ -1749,7 +1840,10 @@ extern "C" int misfit_compact(misfit_ctx* c, int64_t* new_m) {
     g.sval = c->d_sval;
     g.spos = c->d_spos;
     c->S.g = g;
-    reset_overlay(c);
+    {
+        Range r("misfit:compact_reset_overlay");
+        reset_overlay(c);
+    }
     if (!sorted && !merged) {
         int rc = build_row_index(c);
         if (rc) return rc;

@@ -95,21 +95,68 @@ __device__ __forceinline__ int64_t read_lane(int64_t v, int l) {
 }
 __device__ __forceinline__ int32_t read_lane(int32_t v, int l) { return __builtin_amdgcn_readlane(v, l); }
 
-// One 64-entry group of an entry walk, as a lane sees it.
+// Inclusive prefix maximum over the lanes of the wave, in lane order, of
+// values >= 0: within each row of 16 lanes by shifts of 1, 2, 4 and 8, then
+// across rows by the broadcasts of lanes 15 and 31 (DPP, no LDS traffic).
+__device__ __forceinline__ int wave_prefix_max(int v) {
+    int t;
+    t = __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+    v = t > v ? t : v;
+    t = __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+    v = t > v ? t : v;
+    t = __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+    v = t > v ? t : v;
+    t = __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+    v = t > v ? t : v;
+    t = __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1 and 3
+    v = t > v ? t : v;
+    t = __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2 and 3
+    v = t > v ? t : v;
+    return v;
+}
+
+// A row of the current window as the walk's table holds it in LDS: its
+// first entry and the per-row values its entries use.
 template <class Pay>
-struct WalkGroup {
-    int64_t pp;     // the lane's entry
-    int64_t q;      // pp clamped into the window: always an entry index, for unconditional loads
-    int64_t row, start;
-    Pay pay;        // the row's values (Pay::shfl of the window's)
-    bool valid;     // the lane holds an entry (pp == q)
+struct RowRec {
+    int64_t start;
+    Pay pay;
 };
 
-// Visits every entry of rows [r0, r1) once, balanced by entries like
-// entry_walk (chunks of MF_ECHUNK, 64-row windows, rows found by a 6-step
-// shuffle search), for kernels bound by memory: each step takes
-// MF_EW_GROUPS groups of 64 entries, and steps are software-pipelined in
-// three phases,
+// One step of an entry walk: MF_EW_GROUPS groups of 64 consecutive entries
+// from p0, group j's lane l holding entry p0 + 64 j + l.  Everything but
+// the row lanes is wave-uniform; rows are found through the window's table.
+#define MF_EW_GROUPS 4
+template <class Pay>
+struct WalkStep {
+    int64_t p0, lim;        // the step's first entry; entries from lim on are not this step's
+    int64_t x;              // the window's first row
+    int64_t chunk;          // the merge-path chunk of these entries
+    RowRec<Pay>* tab;       // the window's rows (LDS)
+    int rl[MF_EW_GROUPS];   // per lane: the window row (0..63) of its entry in each group
+    __device__ __forceinline__ int64_t pp(int j) const { return p0 + (int64_t)(MF_WAVE * j + lane_id()); }
+    __device__ __forceinline__ bool valid(int j) const { return pp(j) < lim; }
+    // pp clamped into the step: always an entry index, for unconditional loads
+    __device__ __forceinline__ int64_t q(int j) const {
+        const int64_t e = pp(j);
+        return e < lim ? e : lim - 1;
+    }
+    __device__ __forceinline__ int64_t start(int j) const { return tab[rl[j]].start; }
+    __device__ __forceinline__ const Pay& pay(int j) const { return tab[rl[j]].pay; }
+};
+
+// Visits every entry of rows [r0, r1) once, balanced by merge-path chunks
+// (cst, nch), for kernels bound by memory.  A wave walks its chunk through
+// windows of 64 consecutive rows: lane l loads row x + l's start and
+// rowpay(x + l), and the window's rows go to a table in LDS (two per wave,
+// used in turn by the windows that have entries, so the step still
+// waiting for its stores keeps its own).  Each step takes MF_EW_GROUPS
+// groups of 64 entries.  The step's first row is the count of window rows
+// starting at or before it; when the step spans several rows, every row
+// that starts inside it puts its window index at its start in a 256-slot
+// LDS array (max, so the last of several empty rows wins), and a prefix
+// maximum over the slots gives each entry its row.  Steps are
+// software-pipelined in three phases,
 //     load1(next step)  ->  store(this step)  ->  load2(next step),
 // where load2 may depend on load1's values and store on both.  On this
 // GPU one counter tracks loads and stores together and in order, so a
@@ -117,27 +164,29 @@ struct WalkGroup {
 // next step's loads before this step's stores keeps the store latency
 // out of every wait.  The next window's row data is also loaded while the
 // current window is walked.  Phases run on every lane (so `store` may use
-// ballots); groups past the window's end have no valid lane.  rowpay(x)
-// loads the per-row values a lane hands to its row's entries.
-#define MF_EW_GROUPS 4
-template <class Pay>
-struct WalkStep {
-    WalkGroup<Pay> g[MF_EW_GROUPS];
-    int64_t chunk;  // the merge-path chunk of these entries
-};
+// ballots); lanes past the step's end hold no entry (valid() is false) and
+// an arbitrary window row.  Workgroups of MF_BLOCK threads.
 template <class Pay, class RowPay, class Load1, class Load2, class Store>
 __device__ __forceinline__ void entry_walk4(const int64_t* nindex, int32_t r0, int32_t r1, const int64_t* cst,
                                             int64_t nch, RowPay rowpay, Load1 load1, Load2 load2, Store store) {
+    constexpr int STEP = MF_WAVE * MF_EW_GROUPS;
+    __shared__ RowRec<Pay> tabs[MF_BLOCK / MF_WAVE][2][MF_WAVE];
+    __shared__ __align__(16) int marks[MF_BLOCK / MF_WAVE][STEP];
+    static_assert(STEP == 4 * MF_WAVE, "one int4 of marks per lane");
     GC c = grid_coords();
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / MF_WAVE));
+    const int64_t wave0 = __builtin_amdgcn_readfirstlane((int)c.wave);
+    int* mark = marks[wv];
     using D = decltype(load1(WalkStep<Pay>{}));
     WalkStep<Pay> cur;
     D dcur{};
     bool have = false;  // wave-uniform: cur holds a step whose stores are due
-    for (int64_t t = c.wave; t < nch; t += c.nwaves) {
+    int slot = 1;       // the table of the latest window with entries
+    for (int64_t t = wave0; t < nch; t += c.nwaves) {
         int64_t p = cst[t];
         const int64_t tend = cst[t + 1];
         if (p >= tend) continue;  // a chunk of empty rows only
-        int64_t x = row_of(nindex, r0, r1, p, c.lane);
+        int64_t x = read_lane(row_of(nindex, r0, r1, p, c.lane), 0);
         // window loads at clamped indices, selected after
         auto win = [&](int64_t x0, int64_t& off, Pay& pay, int64_t& wend) {
             const int64_t myv = x0 + c.lane;
@@ -157,44 +206,35 @@ __device__ __forceinline__ void entry_walk4(const int64_t* nindex, int32_t r0, i
             Pay npay{};
             const int64_t lim = wend < tend ? wend : tend;
             if (lim < tend && xw < r1) win(xw, noff, npay, nwend);  // wave-uniform: the chunk goes on
-            int lf = 0;  // window row of the step's first entry (wave-uniform; only grows)
-            for (int64_t p0 = p; p0 < lim; p0 += MF_WAVE * MF_EW_GROUPS) {
+            if (lim > p) {  // wave-uniform: the window has entries
+                slot ^= 1;
+                tabs[wv][slot][c.lane] = RowRec<Pay>{off, pay};
+            }
+            RowRec<Pay>* tab = tabs[wv][slot];
+            for (int64_t p0 = p; p0 < lim; p0 += STEP) {
                 WalkStep<Pay> nx;
+                nx.p0 = p0;
+                nx.lim = lim;
+                nx.x = x;
                 nx.chunk = t;
-                while (lf < MF_WAVE - 1 && read_lane(off, lf + 1) <= p0) lf++;
+                nx.tab = tab;
+                // the step's first row: the last window row starting at or before p0
+                const int lf = __popcll(__ballot(off <= p0)) - 1;
                 const int64_t rend = lf < MF_WAVE - 1 ? read_lane(off, lf + 1) : wend;
-                if (p0 + MF_WAVE * MF_EW_GROUPS <= rend) {
-                    // the whole step lies in row lf: no search, the row's
-                    // values read from its lane by scalar reads
-                    const int64_t st = read_lane(off, lf);
-                    const Pay py = pay.read(lf);
+                if (p0 + STEP <= rend) {  // the whole step lies in row lf
 #pragma unroll
-                    for (int j = 0; j < MF_EW_GROUPS; j++) {
-                        WalkGroup<Pay>& g = nx.g[j];
-                        g.pp = p0 + MF_WAVE * j + c.lane;
-                        g.valid = g.pp < lim;
-                        g.q = g.valid ? g.pp : lim - 1;
-                        g.start = st;
-                        g.pay = py;
-                        g.row = x + lf;
-                    }
+                    for (int j = 0; j < MF_EW_GROUPS; j++) nx.rl[j] = lf;
                 } else {
+                    reinterpret_cast<int4*>(mark)[c.lane] = make_int4(0, 0, 0, 0);
+                    const int64_t rel = off - p0;
+                    if (rel >= 1 && rel < STEP) atomicMax(&mark[rel], c.lane);
+                    int carry = lf;
 #pragma unroll
                     for (int j = 0; j < MF_EW_GROUPS; j++) {
-                        WalkGroup<Pay>& g = nx.g[j];
-                        g.pp = p0 + MF_WAVE * j + c.lane;
-                        g.valid = g.pp < lim;
-                        g.q = g.valid ? g.pp : lim - 1;
-                        int l = lf, h = MF_WAVE - 1;  // rows before lf end before p0
-#pragma unroll
-                        for (int s = 0; s < 6; s++) {
-                            int mid = (l + h + 1) >> 1;
-                            int64_t om = __shfl(off, mid, MF_WAVE);
-                            if (om <= g.q) l = mid; else h = mid - 1;
-                        }
-                        g.start = __shfl(off, l, MF_WAVE);
-                        g.pay = pay.shfl(l);
-                        g.row = x + l;
+                        int s = wave_prefix_max(mark[MF_WAVE * j + c.lane]);
+                        s = s > carry ? s : carry;
+                        nx.rl[j] = s;
+                        carry = __builtin_amdgcn_readlane(s, MF_WAVE - 1);
                     }
                 }
                 D dn = load1(nx);
@@ -244,14 +284,9 @@ static void merge_path(const int64_t* nindex, int32_t r0, int32_t r1, int64_t nc
     k_merge_path<<<(unsigned)std::min<int64_t>(nb, 4096), 256, 0, st>>>(nindex, r0, r1, nch, cst);
 }
 
-struct PayNone {
-    __device__ PayNone shfl(int) const { return PayNone{}; }
-    __device__ PayNone read(int) const { return PayNone{}; }
-};
+struct PayNone {};
 struct PayI64 {
     int64_t v;
-    __device__ PayI64 shfl(int l) const { return PayI64{__shfl(v, l, MF_WAVE)}; }
-    __device__ PayI64 read(int l) const { return PayI64{read_lane(v, l)}; }
 };
 
 // ---------------------------------------------------------------- basics
@@ -661,12 +696,13 @@ __global__ __launch_bounds__(256) void k_tomb_words_sorted(const int64_t* nindex
         [&](const WalkStep<PayNone>& st) {
             D d;
 #pragma unroll
-            for (int j = 0; j < MF_EW_GROUPS; j++) d.a[j] = st.g[j].start + (int64_t)spos[st.g[j].q];
+            for (int j = 0; j < MF_EW_GROUPS; j++) d.a[j] = st.start(j) + (int64_t)spos[st.q(j)];
             return d;
         },
         [&](const WalkStep<PayNone>&, D& d) {
+            // clamped: a lane past the step's end may pair its row with another row's entry
 #pragma unroll
-            for (int j = 0; j < MF_EW_GROUPS; j++) d.word[j] = tw[d.a[j] >> 6].mask;
+            for (int j = 0; j < MF_EW_GROUPS; j++) d.word[j] = tw[d.a[j] >> 6 < nw ? d.a[j] >> 6 : nw - 1].mask;
         },
         [&](const WalkStep<PayNone>& st, const D& d) {
             if (st.chunk != chunk) {  // wave-uniform
@@ -679,11 +715,10 @@ __global__ __launch_bounds__(256) void k_tomb_words_sorted(const int64_t* nindex
             }
 #pragma unroll
             for (int j = 0; j < MF_EW_GROUPS; j++) {
-                const WalkGroup<PayNone>& g = st.g[j];
-                const int cnt = __popcll(__ballot(g.valid));
+                const int cnt = __popcll(__ballot(st.valid(j)));
                 if (cnt == 0) break;  // this and later groups are past the window's end (wave-uniform)
-                const int64_t p0 = g.pp - c.lane;  // the group's first entry
-                const u64 bal = __ballot(g.valid && ((d.word[j] >> (d.a[j] & 63)) & 1ull));
+                const int64_t p0 = st.p0 + MF_WAVE * j;  // the group's first entry
+                const u64 bal = __ballot(st.valid(j) && ((d.word[j] >> (d.a[j] & 63)) & 1ull));
                 const int sh = (int)(p0 & 63);
                 mcur |= bal << sh;
                 if (sh + cnt >= 64) {  // the word is complete
@@ -757,7 +792,7 @@ __global__ __launch_bounds__(256) void k_copy_base(DevGraph g, const int64_t* ni
         [&](const WalkStep<PayI64>& st) {
             D d;
 #pragma unroll
-            for (int j = 0; j < MF_EW_GROUPS; j++) d.y[j] = g.nlist[st.g[j].q];
+            for (int j = 0; j < MF_EW_GROUPS; j++) d.y[j] = g.nlist[st.q(j)];
             return d;
         },
         [](const WalkStep<PayI64>&, D&) {},
@@ -768,10 +803,12 @@ __global__ __launch_bounds__(256) void k_copy_base(DevGraph g, const int64_t* ni
             }
 #pragma unroll
             for (int j = 0; j < MF_EW_GROUPS; j++) {
-                const WalkGroup<PayI64>& w = st.g[j];
-                const u64 bal = __ballot(w.valid && d.y[j] < 0);
-                if (w.valid && d.y[j] >= 0)
-                    out[w.pay.v + (w.pp - w.start) - (int64_t)(grun + __popcll(bal & below_mask(lane)))] = d.y[j];
+                const bool valid = st.valid(j);
+                const u64 bal = __ballot(valid && d.y[j] < 0);
+                if (valid && d.y[j] >= 0) {
+                    const RowRec<PayI64>& r = st.tab[st.rl[j]];
+                    out[r.pay.v + (st.pp(j) - r.start) - (int64_t)(grun + __popcll(bal & below_mask(lane)))] = d.y[j];
+                }
                 grun += __popcll(bal);
             }
         });
@@ -996,14 +1033,6 @@ struct PayMerge {
     int64_t d0, t0, dst;
     int32_t tn;
     int32_t tv0, tv1;  // the row's first two sorted tail values (INT32_MAX past its tail)
-    __device__ PayMerge shfl(int l) const {
-        return PayMerge{__shfl(d0, l, MF_WAVE), __shfl(t0, l, MF_WAVE), __shfl(dst, l, MF_WAVE), __shfl(tn, l, MF_WAVE),
-                        __shfl(tv0, l, MF_WAVE), __shfl(tv1, l, MF_WAVE)};
-    }
-    __device__ PayMerge read(int l) const {
-        return PayMerge{read_lane(d0, l), read_lane(t0, l), read_lane(dst, l), read_lane(tn, l), read_lane(tv0, l),
-                        read_lane(tv1, l)};
-    }
 };
 
 template <bool HasSorted>
@@ -1012,6 +1041,7 @@ __global__ __launch_bounds__(256) void k_merge_base(OldRows o, int32_t n, MergeL
                                                     int32_t* nspos) {
     const TombWord* tw = L.tw;
     const TombWord* tws = HasSorted ? L.tws : L.tw;
+    const int64_t nw = mf_tomb_words(o.nindex[n]);
     struct D {
         int32_t v[MF_EW_GROUPS], q[MF_EW_GROUPS];
         TombWord ws[MF_EW_GROUPS], wp[MF_EW_GROUPS];
@@ -1030,41 +1060,46 @@ __global__ __launch_bounds__(256) void k_merge_base(OldRows o, int32_t n, MergeL
             D d;
 #pragma unroll
             for (int j = 0; j < MF_EW_GROUPS; j++) {
-                const WalkGroup<PayMerge>& w = st.g[j];
+                const int64_t q = st.q(j);
                 if (HasSorted) {
-                    d.q[j] = o.spos[w.q];
-                    d.v[j] = o.sval[w.q];
+                    d.q[j] = o.spos[q];
+                    d.v[j] = o.sval[q];
                 } else {
-                    d.v[j] = o.nlist[w.q];
-                    d.q[j] = (int32_t)(w.q - w.start);
+                    d.v[j] = o.nlist[q];
+                    d.q[j] = (int32_t)(q - st.start(j));
                 }
-                d.ws[j] = tws[w.q >> 6];
+                d.ws[j] = tws[q >> 6];
             }
             return d;
         },
         [&](const WalkStep<PayMerge>& st, D& d) {
             if (HasSorted) {
+                // clamped: a lane past the step's end may pair its row with another row's entry
 #pragma unroll
-                for (int j = 0; j < MF_EW_GROUPS; j++) d.wp[j] = tw[(st.g[j].start + d.q[j]) >> 6];
+                for (int j = 0; j < MF_EW_GROUPS; j++) {
+                    const int64_t w = (st.start(j) + d.q[j]) >> 6;
+                    d.wp[j] = tw[w < nw ? w : nw - 1];
+                }
             }
         },
         [&](const WalkStep<PayMerge>& st, const D& d) {
 #pragma unroll
             for (int j = 0; j < MF_EW_GROUPS; j++) {
-                const WalkGroup<PayMerge>& w = st.g[j];
-                if (!w.valid) continue;
-                const PayMerge& r = w.pay;
-                const int64_t dsort = (int64_t)d.ws[j].pre + __popcll(d.ws[j].mask & below_mask(w.pp)) - r.d0;
+                if (!st.valid(j)) continue;
+                const int64_t pp = st.pp(j);
+                const RowRec<PayMerge> rr = st.tab[st.rl[j]];
+                const PayMerge& r = rr.pay;
+                const int64_t dsort = (int64_t)d.ws[j].pre + __popcll(d.ws[j].mask & below_mask(pp)) - r.d0;
                 int64_t dpos;
                 if (HasSorted) {
-                    const int64_t a = w.start + d.q[j];
+                    const int64_t a = rr.start + d.q[j];
                     if ((d.wp[j].mask >> (a & 63)) & 1ull) continue;  // deleted
                     dpos = (int64_t)d.wp[j].pre + __popcll(d.wp[j].mask & below_mask(a)) - r.d0;
                 } else {
                     if (d.v[j] < 0) continue;  // deleted
                     dpos = dsort;
                 }
-                int64_t at = (w.pp - w.start) - dsort;
+                int64_t at = (pp - rr.start) - dsort;
                 if (r.tn <= 2) at += (r.tv0 < d.v[j] ? 1 : 0) + (r.tv1 < d.v[j] ? 1 : 0);
                 else at += lower32(tsv, r.t0, r.t0 + r.tn, d.v[j]) - r.t0;
                 nsval[r.dst + at] = d.v[j];
@@ -1078,13 +1113,6 @@ __global__ __launch_bounds__(256) void k_merge_base(OldRows o, int32_t n, MergeL
 // lock step so their loads are in flight together.
 struct PayTail {
     int64_t b, e, d0, d1, dst;
-    __device__ PayTail shfl(int l) const {
-        return PayTail{__shfl(b, l, MF_WAVE), __shfl(e, l, MF_WAVE), __shfl(d0, l, MF_WAVE), __shfl(d1, l, MF_WAVE),
-                       __shfl(dst, l, MF_WAVE)};
-    }
-    __device__ PayTail read(int l) const {
-        return PayTail{read_lane(b, l), read_lane(e, l), read_lane(d0, l), read_lane(d1, l), read_lane(dst, l)};
-    }
 };
 
 __global__ __launch_bounds__(256) void k_merge_tail(OldRows o, int32_t n, MergeLists L, const int32_t* tsv,
@@ -1104,8 +1132,8 @@ __global__ __launch_bounds__(256) void k_merge_tail(OldRows o, int32_t n, MergeL
             D d;
 #pragma unroll
             for (int j = 0; j < MF_EW_GROUPS; j++) {
-                d.wv[j] = tsv[st.g[j].q];
-                d.wi[j] = tsi[st.g[j].q];
+                d.wv[j] = tsv[st.q(j)];
+                d.wi[j] = tsi[st.q(j)];
             }
             return d;
         },
@@ -1114,8 +1142,9 @@ __global__ __launch_bounds__(256) void k_merge_tail(OldRows o, int32_t n, MergeL
             int64_t lo[MF_EW_GROUPS], hi[MF_EW_GROUPS];
 #pragma unroll
             for (int j = 0; j < MF_EW_GROUPS; j++) {
-                lo[j] = st.g[j].pay.b;
-                hi[j] = st.g[j].valid ? st.g[j].pay.e : st.g[j].pay.b;
+                const PayTail& r = st.pay(j);
+                lo[j] = r.b;
+                hi[j] = st.valid(j) ? r.e : r.b;
             }
             while (true) {
                 bool any = false;
@@ -1146,11 +1175,11 @@ __global__ __launch_bounds__(256) void k_merge_tail(OldRows o, int32_t n, MergeL
         [&](const WalkStep<PayTail>& st, const D& d) {
 #pragma unroll
             for (int j = 0; j < MF_EW_GROUPS; j++) {
-                const WalkGroup<PayTail>& g = st.g[j];
-                if (!g.valid) continue;
-                const PayTail& r = g.pay;
+                if (!st.valid(j)) continue;
+                const RowRec<PayTail> rr = st.tab[st.rl[j]];
+                const PayTail& r = rr.pay;
                 const int64_t dl = (int64_t)d.t[j].pre + __popcll(d.t[j].mask & below_mask(d.lo[j])) - r.d0;
-                const int64_t at = (d.lo[j] - r.b) - dl + (g.pp - g.start);
+                const int64_t at = (d.lo[j] - r.b) - dl + (st.pp(j) - rr.start);
                 const int64_t kept = (r.e - r.b) - (r.d1 - r.d0);
                 nsval[r.dst + at] = d.wv[j];
                 nspos[r.dst + at] = (int32_t)(kept + d.wi[j]);

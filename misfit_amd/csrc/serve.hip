@@ -55,6 +55,8 @@ int mf_ctx_prepare_external(misfit_ctx* c, const misfit_update* ups, int64_t cou
 int mf_ctx_external_ready(misfit_ctx* c, uint64_t ticket);
 // capi.hip: every batch of the context may halt (one batch per graph).
 void mf_ctx_set_halting(misfit_ctx* c, bool on);
+// capi.hip: whether compactions keep their released buffers for the next one.
+void mf_ctx_set_keep_compact(misfit_ctx* c, bool on);
 
 namespace {
 using clk = std::chrono::steady_clock;
@@ -574,6 +576,7 @@ extern "C" int misfit_server_add_graph(misfit_server* s, int32_t n, int64_t m, c
     int rc = misfit_ctx_create(s->device, &g->ctx);
     std::string em = "cannot create a context";
     if (rc == MISFIT_OK) {
+        mf_ctx_set_keep_compact(g->ctx, false);
         rc = misfit_load_csr(g->ctx, n, m, nindex, nindex_is_64, nlist);
         if (rc == MISFIT_OK) rc = misfit_set_mis(g->ctx, mis);
         if (rc == MISFIT_OK) rc = misfit_set_mode(g->ctx, MISFIT_MODE_QUEUED);

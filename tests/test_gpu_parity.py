@@ -113,6 +113,36 @@ def test_compaction_parity(oracle, engine_mod, eng, mode):
     assert eng.check() == (0, 0, 0)
 
 
+def test_repeated_compactions_reuse_buffers(oracle, engine_mod, eng):
+    """One context compacted four times while its base grows a little and
+    shrinks more: each compaction's outputs come from the buffers the one
+    before released where they fit (the old base, offsets, sorted copy and
+    deletion marks), and every compacted base equals the oracle's."""
+    rng = np.random.default_rng(23)
+    ni, nl = H.random_graph(rng, 3000, 0.004)
+    mis = oracle.Graph.from_csr(ni, nl).greedy_mis()
+    eng.load_csr(ni, nl)
+    eng.set_mis(mis)
+    eng.set_rule(True)
+    for k in range(4):
+        if k % 2 == 0:  # insertions of new pairs: rows grow past their ascending order
+            batches = H.random_batches(rng, oracle, ni, nl, 2, 200)
+        else:  # deletions of existing entries
+            src = np.repeat(np.arange(len(ni) - 1, dtype=np.int32), np.diff(ni))
+            pick = [rng.integers(0, len(nl), 1500) for _ in range(2)]
+            batches = [(src[p], nl[p], np.zeros(len(p), np.uint8)) for p in pick]
+        want, _, og = H.oracle_replay(oracle, ni, nl, mis, batches)
+        for (u, v, ins), w in zip(batches, want):
+            eng.process_batch(engine_mod.make_updates(u, v, ins))
+            assert np.array_equal(eng.get_mis(), w), "phase %d" % k
+        oni, onl = og.compact()
+        assert eng.compact() == len(onl)
+        eni, enl = eng.export_csr()
+        assert np.array_equal(eni, oni) and np.array_equal(enl, onl), "compaction %d" % (k + 1)
+        ni, nl, mis = oni, onl, want[-1]
+    assert eng.check() == (0, 0, 0)
+
+
 @pytest.mark.parametrize("mode", MODES)
 def test_er_config_parity(oracle, engine_mod, eng, mode):
     """Erdos-Renyi n=100,000, 800,000 pairs, seed 42; greedy MIS; 10 batches
