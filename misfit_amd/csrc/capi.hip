@@ -1697,7 +1697,6 @@ extern "C" int misfit_compact(misfit_ctx* c, int64_t* new_m) {
     if (e == hipSuccess) e = ensure_compact_scratch(c, std::max<int64_t>(ntail, 1));  // tail lists never empty
     const int64_t nwords = mf_tomb_words(g.m);
     L.ncst = mf_path_chunks(g.n, g.m);
-    L.ntcst = mf_path_chunks(g.n, ntail);
     // counting chunks: per MF_TCHUNK entries (position order) or per merge-path chunk (sorted order)
     const int64_t nchunks = std::max(mf_tomb_chunks(g.m), L.ncst);
     if (e == hipSuccess) e = grab((void**)&L.tw, (size_t)nwords * sizeof(TombWord));
@@ -1706,7 +1705,6 @@ extern "C" int misfit_compact(misfit_ctx* c, int64_t* new_m) {
     if (e == hipSuccess) e = grab((void**)&L.cpre, ((size_t)nchunks + 1) * 8);
     if (e == hipSuccess) e = grab((void**)&L.cpart, ((size_t)nchunks / 2048 + 2) * 8);
     if (e == hipSuccess) e = grab((void**)&L.cst, ((size_t)L.ncst + 1) * 8);
-    if (e == hipSuccess) e = grab((void**)&L.tcst, ((size_t)L.ntcst + 1) * 8);
     L.doff = c->cs_doff;
     L.tval = c->cs_tval;
     L.tidx = c->cs_tidx;

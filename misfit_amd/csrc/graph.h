@@ -75,8 +75,6 @@ struct MergeLists {
     int64_t* cpart = nullptr;        // scan scratch
     int64_t* cst = nullptr;          // merge-path chunk starts of the old rows (ncst + 1 entries)
     int64_t ncst = 0;
-    int64_t* tcst = nullptr;         // ... of the tail lists (ntcst + 1 entries)
-    int64_t ntcst = 0;
     // tail sort: sorted tail values and their insertion indices, rows whose
     // tail is long, a counter, and the library sort's temporary storage
     int32_t* tsv = nullptr;

@@ -1024,37 +1024,62 @@ __device__ __forceinline__ int64_t lower32(const int32_t* a, int64_t lo, int64_t
     return lo;
 }
 
-// Entries of the old sorted order (entry_walk4), with each row's deletion
-// and tail offsets and destination loaded once per 64-row window by the
-// lane that owns the row and shuffled to its entries.  HasSorted: the old
-// rows had a sorted copy (else they were ascending and are their own
-// sorted order).
+// Wave shift by one lane: lane l gets v of lane l - 1, lane 0 gets 0 (DPP).
+__device__ __forceinline__ int wave_shr1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false); }
+
+// The merge, one walk of the old sorted order (entry_walk4).  Each old entry
+// k of a row, value v, kept or deleted, with t(k) = #{tail values < v}:
+//   - a kept entry lands at (kept entries before k) + t(k);
+//   - it places the tail entries of sorted rank j in [t(k-1), t(k)) (values
+//     between the old values k-1 and k) at (kept entries before k) + j, and
+//     the row's last old entry those of rank j >= t(k) at (kept entries of
+//     the row) + j; rows with no old entry get their tails in
+//     k_merge_tail_empty.
+// t(k-1) comes from the lane before (a wave shift), from the group before,
+// from the step before when it ended at k-1, or, at a chunk's start inside a
+// row, from the old value k-1 itself.  A row's values come from the walk's
+// window table, with its first two sorted tail values and their insertion
+// indices; longer tails are searched.  HasSorted: the old rows had a sorted
+// copy (else they were ascending and are their own sorted order).
 struct PayMerge {
-    int64_t d0, t0, dst;
-    int32_t tn;
-    int32_t tv0, tv1;  // the row's first two sorted tail values (INT32_MAX past its tail)
+    int64_t d0, t0, dst;  // deletions before the row; its tail's offset; its new start
+    int64_t end;          // the old row's end
+    int32_t tn, kr;       // tail entries; kept old entries
+    int32_t tv0, tv1;     // the first two sorted tail values (INT32_MAX past the tail)
+    int32_t ti0, ti1;     // ... and their insertion indices
 };
 
 template <bool HasSorted>
 __global__ __launch_bounds__(256) void k_merge_base(OldRows o, int32_t n, MergeLists L, const int32_t* tsv,
-                                                    int64_t ntail, const int64_t* nidx, int32_t* nsval,
-                                                    int32_t* nspos) {
+                                                    const int32_t* tsi, int64_t ntail, const int64_t* nidx,
+                                                    int32_t* nsval, int32_t* nspos) {
     const TombWord* tw = L.tw;
     const TombWord* tws = HasSorted ? L.tws : L.tw;
     const int64_t nw = mf_tomb_words(o.nindex[n]);
+    const int lane = lane_id();
+    int64_t carry_p = -2;  // wave-uniform: the last entry of the step stored before, and its t
+    int carry_t = 0;
     struct D {
         int32_t v[MF_EW_GROUPS], q[MF_EW_GROUPS];
         TombWord ws[MF_EW_GROUPS], wp[MF_EW_GROUPS];
+    };
+    // #{tail values < v} in a row
+    auto tails_below = [&](const PayMerge& r, int32_t v) {
+        if (r.tn <= 2) return (r.tv0 < v ? 1 : 0) + (r.tv1 < v ? 1 : 0);
+        return (int)(lower32(tsv, r.t0, r.t0 + r.tn, v) - r.t0);
     };
     entry_walk4<PayMerge>(
         o.nindex, 0, n, L.cst, L.ncst,
         [&](int64_t x) {
             const int64_t t0 = L.toff[x];
             const int32_t tn = (int32_t)(L.toff[x + 1] - t0);
-            // a row's tail of at most two entries rides along with the row
-            // (clamped loads: tsv holds at least one entry even when no row has a tail)
-            const int32_t a = tsv[t0 < ntail ? t0 : 0], b = tsv[t0 + 1 < ntail ? t0 + 1 : 0];
-            return PayMerge{L.doff[x], t0, nidx[x], tn, tn >= 1 ? a : INT32_MAX, tn >= 2 ? b : INT32_MAX};
+            // a row's first two tail entries ride along with the row (clamped
+            // loads: the tail lists hold at least one entry even when no row has a tail)
+            const int64_t i0 = t0 < ntail ? t0 : 0, i1 = t0 + 1 < ntail ? t0 + 1 : 0;
+            const int32_t a = tsv[i0], b = tsv[i1], ia = tsi[i0], ib = tsi[i1];
+            const int64_t dst = nidx[x];
+            return PayMerge{L.doff[x], t0, dst, o.nindex[x + 1], tn, (int32_t)(nidx[x + 1] - dst - tn),
+                            tn >= 1 ? a : INT32_MAX, tn >= 2 ? b : INT32_MAX, ia, ib};
         },
         [&](const WalkStep<PayMerge>& st) {
             D d;
@@ -1083,108 +1108,94 @@ __global__ __launch_bounds__(256) void k_merge_base(OldRows o, int32_t n, MergeL
             }
         },
         [&](const WalkStep<PayMerge>& st, const D& d) {
+            // t of the entry before the step's first one (wave-uniform, used by lane 0)
+            int tprev0;
+            if (carry_p == st.p0 - 1) {
+                tprev0 = carry_t;
+            } else {  // a chunk's first step: from the old value before it, when the row has one
+                tprev0 = 0;
+                const int64_t s0 = read_lane(st.start(0), 0);
+                if (st.p0 > s0) {
+                    const PayMerge r0 = st.tab[__builtin_amdgcn_readfirstlane(st.rl[0])].pay;
+                    tprev0 = __builtin_amdgcn_readfirstlane(tails_below(r0, old_sorted_val(o, st.p0 - 1)));
+                }
+            }
+            int tk[MF_EW_GROUPS];
 #pragma unroll
             for (int j = 0; j < MF_EW_GROUPS; j++) {
-                if (!st.valid(j)) continue;
                 const int64_t pp = st.pp(j);
-                const RowRec<PayMerge> rr = st.tab[st.rl[j]];
-                const PayMerge& r = rr.pay;
-                const int64_t dsort = (int64_t)d.ws[j].pre + __popcll(d.ws[j].mask & below_mask(pp)) - r.d0;
-                int64_t dpos;
+                const bool valid = pp < st.lim;
+                const RowRec<PayMerge>& rr = st.tab[st.rl[j]];  // LDS
+                const int64_t start = rr.start, d0 = rr.pay.d0, t0 = rr.pay.t0, dst = rr.pay.dst;
+                const int32_t tn = rr.pay.tn;
+                int32_t v = d.v[j];
+                bool deleted;
+                int64_t dpos = 0;
+                const int64_t dsort = (int64_t)d.ws[j].pre + __popcll(d.ws[j].mask & below_mask(pp)) - d0;
                 if (HasSorted) {
-                    const int64_t a = rr.start + d.q[j];
-                    if ((d.wp[j].mask >> (a & 63)) & 1ull) continue;  // deleted
-                    dpos = (int64_t)d.wp[j].pre + __popcll(d.wp[j].mask & below_mask(a)) - r.d0;
+                    const int64_t a = start + d.q[j];
+                    deleted = (d.wp[j].mask >> (a & 63)) & 1ull;
+                    dpos = (int64_t)d.wp[j].pre + __popcll(d.wp[j].mask & below_mask(a)) - d0;
                 } else {
-                    if (d.v[j] < 0) continue;  // deleted
+                    deleted = v < 0;
+                    if (deleted) v = ~v;
                     dpos = dsort;
                 }
-                int64_t at = (pp - rr.start) - dsort;
-                if (r.tn <= 2) at += (r.tv0 < d.v[j] ? 1 : 0) + (r.tv1 < d.v[j] ? 1 : 0);
-                else at += lower32(tsv, r.t0, r.t0 + r.tn, d.v[j]) - r.t0;
-                nsval[r.dst + at] = d.v[j];
-                nspos[r.dst + at] = d.q[j] - (int32_t)dpos;
+                const int32_t tv0 = rr.pay.tv0, tv1 = rr.pay.tv1;
+                tk[j] = !valid ? 0
+                        : tn <= 2 ? (tv0 < v ? 1 : 0) + (tv1 < v ? 1 : 0)
+                                  : (int)(lower32(tsv, t0, t0 + tn, v) - t0);
+                // t of the entry before this lane's
+                const int tfirst = j == 0 ? tprev0 : __builtin_amdgcn_readlane(tk[j - 1], MF_WAVE - 1);
+                int tp = wave_shr1(tk[j]);
+                if (lane == 0) tp = tfirst;
+                if (pp == start) tp = 0;
+                if (!valid) continue;
+                const int64_t kb = (pp - start) - dsort;  // kept entries before this one
+                if (!deleted) {
+                    const int64_t at = kb + tk[j];
+                    nsval[dst + at] = v;
+                    nspos[dst + at] = d.q[j] - (int32_t)dpos;
+                }
+                const int hi = pp + 1 == rr.pay.end ? tn : tk[j];
+                if (tp < hi) {
+                    const int32_t kr = rr.pay.kr, ti0 = rr.pay.ti0, ti1 = rr.pay.ti1;
+                    for (int t = tp; t < hi; t++) {
+                        const int64_t at = t < tk[j] ? kb + t : kb + (deleted ? 0 : 1) + t;
+                        int32_t w = t == 0 ? tv0 : tv1, wi = t == 0 ? ti0 : ti1;  // values, not addresses
+                        if (tn > 2) {
+                            w = tsv[t0 + t];
+                            wi = tsi[t0 + t];
+                        }
+                        nsval[dst + at] = w;
+                        nspos[dst + at] = kr + wi;
+                    }
+                }
             }
+            // the step's last entry, for the next step
+            const int64_t last = (st.lim < st.p0 + MF_WAVE * MF_EW_GROUPS ? st.lim : st.p0 + MF_WAVE * MF_EW_GROUPS) - 1;
+            const int gl = (int)((last - st.p0) >> 6), ll = (int)((last - st.p0) & 63);
+            const int tl = gl == 0 ? tk[0] : gl == 1 ? tk[1] : gl == 2 ? tk[2] : tk[3];
+            carry_t = __builtin_amdgcn_readlane(tl, ll);
+            carry_p = last;
         });
 }
 
-// Tail entries (entry_walk4 over the tail lists): each one's rank among
-// the old row's values by binary search, the searches of all groups in
-// lock step so their loads are in flight together.
-struct PayTail {
-    int64_t b, e, d0, d1, dst;
-};
-
-__global__ __launch_bounds__(256) void k_merge_tail(OldRows o, int32_t n, MergeLists L, const int32_t* tsv,
-                                                    const int32_t* tsi, const int64_t* nidx, int32_t* nsval,
-                                                    int32_t* nspos) {
-    const TombWord* tws = o.sval ? L.tws : L.tw;
-    const int64_t mlast = o.nindex[n] - 1;  // rows with a tail may all be empty in the old graph (then no search runs)
-    struct D {
-        int32_t wv[MF_EW_GROUPS], wi[MF_EW_GROUPS];
-        int64_t lo[MF_EW_GROUPS];
-        TombWord t[MF_EW_GROUPS];
-    };
-    entry_walk4<PayTail>(
-        L.toff, 0, n, L.tcst, L.ntcst,
-        [&](int64_t x) { return PayTail{o.nindex[x], o.nindex[x + 1], L.doff[x], L.doff[x + 1], nidx[x]}; },
-        [&](const WalkStep<PayTail>& st) {
-            D d;
-#pragma unroll
-            for (int j = 0; j < MF_EW_GROUPS; j++) {
-                d.wv[j] = tsv[st.q(j)];
-                d.wi[j] = tsi[st.q(j)];
-            }
-            return d;
-        },
-        [&](const WalkStep<PayTail>& st, D& d) {
-            // old values below w, all searches a step at a time
-            int64_t lo[MF_EW_GROUPS], hi[MF_EW_GROUPS];
-#pragma unroll
-            for (int j = 0; j < MF_EW_GROUPS; j++) {
-                const PayTail& r = st.pay(j);
-                lo[j] = r.b;
-                hi[j] = st.valid(j) ? r.e : r.b;
-            }
-            while (true) {
-                bool any = false;
-#pragma unroll
-                for (int j = 0; j < MF_EW_GROUPS; j++) any |= lo[j] < hi[j];
-                if (!__ballot(any)) break;
-                int32_t mv[MF_EW_GROUPS];
-                int64_t mid[MF_EW_GROUPS];
-#pragma unroll
-                for (int j = 0; j < MF_EW_GROUPS; j++) {
-                    mid[j] = (lo[j] + hi[j]) >> 1;
-                    const int64_t cl = mid[j] < mlast ? mid[j] : mlast;
-                    mv[j] = o.sval ? o.sval[cl] : decode(o.nlist[cl]);
-                }
-#pragma unroll
-                for (int j = 0; j < MF_EW_GROUPS; j++)
-                    if (lo[j] < hi[j]) {
-                        if (mv[j] < d.wv[j]) lo[j] = mid[j] + 1;
-                        else hi[j] = mid[j];
-                    }
-            }
-#pragma unroll
-            for (int j = 0; j < MF_EW_GROUPS; j++) {
-                d.lo[j] = lo[j];
-                d.t[j] = tws[lo[j] >> 6];
-            }
-        },
-        [&](const WalkStep<PayTail>& st, const D& d) {
-#pragma unroll
-            for (int j = 0; j < MF_EW_GROUPS; j++) {
-                if (!st.valid(j)) continue;
-                const RowRec<PayTail> rr = st.tab[st.rl[j]];
-                const PayTail& r = rr.pay;
-                const int64_t dl = (int64_t)d.t[j].pre + __popcll(d.t[j].mask & below_mask(d.lo[j])) - r.d0;
-                const int64_t at = (d.lo[j] - r.b) - dl + (st.pp(j) - rr.start);
-                const int64_t kept = (r.e - r.b) - (r.d1 - r.d0);
-                nsval[r.dst + at] = d.wv[j];
-                nspos[r.dst + at] = (int32_t)(kept + d.wi[j]);
-            }
-        });
+// Tail entries of rows with no old entry: the whole new row, in sorted order.
+__global__ __launch_bounds__(256) void k_merge_tail_empty(const int64_t* nindex, MergeLists L, const int32_t* tsv,
+                                                          const int32_t* tsi, const int64_t* nidx, int32_t* nsval,
+                                                          int32_t* nspos) {
+    GC c = grid_coords();
+    const int64_t nr = (int64_t)*L.ntrows;
+    for (int64_t i = c.tid; i < nr; i += c.nthreads) {
+        const int32_t x = L.trows[i];
+        if (nindex[x + 1] != nindex[x]) continue;
+        const int64_t t0 = L.toff[x], tn = L.toff[x + 1] - t0, dst = nidx[x];
+        for (int64_t j = 0; j < tn; j++) {
+            nsval[dst + j] = tsv[t0 + j];
+            nspos[dst + j] = tsi[t0 + j];
+        }
+    }
 }
 
 // Per-row deletion and tail counts, and the rows with a tail listed (one
@@ -1410,12 +1421,9 @@ hipError_t mf_merge_row_index(GraphState& S, const MergeLists& L, int64_t ntail,
     hipError_t e = sort_lists(S, L.toff, L.trows, L.ntrows, ntail, L.tval, L.tidx, L.tsv, L.tsi, L);
     if (e == hipSuccess) {
         OldRows o{g.nindex, g.nlist, g.sval, g.spos};
-        if (g.sval) k_merge_base<true><<<S.blocks, 256, 0, st>>>(o, g.n, L, L.tsv, ntail, nidx, nsval, nspos);
-        else k_merge_base<false><<<S.blocks, 256, 0, st>>>(o, g.n, L, L.tsv, ntail, nidx, nsval, nspos);
-        if (ntail) {
-            merge_path(L.toff, 0, g.n, L.ntcst, L.tcst, st);
-            k_merge_tail<<<S.blocks, 256, 0, st>>>(o, g.n, L, L.tsv, L.tsi, nidx, nsval, nspos);
-        }
+        if (g.sval) k_merge_base<true><<<S.blocks, 256, 0, st>>>(o, g.n, L, L.tsv, L.tsi, ntail, nidx, nsval, nspos);
+        else k_merge_base<false><<<S.blocks, 256, 0, st>>>(o, g.n, L, L.tsv, L.tsi, ntail, nidx, nsval, nspos);
+        if (ntail) k_merge_tail_empty<<<S.blocks, 256, 0, st>>>(g.nindex, L, L.tsv, L.tsi, nidx, nsval, nspos);
         if (check) k_check_row_index<<<S.blocks, 256, 0, st>>>(nidx, nnl, nsval, nspos, g.n, d_flags);
         e = hipGetLastError();
     }
