@@ -38,6 +38,20 @@ struct Range {
     Range(const Range&) = delete;
     Range& operator=(const Range&) = delete;
 };
+
+// Consecutive roctx ranges of one call's host phases: to() ends the phase
+// before it and starts the next; the last one ends with the object.
+struct Phases {
+    bool open = false;
+    void to(const char* name) {
+        if (open) roctxRangePop();
+        roctxRangePushA(name);
+        open = true;
+    }
+    ~Phases() {
+        if (open) roctxRangePop();
+    }
+};
 }  // namespace
 
 #define MF_NSLOT 4
@@ -705,7 +719,7 @@ extern "C" int misfit_load_csr(misfit_ctx* c, int32_t n, int64_t m, const void* 
     HIPCHK(c, dalloc(A, &c->d_ctr, C_NCOUNTERS * MF_CS));
     HIPCHK(c, dalloc(A, &c->d_round, 1));
     HIPCHK(c, dalloc(A, &c->d_acc, 8));
-    HIPCHK(c, dalloc(A, &c->d_flags, 1));
+    HIPCHK(c, dalloc(A, &c->d_flags, 2));
     P.ctr = c->d_ctr;
     P.round = c->d_round;
     GraphState& S = c->S;
@@ -1683,17 +1697,29 @@ extern "C" int misfit_compact(misfit_ctx* c, int64_t* new_m) {
     int32_t* nl = nullptr;
     int64_t total = 0, ntail = 0;
     MergeLists L;
+    Phases ph;
+    ph.to("compact:plan");
     hipError_t e = grab((void**)&nidx, ((size_t)g.n + 1) * 8);
     if (e == hipSuccess) e = mf_compact_plan(c->S, nidx);
+    ph.to("compact:counts");
     // per-row deletion counts and tail lists; the deletion marks of the old
     // rows (position order, and sorted order when a sorted copy exists) and
     // the counting chunks' sums are transient
     if (e == hipSuccess) e = ensure_compact_scratch(c, 0);
     if (e == hipSuccess)
         e = mf_merge_counts(c->S, c->cs_dcnt, c->cs_tcnt, c->cs_doff, c->cs_toff, c->cs_trows, c->cs_cnt);
-    if (e == hipSuccess) e = hipMemcpyAsync(&total, nidx + g.n, 8, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(&ntail, c->cs_toff + g.n, 8, hipMemcpyDeviceToHost, st);
+    // two read-backs in all, into pinned memory (asynchronous copies): the
+    // sizes here, every flag at the end
+    ph.to("compact:sizes");
+    u64* hs = c->h_small;
+    if (e == hipSuccess) e = hipMemcpyAsync(hs, nidx + g.n, 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(hs + 1, c->cs_toff + g.n, 8, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess) {
+        total = (int64_t)hs[0];
+        ntail = (int64_t)hs[1];
+    }
+    ph.to("compact:alloc");
     if (e == hipSuccess) e = ensure_compact_scratch(c, std::max<int64_t>(ntail, 1));  // tail lists never empty
     const int64_t nwords = mf_tomb_words(g.m);
     L.ncst = mf_path_chunks(g.n, g.m);
@@ -1718,61 +1744,54 @@ extern "C" int misfit_compact(misfit_ctx* c, int64_t* new_m) {
     L.tmp = &c->cs_tmp;
     L.tmp_bytes = &c->cs_tmp_bytes;
     if (e == hipSuccess) e = grab((void**)&nl, (size_t)total * 4);
+    // The sorted row copy is merged unless the new rows are ascending for
+    // sure (they were, and no row has a tail), without waiting for the copy
+    // to tell: when every tail happens to continue its row in ascending
+    // order, the merged copy is dropped afterwards.  Tail lists too long to
+    // merge take the full sort below.
+    const bool merge = total > 0 && !(g.sorted && ntail == 0) && ntail <= (int64_t)INT32_MAX;
+    int32_t *nsval = nullptr, *nspos = nullptr;
+    if (e == hipSuccess && merge) e = grab((void**)&nsval, (size_t)total * 4);
+    if (e == hipSuccess && merge) e = grab((void**)&nspos, (size_t)total * 4);
+    ph.to("compact:copy");
+    // d_flags[0]: 1 when the new rows are not ascending (k_copy_runs tests
+    // whether every tail continues its row in ascending order), 16 and 32
+    // from the merge's self-check; d_flags[1]: the self-check of the rows
     if (e == hipSuccess) e = mf_compact_copy(c->S, nidx, nl, L, c->d_flags);
-    // the new rows are ascending when the old ones were and every tail
-    // continues its row in ascending order (k_copy_runs tests that)
-    int flags = 0;
-    if (e == hipSuccess) e = hipMemcpyAsync(&flags, c->d_flags, sizeof(int), hipMemcpyDeviceToHost, st);
+    GraphState NS = c->S;
+    NS.g.nindex = nidx;
+    NS.g.nlist = nl;
+    NS.g.m = total;
+    if (e == hipSuccess && c->compact_selfcheck) {  // the new rows' offsets, entries and order
+        mf_fill_i32((int32_t*)c->d_flags + 1, 0, 1, st);
+        e = mf_validate_csr(NS, c->d_flags + 1);
+        if (e == hipSuccess) e = mf_validate_entries(NS, c->d_flags + 1);
+    }
+    ph.to("compact:merge");
+    if (e == hipSuccess && merge)
+        e = mf_merge_row_index(c->S, L, ntail, nidx, nl, nsval, nspos, c->d_flags, c->compact_selfcheck);
+    ph.to("compact:flags");
+    if (e == hipSuccess) e = hipMemcpyAsync(hs, c->d_flags, 2 * sizeof(int), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
         drop_all();
         HIPCHK(c, e);
     }
-    const bool sorted = (flags & 1) == 0;
-    GraphState NS = c->S;
-    NS.g.nindex = nidx;
-    NS.g.nlist = nl;
-    NS.g.m = total;
-    if (c->compact_selfcheck) {  // the new rows' offsets, entries and order, against the flag above
-        int vflags = 0;
-        e = mf_validate_csr(NS, c->d_flags);
-        if (e == hipSuccess) e = mf_validate_entries(NS, c->d_flags);
-        if (e == hipSuccess) e = hipMemcpyAsync(&vflags, c->d_flags, sizeof(int), hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
-        if (e != hipSuccess) {
-            drop_all();
-            HIPCHK(c, e);
-        }
-        if ((vflags & ~1) || (sorted && (vflags & 1))) {
-            drop_all();
-            return fail(c, MISFIT_E_CHECK, "compacted rows failed the self-check (flags " + std::to_string(vflags) + ")");
-        }
+    int flags[2];
+    memcpy(flags, hs, sizeof flags);
+    const bool sorted = (flags[0] & 1) == 0;
+    if (c->compact_selfcheck && ((flags[1] & ~1) || (sorted && (flags[1] & 1)))) {
+        drop_all();
+        return fail(c, MISFIT_E_CHECK, "compacted rows failed the self-check (flags " + std::to_string(flags[1]) + ")");
     }
-    int32_t *nsval = nullptr, *nspos = nullptr;
-    bool merged = false;
-    if (!sorted && total > 0) {
-        e = grab((void**)&nsval, (size_t)total * 4);
-        if (e == hipSuccess) e = grab((void**)&nspos, (size_t)total * 4);
-        if (e == hipSuccess) {
-            mf_fill_i32((int32_t*)c->d_flags, 0, 1, st);
-            e = mf_merge_row_index(c->S, L, ntail, nidx, nl, nsval, nspos, c->d_flags, c->compact_selfcheck);
-            if (e == hipSuccess) e = hipMemcpyAsync(&flags, c->d_flags, sizeof(int), hipMemcpyDeviceToHost, st);
-            if (e == hipSuccess) e = hipStreamSynchronize(st);
-            if (e == hipSuccess) merged = true;
-            else if (e == hipErrorInvalidValue) e = hipSuccess;  // lists too long to merge: full sort below
-        }
-        if (e != hipSuccess) {
-            drop_all();
-            HIPCHK(c, e);
-        }
-        if (merged && (flags & 16)) {
-            drop_all();
-            return fail(c, MISFIT_E_UNSUPPORTED, "a row lists a neighbour twice");
-        }
-        if (merged && (flags & 32)) {
-            drop_all();
-            return fail(c, MISFIT_E_CHECK, "merged sorted row copy does not match the compacted rows");
-        }
+    const bool merged = merge && !sorted;
+    if (merged && (flags[0] & 16)) {
+        drop_all();
+        return fail(c, MISFIT_E_UNSUPPORTED, "a row lists a neighbour twice");
+    }
+    if (merged && (flags[0] & 32)) {
+        drop_all();
+        return fail(c, MISFIT_E_CHECK, "merged sorted row copy does not match the compacted rows");
     }
     // the edge walk's split tasks follow the new base's size (allocated
     // before the swap: on failure the old graph stays as it was)
@@ -1786,7 +1805,7 @@ extern "C" int misfit_compact(misfit_ctx* c, int64_t* new_m) {
         }
     }
     // the new graph replaces the old one
-    Range release("misfit:compact_release");
+    ph.to("compact:release");
     const size_t cap_nidx = keep(nidx);
     const size_t cap_nl = keep(nl);
     if (ntasks) {
@@ -1838,10 +1857,8 @@ extern "C" int misfit_compact(misfit_ctx* c, int64_t* new_m) {
     g.sval = c->d_sval;
     g.spos = c->d_spos;
     c->S.g = g;
-    {
-        Range r("misfit:compact_reset_overlay");
-        reset_overlay(c);
-    }
+    ph.to("compact:reset_overlay");
+    reset_overlay(c);
     if (!sorted && !merged) {
         int rc = build_row_index(c);
         if (rc) return rc;

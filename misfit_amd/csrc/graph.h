@@ -101,8 +101,8 @@ hipError_t mf_build_row_index(GraphState& S, int32_t* sval, int32_t* spos, int* 
 hipError_t mf_merge_counts(GraphState& S, int64_t* dcnt, int64_t* tcnt, int64_t* doff, int64_t* toff, int32_t* trows,
                            u64* ntrows);
 // sorted copy of the compacted rows nidx/nnl by merging the old graph's (S) sorted order with the
-// tails compaction collected; with `check`, the result is checked entry by entry (same flags as
-// mf_build_row_index)
+// tails compaction collected, queued on S's stream without waiting; with `check`, the result is
+// checked entry by entry (the flags of mf_build_row_index, ORed into *d_flags)
 hipError_t mf_merge_row_index(GraphState& S, const MergeLists& L, int64_t ntail, const int64_t* nidx,
                               const int32_t* nnl, int32_t* nsval, int32_t* nspos, int* d_flags, bool check);
 void mf_scan_excl(const int64_t* in, int64_t count, int64_t* out, int64_t* part, hipStream_t st);

@@ -610,6 +610,22 @@ __global__ __launch_bounds__(256) void k_newdeg(DevGraph g, int64_t* deg) {
 
 __device__ __forceinline__ u64 below_mask(int64_t p) { return (1ull << (p & 63)) - 1ull; }
 
+// A deletion-mark word read as three 32-bit words: the mask and the low half
+// of pre.  Differences of pre inside one row are below 2^31, so they come out
+// exact in 32-bit arithmetic.
+struct TombWord3 {
+    u64 mask;
+    uint32_t pre;
+};
+__device__ __forceinline__ TombWord3 load_tw3(const TombWord* t) {
+    const uint3 w = *reinterpret_cast<const uint3*>(t);
+    return TombWord3{(u64)w.x | ((u64)w.y << 32), w.z};
+}
+// deletions of the row (whose first deletion-mark count is d0) before entry p, from the word of p
+__device__ __forceinline__ int32_t tomb_before(const TombWord3& t, int64_t p, int64_t d0) {
+    return (int32_t)(t.pre + (uint32_t)__popcll(t.mask & ((1ull << (p & 63)) - 1ull)) - (uint32_t)d0);
+}
+
 // deleted entries of the array before entry p
 __device__ __forceinline__ int64_t tomb_rank(const TombWord* tw, int64_t p) {
     const TombWord t = tw[p >> 6];
@@ -820,28 +836,48 @@ __global__ __launch_bounds__(256) void k_copy_base(DevGraph g, const int64_t* ni
 // ascending after its tail was appended sets flags |= 1 (its kept part is
 // ascending when the old rows were, so only the tail and its first entry's
 // step need testing).
-__global__ void k_copy_runs(DevGraph g, const int64_t* nidx, int32_t* out, MergeLists L, int check_order, int* flags) {
+#define MF_CR_ROWS 4  // rows per thread and step: each level of loads is issued for all of them together
+__global__ __launch_bounds__(256) void k_copy_runs(DevGraph g, const int64_t* nidx, int32_t* out, MergeLists L,
+                                                   int check_order, int* flags) {
     bool unsorted = false;
-    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < g.n; x += (int64_t)gridDim.x * blockDim.x) {
-        int32_t len = g.ov_len[x];
-        if (!len) continue;
-        const int64_t row0 = nidx[x];
-        int64_t dst = row0 + (g.nindex[x + 1] - g.nindex[x]) - g.ov_tomb[x];
-        int64_t t0 = L.tval ? L.toff[x] : 0;
-        int32_t prev = (check_order && dst > row0) ? out[dst - 1] : -1;
-        int32_t j = 0;
-        const int32_t* r = g.pool + g.ov_start[x];
-        for (int32_t i = 0; i < len; i++) {
-            const int32_t y = r[i];
-            if (y < 0) continue;
-            out[dst++] = y;
-            if (y <= prev) unsorted = true;
-            prev = y;
-            if (L.tval) {
-                L.tval[t0 + j] = y;
-                L.tidx[t0 + j] = j;
+    const int64_t stride = (int64_t)gridDim.x * MF_BLOCK;
+    for (int64_t x0 = (int64_t)blockIdx.x * MF_BLOCK + threadIdx.x; x0 < g.n; x0 += MF_CR_ROWS * stride) {
+        int64_t x[MF_CR_ROWS];
+        int32_t len[MF_CR_ROWS];
+#pragma unroll
+        for (int k = 0; k < MF_CR_ROWS; k++) {
+            x[k] = x0 + k * stride;
+            len[k] = g.ov_len[x[k] < g.n ? x[k] : g.n - 1];  // clamped, then dropped below
+            if (x[k] >= g.n) len[k] = 0;
+        }
+        int64_t row0[MF_CR_ROWS], dst[MF_CR_ROWS], t0[MF_CR_ROWS], st[MF_CR_ROWS];
+#pragma unroll
+        for (int k = 0; k < MF_CR_ROWS; k++) {
+            if (!len[k]) continue;
+            row0[k] = nidx[x[k]];
+            dst[k] = row0[k] + (g.nindex[x[k] + 1] - g.nindex[x[k]]) - g.ov_tomb[x[k]];
+            t0[k] = L.tval ? L.toff[x[k]] : 0;
+            st[k] = g.ov_start[x[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < MF_CR_ROWS; k++) {
+            if (!len[k]) continue;
+            int64_t d = dst[k];
+            int32_t prev = (check_order && d > row0[k]) ? out[d - 1] : -1;
+            int32_t j = 0;
+            const int32_t* r = g.pool + st[k];
+            for (int32_t i = 0; i < len[k]; i++) {
+                const int32_t y = r[i];
+                if (y < 0) continue;
+                out[d++] = y;
+                if (y <= prev) unsorted = true;
+                prev = y;
+                if (L.tval) {
+                    L.tval[t0[k] + j] = y;
+                    L.tidx[t0[k] + j] = j;
+                }
+                j++;
             }
-            j++;
         }
     }
     if (check_order && unsorted) atomicOr(flags, 1);
@@ -1059,9 +1095,12 @@ __global__ __launch_bounds__(256) void k_merge_base(OldRows o, int32_t n, MergeL
     const int lane = lane_id();
     int64_t carry_p = -2;  // wave-uniform: the last entry of the step stored before, and its t
     int carry_t = 0;
+    // per group: the entry's value and old position; the sorted-order marks'
+    // word, folded into the row's deletions before the entry (ds) as soon as
+    // load2 runs; the position-order marks' word of its old position
     struct D {
-        int32_t v[MF_EW_GROUPS], q[MF_EW_GROUPS];
-        TombWord ws[MF_EW_GROUPS], wp[MF_EW_GROUPS];
+        int32_t v[MF_EW_GROUPS], q[MF_EW_GROUPS], ds[MF_EW_GROUPS];
+        TombWord3 ws[MF_EW_GROUPS], wp[MF_EW_GROUPS];
     };
     // #{tail values < v} in a row
     auto tails_below = [&](const PayMerge& r, int32_t v) {
@@ -1093,17 +1132,19 @@ __global__ __launch_bounds__(256) void k_merge_base(OldRows o, int32_t n, MergeL
                     d.v[j] = o.nlist[q];
                     d.q[j] = (int32_t)(q - st.start(j));
                 }
-                d.ws[j] = tws[q >> 6];
+                d.ws[j] = load_tw3(&tws[q >> 6]);
             }
             return d;
         },
         [&](const WalkStep<PayMerge>& st, D& d) {
+#pragma unroll
+            for (int j = 0; j < MF_EW_GROUPS; j++) d.ds[j] = tomb_before(d.ws[j], st.pp(j), st.tab[st.rl[j]].pay.d0);
             if (HasSorted) {
                 // clamped: a lane past the step's end may pair its row with another row's entry
 #pragma unroll
                 for (int j = 0; j < MF_EW_GROUPS; j++) {
                     const int64_t w = (st.start(j) + d.q[j]) >> 6;
-                    d.wp[j] = tw[w < nw ? w : nw - 1];
+                    d.wp[j] = load_tw3(&tw[w < nw ? w : nw - 1]);
                 }
             }
         },
@@ -1131,11 +1172,11 @@ __global__ __launch_bounds__(256) void k_merge_base(OldRows o, int32_t n, MergeL
                 int32_t v = d.v[j];
                 bool deleted;
                 int64_t dpos = 0;
-                const int64_t dsort = (int64_t)d.ws[j].pre + __popcll(d.ws[j].mask & below_mask(pp)) - d0;
+                const int64_t dsort = d.ds[j];
                 if (HasSorted) {
                     const int64_t a = start + d.q[j];
                     deleted = (d.wp[j].mask >> (a & 63)) & 1ull;
-                    dpos = (int64_t)d.wp[j].pre + __popcll(d.wp[j].mask & below_mask(a)) - d0;
+                    dpos = tomb_before(d.wp[j], a, d0);
                 } else {
                     deleted = v < 0;
                     if (deleted) v = ~v;
@@ -1181,17 +1222,18 @@ __global__ __launch_bounds__(256) void k_merge_base(OldRows o, int32_t n, MergeL
         });
 }
 
-// Tail entries of rows with no old entry: the whole new row, in sorted order.
-__global__ __launch_bounds__(256) void k_merge_tail_empty(const int64_t* nindex, MergeLists L, const int32_t* tsv,
-                                                          const int32_t* tsi, const int64_t* nidx, int32_t* nsval,
-                                                          int32_t* nspos) {
+// Tail entries of rows with no old entry: the whole new row, in sorted order
+// (a coalesced pass over every row's offsets).
+__global__ __launch_bounds__(256) void k_merge_tail_empty(const int64_t* nindex, int32_t n, MergeLists L,
+                                                          const int32_t* tsv, const int32_t* tsi, const int64_t* nidx,
+                                                          int32_t* nsval, int32_t* nspos) {
     GC c = grid_coords();
-    const int64_t nr = (int64_t)*L.ntrows;
-    for (int64_t i = c.tid; i < nr; i += c.nthreads) {
-        const int32_t x = L.trows[i];
-        if (nindex[x + 1] != nindex[x]) continue;
-        const int64_t t0 = L.toff[x], tn = L.toff[x + 1] - t0, dst = nidx[x];
-        for (int64_t j = 0; j < tn; j++) {
+#pragma unroll 4
+    for (int64_t x = c.tid; x < n; x += c.nthreads) {
+        const int64_t t0 = L.toff[x], t1 = L.toff[x + 1], b = nindex[x], e = nindex[x + 1];
+        if (t1 == t0 || e != b) continue;
+        const int64_t dst = nidx[x];
+        for (int64_t j = 0; j < t1 - t0; j++) {
             nsval[dst + j] = tsv[t0 + j];
             nspos[dst + j] = tsi[t0 + j];
         }
@@ -1369,11 +1411,11 @@ hipError_t mf_build_row_index(GraphState& S, int32_t* sval, int32_t* spos, int* 
 // compacted rows.  Lists of more than 2^31 entries are refused.
 namespace {
 // begin (end: past = 1) of the i-th listed row's slice of a list, for the library sort
-struct RowOffset {
+struct RowOffset {  // a listed row's list bounds; a slot naming no row (-1) is an empty list
     const int64_t* off;
     const int32_t* rows;
     int past;
-    __host__ __device__ int operator()(const int& i) const { return (int)off[rows[i] + past]; }
+    __host__ __device__ int operator()(const int& i) const { return rows[i] < 0 ? 0 : (int)off[rows[i] + past]; }
 };
 
 // Sorts the per-row lists of the listed rows: short ones a wave each, the
@@ -1382,11 +1424,14 @@ hipError_t sort_lists(GraphState& S, const int64_t* off, const int32_t* rows, co
                       const int32_t* kin, const int32_t* vin, int32_t* kout, int32_t* vout, const MergeLists& L) {
     hipStream_t st = S.stream;
     if (total == 0) return hipSuccess;
+    // Without reading back how many lists are long: there are at most
+    // total / (MF_SORT_WAVE + 1) of them (and at most one per row), and the
+    // slots k_sort_small leaves unused name no row, so they sort nothing.
+    const int64_t nbig = std::min<int64_t>(total / (MF_SORT_WAVE + 1), S.g.n);
+    if (nbig) k_fill<int32_t><<<(unsigned)std::min<int64_t>((nbig + 255) / 256, 1024), 256, 0, st>>>(L.big, -1, nbig);
     k_fill<u64><<<1, 64, 0, st>>>(L.nbig, 0ull, 1);
     k_sort_small<<<S.blocks, 256, 0, st>>>(off, rows, nrows, kin, vin, kout, vout, L.big, L.nbig);
-    u64 nbig = 0;
-    hipError_t e = hipMemcpyAsync(&nbig, L.nbig, 8, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess || nbig == 0) return e;
     rocprim::counting_iterator<int> idx(0);
     rocprim::transform_iterator<rocprim::counting_iterator<int>, RowOffset, int> beg(idx, RowOffset{off, L.big, 0});
@@ -1423,10 +1468,9 @@ hipError_t mf_merge_row_index(GraphState& S, const MergeLists& L, int64_t ntail,
         OldRows o{g.nindex, g.nlist, g.sval, g.spos};
         if (g.sval) k_merge_base<true><<<S.blocks, 256, 0, st>>>(o, g.n, L, L.tsv, L.tsi, ntail, nidx, nsval, nspos);
         else k_merge_base<false><<<S.blocks, 256, 0, st>>>(o, g.n, L, L.tsv, L.tsi, ntail, nidx, nsval, nspos);
-        if (ntail) k_merge_tail_empty<<<S.blocks, 256, 0, st>>>(g.nindex, L, L.tsv, L.tsi, nidx, nsval, nspos);
+        if (ntail) k_merge_tail_empty<<<S.blocks, 256, 0, st>>>(g.nindex, g.n, L, L.tsv, L.tsi, nidx, nsval, nspos);
         if (check) k_check_row_index<<<S.blocks, 256, 0, st>>>(nidx, nnl, nsval, nspos, g.n, d_flags);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
     return e;
 }
