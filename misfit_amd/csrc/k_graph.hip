@@ -674,84 +674,6 @@ __global__ __launch_bounds__(256) void k_tomb_words(const int32_t* nlist, int64_
     }
 }
 
-// Deletion marks of the old sorted copy (entry p of row x is deleted when
-// its row position spos[p] is): each merge-path chunk's entries are walked
-// in order (entry_walk4), each group's ballot is merged into the aligned
-// word it covers (a group of at most 64 contiguous entries straddles at
-// most one word boundary), and every word of the chunk is written, `pre`
-// counted from the chunk's start, with the chunk's count in ccnt (zeroed
-// before: chunks of empty rows are not visited).  Chunk starts are
-// multiples of 64 entries, so every word belongs to one chunk; the last
-// chunk also owns the word past the end.
-__global__ __launch_bounds__(256) void k_tomb_words_sorted(const int64_t* nindex, const int32_t* spos, int32_t n,
-                                                           const TombWord* tw, TombWord* tws, const int64_t* cst,
-                                                           int64_t nch, int64_t* ccnt) {
-    GC c = grid_coords();
-    const int64_t m = nindex[n];
-    const int64_t nw = mf_tomb_words(m);
-    auto chunk_end_word = [&](int64_t t) { return t + 1 < nch ? cst[t + 1] >> 6 : nw; };
-    // state of the wave's current chunk (chunks come in order, and a
-    // chunk's groups in order)
-    int64_t chunk = -1, wcur = 0, wend = 0;
-    u64 mcur = 0, run = 0;
-    auto close_chunk = [&]() {
-        if (chunk < 0) return;
-        for (; wcur < wend; wcur++) {  // the last, partial word and any word past the end
-            if (c.lane == 0) tws[wcur] = TombWord{mcur, run};
-            run += __popcll(mcur);
-            mcur = 0;
-        }
-        if (c.lane == 0) ccnt[chunk] = (int64_t)run;
-    };
-    struct D {
-        int64_t a[MF_EW_GROUPS];
-        u64 word[MF_EW_GROUPS];
-    };
-    entry_walk4<PayNone>(
-        nindex, 0, n, cst, nch, [](int64_t) { return PayNone{}; },
-        [&](const WalkStep<PayNone>& st) {
-            D d;
-#pragma unroll
-            for (int j = 0; j < MF_EW_GROUPS; j++) d.a[j] = st.start(j) + (int64_t)spos[st.q(j)];
-            return d;
-        },
-        [&](const WalkStep<PayNone>&, D& d) {
-            // clamped: a lane past the step's end may pair its row with another row's entry
-#pragma unroll
-            for (int j = 0; j < MF_EW_GROUPS; j++) d.word[j] = tw[d.a[j] >> 6 < nw ? d.a[j] >> 6 : nw - 1].mask;
-        },
-        [&](const WalkStep<PayNone>& st, const D& d) {
-            if (st.chunk != chunk) {  // wave-uniform
-                close_chunk();
-                chunk = st.chunk;
-                wcur = cst[chunk] >> 6;
-                wend = chunk_end_word(chunk);
-                mcur = 0;
-                run = 0;
-            }
-#pragma unroll
-            for (int j = 0; j < MF_EW_GROUPS; j++) {
-                const int cnt = __popcll(__ballot(st.valid(j)));
-                if (cnt == 0) break;  // this and later groups are past the window's end (wave-uniform)
-                const int64_t p0 = st.p0 + MF_WAVE * j;  // the group's first entry
-                const u64 bal = __ballot(st.valid(j) && ((d.word[j] >> (d.a[j] & 63)) & 1ull));
-                const int sh = (int)(p0 & 63);
-                mcur |= bal << sh;
-                if (sh + cnt >= 64) {  // the word is complete
-                    if (c.lane == 0) tws[wcur] = TombWord{mcur, run};
-                    run += __popcll(mcur);
-                    wcur++;
-                    mcur = sh ? (bal >> (64 - sh)) : 0ull;
-                }
-            }
-        });
-    close_chunk();
-    // the last chunk holds no entry (m a multiple of 64, its only word is
-    // the one past the end): written here
-    if (c.tid == 0 && cst[nch - 1] == cst[nch])
-        for (int64_t w = cst[nch - 1] >> 6; w < nw; w++) tws[w] = TombWord{0ull, 0ull};
-}
-
 __global__ void k_words_fix(TombWord* tw, int64_t nw, const int64_t* cpre) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += (int64_t)gridDim.x * blockDim.x)
         tw[i].pre += (u64)cpre[i / (MF_TCHUNK / 64)];
@@ -769,53 +691,89 @@ __global__ __launch_bounds__(256) void k_words_fix_path(TombWord* tw, int64_t nw
     }
 }
 
-static void tomb_marks(GraphState& S, const int32_t* spos, const TombWord* tw_pos, TombWord* out, const MergeLists& L) {
+// Deletion marks of the old rows in position order, counted per MF_TCHUNK entries.
+static void tomb_marks(GraphState& S, TombWord* out, const MergeLists& L) {
     const int64_t m = S.g.m;
-    const int64_t nw = mf_tomb_words(m);
-    if (spos) {  // sorted order: counted per merge-path chunk of the old rows
-        k_fill<int64_t><<<1024, 256, 0, S.stream>>>(L.ccnt, 0, L.ncst);
-        k_tomb_words_sorted<<<S.blocks, 256, 0, S.stream>>>(S.g.nindex, spos, S.g.n, tw_pos, out, L.cst, L.ncst,
-                                                            L.ccnt);
-        mf_scan_excl(L.ccnt, L.ncst, L.cpre, L.cpart, S.stream);
-        k_words_fix_path<<<S.blocks, 256, 0, S.stream>>>(out, nw, L.cst, L.ncst, L.cpre);
-    } else {  // position order: counted per MF_TCHUNK entries
-        const int64_t nch = mf_tomb_chunks(m);
-        const int B = (int)std::min<int64_t>(S.blocks, (nch + 3) / 4);
-        k_tomb_words<<<B, 256, 0, S.stream>>>(S.g.nlist, m, out, nch, L.ccnt);
-        mf_scan_excl(L.ccnt, nch, L.cpre, L.cpart, S.stream);
-        k_words_fix<<<S.blocks, 256, 0, S.stream>>>(out, nw, L.cpre);
-    }
+    const int64_t nch = mf_tomb_chunks(m);
+    const int B = (int)std::min<int64_t>(S.blocks, (nch + 3) / 4);
+    k_tomb_words<<<B, 256, 0, S.stream>>>(S.g.nlist, m, out, nch, L.ccnt);
+    mf_scan_excl(L.ccnt, nch, L.cpre, L.cpart, S.stream);
+    k_words_fix<<<S.blocks, 256, 0, S.stream>>>(out, mf_tomb_words(m), L.cpre);
 }
 
 // Every kept base entry to its compacted position: row x starts at nidx[x],
 // entry p keeps its place minus the deletions before it in the row, i.e.
 // dst = nidx[x] + doff[x] + (p - start) - (deletions before p in the array).
 // Balanced by entries over all rows, hub rows spread over many waves.
+//
+// Sorted (the old rows have a sorted copy): the same walk also writes the
+// deletion marks of the old sorted copy, since entry p of the sorted copy
+// lies in the same row as entry p of the rows: it is deleted when the entry
+// at its row position spos[p] is (bit of the position marks tw).  Each
+// group's ballot goes into the aligned word it covers (64 contiguous entries
+// straddle at most one word boundary), every word of the chunk is written
+// with `pre` counted from the chunk's start, and the chunk's count goes to
+// ccnt (zeroed before: chunks of empty rows are not visited); merge-path
+// chunk starts are multiples of 64 entries, so every word belongs to one
+// chunk, and the last chunk also owns the word past the end.
+template <bool Sorted>
 __global__ __launch_bounds__(256) void k_copy_base(DevGraph g, const int64_t* nidx, const int64_t* doff,
                                                    const TombWord* tw, const int64_t* cst, int64_t nch,
-                                                   int32_t* out) {
+                                                   int32_t* out, TombWord* tws, int64_t* ccnt) {
     // Entries come in order within a chunk, so the deletions before an
     // entry are counted as the walk goes (a ballot per group), from the
     // chunk's start (a multiple of 64 entries: its word's `pre`).
     const int lane = lane_id();
+    const int64_t m = g.m;
+    const int64_t nw = mf_tomb_words(m);
     int64_t chunk = -1;
     u64 grun = 0;
+    // sorted-order marks of the wave's current chunk
+    int64_t wcur = 0, wend = 0;
+    u64 mcur = 0, run = 0;
+    auto close_chunk = [&]() {
+        if (chunk < 0) return;
+        for (; wcur < wend; wcur++) {  // the last, partial word and any word past the end
+            if (lane == 0) tws[wcur] = TombWord{mcur, run};
+            run += __popcll(mcur);
+            mcur = 0;
+        }
+        if (lane == 0) ccnt[chunk] = (int64_t)run;
+    };
     struct D {
         int32_t y[MF_EW_GROUPS];
+        int64_t a[MF_EW_GROUPS];   // Sorted: the sorted entry's row position
+        u64 word[MF_EW_GROUPS];    // ... and its position-marks word
     };
     entry_walk4<PayI64>(
         g.nindex, 0, g.n, cst, nch, [&](int64_t x) { return PayI64{nidx[x] + doff[x]}; },
         [&](const WalkStep<PayI64>& st) {
             D d;
 #pragma unroll
-            for (int j = 0; j < MF_EW_GROUPS; j++) d.y[j] = g.nlist[st.q(j)];
+            for (int j = 0; j < MF_EW_GROUPS; j++) {
+                d.y[j] = g.nlist[st.q(j)];
+                if (Sorted) d.a[j] = st.start(j) + (int64_t)g.spos[st.q(j)];
+            }
             return d;
         },
-        [](const WalkStep<PayI64>&, D&) {},
+        [&](const WalkStep<PayI64>&, D& d) {
+            if (Sorted) {
+                // clamped: a lane past the step's end may pair its row with another row's entry
+#pragma unroll
+                for (int j = 0; j < MF_EW_GROUPS; j++) d.word[j] = tw[d.a[j] >> 6 < nw ? d.a[j] >> 6 : nw - 1].mask;
+            }
+        },
         [&](const WalkStep<PayI64>& st, const D& d) {
             if (st.chunk != chunk) {  // wave-uniform
+                if (Sorted) close_chunk();
                 chunk = st.chunk;
                 grun = tw[cst[chunk] >> 6].pre;
+                if (Sorted) {
+                    wcur = cst[chunk] >> 6;
+                    wend = chunk + 1 < nch ? cst[chunk + 1] >> 6 : nw;
+                    mcur = 0;
+                    run = 0;
+                }
             }
 #pragma unroll
             for (int j = 0; j < MF_EW_GROUPS; j++) {
@@ -826,8 +784,29 @@ __global__ __launch_bounds__(256) void k_copy_base(DevGraph g, const int64_t* ni
                     out[r.pay.v + (st.pp(j) - r.start) - (int64_t)(grun + __popcll(bal & below_mask(lane)))] = d.y[j];
                 }
                 grun += __popcll(bal);
+                if (Sorted) {
+                    const int cnt = __popcll(__ballot(valid));
+                    if (cnt) {  // groups past the window's end hold no entry (wave-uniform)
+                        const u64 sb = __ballot(valid && ((d.word[j] >> (d.a[j] & 63)) & 1ull));
+                        const int sh = (int)((st.p0 + MF_WAVE * j) & 63);  // the group's first entry's bit
+                        mcur |= sb << sh;
+                        if (sh + cnt >= 64) {  // the word is complete
+                            if (lane == 0) tws[wcur] = TombWord{mcur, run};
+                            run += __popcll(mcur);
+                            wcur++;
+                            mcur = sh ? (sb >> (64 - sh)) : 0ull;
+                        }
+                    }
+                }
             }
         });
+    if (Sorted) {
+        close_chunk();
+        // the last chunk holds no entry (m a multiple of 64, its only word is
+        // the one past the end): written here
+        if (blockIdx.x == 0 && threadIdx.x == 0 && cst[nch - 1] == cst[nch])
+            for (int64_t w = cst[nch - 1] >> 6; w < nw; w++) tws[w] = TombWord{0ull, 0ull};
+    }
 }
 
 // live overlay entries follow the base part, in insertion order; with
@@ -891,8 +870,15 @@ hipError_t mf_compact_plan(GraphState& S, int64_t* d_newidx) {
 
 hipError_t mf_compact_copy(GraphState& S, const int64_t* d_newidx, int32_t* d_out, const MergeLists& L, int* d_flags) {
     merge_path(S.g.nindex, 0, S.g.n, L.ncst, L.cst, S.stream);
-    tomb_marks(S, nullptr, nullptr, L.tw, L);
-    k_copy_base<<<S.blocks, 256, 0, S.stream>>>(S.g, d_newidx, L.doff, L.tw, L.cst, L.ncst, d_out);
+    tomb_marks(S, L.tw, L);
+    if (S.g.sval) {  // the sorted copy's marks in the same walk, counted per merge-path chunk
+        k_fill<int64_t><<<1024, 256, 0, S.stream>>>(L.ccnt, 0, L.ncst);
+        k_copy_base<true><<<S.blocks, 256, 0, S.stream>>>(S.g, d_newidx, L.doff, L.tw, L.cst, L.ncst, d_out, L.tws,
+                                                          L.ccnt);
+    } else {
+        k_copy_base<false><<<S.blocks, 256, 0, S.stream>>>(S.g, d_newidx, L.doff, L.tw, L.cst, L.ncst, d_out, nullptr,
+                                                           nullptr);
+    }
     k_fill<int><<<1, 64, 0, S.stream>>>(d_flags, S.g.sorted ? 0 : 1, 1);
     k_copy_runs<<<S.blocks, 256, 0, S.stream>>>(S.g, d_newidx, d_out, L, S.g.sorted ? 1 : 0, d_flags);
     return hipGetLastError();
@@ -1462,7 +1448,10 @@ hipError_t mf_merge_row_index(GraphState& S, const MergeLists& L, int64_t ntail,
     const DevGraph& g = S.g;
     hipStream_t st = S.stream;
     if (ntail > (int64_t)INT32_MAX) return hipErrorInvalidValue;
-    if (g.sval) tomb_marks(S, g.spos, L.tw, L.tws, L);  // deletion marks in sorted order
+    if (g.sval) {  // the sorted copy's marks (written by the copy, per merge-path chunk) made global
+        mf_scan_excl(L.ccnt, L.ncst, L.cpre, L.cpart, st);
+        k_words_fix_path<<<S.blocks, 256, 0, st>>>(L.tws, mf_tomb_words(g.m), L.cst, L.ncst, L.cpre);
+    }
     hipError_t e = sort_lists(S, L.toff, L.trows, L.ntrows, ntail, L.tval, L.tidx, L.tsv, L.tsi, L);
     if (e == hipSuccess) {
         OldRows o{g.nindex, g.nlist, g.sval, g.spos};
