@@ -434,6 +434,13 @@ def single_graph_line(M, args, name, ni, nl, mis0, u, v, W, K, B, index_bytes, c
         eng.load_csr(ni, nl)
         ins = eng.flags(u, v)  # fixed against the loaded graph
         ups = M.make_updates(u, v, ins)
+        if args.served_only:  # profiling runs: the served batches alone, with their final check
+            served = run_served(M, eng, ni, nl, mis0, ups, W, K, B)
+            if served["check"] != (0, 0, 0):
+                raise SystemExit("served: final check failed: %s" % (served["check"],))
+            return {"metric": "edge updates/s (served batches only: a profiling run)",
+                    "value": round(K * B / served["wall_s"], 1), "unit": "edge updates/s", "n_gpus": 1,
+                    "steps": K, "warmup": W, "config": {"workload": workload, "name": name, "device": dev_name}}
         plain = run_plain(M, eng, ni, nl, mis0, ups, W, K, B, index_bytes=index_bytes)
         print("plain: %.1f updates/s" % (K * B / plain["wall_s"]), file=sys.stderr, flush=True)
         served = run_served(M, eng, ni, nl, mis0, ups, W, K, B)
