@@ -261,7 +261,9 @@ const char* misfit_server_last_error(const misfit_server* srv);
 /* MISFIT_E_NOMEM (nothing allocated) when the graph does not fit: its
  * resident bytes (base, sorted row copy, overlay, per-vertex state) plus the
  * headroom of the largest compaction a resident graph may run (compactions
- * run one at a time) must fit in free device memory minus hbm_reserve. */
+ * run one at a time) must fit in free device memory minus hbm_reserve.  With
+ * MISFIT_SERVE_COMPACT_KEEP=1 (read when the server is created) every graph
+ * keeps its compactions' buffers and is charged its own headroom instead. */
 int misfit_server_add_graph(misfit_server* srv, int32_t n, int64_t m, const void* nindex, int nindex_is_64,
                             const int32_t* nlist, const uint8_t* mis, int32_t* graph);
 int misfit_server_remove_graph(misfit_server* srv, int32_t graph);

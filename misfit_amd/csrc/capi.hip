@@ -1582,11 +1582,19 @@ static void release_buf(misfit_ctx* c, void* p, size_t cap) {
     else mf_free_device(p);
 }
 
-// The serving layer's graphs share one compaction headroom: they keep no
-// buffers between compactions.
+// Whether compactions keep the buffers they release (the serving layer
+// decides for its graphs: by default they share one compaction headroom and
+// keep none).
 void mf_ctx_set_keep_compact(misfit_ctx* c, bool on) {
     c->keep_compact = on;
     if (!on) free_spares(c);
+}
+
+// Device bytes the context keeps for its next compaction.
+int64_t mf_ctx_spare_bytes(const misfit_ctx* c) {
+    int64_t b = 0;
+    for (const misfit_ctx::Spare& s : c->spares) b += (int64_t)s.cap;
+    return b;
 }
 
 // Compaction scratch and kept buffers: freed with the graph (reload, destroy).

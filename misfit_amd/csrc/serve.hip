@@ -55,8 +55,10 @@ int mf_ctx_prepare_external(misfit_ctx* c, const misfit_update* ups, int64_t cou
 int mf_ctx_external_ready(misfit_ctx* c, uint64_t ticket);
 // capi.hip: every batch of the context may halt (one batch per graph).
 void mf_ctx_set_halting(misfit_ctx* c, bool on);
-// capi.hip: whether compactions keep their released buffers for the next one.
+// capi.hip: whether compactions keep their released buffers for the next
+// one, and how many bytes a context keeps.
 void mf_ctx_set_keep_compact(misfit_ctx* c, bool on);
+int64_t mf_ctx_spare_bytes(const misfit_ctx* c);
 
 namespace {
 using clk = std::chrono::steady_clock;
@@ -271,6 +273,11 @@ struct misfit_server {
     int64_t headroom_max = 0;   // the largest compaction headroom among the resident graphs
     Runner runner;              // the serving kernel (opts.persistent)
     bool check_each = false;    // MISFIT_SERVE_CHECK=1: the full check after every batch (a debugging aid)
+    // MISFIT_SERVE_COMPACT_KEEP=1: every graph keeps the buffers its
+    // compactions release for its next one (no free and allocation of the
+    // base and its sorted copy per compaction); admission then charges each
+    // graph its own compaction headroom instead of sharing the largest one
+    bool keep_compact = false;
 
     // Whether request r of graph g runs its rounds in the serving kernel:
     // every batch (persistent 2), or (persistent 1) a small one, or one of a
@@ -400,7 +407,10 @@ bool misfit_server::service(Graph* g, std::unique_lock<std::mutex>& lk) {
         int64_t new_m = 0;
         int rc = MISFIT_OK;
         std::string em;
-        const int64_t need = compaction_headroom(g->n, g->base_m + g->overlay, opts.compact_fraction);
+        // a graph that keeps its buffers needs only what they do not cover
+        const int64_t need =
+            std::max<int64_t>(0, compaction_headroom(g->n, g->base_m + g->overlay, opts.compact_fraction) -
+                                     (keep_compact ? mf_ctx_spare_bytes(g->ctx) : 0));
         const int64_t fr = device_free_bytes(device);
         if (fr - need < opts.hbm_reserve) {
             rc = MISFIT_E_NOMEM;
@@ -522,6 +532,8 @@ extern "C" int misfit_server_create(int device, const misfit_server_opts* opts, 
     s->opts.persistent = opts ? opts->persistent : 0;
     const char* ck = getenv("MISFIT_SERVE_CHECK");
     s->check_each = ck && ck[0] == '1';
+    const char* kc = getenv("MISFIT_SERVE_COMPACT_KEEP");
+    s->keep_compact = kc && kc[0] == '1';
     if (s->opts.persistent > 0) {
         const char* idle = getenv("MISFIT_SERVE_IDLE_MS");
         const int rc = s->runner.init(device, idle && atof(idle) > 0 ? atof(idle) : 20.0);
@@ -556,14 +568,16 @@ extern "C" int misfit_server_add_graph(misfit_server* s, int32_t n, int64_t m, c
                                        const int32_t* nlist, const uint8_t* mis, int32_t* graph) {
     if (!s || !graph || !mis) return MISFIT_E_ARG;
     if (n < 1 || m < 0) return s->fail(MISFIT_E_ARG, "invalid graph size");
-    const int64_t need = graph_footprint(n, m);
     const int64_t headroom = s->opts.compact_fraction > 0 ? compaction_headroom(n, m, s->opts.compact_fraction) : 0;
+    // graphs that keep their compaction buffers hold their own headroom from
+    // their first compaction on: it is charged with their resident bytes
+    const int64_t need = graph_footprint(n, m) + (s->keep_compact ? headroom : 0);
     {
         // the graph's resident bytes, and the headroom of the largest
         // compaction any resident graph (or this one) may run, must fit
         std::lock_guard<std::mutex> lk(s->mu);
         const int64_t fr = device_free_bytes(s->device) - s->loading_bytes;
-        const int64_t room = std::max(headroom, s->headroom_max);
+        const int64_t room = s->keep_compact ? 0 : std::max(headroom, s->headroom_max);
         if (fr - need - room < s->opts.hbm_reserve)
             return s->fail(MISFIT_E_NOMEM, "graph not admitted: needs " + std::to_string(need >> 20) + " MiB and " +
                                                std::to_string(room >> 20) + " MiB of compaction headroom, " +
@@ -576,7 +590,7 @@ extern "C" int misfit_server_add_graph(misfit_server* s, int32_t n, int64_t m, c
     int rc = misfit_ctx_create(s->device, &g->ctx);
     std::string em = "cannot create a context";
     if (rc == MISFIT_OK) {
-        mf_ctx_set_keep_compact(g->ctx, false);
+        mf_ctx_set_keep_compact(g->ctx, s->keep_compact);
         rc = misfit_load_csr(g->ctx, n, m, nindex, nindex_is_64, nlist);
         if (rc == MISFIT_OK) rc = misfit_set_mis(g->ctx, mis);
         if (rc == MISFIT_OK) rc = misfit_set_mode(g->ctx, MISFIT_MODE_QUEUED);

@@ -45,8 +45,12 @@ def _oracle_with_compactions(oracle, ni, nl, mis0, batches, before):
     return mis
 
 
-@pytest.mark.parametrize("inflight", [1, 3])
-def test_served_graphs_match_oracle_with_compactions(oracle, engine_mod, inflight):
+@pytest.mark.parametrize("inflight,keep", [(1, "0"), (3, "0"), (1, "1")])
+def test_served_graphs_match_oracle_with_compactions(oracle, engine_mod, inflight, keep, monkeypatch):
+    """Three graphs served with compactions; keep "1": every graph keeps the
+    buffers its compactions release (MISFIT_SERVE_COMPACT_KEEP, read when
+    the server is created) and takes its next compaction's outputs from them."""
+    monkeypatch.setenv("MISFIT_SERVE_COMPACT_KEEP", keep)
     work = [_workload(engine_mod, oracle, k, s, seed)
             for (k, s, seed) in [("er", 1 << 14, 31), ("rmat", 14, 32), ("rgg", 1 << 14, 33)]]
     with engine_mod.Server(0, compact_fraction=0.02, max_inflight=inflight) as srv:
