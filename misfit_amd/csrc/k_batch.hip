@@ -1564,18 +1564,34 @@ __device__ void ph_comp_apply(const BP& P, const GC& c, int par) {
 // Decisions of committed edges stay in kind/res (later rounds only classify
 // edges past the committed prefix), so the rule's counters are taken once
 // per batch instead of with per-edge atomics.
+#define MF_STATS_UNROLL 4  // quads per thread whose loads are all issued before the first is used
 __device__ void ph_stats(const BP& P, const GC& c) {
     u64 ai = 0, ti = 0, ad = 0, td = 0;
-    // four kinds per 32-bit load (kind is a whole allocation)
+    // four kinds per 32-bit load (kind is a whole allocation); every load of
+    // a step is issued at a clamped index before any is used, so a thread
+    // waits once per step instead of once per quad and again per result
     const int64_t nq = P.U / 4;
     const uint32_t* k4 = reinterpret_cast<const uint32_t*>(P.kind);
-    for (int64_t q = c.tid; q < nq; q += c.nthreads) {
-        const uint32_t w = k4[q];
-        if (!w) continue;
-        for (int j = 0; j < 4; j++) {
-            const uint32_t k = (w >> (8 * j)) & 0xFFu;
-            if (k == 1) { ai++; ti += P.res_x[4 * q + j] >= 0; }
-            else if (k == 2) { ad++; td += P.res_x[4 * q + j] >= 0; }
+    for (int64_t q0 = c.tid; q0 < nq; q0 += MF_STATS_UNROLL * c.nthreads) {
+        uint32_t w[MF_STATS_UNROLL];
+        int32_t r[MF_STATS_UNROLL][4];
+#pragma unroll
+        for (int u = 0; u < MF_STATS_UNROLL; u++) {
+            const int64_t q = q0 + u * c.nthreads;
+            const int64_t qc = q < nq ? q : nq - 1;
+            w[u] = k4[qc];
+#pragma unroll
+            for (int j = 0; j < 4; j++) r[u][j] = P.res_x[4 * qc + j];
+            if (q >= nq) w[u] = 0;
+        }
+#pragma unroll
+        for (int u = 0; u < MF_STATS_UNROLL; u++) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint32_t k = (w[u] >> (8 * j)) & 0xFFu;
+                if (k == 1) { ai++; ti += r[u][j] >= 0; }
+                else if (k == 2) { ad++; td += r[u][j] >= 0; }
+            }
         }
     }
     for (int64_t e = 4 * nq + c.tid; e < P.U; e += c.nthreads) {
