@@ -113,6 +113,38 @@ def test_compaction_parity(oracle, engine_mod, eng, mode):
     assert eng.check() == (0, 0, 0)
 
 
+def test_compaction_keeps_ascending_rows(oracle, engine_mod, eng):
+    """Insertions that continue every row in ascending order: the merged
+    sorted copy, built before the copy's order test is read, is dropped and
+    the compacted base is used as its own sorted order by later batches."""
+    rng = np.random.default_rng(5)
+    ni, nl = H.random_graph(rng, 200, 0.05)
+    # new vertices 200..219 hang off nothing; edges among them inserted in
+    # ascending order of both endpoints keep every row ascending
+    n = 220
+    ni = np.concatenate([ni, np.full(n - 200, ni[-1], np.int64)])
+    mis = oracle.Graph.from_csr(ni, nl).greedy_mis()
+    pairs = [(a, b) for a in range(200, n) for b in range(a + 1, n) if (a * 7 + b) % 3 == 0]
+    u = np.array([p[0] for p in pairs], np.int32)
+    v = np.array([p[1] for p in pairs], np.int32)
+    batches = [(u, v, np.ones(len(u), np.uint8))]
+    want, _, og = H.oracle_replay(oracle, ni, nl, mis, batches)
+    got, _, _ = H.engine_replay(engine_mod, eng, ni, nl, mis, batches)
+    assert np.array_equal(want[-1], got[-1])
+    oni, onl = og.compact()
+    assert all(np.all(np.diff(onl[oni[x]:oni[x + 1]]) > 0) for x in range(n))  # the case under test
+    assert eng.compact() == len(onl)
+    eni, enl = eng.export_csr()
+    assert np.array_equal(eni, oni) and np.array_equal(enl, onl)
+    # later batches search the compacted rows (deletions and insertions of existing pairs)
+    more = H.random_batches(rng, oracle, oni, onl, 3, 400, general=True)
+    w2, _, _ = H.oracle_replay(oracle, oni, onl, want[-1], more)
+    for (bu, bv, bins), w in zip(more, w2):
+        eng.process_batch(engine_mod.make_updates(bu, bv, bins))
+        assert np.array_equal(eng.get_mis(), w)
+    assert eng.check() == (0, 0, 0)
+
+
 def test_repeated_compactions_reuse_buffers(oracle, engine_mod, eng):
     """One context compacted four times while its base grows a little and
     shrinks more: each compaction's outputs come from the buffers the one
