@@ -113,6 +113,61 @@ def test_compaction_parity(oracle, engine_mod, eng, mode):
     assert eng.check() == (0, 0, 0)
 
 
+def test_compaction_hub_rows_across_chunks(oracle, engine_mod, eng):
+    """Two hub rows of 30,000 and 20,000 neighbours, longer than the
+    compaction's walk chunks (8,192 rows plus entries), with insertions into
+    them and deletions from them: chunks start inside the hubs' rows and
+    inside their tails' value ranges.  Compacted twice (ascending rows, then
+    the merged sorted copy), each result equal to the oracle's."""
+    rng = np.random.default_rng(17)
+    n = 60000
+    pairs = set()
+    for hub, deg in ((0, 30000), (1, 20000)):
+        for y in rng.choice(np.arange(2, n), deg, replace=False):
+            pairs.add((hub, int(y)))
+    a = rng.integers(2, n, 40000)
+    b = rng.integers(2, n, 40000)
+    pairs |= {(int(min(x, y)), int(max(x, y))) for x, y in zip(a, b) if x != y}
+    rows = [[] for _ in range(n)]
+    for x, y in pairs:
+        rows[x].append(y)
+        rows[y].append(x)
+    ni = np.zeros(n + 1, np.int64)
+    ni[1:] = np.cumsum([len(r) for r in rows])
+    nl = np.concatenate([np.sort(np.array(r, np.int32)) for r in rows]).astype(np.int32)
+    mis = oracle.Graph.from_csr(ni, nl).greedy_mis()
+    eng.load_csr(ni, nl)
+    eng.set_mis(mis)
+    eng.set_rule(True)
+    for k in range(2):
+        g = oracle.Graph.from_csr(ni, nl)
+        hub_rows = [nl[ni[h]:ni[h + 1]] for h in (0, 1)]
+        batches = []
+        for _ in range(2):
+            # new neighbours of the hubs, deletions of hub entries, and random pairs
+            hu = np.concatenate([np.zeros(600, np.int32), np.ones(400, np.int32)])
+            hv = rng.integers(2, n, 1000).astype(np.int32)
+            du = np.concatenate([np.zeros(500, np.int32), np.ones(300, np.int32)])
+            dv = np.concatenate([rng.choice(hub_rows[0], 500), rng.choice(hub_rows[1], 300)]).astype(np.int32)
+            ru = rng.integers(0, n, 2000).astype(np.int32)
+            rv = rng.integers(0, n, 2000).astype(np.int32)
+            u = np.concatenate([hu, du, ru])
+            v = np.concatenate([hv, dv, rv])
+            keep = u != v
+            u, v = u[keep], v[keep]
+            batches.append((u, v, g.flags(u, v)))
+        want, _, og = H.oracle_replay(oracle, ni, nl, mis, batches)
+        for (u, v, ins), w in zip(batches, want):
+            eng.process_batch(engine_mod.make_updates(u, v, ins))
+            assert np.array_equal(eng.get_mis(), w), "phase %d" % k
+        oni, onl = og.compact()
+        assert eng.compact() == len(onl)
+        eni, enl = eng.export_csr()
+        assert np.array_equal(eni, oni) and np.array_equal(enl, onl), "compaction %d" % (k + 1)
+        ni, nl, mis = oni, onl, want[-1]
+    assert eng.check() == (0, 0, 0)
+
+
 def test_compaction_keeps_ascending_rows(oracle, engine_mod, eng):
     """Insertions that continue every row in ascending order: the merged
     sorted copy, built before the copy's order test is read, is dropped and
