@@ -133,11 +133,11 @@ struct misfit_ctx {
     int32_t* d_sval = nullptr; // sorted copy of the base rows when they are not ascending
     int32_t* d_spos = nullptr;
     // compaction scratch, kept between compactions (grown as needed, freed
-    // with the graph): per-vertex counts, offsets and row lists, and the
-    // tail lists of the merge
+    // with the graph): the plan scan's block sums, per-vertex offsets and
+    // row lists, and the tail lists of the merge
     std::vector<void*> cs_allocs;
     int64_t cs_n = 0, cs_t = 0;
-    int64_t *cs_dcnt = nullptr, *cs_tcnt = nullptr, *cs_doff = nullptr, *cs_toff = nullptr;
+    int64_t *cs_part = nullptr, *cs_doff = nullptr, *cs_toff = nullptr;
     int32_t *cs_trows = nullptr, *cs_big = nullptr;
     u64* cs_cnt = nullptr;  // [0] rows with a tail, [1] long lists
     int32_t *cs_tval = nullptr, *cs_tidx = nullptr, *cs_tsv = nullptr, *cs_tsi = nullptr;
@@ -1606,7 +1606,7 @@ static void free_compact_scratch(misfit_ctx* c) {
     c->cs_tmp = nullptr;
     c->cs_tmp_bytes = 0;
     c->cs_n = c->cs_t = 0;
-    c->cs_dcnt = c->cs_tcnt = c->cs_doff = c->cs_toff = nullptr;
+    c->cs_part = c->cs_doff = c->cs_toff = nullptr;
     c->cs_trows = c->cs_big = nullptr;
     c->cs_cnt = nullptr;
     c->cs_tval = c->cs_tidx = c->cs_tsv = c->cs_tsi = nullptr;
@@ -1634,8 +1634,7 @@ static hipError_t ensure_compact_scratch(misfit_ctx* c, int64_t ntail) {
     hipError_t e = hipSuccess;
     if (c->cs_n != n) {
         free_compact_scratch(c);
-        e = cs_alloc(c, (void**)&c->cs_dcnt, (size_t)n * 8);
-        if (e == hipSuccess) e = cs_alloc(c, (void**)&c->cs_tcnt, (size_t)n * 8);
+        e = cs_alloc(c, (void**)&c->cs_part, (((size_t)n + 2047) / 2048 + 2) * 3 * 8);
         if (e == hipSuccess) e = cs_alloc(c, (void**)&c->cs_doff, ((size_t)n + 1) * 8);
         if (e == hipSuccess) e = cs_alloc(c, (void**)&c->cs_toff, ((size_t)n + 1) * 8);
         if (e == hipSuccess) e = cs_alloc(c, (void**)&c->cs_trows, (size_t)n * 4);
@@ -1707,15 +1706,14 @@ extern "C" int misfit_compact(misfit_ctx* c, int64_t* new_m) {
     MergeLists L;
     Phases ph;
     ph.to("compact:plan");
+    // the new row starts, the deletions and tails before each row and the
+    // rows with a tail, in one scan (cs_part: its scratch); the deletion
+    // marks of the old rows (position order, and sorted order when a sorted
+    // copy exists) and the counting chunks' sums are transient
     hipError_t e = grab((void**)&nidx, ((size_t)g.n + 1) * 8);
-    if (e == hipSuccess) e = mf_compact_plan(c->S, nidx);
-    ph.to("compact:counts");
-    // per-row deletion counts and tail lists; the deletion marks of the old
-    // rows (position order, and sorted order when a sorted copy exists) and
-    // the counting chunks' sums are transient
     if (e == hipSuccess) e = ensure_compact_scratch(c, 0);
     if (e == hipSuccess)
-        e = mf_merge_counts(c->S, c->cs_dcnt, c->cs_tcnt, c->cs_doff, c->cs_toff, c->cs_trows, c->cs_cnt);
+        e = mf_compact_plan_counts(c->S, nidx, c->cs_doff, c->cs_toff, c->cs_trows, c->cs_cnt, c->cs_part);
     // two read-backs in all, into pinned memory (asynchronous copies): the
     // sizes here, every flag at the end
     ph.to("compact:sizes");

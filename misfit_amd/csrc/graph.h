@@ -97,9 +97,11 @@ hipError_t mf_validate_entries(GraphState& S, int* d_flags);
 // sorted copy of every row with positions (rows not ascending); flags |= 16 on a repeated neighbour,
 // |= 32 when the copy does not match the rows
 hipError_t mf_build_row_index(GraphState& S, int32_t* sval, int32_t* spos, int* d_flags);
-// per-row deletion and tail counts of the overlay and their offsets (n+1 each)
-hipError_t mf_merge_counts(GraphState& S, int64_t* dcnt, int64_t* tcnt, int64_t* doff, int64_t* toff, int32_t* trows,
-                           u64* ntrows);
+// the overlay's deletions and tails before each row (doff, toff) and the new
+// row starts (nidx), n+1 each, in one scan; the rows with a tail listed in
+// trows in row order (*ntrows of them); part: scan scratch of 3 * ceil(n / 2048) entries
+hipError_t mf_compact_plan_counts(GraphState& S, int64_t* nidx, int64_t* doff, int64_t* toff, int32_t* trows,
+                                  u64* ntrows, int64_t* part);
 // sorted copy of the compacted rows nidx/nnl by merging the old graph's (S) sorted order with the
 // tails compaction collected, queued on S's stream without waiting; with `check`, the result is
 // checked entry by entry (the flags of mf_build_row_index, ORed into *d_flags)

@@ -601,6 +601,117 @@ void mf_scan_excl(const int64_t* in, int64_t count, int64_t* out, int64_t* part,
     k_scan_final<<<(unsigned)nb, SCAN_T, 0, st>>>(in, count, part, out, out + count);
 }
 
+// ---------------------------------------------------------------- compaction plan
+// The overlay's per-row counts, deleted base entries (ov_tomb) and live
+// overlay entries (ov_alive), scanned together in one three-pass scan of
+// three columns: doff and toff, the entries of each kind in the rows before,
+// the new row starts nidx = nindex - doff + toff (a new row is its old row
+// minus its deletions plus its tail), each with its total at [n], and the
+// rows with a tail listed in row order, each at the count of such rows
+// before it (no atomics).  part: three columns of block sums.
+#define MF_PLAN_COLS 3
+__global__ __launch_bounds__(SCAN_T) void k_plan_reduce(DevGraph g, int64_t* part) {
+    __shared__ int64_t sh[SCAN_T / 64 + 1];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_PER;
+    int64_t sa = 0, sb = 0, sc = 0;
+    for (int i = 0; i < SCAN_PER; i++)
+        if (base + i < g.n) {
+            const int32_t b = g.ov_alive[base + i];
+            sa += g.ov_tomb[base + i];
+            sb += b;
+            sc += b > 0 ? 1 : 0;
+        }
+    int64_t ta, tb, tc;
+    block_excl_scan(sa, sh, ta);
+    block_excl_scan(sb, sh, tb);
+    block_excl_scan(sc, sh, tc);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = ta;
+        part[gridDim.x + blockIdx.x] = tb;
+        part[2 * (int64_t)gridDim.x + blockIdx.x] = tc;
+    }
+}
+
+// the columns of block sums, one workgroup each
+__global__ __launch_bounds__(SCAN_T) void k_plan_parts(int64_t* part, int64_t nb) {
+    __shared__ int64_t sh[SCAN_T / 64 + 1];
+    int64_t* col = part + (int64_t)blockIdx.x * nb;
+    int64_t carry = 0;
+    for (int64_t b0 = 0; b0 < nb; b0 += SCAN_T) {
+        const int64_t i = b0 + threadIdx.x;
+        const int64_t v = i < nb ? col[i] : 0;
+        int64_t tot;
+        const int64_t ex = block_excl_scan(v, sh, tot);
+        if (i < nb) col[i] = carry + ex;
+        carry += tot;
+    }
+}
+
+// A thread scans SCAN_PER consecutive rows; the offsets are read and the
+// three output columns written through LDS in the other order (row
+// base + i * SCAN_T + thread), so every load and store of a wave covers one
+// contiguous span instead of 64 scattered lines.
+__global__ __launch_bounds__(SCAN_T) void k_plan_final(DevGraph g, const int64_t* part, int64_t* doff, int64_t* toff,
+                                                       int64_t* nidx, int32_t* trows, u64* ntrows) {
+    __shared__ int64_t sh[SCAN_T / 64 + 1];
+    __shared__ int64_t tr[SCAN_TILE];
+    const int64_t n = g.n;
+    const int t = threadIdx.x;
+    const int64_t blk = (int64_t)blockIdx.x * SCAN_TILE;
+    const int64_t base = blk + (int64_t)t * SCAN_PER;
+    for (int i = 0; i < SCAN_PER; i++) {
+        const int64_t x = blk + i * SCAN_T + t;
+        tr[i * SCAN_T + t] = x < n ? g.nindex[x] : 0;
+    }
+    int32_t a[SCAN_PER], b[SCAN_PER];
+    int64_t sa = 0, sb = 0, sc = 0;
+    for (int i = 0; i < SCAN_PER; i++) {
+        a[i] = base + i < n ? g.ov_tomb[base + i] : 0;
+        b[i] = base + i < n ? g.ov_alive[base + i] : 0;
+        sa += a[i];
+        sb += b[i];
+        sc += b[i] > 0 ? 1 : 0;
+    }
+    int64_t ta, tb, tc;
+    int64_t ea = block_excl_scan(sa, sh, ta) + part[blockIdx.x];  // its barriers also order tr
+    int64_t eb = block_excl_scan(sb, sh, tb) + part[gridDim.x + blockIdx.x];
+    int64_t ec = block_excl_scan(sc, sh, tc) + part[2 * (int64_t)gridDim.x + blockIdx.x];
+    int64_t vd[SCAN_PER], vt[SCAN_PER], vn[SCAN_PER];
+    for (int i = 0; i < SCAN_PER; i++) {
+        vd[i] = ea;
+        vt[i] = eb;
+        vn[i] = tr[t * SCAN_PER + i] - ea + eb;
+        if (base + i < n && b[i] > 0) trows[ec++] = (int32_t)(base + i);
+        ea += a[i];
+        eb += b[i];
+    }
+    if (base < n && base + SCAN_PER >= n) {
+        doff[n] = ea;
+        toff[n] = eb;
+        nidx[n] = g.nindex[n] - ea + eb;
+        *ntrows = (u64)ec;
+    }
+    int64_t* const out[3] = {doff, toff, nidx};
+    for (int k = 0; k < 3; k++) {
+        __syncthreads();  // tr is free
+        for (int i = 0; i < SCAN_PER; i++) tr[t * SCAN_PER + i] = k == 0 ? vd[i] : k == 1 ? vt[i] : vn[i];
+        __syncthreads();
+        for (int i = 0; i < SCAN_PER; i++) {
+            const int64_t x = blk + i * SCAN_T + t;
+            if (x < n) out[k][x] = tr[i * SCAN_T + t];
+        }
+    }
+}
+
+hipError_t mf_compact_plan_counts(GraphState& S, int64_t* nidx, int64_t* doff, int64_t* toff, int32_t* trows,
+                                  u64* ntrows, int64_t* part) {
+    const int64_t nb = (S.g.n + SCAN_TILE - 1) / SCAN_TILE;
+    k_plan_reduce<<<(unsigned)nb, SCAN_T, 0, S.stream>>>(S.g, part);
+    k_plan_parts<<<MF_PLAN_COLS, SCAN_T, 0, S.stream>>>(part, nb);
+    k_plan_final<<<(unsigned)nb, SCAN_T, 0, S.stream>>>(S.g, part, doff, toff, nidx, trows, ntrows);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------- compaction
 // New row lengths: old length - deleted base entries + live overlay entries.
 __global__ __launch_bounds__(256) void k_newdeg(DevGraph g, int64_t* deg) {
@@ -1226,34 +1337,6 @@ __global__ __launch_bounds__(256) void k_merge_tail_empty(const int64_t* nindex,
     }
 }
 
-// Per-row deletion and tail counts, and the rows with a tail listed (one
-// atomic per workgroup, as in k_newdeg).
-__global__ __launch_bounds__(256) void k_merge_counts(DevGraph g, int64_t* dcnt, int64_t* tcnt, int32_t* trows,
-                                                      u64* ntrows) {
-    __shared__ unsigned int cnt, cur;
-    __shared__ u64 base;
-    if (threadIdx.x == 0) {
-        cnt = 0;
-        cur = 0;
-    }
-    __syncthreads();
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    unsigned int mine = 0;
-    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < g.n; x += stride) {
-        dcnt[x] = g.ov_tomb[x];
-        int32_t a = g.ov_alive[x];
-        tcnt[x] = a;
-        mine += a ? 1u : 0u;
-    }
-    if (mine) atomicAdd(&cnt, mine);
-    __syncthreads();
-    if (threadIdx.x == 0 && cnt) base = atomicAdd(ntrows, (u64)cnt);
-    __syncthreads();
-    if (!cnt) return;
-    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < g.n; x += stride)
-        if (g.ov_alive[x]) trows[base + atomicAdd(&cur, 1u)] = (int32_t)x;
-}
-
 // Sorts the per-row lists of the rows listed in rows[0, *nrows) by rank
 // (values are distinct within a row: each entry goes to the number of
 // smaller ones).  A wave takes 64 listed rows at a time: a list of at most
@@ -1308,15 +1391,6 @@ __global__ __launch_bounds__(256) void k_sort_small(const int64_t* off, const in
             }
         }
     }
-}
-
-hipError_t mf_merge_counts(GraphState& S, int64_t* dcnt, int64_t* tcnt, int64_t* doff, int64_t* toff, int32_t* trows,
-                           u64* ntrows) {
-    k_fill<u64><<<1, 64, 0, S.stream>>>(ntrows, 0ull, 1);
-    k_merge_counts<<<S.blocks, 256, 0, S.stream>>>(S.g, dcnt, tcnt, trows, ntrows);
-    mf_scan_excl(dcnt, S.g.n, doff, S.d_scan_part, S.stream);
-    mf_scan_excl(tcnt, S.g.n, toff, S.d_scan_part, S.stream);
-    return hipGetLastError();
 }
 
 hipError_t mf_build_row_index(GraphState& S, int32_t* sval, int32_t* spos, int* d_flags) {
