@@ -140,6 +140,17 @@ def test_update_streams(engine_mod, oracle):
     assert deg[u3].max() <= q and deg[v3].max() <= q
 
 
+def test_bench_refuses_more_than_one_gpu():
+    """The bench measures one GPU: --gpus above 1 exits with a message
+    before anything touches a device."""
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in r.stderr and "--gpus 2" in r.stderr
+    assert r.stdout.strip() == ""  # no JSON line
+
+
 def test_binaries_usage_without_gpu(engine_mod):
     for b in ("misfit", "misfit_async"):
         exe = os.path.join(engine_mod.BIN_DIR, b)
