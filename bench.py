@@ -577,6 +577,18 @@ def compact_line(M, args, copy):
         shutil.rmtree(tmpdir, ignore_errors=True)
     mean = float(np.mean(ms))
     bc = 8 * (n + 1) + 4 * m + 4 * pool + 8 * (n + 1) + 4 * int(new_m)
+    # HBM bytes of one compaction (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated), committed
+    # for this workload
+    traffic, traffic_src = None, None
+    if args.seed == 7:
+        rel = os.path.join("profiles", "compaction_rmat22_traffic.md")
+        try:
+            with open(os.path.join(ROOT, rel)) as f:
+                for line in f:
+                    if line.startswith("traffic_bytes_per_compaction:"):
+                        traffic, traffic_src = int(line.split(":")[1]), rel
+        except OSError:
+            pass
     ach = bc / (mean / 1e3) / 1e9
     return {"metric": "overlay compaction time (merge into a new base, sorted row copy rebuilt)",
             "value": round(mean, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -588,7 +600,7 @@ def compact_line(M, args, copy):
                        "name": "compact", "graph_entries": m, "overlay_slots": pool, "entries_after": int(new_m),
                        "gpu_compact_plus_export_and_write_ms": round(with_write, 2), "device": name},
             "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None,
+                         "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": traffic_src,
                          "kernel": "misfit_compact (all its kernels) timed by the host around the call",
                          "algorithmic_bytes_per_compaction": int(bc),
                          "stream_copy_gbs": copy["copy_gbs"] if copy else None},
