@@ -10,6 +10,24 @@ def sample_inputs(oracle, sample_dir):
     return g, mis, batches
 
 
+# The rule's per-batch counters, reported alike by the oracle and the engine:
+# insertions with both ends in the MIS (each evicts), admissions on that
+# branch, deletions with exactly one end in the MIS, admissions on that
+# branch, and completion additions.
+RULE_COUNTERS = ("active_ins", "evictions", "admit_ins", "active_del", "admit_del", "completion_adds")
+
+
+def counter_mismatch(want_stats, got_stats):
+    """The first rule counter on which engine and oracle differ ("" when none)."""
+    if len(want_stats) != len(got_stats):
+        return "%d oracle batches, %d engine batches" % (len(want_stats), len(got_stats))
+    for b, (w, g) in enumerate(zip(want_stats, got_stats)):
+        for k in RULE_COUNTERS:
+            if w[k] != g[k]:
+                return "batch %d %s: oracle %d, engine %d" % (b, k, w[k], g[k])
+    return ""
+
+
 def oracle_replay(oracle, nindex, nlist, mis0, batches, completion=True, literal=False):
     """Per-batch MIS (list of uint8 arrays), stats and the compacted CSR."""
     g = oracle.Graph.from_csr(nindex, nlist)

@@ -51,11 +51,12 @@ def test_random_parity(oracle, engine_mod, eng, mode, seed, general):
     mis0 = oracle.Graph.from_csr(ni, nl).greedy_mis()
     batches = H.random_batches(rng, oracle, ni, nl, 6, 80, general=general)
     for completion in (False, True):
-        want, _, og = H.oracle_replay(oracle, ni, nl, mis0, batches, completion=completion, literal=True)
-        got, _, checks = H.engine_replay(engine_mod, eng, ni, nl, mis0, batches, completion=completion, mode=mode)
+        want, wst, og = H.oracle_replay(oracle, ni, nl, mis0, batches, completion=completion, literal=True)
+        got, gst, checks = H.engine_replay(engine_mod, eng, ni, nl, mis0, batches, completion=completion, mode=mode)
         for b in range(len(batches)):
             assert np.array_equal(want[b], got[b]), "seed %d batch %d completion %s" % (seed, b, completion)
             assert checks[b][0] == 0 and checks[b][2] == 0
+        assert H.counter_mismatch(wst, gst) == "", "seed %d completion %s" % (seed, completion)
         for v in range(0, n, 7):
             assert np.array_equal(eng.neighbors(v), og.neighbors(v)), "row %d order" % v
 
@@ -69,11 +70,12 @@ def test_deletion_heavy_parity(oracle, engine_mod, eng, mode, seed):
     mis0 = g.greedy_mis()
     raw = H.existing_edge_batches(rng, ni, nl, 6, 120)
     batches = [(u, v, g.flags(u, v)) for (u, v) in raw]
-    want, _, _ = H.oracle_replay(oracle, ni, nl, mis0, batches, completion=True, literal=True)
-    got, _, checks = H.engine_replay(engine_mod, eng, ni, nl, mis0, batches, completion=True, mode=mode)
+    want, wst, _ = H.oracle_replay(oracle, ni, nl, mis0, batches, completion=True, literal=True)
+    got, gst, checks = H.engine_replay(engine_mod, eng, ni, nl, mis0, batches, completion=True, mode=mode)
     for b in range(len(batches)):
         assert np.array_equal(want[b], got[b]), "batch %d" % b
         assert checks[b] == (0, 0, 0)
+    assert H.counter_mismatch(wst, gst) == ""
 
 
 def test_flags_parity(oracle, engine_mod, eng):
@@ -517,6 +519,35 @@ def test_tuning_knobs_keep_results(oracle, engine_mod, mode, knobs, monkeypatch)
     want, _, _ = H.oracle_replay(oracle, ni, nl, mis0, batches, completion=True, literal=False)
     for a, b in zip(want, got):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("knobs", [{"MISFIT_GRID_ROUNDS": "200"}, {}, {"MISFIT_GRID_ROUNDS": "1"}])
+def test_rule_counters_match_oracle(oracle, engine_mod, knobs, monkeypatch):
+    """Queued batches report the oracle's rule counters (active insertions,
+    evictions, admissions on both branches, completion additions) whether
+    their edges are committed by grid-wide rounds (200 fixed rounds: every
+    batch ends in the grid, the rest run empty), by the planned rounds, or
+    mostly by the one-workgroup tail (one grid round), with the MIS equal
+    after every batch."""
+    monkeypatch.delenv("MISFIT_GRID_ROUNDS", raising=False)
+    for k, v in knobs.items():
+        monkeypatch.setenv(k, v)
+    ni, nl = engine_mod.gen_rmat(15, 16, 17)
+    mis0 = engine_mod.greedy_mis(ni, nl)
+    u, v = engine_mod.gen_updates(engine_mod.MIXED_EXISTING, ni, nl, 9000, 17)
+    g = oracle.Graph.from_csr(ni, nl)
+    ins = g.flags(u, v)
+    sizes = [400, 2600, 5000, 1000]
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    batches = [(u[a:b], v[a:b], ins[a:b]) for a, b in zip(offs[:-1], offs[1:])]
+    with engine_mod.Engine(0) as e:
+        got, gst, checks = H.engine_replay(engine_mod, e, ni, nl, mis0, batches, mode=1)
+    assert all(c == (0, 0, 0) for c in checks)
+    want, wst, _ = H.oracle_replay(oracle, ni, nl, mis0, batches, completion=True, literal=False)
+    for a, b in zip(want, got):
+        assert np.array_equal(a, b)
+    assert sum(s["evictions"] for s in wst) > 0 and sum(s["admit_del"] for s in wst) > 0
+    assert H.counter_mismatch(wst, gst) == ""
 
 
 @pytest.mark.parametrize("mode", MODES)
