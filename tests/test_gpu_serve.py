@@ -30,9 +30,9 @@ def _workload(engine_mod, oracle, kind, size, seed, nb=5, bs=3000):
     return ni, nl, mis0, batches
 
 
-def _oracle_with_compactions(oracle, ni, nl, mis0, batches, before):
+def _oracle_with_compactions(oracle, ni, nl, mis0, batches, before, stats=None):
     """Replay with a compaction wherever `before` (compactions before each
-    batch) steps up."""
+    batch) steps up; each batch's rule counters go to `stats` when given."""
     g = oracle.Graph.from_csr(ni, nl)
     mis = np.array(mis0, np.uint8)
     done = 0
@@ -41,7 +41,9 @@ def _oracle_with_compactions(oracle, ni, nl, mis0, batches, before):
             cni, cnl = g.compact()
             g = oracle.Graph.from_csr(cni, cnl)
             done += 1
-        g.process_batch(mis, u, v, ins, completion=True, literal=False)
+        st = g.process_batch(mis, u, v, ins, completion=True, literal=False)
+        if stats is not None:
+            stats.append(st)
     return mis
 
 
@@ -67,10 +69,12 @@ def test_served_graphs_match_oracle_with_compactions(oracle, engine_mod, infligh
             for s in stats:
                 assert s["graph"] == gid and s["updates"] == 3000
                 assert 0 <= s["queue_ms"] <= s["latency_ms"] and s["device_ms"] > 0
-            want = _oracle_with_compactions(oracle, ni, nl, mis0, batches, before)
+            ost = []
+            want = _oracle_with_compactions(oracle, ni, nl, mis0, batches, before, ost)
             got = srv.get_mis(gid, len(ni) - 1)
             assert np.array_equal(got, want), "graph %d" % gid
             assert srv.check(gid) == (0, 0, 0)
+            assert H.counter_mismatch(ost, [s["batch"] for s in stats]) == "", "graph %d" % gid
         # the bound is low enough that compactions did run
         assert any(srv.wait(srv.submit(g, engine_mod.make_updates(np.zeros(0, np.int32), np.zeros(0, np.int32),
                                                                     np.zeros(0, np.uint8))))["compactions_before"] > 0
@@ -288,13 +292,15 @@ def test_serving_kernel_graphs_alone_and_in_the_mix(oracle, engine_mod):
     from one queue): each graph's MIS is the same served alone and in the
     mix of all sixteen, and the oracle's."""
     graphs = _mixed_graphs(engine_mod, oracle, 16, 3, 7)
-    want = [H.oracle_replay(oracle, ni, nl, mis0, batches, completion=True, literal=False)[0][-1]
+    reps = [H.oracle_replay(oracle, ni, nl, mis0, batches, completion=True, literal=False)
             for (ni, nl, mis0, batches) in graphs]
+    want = [r[0][-1] for r in reps]
     mix, stats, checks = _serve_all(engine_mod, graphs, compact_fraction=0.0, max_inflight=1, workers=4,
                                     persistent=2)
     for gi in range(len(graphs)):
         assert np.array_equal(mix[gi], want[gi]), "graph %d in the mix" % gi
         assert checks[gi] == (0, 0, 0)
+        assert H.counter_mismatch(reps[gi][1], [s["batch"] for s in stats[gi]]) == "", "graph %d" % gi
         assert all(s["device_ms"] > 0 and s["batch"]["rounds"] >= 0 for s in stats[gi])
     for gi in (0, 5, 10, 15):  # alone: a server holding only this graph
         alone, _, checks1 = _serve_all(engine_mod, graphs[gi:gi + 1], compact_fraction=0.0, max_inflight=1,
@@ -318,7 +324,9 @@ def test_serving_kernel_beside_streams_with_compactions(oracle, engine_mod):
     for gi, (ni, nl, mis0, batches) in enumerate(graphs):
         before = [s["compactions_before"] for s in stats[gi]]
         compacted += before[-1]
-        want = _oracle_with_compactions(oracle, ni, nl, mis0, batches, before)
+        ost = []
+        want = _oracle_with_compactions(oracle, ni, nl, mis0, batches, before, ost)
         assert np.array_equal(mis[gi], want), "graph %d" % gi
         assert checks[gi] == (0, 0, 0)
+        assert H.counter_mismatch(ost, [s["batch"] for s in stats[gi]]) == "", "graph %d" % gi
     assert compacted > 0
