@@ -18,13 +18,14 @@ RULE_COUNTERS = ("active_ins", "evictions", "admit_ins", "active_del", "admit_de
 
 
 def counter_mismatch(want_stats, got_stats):
-    """The first rule counter on which engine and oracle differ ("" when none)."""
+    """The first rule counter on which two runs' per-batch stats differ ("" when
+    none): the oracle's (or a reference run's) first, the engine's second."""
     if len(want_stats) != len(got_stats):
-        return "%d oracle batches, %d engine batches" % (len(want_stats), len(got_stats))
+        return "%d batches against %d" % (len(want_stats), len(got_stats))
     for b, (w, g) in enumerate(zip(want_stats, got_stats)):
         for k in RULE_COUNTERS:
             if w[k] != g[k]:
-                return "batch %d %s: oracle %d, engine %d" % (b, k, w[k], g[k])
+                return "batch %d %s: %d expected, %d reported" % (b, k, w[k], g[k])
     return ""
 
 

@@ -38,14 +38,15 @@ def test_rgg_wide_form_parity(oracle, engine_mod, tmp_path):
     u, v = u[keep], v[keep]
     ins = oracle.Graph.from_csr(ni, nl).flags(u, v)
     batches = _batches(u, v, ins, 4000)
-    want, _, _ = H.oracle_replay(oracle, ni, nl, mis0, batches, completion=True, literal=False)
+    want, wst, _ = H.oracle_replay(oracle, ni, nl, mis0, batches, completion=True, literal=False)
     with engine_mod.Engine(0) as eng:
         eng.load_egr(p)
         for mode in (engine_mod.MODE_QUEUED, engine_mod.MODE_PLAIN):
-            got, _, checks = H.engine_replay(engine_mod, eng, ni2, nl2, mis0, batches, mode=mode)
+            got, gst, checks = H.engine_replay(engine_mod, eng, ni2, nl2, mis0, batches, mode=mode)
             for b in range(len(batches)):
                 assert np.array_equal(want[b], got[b]), "mode %d batch %d" % (mode, b)
                 assert checks[b] == (0, 0, 0)
+            assert H.counter_mismatch(wst, gst) == "", "mode %d" % mode
 
 
 def test_low_degree_stream_parity(oracle, engine_mod):
@@ -57,12 +58,13 @@ def test_low_degree_stream_parity(oracle, engine_mod):
     u, v = u[keep], v[keep]
     ins = oracle.Graph.from_csr(ni, nl).flags(u, v)
     batches = _batches(u, v, ins, 3000)
-    want, _, _ = H.oracle_replay(oracle, ni, nl, mis0, batches, completion=True, literal=False)
+    want, wst, _ = H.oracle_replay(oracle, ni, nl, mis0, batches, completion=True, literal=False)
     with engine_mod.Engine(0) as eng:
-        got, _, checks = H.engine_replay(engine_mod, eng, ni, nl, mis0, batches)
+        got, gst, checks = H.engine_replay(engine_mod, eng, ni, nl, mis0, batches)
     for b in range(len(batches)):
         assert np.array_equal(want[b], got[b]), "batch %d" % b
         assert checks[b] == (0, 0, 0)
+    assert H.counter_mismatch(wst, gst) == ""
 
 
 def test_rgg_full_size_properties(engine_mod):

@@ -242,11 +242,12 @@ def test_er_config_parity(oracle, engine_mod, eng, mode):
     g = oracle.Graph.from_csr(ni, nl)
     ins = g.flags(u, v)
     batches = [(u[i:i + 2000], v[i:i + 2000], ins[i:i + 2000]) for i in range(0, 20000, 2000)]
-    want, _, _ = H.oracle_replay(oracle, ni, nl, mis0, batches, completion=True, literal=False)
+    want, wst, _ = H.oracle_replay(oracle, ni, nl, mis0, batches, completion=True, literal=False)
     got, gst, checks = H.engine_replay(engine_mod, eng, ni, nl, mis0, batches, completion=True, mode=mode)
     for b in range(len(batches)):
         assert np.array_equal(want[b], got[b]), "batch %d" % b
         assert checks[b] == (0, 0, 0)
+    assert H.counter_mismatch(wst, gst) == ""
 
 
 def test_determinism(oracle, engine_mod, eng):

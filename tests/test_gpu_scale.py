@@ -44,7 +44,7 @@ def test_rmat22_stream_plain_checked_queued_equal(engine_mod):
     u, v = engine_mod.gen_updates(engine_mod.MIXED_EXISTING, ni, nl, nb * bs, 7)
     _say("R-MAT 22: %d entries, %d updates generated in %.1f s" % (len(nl), nb * bs, time.time() - t))
     assert len(nl) == 128_303_020
-    finals = {}
+    finals, counters = {}, {mode: [] for mode in (engine_mod.MODE_PLAIN, engine_mod.MODE_QUEUED)}
     for mode in (engine_mod.MODE_PLAIN, engine_mod.MODE_QUEUED):
         with engine_mod.Engine(0) as eng:
             eng.load_csr(ni, nl)
@@ -55,19 +55,23 @@ def test_rmat22_stream_plain_checked_queued_equal(engine_mod):
                 for b in range(nb):
                     st = eng.process_batch(ups[b * bs:(b + 1) * bs])
                     assert st["updates"] == bs
+                    counters[mode].append(st)
                     assert eng.check() == (0, 0, 0), "plain batch %d" % b
             else:
                 tickets = []
                 for b in range(nb):
                     tickets.append(eng.submit(ups[b * bs:(b + 1) * bs]))
                     if len(tickets) == 2:
-                        eng.wait(tickets.pop(0))
+                        counters[mode].append(eng.wait(tickets.pop(0)))
                 for tk in tickets:
-                    eng.wait(tk)
+                    counters[mode].append(eng.wait(tk))
                 assert eng.check() == (0, 0, 0)
             finals[mode] = eng.get_mis()
         _say("mode %d done" % mode)
     assert np.array_equal(finals[engine_mod.MODE_PLAIN], finals[engine_mod.MODE_QUEUED])
+    # the rule's counters per batch agree between the drivers too
+    assert H.counter_mismatch(counters[engine_mod.MODE_PLAIN], counters[engine_mod.MODE_QUEUED]) == ""
+    assert sum(s["evictions"] for s in counters[engine_mod.MODE_PLAIN]) > 0
 
 
 @pytest.mark.timeout(900)
@@ -88,8 +92,10 @@ def test_rmat16_serving_large_batches_with_compactions_match_oracle(oracle, engi
     before = [s["compactions_before"] for s in stats]
     _say("compactions before each batch: %s" % before)
     assert before == sorted(before) and before[-1] >= 1
-    want = _oracle_with_compactions(oracle, ni, nl, mis0, batches, before)
+    ost = []
+    want = _oracle_with_compactions(oracle, ni, nl, mis0, batches, before, ost)
     assert np.array_equal(got, want)
+    assert H.counter_mismatch(ost, [s["batch"] for s in stats]) == ""
 
 
 @pytest.mark.timeout(1200)
