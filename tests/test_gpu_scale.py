@@ -124,12 +124,13 @@ def test_rmat26_serving_full_size(engine_mod):
         plain.set_mode(engine_mod.MODE_PLAIN)
         with engine_mod.Server(0, compact_fraction=0.05, max_inflight=2) as srv:
             gid = srv.add_graph(ni, nl, mis0)
-            before = []
+            before, served_stats = [], []
             t = time.time()
             for i, (a, b) in enumerate(zip(offs[:-1], offs[1:])):
                 st = srv.wait(srv.submit(gid, ups[a:b]))
                 assert st["updates"] == b - a
                 before.append(st["compactions_before"])
+                served_stats.append(st["batch"])
                 assert srv.check(gid) == (0, 0, 0), "served batch %d" % i
                 if i % 50 == 0:
                     _say("served batch %d / %d (%.1f s), compactions %d" % (i, len(sizes), time.time() - t,
@@ -143,12 +144,15 @@ def test_rmat26_serving_full_size(engine_mod):
         assert before[-1] >= 1
         done = 0
         t = time.time()
+        plain_stats = []
         for i in range(nserved):
             while done < before[i]:
                 plain.compact()
                 done += 1
-            plain.process_batch(ups[offs[i]:offs[i + 1]])
+            plain_stats.append(plain.process_batch(ups[offs[i]:offs[i + 1]]))
             if i % 50 == 0:
                 _say("plain batch %d / %d (%.1f s)" % (i, nserved, time.time() - t))
         assert plain.check() == (0, 0, 0)
         assert np.array_equal(plain.get_mis(), served)
+        # the rule's counters per batch agree between the served and the plain driver
+        assert H.counter_mismatch(plain_stats, served_stats) == ""
