@@ -308,27 +308,29 @@ def test_oracle_matches_brute_force(oracle, seed):
         for literal in (True, False):
             gg = oracle.Graph.from_csr(ni, nl)
             m = mis0.copy()
-            for (u, v, ins) in batches:
-                gg.process_batch(m, u, v, ins, completion=completion, literal=literal)
-            bm, rows_state = _brute_multi(n, adj0, [int(x) for x in mis0], batches, completion)
+            ost = [gg.process_batch(m, u, v, ins, completion=completion, literal=literal) for (u, v, ins) in batches]
+            bm, rows_state, bst = _brute_multi(n, adj0, [int(x) for x in mis0], batches, completion)
             assert [int(x) for x in m] == bm, (seed, completion, literal)
             for x in range(n):
                 assert gg.neighbors(x).tolist() == rows_state[x]
+            # the rule's counters, batch by batch
+            assert H.counter_mismatch(bst, ost) == "", (seed, completion, literal)
 
 
 def _brute_multi(n, adj0, mis, batches, completion):
     """Independent restatement on Python lists, batch after batch: the overlay
     before a batch is rebuilt from the base plus every earlier update, rows
     are explicit BFS lists, the rule runs in batch order, then completion."""
-    hist = []
+    hist, counts = [], []
     for (u, v, ins) in batches:
         cur = list(zip(u.tolist(), v.tolist(), ins.tolist()))
         # graph state before this batch = base plus earlier updates; realise it
         # by replaying them as plain graph updates, then the batch with the rule
         state_adj, removed_added = _materialise(n, adj0, hist)
-        mis, rows = _brute_from_state(n, adj0, state_adj, removed_added, mis, cur, completion)
+        mis, rows, cnt = _brute_from_state(n, adj0, state_adj, removed_added, mis, cur, completion)
+        counts.append(cnt)
         hist.extend(cur)
-    return mis, rows
+    return mis, rows, counts
 
 
 def _materialise(n, adj0, hist):
@@ -417,25 +419,33 @@ def _brute_from_state(n, adj0, _unused, ra, mis, batch, completion):
                 rows[x] = nbrs(x)
     mis = list(mis)
     evicted = []
+    cnt = dict.fromkeys(H.RULE_COUNTERS, 0)
     for (a, b, ins) in batch:
         if ins:
             if mis[a] and mis[b]:
+                cnt["active_ins"] += 1
                 w = max(a, b)
                 mis[w] = 0
                 evicted.append(w)
+                cnt["evictions"] += 1
                 for x in rows.get(w, []):
                     if mis[x] or x not in rows:
                         continue
                     if not any(mis[y] for y in rows[x]):
                         mis[x] = 1
+                        cnt["admit_ins"] += 1
                         break
         else:
             if mis[a] and not mis[b]:
+                cnt["active_del"] += 1
                 if not any(mis[y] for y in rows[b] if y != a):
                     mis[b] = 1
+                    cnt["admit_del"] += 1
             elif not mis[a] and mis[b]:
+                cnt["active_del"] += 1
                 if not any(mis[y] for y in rows[a] if y != b):
                     mis[a] = 1
+                    cnt["admit_del"] += 1
     if completion:
         C = set(seeds) | set(evicted)
         for w in evicted:
@@ -443,7 +453,8 @@ def _brute_from_state(n, adj0, _unused, ra, mis, batch, completion):
         for x in sorted(C):
             if not mis[x] and not any(mis[y] for y in nbrs(x)):
                 mis[x] = 1
-    return mis, [nbrs(x) for x in range(n)]
+                cnt["completion_adds"] += 1
+    return mis, [nbrs(x) for x in range(n)], cnt
 
 
 # ------------------------------------------------------------ engines + invariants
@@ -454,10 +465,29 @@ def test_literal_and_lazy_agree(oracle, seed):
     mis0 = oracle.Graph.from_csr(ni, nl).greedy_mis()
     batches = H.random_batches(rng, oracle, ni, nl, 5, 60, general=bool(seed % 2))
     for completion in (False, True):
-        a, _, _ = H.oracle_replay(oracle, ni, nl, mis0, batches, completion, literal=True)
-        b, _, _ = H.oracle_replay(oracle, ni, nl, mis0, batches, completion, literal=False)
+        a, ast, _ = H.oracle_replay(oracle, ni, nl, mis0, batches, completion, literal=True)
+        b, bst, _ = H.oracle_replay(oracle, ni, nl, mis0, batches, completion, literal=False)
         for x, y in zip(a, b):
             assert np.array_equal(x, y)
+        assert H.counter_mismatch(ast, bst) == ""
+
+
+def test_counter_check_catches_a_planted_miscount(oracle):
+    """The per-batch counter comparison names the batch and the counter of a
+    single miscount planted in a copy of the oracle's stats."""
+    rng = np.random.default_rng(77)
+    ni, nl = H.random_graph(rng, 250, 0.04)
+    mis0 = oracle.Graph.from_csr(ni, nl).greedy_mis()
+    batches = H.random_batches(rng, oracle, ni, nl, 4, 60, general=False)
+    _, st, _ = H.oracle_replay(oracle, ni, nl, mis0, batches, completion=True, literal=False)
+    assert H.counter_mismatch(st, [dict(s) for s in st]) == ""
+    assert sum(s["evictions"] for s in st) > 0
+    for k in H.RULE_COUNTERS:
+        planted = [dict(s) for s in st]
+        planted[2][k] += 1
+        msg = H.counter_mismatch(st, planted)
+        assert msg.startswith("batch 2 %s:" % k), msg
+    assert H.counter_mismatch(st, st[:-1]) != ""
 
 
 @pytest.mark.parametrize("seed", range(4))
