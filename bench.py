@@ -108,21 +108,26 @@ def stream_copy_gbs(mib=2048, reps=20):
     return json.loads(r.stdout.strip().splitlines()[-1])
 
 
+def committed_figure(rel, key):
+    """The integer after `key:` at the start of a line of a committed rocprofv3
+    summary, with the summary's path; (None, None) when either is missing."""
+    try:
+        with open(os.path.join(ROOT, rel)) as f:
+            for line in f:
+                if line.startswith(key + ":"):
+                    return int(line.split(":", 1)[1]), rel
+    except OSError:
+        pass
+    return None, None
+
+
 def committed_traffic(args):
     """HBM bytes per batch measured by rocprofv3 FETCH_SIZE + WRITE_SIZE (scaled by
     the copy-kernel calibration) for the default workload, from the committed
     summary; None for any other workload."""
     if (args.scale, args.batch, args.seed) != (22, 10000, 7):
         return None, None
-    rel = os.path.join("profiles", "queued_batch_rmat22_traffic.md")
-    try:
-        with open(os.path.join(ROOT, rel)) as f:
-            for line in f:
-                if line.startswith("traffic_bytes_per_batch:"):
-                    return int(line.split(":")[1]), rel
-    except OSError:
-        pass
-    return None, None
+    return committed_figure(os.path.join("profiles", "queued_batch_rmat22_traffic.md"), "traffic_bytes_per_batch")
 
 
 KERNEL_NOTE = ("queued batch (prologue, grid rounds, finisher): device time of the timed region by a pair of HIP "
@@ -581,14 +586,8 @@ def compact_line(M, args, copy):
     # for this workload
     traffic, traffic_src = None, None
     if args.seed == 7:
-        rel = os.path.join("profiles", "compaction_rmat22_traffic.md")
-        try:
-            with open(os.path.join(ROOT, rel)) as f:
-                for line in f:
-                    if line.startswith("traffic_bytes_per_compaction:"):
-                        traffic, traffic_src = int(line.split(":")[1]), rel
-        except OSError:
-            pass
+        traffic, traffic_src = committed_figure(os.path.join("profiles", "compaction_rmat22_traffic.md"),
+                                                "traffic_bytes_per_compaction")
     ach = bc / (mean / 1e3) / 1e9
     return {"metric": "overlay compaction time (merge into a new base, sorted row copy rebuilt)",
             "value": round(mean, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

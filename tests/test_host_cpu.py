@@ -151,6 +151,24 @@ def test_bench_refuses_more_than_one_gpu():
     assert r.stdout.strip() == ""  # no JSON line
 
 
+def test_bench_reads_the_committed_traffic_figures():
+    """The bench lines' measured HBM traffic comes from the committed rocprofv3
+    summaries: both figures parse, and a missing file or key gives none."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench_for_test", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    b, src = bench.committed_figure(os.path.join("profiles", "queued_batch_rmat22_traffic.md"),
+                                    "traffic_bytes_per_batch")
+    assert b is not None and b > 0 and src.endswith("queued_batch_rmat22_traffic.md")
+    c, src = bench.committed_figure(os.path.join("profiles", "compaction_rmat22_traffic.md"),
+                                    "traffic_bytes_per_compaction")
+    assert c is not None and c > b and src.endswith("compaction_rmat22_traffic.md")
+    assert bench.committed_figure(os.path.join("profiles", "no_such_summary.md"), "traffic") == (None, None)
+    assert bench.committed_figure(os.path.join("profiles", "compaction_rmat22_traffic.md"), "no_such_key") == \
+        (None, None)
+
+
 def test_binaries_usage_without_gpu(engine_mod):
     for b in ("misfit", "misfit_async"):
         exe = os.path.join(engine_mod.BIN_DIR, b)
